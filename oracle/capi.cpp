@@ -1,0 +1,404 @@
+// Oracle C API for ctypes (tests/, smoke check, bench cpu_baseline only).
+// Test infrastructure: the product never links this library.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+
+#include "oracle.hpp"
+
+using namespace orc;
+using i64 = int64_t;
+
+namespace {
+struct SpaceH {
+  SpaceBox sp;
+  bool have_em = false;
+  ElementMatrices em;
+  std::unique_ptr<CsrMatrix> csr;
+};
+struct ImmH {
+  ImmersedProblem pb;
+};
+struct TopoH {
+  std::unique_ptr<CantileverTopOpt> t;
+};
+}  // namespace
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+ORC_API void* orc_space_new(i64 dim, const double* bounds, const i64* nsub, i64 degree) {
+  auto* h = new SpaceH();
+  std::vector<std::array<double, 2>> b;
+  std::vector<usize> n;
+  for (i64 d = 0; d < dim; ++d) {
+    b.push_back({bounds[2 * d], bounds[2 * d + 1]});
+    n.push_back((usize)nsub[d]);
+  }
+  h->sp = SpaceBox::make(b, n, (usize)degree);
+  return h;
+}
+ORC_API void orc_space_free(void* p) { delete (SpaceH*)p; }
+
+// out: dim, nel, ndof_sc, ndof, nsh, nsh_sc, ndof_dir[3], nel_dir[3]
+ORC_API void orc_space_sizes(void* p, i64* out) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  out[0] = (i64)s.dim;
+  out[1] = (i64)s.nel;
+  out[2] = (i64)s.ndof_sc;
+  out[3] = (i64)s.ndof;
+  out[4] = (i64)s.nsh;
+  out[5] = (i64)s.nsh_sc;
+  for (int d = 0; d < 3; ++d) {
+    out[6 + d] = d < (int)s.dim ? (i64)s.ndof_dir[d] : 1;
+    out[9 + d] = d < (int)s.dim ? (i64)s.nel_dir[d] : 1;
+  }
+}
+
+ORC_API void orc_space_connectivity(void* p, i64* out) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  for (usize i = 0; i < s.connectivity.size(); ++i) out[i] = (i64)s.connectivity[i];
+}
+
+ORC_API i64 orc_space_elem_mats(void* p, double lambda, double mu) {
+  auto* h = (SpaceH*)p;
+  h->em = elasticity_element_matrices(h->sp, lambda, mu);
+  h->have_em = true;
+  return (i64)h->em.types.size();
+}
+
+ORC_API void orc_space_types(void* p, double* types, i64* type_id) {
+  auto* h = (SpaceH*)p;
+  usize nn = h->em.nsh * h->em.nsh;
+  if (types)
+    for (usize t = 0; t < h->em.types.size(); ++t) std::memcpy(types + t * nn, h->em.types[t].data(), nn * sizeof(double));
+  if (type_id)
+    for (usize e = 0; e < h->em.type_id.size(); ++e) type_id[e] = (i64)h->em.type_id[e];
+}
+
+// Body force of the reference's kernel fixture (iga_vs_matlab.rs:62-68).
+ORC_API void orc_space_load_fixture(void* p, double* out) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  usize dim = s.dim;
+  Vec f = load_vector(s, [dim](const std::array<double, 3>& x) -> std::array<double, 3> {
+    if (dim == 2) return {x[0] * x[1], std::sin(x[0]) + x[1] * x[1], 0.0};
+    return {x[0] * x[1], std::sin(x[0]) + x[2], x[1] * x[1]};
+  });
+  std::memcpy(out, f.data(), f.size() * sizeof(double));
+}
+
+// Consistent traction t on the face x = x_max: F(A,c) = t_c * int_face N_A dA,
+// by (p+1)-point Gauss on each face element (exact for the spline trace).
+ORC_API void orc_space_traction_xmax(void* p, const double* t, double* out) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  std::memset(out, 0, s.ndof * sizeof(double));
+  const Space1d& uy = s.univ[1];
+  const Space1d& uz = s.univ[2];
+  usize i_last = s.ndof_dir[0] - 1;
+  for (usize ez = 0; ez < uz.nel; ++ez)
+    for (usize ey = 0; ey < uy.nel; ++ey)
+      for (usize qz = 0; qz < uz.nquad; ++qz)
+        for (usize qy = 0; qy < uy.nquad; ++qy) {
+          double w = uy.qw[ey * uy.nquad + qy] * s.lengths[1] * uz.qw[ez * uz.nquad + qz] * s.lengths[2];
+          for (usize az = 0; az < uz.nsh(); ++az)
+            for (usize ay = 0; ay < uy.nsh(); ++ay) {
+              double n = uy.shape[(ey * uy.nquad + qy) * uy.nsh() + ay] * uz.shape[(ez * uz.nquad + qz) * uz.nsh() + az];
+              usize gy = uy.connectivity[ey * uy.nsh() + ay], gz = uz.connectivity[ez * uz.nsh() + az];
+              usize g = i_last + s.ndof_dir[0] * (gy + s.ndof_dir[1] * gz);
+              for (usize c = 0; c < 3; ++c) out[c * s.ndof_sc + g] += w * n * t[c];
+            }
+        }
+}
+
+ORC_API void orc_space_greville(void* p, double* out) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  auto g = s.greville_points();
+  for (usize i = 0; i < g.size(); ++i)
+    for (int d = 0; d < 3; ++d) out[i * 3 + d] = g[i][d];
+}
+
+// Assemble sum_e scale_e K_e (sparse.rs:179-200; scale NULL = 1) and cache it.
+ORC_API i64 orc_space_assemble(void* p, const double* scale) {
+  auto* h = (SpaceH*)p;
+  if (!h->have_em) return -1;
+  h->csr.reset(new CsrMatrix(orc::assemble(h->sp.ndof, h->sp.connectivity, h->sp.nsh, h->sp.nel, h->em, scale)));
+  return (i64)h->csr->nnz();
+}
+
+// Assemble with the reusable pattern path (AssemblyPattern, sparse.rs:128-175).
+ORC_API i64 orc_space_assemble_pattern(void* p, const double* scale) {
+  auto* h = (SpaceH*)p;
+  if (!h->have_em) return -1;
+  AssemblyPattern pat(h->sp.ndof, h->sp.connectivity, h->sp.nsh, h->sp.nel);
+  Vec s(scale, scale + h->sp.nel);
+  h->csr.reset(new CsrMatrix(pat.assemble(h->em, s)));
+  return (i64)h->csr->nnz();
+}
+
+ORC_API void orc_csr_matvec(void* p, const double* x, double* y) { ((SpaceH*)p)->csr->matvec(x, y); }
+ORC_API void orc_csr_diag(void* p, double* d) {
+  Vec v = ((SpaceH*)p)->csr->diagonal();
+  std::memcpy(d, v.data(), v.size() * sizeof(double));
+}
+ORC_API double orc_csr_frobenius(void* p) { return ((SpaceH*)p)->csr->frobenius(); }
+
+// Masked Jacobi PCG on the cached CSR (the closure of topopt_iga.rs:151-159).
+ORC_API void orc_csr_pcg(void* p, const double* b, const double* diag, const uint8_t* fr, double tol, i64 maxit,
+                         double* x, i64* iters, double* res) {
+  auto* h = (SpaceH*)p;
+  usize n = h->sp.ndof;
+  Vec bb(b, b + n), dd(diag, diag + n), pm(n), tmp(n);
+  PcgResult r = pcg_solve((usize)maxit, tol, n, bb, dd, [&](const double* pv, double* ap) {
+    for (usize i = 0; i < n; ++i) pm[i] = fr[i] ? pv[i] : 0.0;
+    h->csr->matvec(pm.data(), tmp.data());
+    for (usize i = 0; i < n; ++i) ap[i] = fr[i] ? tmp[i] : 0.0;
+  });
+  std::memcpy(x, r.x.data(), n * sizeof(double));
+  *iters = (i64)r.iters;
+  *res = r.res;
+}
+
+// Unscaled element energies u_e^T K_e u_e (topopt_iga.rs:165-184).
+ORC_API void orc_space_energies(void* p, const double* u, double* out) {
+  auto* h = (SpaceH*)p;
+  const SpaceBox& s = h->sp;
+  usize nsh = s.nsh;
+#pragma omp parallel for schedule(static)
+  for (long e = 0; e < (long)s.nel; ++e) {
+    const double* ke = h->em.of_element((usize)e);
+    const usize* ce = &s.connectivity[(usize)e * nsh];
+    double acc = 0.0;
+    for (usize a = 0; a < nsh; ++a) {
+      double row = 0.0;
+      for (usize bb = 0; bb < nsh; ++bb) row += ke[a * nsh + bb] * u[ce[bb]];
+      acc += u[ce[a]] * row;
+    }
+    out[e] = acc;
+  }
+}
+
+ORC_API void orc_fast_sens(void* p, const double* u, const double* x, i64 spline, double penal, double emin,
+                           double young, double poisson, double* out) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  Vec uu(u, u + s.ndof), xx(x, x + (spline ? s.ndof_sc : s.nel));
+  Vec o = fast_sensitivities(s, uu, xx, spline != 0, penal, emin, young, poisson);
+  std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
+// Sensitivity filter rows of topopt_iga.rs:100-126 (structured enumeration).
+// mode 0: sensitivity filter dc~ = sum_f w x_f dc_f / (Hs_e max(x_e, 1e-3))
+// mode 1: density filter forward  x~ = H x / Hs
+// mode 2: density filter backward g_x = H^T (g / Hs)
+ORC_API void orc_filter_apply(void* p, double rmin, i64 mode, const double* x, const double* v, double* out) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  auto rows = build_filter_structured(s, rmin);
+  usize nel = s.nel;
+  Vec hs(nel);
+  for (usize e = 0; e < nel; ++e) {
+    double a = 0.0;
+    for (double w : rows[e].w) a += w;
+    hs[e] = a;
+  }
+  for (usize e = 0; e < nel; ++e) {
+    double acc = 0.0;
+    const auto& r = rows[e];
+    for (usize k = 0; k < r.f.size(); ++k) {
+      usize f = r.f[k];
+      if (mode == 0)
+        acc += r.w[k] * x[f] * v[f];
+      else if (mode == 1)
+        acc += r.w[k] * v[f];
+      else
+        acc += r.w[k] * (v[f] / hs[f]);  // H symmetric: (H^T g)_e = sum_f H_ef g_f
+    }
+    if (mode == 0)
+      out[e] = acc / (hs[e] * std::fmax(x[e], 1e-3));
+    else if (mode == 1)
+      out[e] = acc / hs[e];
+    else
+      out[e] = acc;
+  }
+}
+
+// Filter row check: returns 1 if the structured rows equal the O(nel^2) scan.
+ORC_API i64 orc_filter_rows_match(void* p, double rmin) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  auto a = build_filter_structured(s, rmin), b = build_filter_bruteforce(s, rmin);
+  for (usize e = 0; e < s.nel; ++e) {
+    if (a[e].f != b[e].f) return 0;
+    if (a[e].w != b[e].w) return 0;
+  }
+  return 1;
+}
+
+// ---- structured SIMP cantilever -----------------------------------------
+static CantileverCfg make_cfg(const i64* nel, const double* size, const double* par) {
+  CantileverCfg c;
+  for (int d = 0; d < 3; ++d) {
+    c.nel[d] = (usize)nel[d];
+    c.size[d] = size[d];
+  }
+  c.volfrac = par[0];
+  c.penal = par[1];
+  c.rmin = par[2];
+  c.max_iter = (usize)par[3];
+  c.young = par[4];
+  c.poisson = par[5];
+  c.emin = par[6];
+  c.move_limit = par[7];
+  c.pcg_tol = par[8];
+  return c;
+}
+
+// par = volfrac, penal, rmin, max_iter, young, poisson, emin, move, pcg_tol
+ORC_API i64 orc_topopt_run(const i64* nel, const double* size, const double* par, double* compliance, double* change,
+                           i64* pcg_iters, double* density) {
+  CantileverTopOpt t(make_cfg(nel, size, par));
+  auto h = t.run();
+  for (usize i = 0; i < h.compliance.size(); ++i) {
+    compliance[i] = h.compliance[i];
+    change[i] = h.change[i];
+    pcg_iters[i] = (i64)h.pcg_iterations[i];
+  }
+  std::memcpy(density, h.density.data(), h.density.size() * sizeof(double));
+  return (i64)h.compliance.size();
+}
+
+ORC_API void* orc_topopt_new(const i64* nel, const double* size, const double* par) {
+  auto* h = new TopoH();
+  h->t.reset(new CantileverTopOpt(make_cfg(nel, size, par)));
+  return h;
+}
+ORC_API void orc_topopt_free(void* p) { delete (TopoH*)p; }
+ORC_API void orc_topopt_force_free(void* p, double* force, uint8_t* fr) {
+  auto& t = *((TopoH*)p)->t;
+  std::memcpy(force, t.force.data(), t.force.size() * sizeof(double));
+  std::memcpy(fr, t.free_.data(), t.free_.size());
+}
+ORC_API void orc_topopt_solve_state(void* p, const double* x, const double* u0, double* u, i64* iters, double* res) {
+  auto& t = *((TopoH*)p)->t;
+  Vec xx(x, x + t.sp.nel), uu(u0, u0 + t.sp.ndof);
+  PcgResult r = t.solve_state(xx, uu);
+  std::memcpy(u, r.x.data(), r.x.size() * sizeof(double));
+  *iters = (i64)r.iters;
+  *res = r.res;
+}
+
+// ---- immersed -------------------------------------------------------------
+static TriangleMesh3D make_mesh(const double* nodes, i64 nv, const i64* tris, i64 nt) {
+  TriangleMesh3D m;
+  for (i64 i = 0; i < nv; ++i) m.vertices.push_back({nodes[3 * i], nodes[3 * i + 1], nodes[3 * i + 2]});
+  for (i64 i = 0; i < nt; ++i) m.triangles.push_back({(usize)tris[3 * i], (usize)tris[3 * i + 1], (usize)tris[3 * i + 2]});
+  return m;
+}
+
+// par = young, poisson, gamma, emin, padding; ipar = res[3], degree, stab_kind, clamp, sub[3]
+ORC_API void* orc_immersed_new(const double* nodes, i64 nv, const i64* tris, i64 nt, const i64* ipar, const double* par) {
+  auto* h = new ImmH();
+  TriangleMesh3D m = make_mesh(nodes, nv, tris, nt);
+  h->pb = build_immersed_problem(m, {(usize)ipar[0], (usize)ipar[1], (usize)ipar[2]}, (usize)ipar[3], par[0], par[1],
+                                 (int)ipar[4], par[2], par[3], {(usize)ipar[6], (usize)ipar[7], (usize)ipar[8]},
+                                 (int)ipar[5], par[4]);
+  return h;
+}
+ORC_API void orc_immersed_free(void* p) { delete (ImmH*)p; }
+ORC_API i64 orc_immersed_ndof(void* p) { return (i64)((ImmH*)p)->pb.sp.ndof; }
+// out: bounds[6], stab_frobenius
+ORC_API void orc_immersed_info(void* p, double* bounds, double* stab_fro, int8_t* status, double* weights,
+                               double* force, uint8_t* fr) {
+  auto& pb = ((ImmH*)p)->pb;
+  for (int d = 0; d < 3; ++d) {
+    bounds[2 * d] = pb.grid_bounds[d][0];
+    bounds[2 * d + 1] = pb.grid_bounds[d][1];
+  }
+  *stab_fro = pb.stabilization_frobenius;
+  std::memcpy(status, pb.status.data(), pb.status.size());
+  std::memcpy(weights, pb.weights.data(), pb.weights.size() * sizeof(double));
+  std::memcpy(force, pb.force.data(), pb.force.size() * sizeof(double));
+  std::memcpy(fr, pb.free_.data(), pb.free_.size());
+}
+ORC_API void orc_immersed_matvec(void* p, const double* x, double* y) { ((ImmH*)p)->pb.stiffness.matvec(x, y); }
+ORC_API void orc_immersed_solve(void* p, double tol, i64 maxit, double* u, double* compliance, i64* iters, double* res) {
+  auto& pb = ((ImmH*)p)->pb;
+  ImmersedSolution s = solve_immersed_problem(pb, tol, (usize)maxit);
+  std::memcpy(u, s.u.data(), s.u.size() * sizeof(double));
+  *compliance = s.compliance;
+  *iters = (i64)s.iterations;
+  *res = s.residual;
+}
+
+// ipar = res[3], max_iter, stab_kind ; par = volfrac, penal, rmin, young, poisson, emin, gamma, pcg_tol
+ORC_API i64 orc_immersed_topopt_run(const double* nodes, i64 nv, const i64* tris, i64 nt, const i64* ipar,
+                                    const double* par, double* compliance, double* volume, double* density) {
+  ImmersedTopOptCfg c;
+  for (int d = 0; d < 3; ++d) c.grid_res[d] = (usize)ipar[d];
+  c.max_iter = (usize)ipar[3];
+  c.stab_kind = (int)ipar[4];
+  c.volfrac = par[0];
+  c.penal = par[1];
+  c.rmin = par[2];
+  c.young = par[3];
+  c.poisson = par[4];
+  c.emin = par[5];
+  c.gamma = par[6];
+  c.pcg_tol = par[7];
+  auto h = immersed_topopt_run(make_mesh(nodes, nv, tris, nt), c);
+  for (usize i = 0; i < h.compliance.size(); ++i) {
+    compliance[i] = h.compliance[i];
+    volume[i] = h.volume[i];
+  }
+  std::memcpy(density, h.density.data(), h.density.size() * sizeof(double));
+  return (i64)h.compliance.size();
+}
+
+// Classification with the cylinder-hole SDF predicate (a box minus cylinders
+// parallel to z): inside iff inside the box and outside every cylinder, with a
+// strict sdf < 0.  cyl = [cx, cy, r] * ncyl.  Uses the reference's corner and
+// sub-cell logic (immersed.rs:88-155) with this predicate in place of the ray
+// caster.
+ORC_API void orc_classify_cylinders(const double* box_lo, const double* box_hi, const double* cyl, i64 ncyl,
+                                    const double* grid_bounds, const i64* res, const i64* sub, int8_t* status,
+                                    double* weights) {
+  InsideFn inside = [&](const std::array<double, 3>& q) {
+    double sdf_box = -1e300;
+    for (int d = 0; d < 3; ++d) {
+      double a = box_lo[d] - q[d], b = q[d] - box_hi[d];
+      sdf_box = std::fmax(sdf_box, std::fmax(a, b));
+    }
+    double sdf = sdf_box;
+    for (i64 c = 0; c < ncyl; ++c) {
+      double dx = q[0] - cyl[3 * c], dy = q[1] - cyl[3 * c + 1];
+      double dcyl = cyl[3 * c + 2] - std::sqrt(dx * dx + dy * dy);  // > 0 inside the hole
+      sdf = std::fmax(sdf, dcyl);
+    }
+    return sdf < 0.0;
+  };
+  std::array<std::array<double, 2>, 3> gb;
+  for (int d = 0; d < 3; ++d) gb[d] = {grid_bounds[2 * d], grid_bounds[2 * d + 1]};
+  Vec w;
+  std::vector<int8_t> st;
+  element_weights(inside, gb, {(usize)res[0], (usize)res[1], (usize)res[2]}, {(usize)sub[0], (usize)sub[1], (usize)sub[2]},
+                  w, st);
+  std::memcpy(status, st.data(), st.size());
+  std::memcpy(weights, w.data(), w.size() * sizeof(double));
+}
+
+// ---- weighted quadrature ---------------------------------------------------
+// which: 0 points, 1.. = weights[t][i] with t = which - 1
+ORC_API i64 orc_wq_rule(i64 nel, i64 p, i64 layout, i64 which, i64 i, double* out) {
+  Vec knots = open_knots((usize)nel, (usize)p, p - 1);
+  WqRules1d r = wq_rules_1d(knots, (usize)p, (int)layout);
+  const Vec& v = which == 0 ? r.points : r.weights[which - 1][(usize)i];
+  if (out) std::memcpy(out, v.data(), v.size() * sizeof(double));
+  return (i64)v.size();
+}
+
+// Assemble the WQ stiffness into the space handle's cached CSR.
+ORC_API i64 orc_space_assemble_wq(void* p, double young, double poisson, i64 density_kind, const double* x,
+                                  double penal, double emin, i64 layout) {
+  auto* h = (SpaceH*)p;
+  h->csr.reset(new CsrMatrix(
+      wq_stiffness_layout(h->sp, young, poisson, (int)density_kind, x, penal, emin, (int)layout)));
+  return (i64)h->csr->nnz();
+}

@@ -1,0 +1,460 @@
+// Oracle: immersed background-grid classification, volume fractions, the
+// legacy cut-cell stabilisation, the assembled immersed problem and the
+// immersed SIMP loop.  Restates rust/src/cut_cell.rs (new_box),
+// rust/src/immersed.rs and rust/src/topopt_immersed.rs of the reference.
+// Test infrastructure only (see oracle.hpp).
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "oracle.hpp"
+
+namespace orc {
+
+static constexpr int8_t OUTSIDE = 0, INSIDE = 1, CUT = -1;
+
+// cut_cell.rs:27-53 TriangleMesh3D::new_box
+TriangleMesh3D TriangleMesh3D::new_box(std::array<double, 3> mn, std::array<double, 3> mx) {
+  TriangleMesh3D m;
+  m.vertices = {{mn[0], mn[1], mn[2]}, {mx[0], mn[1], mn[2]}, {mx[0], mx[1], mn[2]}, {mn[0], mx[1], mn[2]},
+                {mn[0], mn[1], mx[2]}, {mx[0], mn[1], mx[2]}, {mx[0], mx[1], mx[2]}, {mn[0], mx[1], mx[2]}};
+  m.triangles = {{0, 2, 1}, {0, 3, 2}, {4, 5, 6}, {4, 6, 7}, {0, 1, 5}, {0, 5, 4},
+                 {2, 3, 7}, {2, 7, 6}, {0, 4, 7}, {0, 7, 3}, {1, 2, 6}, {1, 6, 5}};
+  return m;
+}
+
+// immersed.rs:35-46 ParityRayCaster::new
+ParityRayCaster::ParityRayCaster(const TriangleMesh3D& mesh) {
+  for (auto& t : mesh.triangles) {
+    std::array<std::array<double, 3>, 3> tri{mesh.vertices[t[0]], mesh.vertices[t[1]], mesh.vertices[t[2]]};
+    tris.push_back(tri);
+    auto& a = tri[0];
+    auto& b = tri[1];
+    auto& c = tri[2];
+    yz_box.push_back({std::fmin(std::fmin(a[1], b[1]), c[1]), std::fmax(std::fmax(a[1], b[1]), c[1]),
+                      std::fmin(std::fmin(a[2], b[2]), c[2]), std::fmax(std::fmax(a[2], b[2]), c[2])});
+  }
+}
+
+// immersed.rs:48-75 is_inside (ray parity along +x, barycentric test in y-z)
+bool ParityRayCaster::is_inside(const std::array<double, 3>& q) const {
+  double qx = q[0], qy = q[1], qz = q[2];
+  usize count = 0;
+  for (usize t = 0; t < tris.size(); ++t) {
+    const auto& bb = yz_box[t];
+    if (!(qy >= bb[0] && qy <= bb[1] && qz >= bb[2] && qz <= bb[3])) continue;
+    const auto& a = tris[t][0];
+    const auto& b = tris[t][1];
+    const auto& c = tris[t][2];
+    double v0[2] = {c[1] - a[1], c[2] - a[2]};
+    double v1[2] = {b[1] - a[1], b[2] - a[2]};
+    double v2[2] = {qy - a[1], qz - a[2]};
+    double dot00 = v0[0] * v0[0] + v0[1] * v0[1];
+    double dot01 = v0[0] * v1[0] + v0[1] * v1[1];
+    double dot02 = v0[0] * v2[0] + v0[1] * v2[1];
+    double dot11 = v1[0] * v1[0] + v1[1] * v1[1];
+    double dot12 = v1[0] * v2[0] + v1[1] * v2[1];
+    double inv = 1.0 / (dot00 * dot11 - dot01 * dot01 + 1e-15);
+    double u = (dot11 * dot02 - dot01 * dot12) * inv;
+    double v = (dot00 * dot12 - dot01 * dot02) * inv;
+    if (u >= 0.0 && v >= 0.0 && u + v <= 1.0) {
+      double x_hit = a[0] + v * (b[0] - a[0]) + u * (c[0] - a[0]);
+      if (x_hit > qx) count += 1;
+    }
+  }
+  return count % 2 == 1;
+}
+
+// immersed.rs:79-84 MATLAB-compatible linspace
+Vec matlab_linspace(double a, double b, usize n) {
+  double step = (b - a) / (double)(n - 1);
+  Vec v(n);
+  for (usize i = 0; i < n; ++i) v[i] = a + (double)i * step;
+  v[n - 1] = b;
+  return v;
+}
+
+// immersed.rs:88-117 classify_background_cells (8 corners)
+std::vector<int8_t> classify_background_cells(const InsideFn& inside, const std::array<std::array<double, 2>, 3>& bounds,
+                                              const std::array<usize, 3>& res) {
+  std::vector<Vec> xs(3);
+  for (int d = 0; d < 3; ++d) xs[d] = matlab_linspace(bounds[d][0], bounds[d][1], res[d] + 1);
+  usize nv[3] = {res[0] + 1, res[1] + 1, res[2] + 1};
+  std::vector<uint8_t> inside_v(nv[0] * nv[1] * nv[2]);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)inside_v.size(); ++i) {
+    usize a = (usize)i % nv[0], b = ((usize)i / nv[0]) % nv[1], c = (usize)i / (nv[0] * nv[1]);
+    inside_v[(usize)i] = inside({xs[0][a], xs[1][b], xs[2][c]}) ? 1 : 0;
+  }
+  std::vector<int8_t> status(res[0] * res[1] * res[2], OUTSIDE);
+  static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+  for (usize k = 0; k < res[2]; ++k)
+    for (usize j = 0; j < res[1]; ++j)
+      for (usize i = 0; i < res[0]; ++i) {
+        int s = 0;
+        for (auto& o : off)
+          if (inside_v[(i + o[0]) + nv[0] * ((j + o[1]) + nv[1] * (k + o[2]))]) s += 1;
+        status[i + res[0] * (j + res[1] * k)] = s == 8 ? INSIDE : (s == 0 ? OUTSIDE : CUT);
+      }
+  return status;
+}
+
+// immersed.rs:121-155 element_weights (sub^3 midpoint sampling of cut cells)
+void element_weights(const InsideFn& inside, const std::array<std::array<double, 2>, 3>& bounds,
+                     const std::array<usize, 3>& res, const std::array<usize, 3>& sub, Vec& weights,
+                     std::vector<int8_t>& status) {
+  status = classify_background_cells(inside, bounds, res);
+  double h[3];
+  for (int d = 0; d < 3; ++d) h[d] = (bounds[d][1] - bounds[d][0]) / (double)res[d];
+  weights.assign(status.size(), 0.0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long ee = 0; ee < (long)status.size(); ++ee) {
+    usize e = (usize)ee;
+    int8_t s = status[e];
+    if (s == INSIDE) {
+      weights[e] = 1.0;
+    } else if (s == OUTSIDE) {
+      weights[e] = 0.0;
+    } else {
+      usize idx[3] = {e % res[0], (e / res[0]) % res[1], e / (res[0] * res[1])};
+      double lo[3], hi[3], dx[3];
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = bounds[d][0] + (double)idx[d] * h[d];
+        hi[d] = bounds[d][0] + (double)(idx[d] + 1) * h[d];
+        dx[d] = (hi[d] - lo[d]) / (double)sub[d];
+      }
+      usize n_in = 0;
+      for (usize c = 0; c < sub[2]; ++c)
+        for (usize b = 0; b < sub[1]; ++b)
+          for (usize a = 0; a < sub[0]; ++a) {
+            std::array<double, 3> q{lo[0] + ((double)a + 0.5) * dx[0], lo[1] + ((double)b + 0.5) * dx[1],
+                                    lo[2] + ((double)c + 0.5) * dx[2]};
+            if (inside(q)) n_in += 1;
+          }
+      weights[e] = (double)n_in / (double)(sub[0] * sub[1] * sub[2]);
+    }
+  }
+}
+
+// immersed.rs:168-222 legacy_stabilization (MATLAB "ghost penalty" port)
+void legacy_stabilization(const SpaceBox& sp, const std::vector<int8_t>& status, double gamma, std::vector<usize>& ri,
+                          std::vector<usize>& ci, Vec& vi) {
+  usize res[3] = {sp.nel_dir[0], sp.nel_dir[1], sp.nel_dir[2]};
+  Vec h = sp.element_size();
+  double alpha = gamma * ((h[0] + h[1] + h[2]) / 3.0);
+  auto lin = [&](usize i, usize j, usize k) { return i + res[0] * (j + res[1] * k); };
+  auto active = [&](usize e) { return status[e] == INSIDE || status[e] == CUT; };
+  auto cut = [&](usize e) { return status[e] == CUT; };
+  std::vector<std::pair<usize, usize>> faces;
+  static const usize dirs[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (auto& dd : dirs)
+    for (usize i = 0; i < res[0] - dd[0]; ++i)
+      for (usize j = 0; j < res[1] - dd[1]; ++j)
+        for (usize k = 0; k < res[2] - dd[2]; ++k) {
+          usize e1 = lin(i, j, k), e2 = lin(i + dd[0], j + dd[1], k + dd[2]);
+          if (active(e1) && active(e2) && (cut(e1) || cut(e2))) faces.push_back({e1, e2});
+        }
+  usize nsh_sc = sp.nsh_sc;
+  for (auto& f : faces) {
+    usize e1 = f.first, e2 = f.second;
+    for (usize c = 0; c < 3; ++c) {
+      std::vector<usize> d1(sp.connectivity.begin() + (long)(e1 * sp.nsh + c * nsh_sc),
+                            sp.connectivity.begin() + (long)(e1 * sp.nsh + (c + 1) * nsh_sc));
+      std::vector<usize> d2(sp.connectivity.begin() + (long)(e2 * sp.nsh + c * nsh_sc),
+                            sp.connectivity.begin() + (long)(e2 * sp.nsh + (c + 1) * nsh_sc));
+      std::sort(d1.begin(), d1.end());
+      d1.erase(std::unique(d1.begin(), d1.end()), d1.end());
+      std::sort(d2.begin(), d2.end());
+      d2.erase(std::unique(d2.begin(), d2.end()), d2.end());
+      std::vector<usize> common, only1, only2;
+      for (usize x : d1) (std::binary_search(d2.begin(), d2.end(), x) ? common : only1).push_back(x);
+      for (usize x : d2)
+        if (!std::binary_search(d1.begin(), d1.end(), x)) only2.push_back(x);
+      usize n = common.size();
+      if (n > 0) {
+        double nf = (double)n;
+        for (usize m = 0; m < n * n; ++m) {
+          usize i = m % n, j = m / n;
+          double s = (alpha / nf) * ((i == j ? 1.0 : 0.0) - 1.0 / nf);
+          ri.push_back(common[i]);
+          ci.push_back(common[i]);
+          vi.push_back(s);
+        }
+      }
+      usize np = std::min(only1.size(), only2.size());
+      for (usize k = 0; k < np; ++k) {
+        usize p1 = only1[k], p2 = only2[k];
+        ri.insert(ri.end(), {p1, p2, p1, p2});
+        ci.insert(ci.end(), {p1, p2, p2, p1});
+        vi.insert(vi.end(), {alpha, alpha, -alpha, -alpha});
+      }
+    }
+  }
+}
+
+// immersed.rs:450-463 padded_bounds
+std::array<std::array<double, 2>, 3> padded_bounds(const TriangleMesh3D& mesh, double padding) {
+  std::array<std::array<double, 2>, 3> b;
+  for (int d = 0; d < 3; ++d) b[d] = {std::numeric_limits<double>::max(), std::numeric_limits<double>::lowest()};
+  for (auto& v : mesh.vertices)
+    for (int d = 0; d < 3; ++d) {
+      b[d][0] = std::fmin(b[d][0], v[d]);
+      b[d][1] = std::fmax(b[d][1], v[d]);
+    }
+  for (int d = 0; d < 3; ++d) {
+    double pad = padding * (b[d][1] - b[d][0]);
+    b[d] = {b[d][0] - pad, b[d][1] + pad};
+  }
+  return b;
+}
+
+// immersed.rs:466-538 build_immersed_problem (assembled form)
+ImmersedProblem build_immersed_problem(const TriangleMesh3D& mesh, const std::array<usize, 3>& res, usize degree,
+                                       double young, double poisson, int stab_kind, double gamma, double emin,
+                                       const std::array<usize, 3>& sub, int clamp, double padding) {
+  ImmersedProblem pb;
+  pb.grid_bounds = padded_bounds(mesh, padding);
+  auto& gb = pb.grid_bounds;
+  pb.sp = SpaceBox::make({gb[0], gb[1], gb[2]}, {res[0], res[1], res[2]}, degree);
+  const SpaceBox& sp = pb.sp;
+  ParityRayCaster rc(mesh);
+  element_weights([&](const std::array<double, 3>& q) { return rc.is_inside(q); }, gb, res, sub, pb.weights, pb.status);
+  double lambda, mu;
+  lame(young, poisson, lambda, mu);
+  ElementMatrices em = elasticity_element_matrices(sp, lambda, mu);
+  std::vector<usize> rows, cols;
+  Vec vals;
+  if (stab_kind == 1) legacy_stabilization(sp, pb.status, gamma, rows, cols, vals);
+  usize n_gp = vals.size();
+  CsrMatrix stab = CsrMatrix::from_triplets(sp.ndof, sp.ndof, rows, cols, vals);
+  pb.stabilization_frobenius = stab.frobenius();
+  (void)n_gp;
+  usize nsh = sp.nsh;
+  rows.reserve(rows.size() + nsh * nsh * sp.nel);
+  for (usize e = 0; e < sp.nel; ++e) {
+    double s = emin + (1.0 - emin) * pb.weights[e];
+    const double* ke = em.of_element(e);
+    const usize* ce = &sp.connectivity[e * nsh];
+    for (usize a = 0; a < nsh; ++a)
+      for (usize b = 0; b < nsh; ++b) {
+        rows.push_back(ce[a]);
+        cols.push_back(ce[b]);
+        vals.push_back(s * ke[a * nsh + b]);
+      }
+  }
+  pb.stiffness = CsrMatrix::from_triplets(sp.ndof, sp.ndof, rows, cols, vals);
+  const auto& nd = sp.ndof_dir;
+  pb.free_.assign(sp.ndof, 1);
+  for (usize i = 0; i < sp.ndof_sc; ++i) {
+    usize i1 = i % nd[0], i3 = i / (nd[0] * nd[1]);
+    bool cl = (clamp == 0 && i3 == 0) || (clamp == 1 && i3 == nd[2] - 1) || (clamp == 2 && i1 == 0);
+    if (cl)
+      for (usize c = 0; c < 3; ++c) pb.free_[c * sp.ndof_sc + i] = 0;
+  }
+  long mid_x = (long)std::round((double)nd[0] / 2.0), mid_y = (long)std::round((double)nd[1] / 2.0);
+  std::vector<usize> load;
+  for (usize i = 0; i < sp.ndof_sc; ++i) {
+    usize i1 = i % nd[0], i2 = (i / nd[0]) % nd[1], i3 = i / (nd[0] * nd[1]);
+    if (i3 == nd[2] - 1 && std::labs((long)i1 + 1 - mid_x) <= 2 && std::labs((long)i2 + 1 - mid_y) <= 2)
+      load.push_back(i);
+  }
+  pb.force.assign(sp.ndof, 0.0);
+  for (usize i : load) pb.force[2 * sp.ndof_sc + i] = -1.0 / (double)load.size();
+  return pb;
+}
+
+// immersed.rs:551-570 solve_immersed_problem
+ImmersedSolution solve_immersed_problem(const ImmersedProblem& pb, double tol, usize maxit) {
+  usize n = pb.sp.ndof;
+  const auto& fr = pb.free_;
+  Vec b(n);
+  for (usize i = 0; i < n; ++i) b[i] = fr[i] ? pb.force[i] : 0.0;
+  Vec diag = pb.stiffness.diagonal();
+  for (double& d : diag) d = std::fmax(d, 1e-6);
+  Vec pm(n, 0.0), tmp(n, 0.0);
+  PcgResult r = pcg_solve(maxit, tol, n, b, diag, [&](const double* p, double* ap) {
+    for (usize i = 0; i < n; ++i) pm[i] = fr[i] ? p[i] : 0.0;
+    pb.stiffness.matvec(pm.data(), tmp.data());
+    for (usize i = 0; i < n; ++i) ap[i] = fr[i] ? tmp[i] : 0.0;
+  });
+  ImmersedSolution sol;
+  sol.u.resize(n);
+  for (usize i = 0; i < n; ++i) sol.u[i] = fr[i] ? r.x[i] : 0.0;
+  double c = 0.0;
+  for (usize i = 0; i < n; ++i) c += pb.force[i] * sol.u[i];
+  sol.compliance = c;
+  sol.iterations = r.iters;
+  sol.residual = r.res;
+  return sol;
+}
+
+// topopt_immersed.rs:78-153 (new) and 155-247 (run)
+ImmersedTopOptHistory immersed_topopt_run(const TriangleMesh3D& brep, const ImmersedTopOptCfg& cfg) {
+  auto gb = padded_bounds(brep, 0.05);
+  std::array<usize, 3> res{cfg.grid_res[0], cfg.grid_res[1], cfg.grid_res[2]};
+  SpaceBox sp = SpaceBox::make({gb[0], gb[1], gb[2]}, {res[0], res[1], res[2]}, 2);
+  ParityRayCaster rc(brep);
+  Vec weights;
+  std::vector<int8_t> status;
+  element_weights([&](const std::array<double, 3>& q) { return rc.is_inside(q); }, gb, res, {4, 4, 4}, weights, status);
+  usize nel = sp.nel, n = sp.ndof;
+  std::vector<uint8_t> active(nel);
+  double cad_volume = 0.0;
+  for (usize e = 0; e < nel; ++e) {
+    active[e] = weights[e] >= 0.05;
+    if (active[e]) cad_volume += weights[e];
+  }
+  double lambda, mu;
+  lame(cfg.young, cfg.poisson, lambda, mu);
+  ElementMatrices em = elasticity_element_matrices(sp, lambda, mu);
+  AssemblyPattern pattern(sp.ndof, sp.connectivity, sp.nsh, sp.nel);
+  std::vector<usize> r_, c_;
+  Vec v_;
+  if (cfg.stab_kind == 1) legacy_stabilization(sp, status, cfg.gamma, r_, c_, v_);
+  CsrMatrix stab = CsrMatrix::from_triplets(sp.ndof, sp.ndof, r_, c_, v_);
+  auto nd = sp.ndof_dir;
+  std::vector<uint8_t> fr(n, 1);
+  long mid_x = (long)std::round((double)nd[0] / 2.0), mid_y = (long)std::round((double)nd[1] / 2.0);
+  std::vector<usize> load;
+  for (usize i = 0; i < sp.ndof_sc; ++i) {
+    auto s = unravel(i, nd);
+    if (s[2] == 0)
+      for (usize cc = 0; cc < 3; ++cc) fr[cc * sp.ndof_sc + i] = 0;
+    if (s[2] == nd[2] - 1 && std::labs((long)s[0] + 1 - mid_x) <= 2 && std::labs((long)s[1] + 1 - mid_y) <= 2)
+      load.push_back(i);
+  }
+  Vec force(n, 0.0);
+  for (usize i : load) force[2 * sp.ndof_sc + i] = -1.0 / (double)load.size();
+  // filter on the design domain (offset stencil, topopt_immersed.rs:117-151)
+  Vec h = sp.element_size();
+  double r = cfg.rmin * (h[0] + h[1] + h[2]) / 3.0;
+  long rmax[3];
+  for (int d = 0; d < 3; ++d) rmax[d] = (long)std::ceil(r / h[d]);
+  struct Off {
+    long di, dj, dk;
+    double w;
+  };
+  std::vector<Off> offsets;
+  for (long dk = -rmax[2]; dk <= rmax[2]; ++dk)
+    for (long dj = -rmax[1]; dj <= rmax[1]; ++dj)
+      for (long di = -rmax[0]; di <= rmax[0]; ++di) {
+        double dist = std::sqrt(((double)di * h[0]) * ((double)di * h[0]) + ((double)dj * h[1]) * ((double)dj * h[1]) +
+                                ((double)dk * h[2]) * ((double)dk * h[2]));
+        if (dist <= r) offsets.push_back({di, dj, dk, r - dist});
+      }
+  std::vector<FilterRow> filter(nel);
+  for (usize e = 0; e < nel; ++e) {
+    if (!active[e]) continue;
+    auto s = unravel(e, sp.nel_dir);
+    for (auto& o : offsets) {
+      long ni = (long)s[0] + o.di, nj = (long)s[1] + o.dj, nk = (long)s[2] + o.dk;
+      if (ni < 0 || nj < 0 || nk < 0 || ni >= (long)res[0] || nj >= (long)res[1] || nk >= (long)res[2]) continue;
+      usize f = (usize)ni + res[0] * ((usize)nj + res[1] * (usize)nk);
+      if (active[f]) {
+        filter[e].f.push_back(f);
+        filter[e].w.push_back(o.w);
+      }
+    }
+  }
+  Vec filter_sum(nel);
+  for (usize e = 0; e < nel; ++e) {
+    double s = 0.0;
+    for (double w : filter[e].w) s += w;
+    filter_sum[e] = s < 1e-6 ? 1.0 : s;
+  }
+  // run
+  Vec x(nel), u(n, 0.0);
+  for (usize e = 0; e < nel; ++e) x[e] = active[e] ? cfg.volfrac : 0.0;
+  ImmersedTopOptHistory hist;
+  double nf = 0.0;
+  for (usize i = 0; i < n; ++i)
+    if (fr[i]) nf += force[i] * force[i];
+  nf = std::sqrt(nf);
+  for (usize iter = 1; iter <= cfg.max_iter; ++iter) {
+    Vec x_old = x;
+    Vec scale(nel);
+    for (usize e = 0; e < nel; ++e)
+      scale[e] = active[e] ? weights[e] * (cfg.emin + (1.0 - cfg.emin) * std::pow(x[e], cfg.penal)) : cfg.emin;
+    CsrMatrix k = pattern.assemble(em, scale);
+    Vec dk = k.diagonal(), ds = stab.diagonal(), diag(n);
+    for (usize i = 0; i < n; ++i) diag[i] = std::fmax(dk[i] + ds[i], 1e-6);
+    Vec tmp(n, 0.0), tmp2(n, 0.0), pm(n, 0.0);
+    auto op = [&](const double* p, double* ap) {
+      for (usize i = 0; i < n; ++i) pm[i] = fr[i] ? p[i] : 0.0;
+      k.matvec(pm.data(), tmp.data());
+      stab.matvec(pm.data(), tmp2.data());
+      for (usize i = 0; i < n; ++i) ap[i] = fr[i] ? tmp[i] + tmp2[i] : 0.0;
+    };
+    Vec ku(n, 0.0);
+    op(u.data(), ku.data());
+    Vec b(n);
+    for (usize i = 0; i < n; ++i) b[i] = fr[i] ? force[i] - ku[i] : 0.0;
+    double rb = 0.0;
+    for (double v : b) rb += v * v;
+    rb = std::sqrt(rb);
+    double tol = rb > 0.0 ? std::fmin(cfg.pcg_tol * nf / rb, 0.5) : 1.0;
+    PcgResult pr = pcg_solve(cfg.pcg_maxit, tol, n, b, diag, op);
+    for (usize i = 0; i < n; ++i) u[i] = fr[i] ? u[i] + pr.x[i] : 0.0;
+    double c = 0.0;
+    for (usize i = 0; i < n; ++i) c += force[i] * u[i];
+    usize nsh = sp.nsh;
+    Vec ee(nel);
+    for (usize e = 0; e < nel; ++e) {
+      const double* ke = em.of_element(e);
+      const usize* ce = &sp.connectivity[e * nsh];
+      double s = 0.0;
+      for (usize a = 0; a < nsh; ++a) {
+        double row = 0.0;
+        for (usize bb = 0; bb < nsh; ++bb) row += ke[a * nsh + bb] * u[ce[bb]];
+        s += u[ce[a]] * row;
+      }
+      ee[e] = s;
+    }
+    Vec dc_raw(nel), dc(nel);
+    for (usize e = 0; e < nel; ++e)
+      dc_raw[e] = -weights[e] * (cfg.penal * (1.0 - cfg.emin) * std::pow(x[e], cfg.penal - 1.0) * ee[e]);
+    for (usize e = 0; e < nel; ++e) {
+      double s = 0.0;
+      for (usize kk = 0; kk < filter[e].f.size(); ++kk) s += filter[e].w[kk] * x[filter[e].f[kk]] * dc_raw[filter[e].f[kk]];
+      dc[e] = s / (filter_sum[e] * std::fmax(x[e], 1e-3));
+    }
+    std::vector<usize> act;
+    for (usize e = 0; e < nel; ++e)
+      if (active[e]) act.push_back(e);
+    double l1 = 0.0, mx = 0.0;
+    for (usize e : act) mx = std::fmax(mx, std::fabs(dc[e]));
+    double l2 = 2.0 * mx;
+    double target = cfg.volfrac * cad_volume;
+    Vec cand(act.size());
+    for (usize k2 = 0; k2 < act.size(); ++k2) cand[k2] = x_old[act[k2]];
+    while ((l2 - l1) / (l1 + l2 + 1e-10) > 1e-4) {
+      double lmid = 0.5 * (l1 + l2);
+      for (usize k2 = 0; k2 < act.size(); ++k2) {
+        usize e = act[k2];
+        double be = std::sqrt(-dc[e] / lmid);
+        cand[k2] = std::fmax(std::fmax(std::fmin(std::fmin(x_old[e] * be, x_old[e] + 0.2), 1.0), x_old[e] - 0.2), 0.001);
+      }
+      double vol = 0.0;
+      for (usize k2 = 0; k2 < act.size(); ++k2) vol += weights[act[k2]] * cand[k2];
+      if (vol > target)
+        l1 = lmid;
+      else
+        l2 = lmid;
+    }
+    x.assign(nel, 0.0);
+    for (usize k2 = 0; k2 < act.size(); ++k2) x[act[k2]] = cand[k2];
+    double change = 0.0;
+    for (usize e : act) change = std::fmax(change, std::fabs(x[e] - x_old[e]));
+    double vol = 0.0;
+    for (usize e : act) vol += weights[e] * x[e];
+    vol /= cad_volume;
+    hist.compliance.push_back(c);
+    hist.volume.push_back(vol);
+    hist.change.push_back(change);
+    hist.pcg_iterations.push_back(pr.iters);
+  }
+  hist.density = x;
+  return hist;
+}
+
+}  // namespace orc
