@@ -1,0 +1,10 @@
+"""MI355X-native matrix-free p = 2 immersed-IGA elasticity hot path.
+
+Host API over libicc.so (HIP kernels for gfx950 behind a C ABI, include/icc.h).
+"""
+from ._abi import (ICC_F32, ICC_F64, ICC_OK, ICC_EINVAL, ICC_ENOMEM, ICC_EHIP, ICC_EBREAKDOWN,  # noqa: F401
+                   ICC_ENOCONV, ICC_ENOTSUP, IccError, LIB_PATH, lib)
+from .core import Context, Filter, Grid, Operator, classify_cylinders  # noqa: F401
+from .topopt import CantileverTopOpt, CantileverTopOptConfig, TopOptHistory  # noqa: F401
+
+__all__ = ["Context", "Grid", "Operator", "Filter", "classify_cylinders", "ICC_F64", "ICC_F32", "IccError"]

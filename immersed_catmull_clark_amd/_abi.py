@@ -1,0 +1,116 @@
+"""ctypes binding of libicc.so (the C ABI declared in include/icc.h).
+
+The library is required: importing the package on a machine without the
+built library raises, and every entry point runs the HIP path; there is no
+CPU fallback.
+"""
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libicc.so")
+
+ICC_OK, ICC_EINVAL, ICC_ENOMEM, ICC_EHIP, ICC_EBREAKDOWN, ICC_ENOCONV, ICC_ENOTSUP = range(7)
+ICC_F64, ICC_F32 = 0, 1
+
+i64 = C.c_int64
+dbl = C.c_double
+vp = C.c_void_p
+cint = C.c_int
+vpp = C.POINTER(C.c_void_p)
+
+# (name, restype, argtypes)
+SIGNATURES = [
+    ("icc_last_error", C.c_char_p, []),
+    ("icc_version", C.c_char_p, []),
+    ("icc_ctx_create", cint, [cint, vpp]),
+    ("icc_ctx_destroy", cint, [vp]),
+    ("icc_ctx_sync", cint, [vp]),
+    ("icc_ctx_stream", vp, [vp]),
+    ("icc_grid_create", cint, [vp, vp, vp, i64, dbl, dbl, vpp]),
+    ("icc_grid_destroy", cint, [vp]),
+    ("icc_grid_sizes", cint, [vp, vp]),
+    ("icc_grid_connectivity", cint, [vp, vp]),
+    ("icc_grid_type_id", cint, [vp, vp]),
+    ("icc_grid_element_matrix", cint, [vp, i64, vp]),
+    ("icc_op_create", cint, [vp, vpp]),
+    ("icc_op_destroy", cint, [vp]),
+    ("icc_op_set_scale", cint, [vp, vp]),
+    ("icc_op_set_scale_dev", cint, [vp, vp]),
+    ("icc_op_set_free_mask", cint, [vp, vp]),
+    ("icc_op_set_sparse", cint, [vp, i64, vp, vp, vp]),
+    ("icc_apply", cint, [vp, cint, vp, vp]),
+    ("icc_apply_dev", cint, [vp, cint, vp, vp]),
+    ("icc_diag", cint, [vp, vp]),
+    ("icc_diag_dev", cint, [vp, vp]),
+    ("icc_pcg", cint, [vp, vp, vp, dbl, i64, vp, vp, vp]),
+    ("icc_pcg_dev", cint, [vp, vp, vp, dbl, i64, vp, vp, vp]),
+    ("icc_solve_ir_dev", cint, [vp, cint, vp, dbl, dbl, i64, i64, vp, vp]),
+    ("icc_solve_ir", cint, [vp, cint, vp, dbl, dbl, i64, i64, vp, vp]),
+    ("icc_energies", cint, [vp, vp, vp]),
+    ("icc_energies_dev", cint, [vp, vp, vp]),
+    ("icc_sens_dev", cint, [vp, vp, vp, dbl, dbl, vp, vp]),
+    ("icc_sens", cint, [vp, vp, vp, dbl, dbl, vp, vp]),
+    ("icc_filter_create", cint, [vp, dbl, vpp]),
+    ("icc_filter_destroy", cint, [vp]),
+    ("icc_filter_sens_dev", cint, [vp, vp, vp, vp]),
+    ("icc_filter_fwd_dev", cint, [vp, vp, vp]),
+    ("icc_filter_bwd_dev", cint, [vp, vp, vp]),
+    ("icc_filter_sens", cint, [vp, vp, vp, vp]),
+    ("icc_filter_fwd", cint, [vp, vp, vp]),
+    ("icc_filter_bwd", cint, [vp, vp, vp]),
+    ("icc_wq_diag", cint, [vp, vp, dbl, dbl, vp]),
+    ("icc_wq_diag_dev", cint, [vp, vp, dbl, dbl, vp]),
+    ("icc_classify_cylinders", cint, [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]),
+    ("icc_debug_mfma_layout", cint, [vp, vp]),
+    ("icc_topopt_config_default", None, [vp]),
+    ("icc_topopt_create", cint, [vp, vp, vpp]),
+    ("icc_topopt_destroy", cint, [vp]),
+    ("icc_topopt_sizes", cint, [vp, vp]),
+    ("icc_topopt_force", cint, [vp, vp, vp]),
+    ("icc_topopt_solve_state", cint, [vp, vp, vp, vp, vp, vp]),
+    ("icc_topopt_element_energies", cint, [vp, vp, vp]),
+    ("icc_topopt_compliance", cint, [vp, vp, vp, vp, vp]),
+    ("icc_topopt_run", cint, [vp, vp, vp, vp, vp, vp, vp, vp]),
+]
+
+REPORT_FN = C.CFUNCTYPE(None, i64, dbl, dbl, dbl, vp)
+
+
+class TopOptConfigC(C.Structure):
+    """icc_topopt_config (include/icc.h)."""
+    _fields_ = [("nel", i64 * 3), ("size", dbl * 3), ("volfrac", dbl), ("penal", dbl), ("rmin", dbl),
+                ("max_iter", i64), ("young", dbl), ("poisson", dbl), ("emin", dbl), ("move_limit", dbl),
+                ("pcg_tol", dbl), ("load", C.c_int32), ("filter", C.c_int32), ("solver", C.c_int32),
+                ("pad", C.c_int32)]
+
+_lib = None
+
+
+class IccError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libicc error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m immersed_catmull_clark_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, allowed=(ICC_OK,)):
+    if rc not in allowed:
+        msg = lib().icc_last_error()
+        raise IccError(rc, msg.decode() if msg else "")
+    return rc
+
+ICC_BREAKDOWN_CODES = (ICC_EBREAKDOWN, ICC_ENOCONV)

@@ -1,0 +1,71 @@
+"""Builds libicc.so (HIP kernels for gfx950 + C ABI) in-tree with hipcc.
+
+Usage: python -m immersed_catmull_clark_amd.build   (or build() from Python)
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libicc.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
+    # packed FP32 (v_pk_*_f32) stays off: the scalar forms are used throughout
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+HOST_FLAGS = ["-x", "c++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-Wno-unused-result",
+              "-Wno-unused-value",
+              "-I/opt/rocm/include"]
+
+SOURCES = [
+    # (file, extra flags)
+    ("k_apply.hip", []),
+    ("k_vec.hip", []),
+    ("k_misc.hip", ["-ffp-contract=off"]),
+    ("k_immersed.hip", ["-ffp-contract=off"]),
+    ("setup.cpp", None),
+    ("wq.cpp", None),
+    ("api.cpp", None),
+    ("api_ext.cpp", None),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "is not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + out)
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for name, extra in SOURCES:
+        src = os.path.join(CSRC, name)
+        if not os.path.exists(src):
+            continue
+        obj = os.path.join(BUILD, name + ".o")
+        if extra is None:
+            cmd = [HIPCC] + HOST_FLAGS + ["-c", src, "-o", obj]
+        else:
+            cmd = [HIPCC] + HIP_FLAGS + extra + ["-c", src, "-o", obj]
+        out = _run(cmd)
+        if verbose and out.strip():
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

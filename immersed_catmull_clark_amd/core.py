@@ -1,0 +1,257 @@
+"""Python host API over libicc: contexts, grids, operators, filters.
+
+numpy arrays go through the host-pointer entry points; torch CUDA tensors go
+through the *_dev entry points on the context's HIP stream (call
+Context.sync() before reading them from torch's stream).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._abi import (ICC_BREAKDOWN_CODES, ICC_F32, ICC_F64, ICC_OK, check, lib)
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(a.data_ptr())  # torch tensor
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _is_dev(a):
+    return a is not None and not isinstance(a, np.ndarray) and getattr(a, "is_cuda", False)
+
+
+class Context:
+    """One device + one HIP stream (icc_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().icc_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def sync(self):
+        check(lib().icc_ctx_sync(self.h))
+
+    @property
+    def stream(self):
+        return lib().icc_ctx_stream(self.h)
+
+    def close(self):
+        if self.h:
+            lib().icc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mfma_layout(self):
+        out = np.zeros(256)
+        check(lib().icc_debug_mfma_layout(self.h, _p(out)))
+        return out.reshape(64, 4)
+
+
+class Grid:
+    """p = 2 B-spline space on a box with its exact element matrices
+    (SpaceBox::new + elasticity_element_matrices, rust/src/iga)."""
+
+    def __init__(self, ctx, nel, bounds, young=1.0, poisson=0.3, degree=2):
+        self.ctx = ctx
+        n = np.asarray(nel, dtype=np.int64)
+        b = _f64(np.asarray(bounds, dtype=np.float64).ravel())
+        h = C.c_void_p()
+        check(lib().icc_grid_create(ctx.h, _p(n), _p(b), int(degree), float(young), float(poisson), C.byref(h)))
+        self.h = h
+        s = np.zeros(11, dtype=np.int64)
+        check(lib().icc_grid_sizes(h, _p(s)))
+        self.nel_tot, self.ndof_sc, self.ndof, self.ntypes, self.nsh = (int(v) for v in s[:5])
+        self.ncp = [int(v) for v in s[5:8]]
+        self.nel = [int(v) for v in s[8:11]]
+        self.young, self.poisson = float(young), float(poisson)
+        self.bounds = b.reshape(3, 2)
+
+    def connectivity(self):
+        out = np.zeros(self.nel_tot * 81, dtype=np.int64)
+        check(lib().icc_grid_connectivity(self.h, _p(out)))
+        return out
+
+    def type_id(self):
+        out = np.zeros(self.nel_tot, dtype=np.int64)
+        check(lib().icc_grid_type_id(self.h, _p(out)))
+        return out
+
+    def element_matrix(self, t):
+        out = np.zeros((81, 81))
+        check(lib().icc_grid_element_matrix(self.h, int(t), _p(out)))
+        return out
+
+    def wq_diag(self, x=None, penal=3.0, emin=1e-3):
+        d = np.zeros(self.ndof)
+        xx = None if x is None else _f64(x)
+        check(lib().icc_wq_diag(self.h, _p(xx), float(penal), float(emin), _p(d)))
+        return d
+
+    def close(self):
+        if self.h:
+            lib().icc_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Operator:
+    """y = M sum_e s_e C_e^T K_t(e) C_e (M x): the matvec closure of the
+    reference's PCG (topopt_iga.rs:151-159), matrix-free on the GPU."""
+
+    def __init__(self, grid, scale=None, free=None):
+        self.grid = grid
+        h = C.c_void_p()
+        check(lib().icc_op_create(grid.h, C.byref(h)))
+        self.h = h
+        if scale is not None:
+            self.set_scale(scale)
+        if free is not None:
+            self.set_free(free)
+
+    @property
+    def ndof(self):
+        return self.grid.ndof
+
+    def set_scale(self, s):
+        if _is_dev(s):
+            check(lib().icc_op_set_scale_dev(self.h, _p(s)))
+        else:
+            check(lib().icc_op_set_scale(self.h, _p(_f64(s))))
+
+    def set_free(self, free):
+        fr = None if free is None else np.ascontiguousarray(free, dtype=np.uint8)
+        check(lib().icc_op_set_free_mask(self.h, _p(fr)))
+
+    def set_sparse(self, indptr, indices, data):
+        ip = np.ascontiguousarray(indptr, dtype=np.int64)
+        ix = np.ascontiguousarray(indices, dtype=np.int64)
+        dd = _f64(data)
+        check(lib().icc_op_set_sparse(self.h, len(ip) - 1, _p(ip), _p(ix), _p(dd)))
+
+    def apply(self, x, prec=ICC_F64, out=None):
+        if _is_dev(x):
+            check(lib().icc_apply_dev(self.h, prec, _p(x), _p(out)))
+            return out
+        y = np.zeros(self.ndof)
+        check(lib().icc_apply(self.h, prec, _p(_f64(x)), _p(y)))
+        return y
+
+    def diag(self, out=None):
+        if out is not None and _is_dev(out):
+            check(lib().icc_diag_dev(self.h, _p(out)))
+            return out
+        d = np.zeros(self.ndof)
+        check(lib().icc_diag(self.h, _p(d)))
+        return d
+
+    def pcg(self, b, diag, tol, maxit):
+        """PcgSolver::solve (solver.rs:14-76) with this operator as the closure.
+        Returns (x, iters, res, status)."""
+        it = np.zeros(1, dtype=np.int64)
+        res = np.zeros(1)
+        x = np.zeros(self.ndof)
+        rc = lib().icc_pcg(self.h, _p(_f64(b)), _p(_f64(diag)), float(tol), int(maxit), _p(x), _p(it), _p(res))
+        check(rc, allowed=(ICC_OK,) + ICC_BREAKDOWN_CODES)
+        return x, int(it[0]), float(res[0]), rc
+
+    def solve_ir(self, f, tol=1e-8, inner=ICC_F32, inner_tol=1e-3, max_outer=50, max_inner=100000, u0=None):
+        """Mixed-precision iterative refinement (FP64 outer residual).
+        Returns (u, stats dict, status)."""
+        u = np.zeros(self.ndof) if u0 is None else _f64(u0).copy()
+        stats = np.zeros(8)
+        rc = lib().icc_solve_ir(self.h, int(inner), _p(_f64(f)), float(tol), float(inner_tol), int(max_outer),
+                                int(max_inner), _p(u), _p(stats))
+        check(rc, allowed=(ICC_OK,) + ICC_BREAKDOWN_CODES)
+        st = dict(outer=int(stats[0]), inner_iters=int(stats[1]), true_res=float(stats[2]),
+                  fp64_applies=int(stats[3]), inner_applies=int(stats[4]))
+        return u, st, rc
+
+    def energies(self, u):
+        ee = np.zeros(self.grid.nel_tot)
+        check(lib().icc_energies(self.h, _p(_f64(u)), _p(ee)))
+        return ee
+
+    def sens(self, u, x, penal, emin, w=None):
+        dc = np.zeros(self.grid.nel_tot)
+        ww = None if w is None else _f64(w)
+        check(lib().icc_sens(self.h, _p(_f64(u)), _p(_f64(x)), float(penal), float(emin), _p(ww), _p(dc)))
+        return dc
+
+    def close(self):
+        if self.h:
+            lib().icc_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Filter:
+    """Cartesian filter on element centres (topopt_iga.rs:100-126)."""
+
+    def __init__(self, grid, rmin):
+        self.grid = grid
+        h = C.c_void_p()
+        check(lib().icc_filter_create(grid.h, float(rmin), C.byref(h)))
+        self.h = h
+
+    def sens(self, x, dc):
+        out = np.zeros(self.grid.nel_tot)
+        check(lib().icc_filter_sens(self.h, _p(_f64(x)), _p(_f64(dc)), _p(out)))
+        return out
+
+    def fwd(self, x):
+        out = np.zeros(self.grid.nel_tot)
+        check(lib().icc_filter_fwd(self.h, _p(_f64(x)), _p(out)))
+        return out
+
+    def bwd(self, g):
+        out = np.zeros(self.grid.nel_tot)
+        check(lib().icc_filter_bwd(self.h, _p(_f64(g)), _p(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().icc_filter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def classify_cylinders(ctx, box_lo, box_hi, cyl, grid_bounds, res, sub=(4, 4, 4)):
+    """classify_background_cells + element_weights (immersed.rs:88-155) with
+    the box-minus-cylinders SDF predicate.  Returns (status int8, weights)."""
+    nel = int(np.prod(res))
+    status = np.zeros(nel, dtype=np.int8)
+    w = np.zeros(nel)
+    cyl = _f64(np.asarray(cyl, dtype=np.float64).ravel())
+    check(lib().icc_classify_cylinders(ctx.h, _p(_f64(box_lo)), _p(_f64(box_hi)), _p(cyl), len(cyl) // 3,
+                                       _p(_f64(np.asarray(grid_bounds).ravel())),
+                                       _p(np.asarray(res, dtype=np.int64)), _p(np.asarray(sub, dtype=np.int64)),
+                                       _p(status), _p(w)))
+    return status, w

@@ -1,0 +1,897 @@
+// C ABI of libicc (host side): contexts, grids, operators, solvers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <functional>
+#include <type_traits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "icc_internal.hpp"
+#include "icc_kernels.hpp"
+
+#define ICC_API extern "C" __attribute__((visibility("default")))
+
+namespace icc {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return ICC_EHIP;
+}
+
+template <typename T>
+static int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    g_err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return ICC_ENOMEM;
+  }
+  return ICC_OK;
+}
+
+#define ICC_TRY(call)            \
+  do {                           \
+    int rc__ = (call);           \
+    if (rc__ != ICC_OK) return rc__; \
+  } while (0)
+
+// Workspace of one operator: solver vectors and reduction scratch.
+struct Work {
+  int64_t n = 0;
+  double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *dinv = nullptr, *tmp = nullptr, *f = nullptr;
+  float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *dinv32 = nullptr;
+  double* parts = nullptr;  // 2 * 1024 vector partials
+  double* pap = nullptr;    // ntiles + csr blocks
+  PcgState* st = nullptr;
+  PcgState* st_host = nullptr;  // pinned
+};
+
+static Work* work_of(icc_op* op);
+
+}  // namespace icc
+
+using namespace icc;
+
+struct icc_op_ext {
+  Work w;
+};
+
+// ---------------------------------------------------------------------------
+ICC_API const char* icc_last_error(void) { return g_err.c_str(); }
+ICC_API const char* icc_version(void) { return "libicc 0.1 (gfx950)"; }
+
+ICC_API int icc_ctx_create(int device, icc_ctx** out) {
+  if (!out) {
+    set_error("icc_ctx_create: null out");
+    return ICC_EINVAL;
+  }
+  int ndev = 0;
+  ICC_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) {
+    set_error("icc_ctx_create: no such device");
+    return ICC_EINVAL;
+  }
+  ICC_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  ICC_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0) {
+    set_error(std::string("icc_ctx_create: built for gfx950, device is ") + prop.gcnArchName);
+    return ICC_ENOTSUP;
+  }
+  auto* c = new icc_ctx();
+  c->device = device;
+  c->num_cu = prop.multiProcessorCount;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = c;
+  return ICC_OK;
+}
+
+ICC_API int icc_ctx_destroy(icc_ctx* ctx) {
+  if (!ctx) return ICC_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ICC_OK;
+}
+
+ICC_API int icc_ctx_sync(icc_ctx* ctx) {
+  ICC_HIP(hipStreamSynchronize(ctx->stream));
+  return ICC_OK;
+}
+
+ICC_API void* icc_ctx_stream(icc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bounds[6], int64_t degree, double young,
+                            double poisson, icc_grid** out) {
+  if (!ctx || !nel || !bounds || !out) {
+    set_error("icc_grid_create: null argument");
+    return ICC_EINVAL;
+  }
+  if (degree != 2) {
+    set_error("icc_grid_create: only degree 2 is implemented");
+    return ICC_ENOTSUP;
+  }
+  auto* g = new icc_grid();
+  g->ctx = ctx;
+  int rc = build_setup(nel, bounds, young, poisson, g->s);
+  if (rc != ICC_OK) {
+    delete g;
+    return rc;
+  }
+  const GridSetup& s = g->s;
+  g->ntypes = (int)s.kmat.size();
+  g->ntiles = (int)s.tiles.size();
+  for (int d = 0; d < 3; ++d) g->T[d] = (int)s.mats1[d].size();
+  g->maxnel = (int)std::max(s.nel[0], std::max(s.nel[1], s.nel[2]));
+  // MFMA A fragments: lane l of (type, mt, ks) holds K[16 mt + l%16][4 ks + l/16]
+  const size_t nfrag = (size_t)g->ntypes * MT * KS * 64;
+  std::vector<double> f64(nfrag, 0.0);
+  std::vector<float> f32(nfrag, 0.0f);
+  std::vector<double> kd((size_t)g->ntypes * NSH);
+  for (int t = 0; t < g->ntypes; ++t) {
+    const double* K = s.kmat[t].data();
+    for (int mt = 0; mt < MT; ++mt)
+      for (int ks = 0; ks < KS; ++ks)
+        for (int l = 0; l < 64; ++l) {
+          int row = 16 * mt + (l % 16), colk = 4 * ks + (l / 16);
+          double v = (row < NSH && colk < NSH) ? K[row * NSH + colk] : 0.0;
+          size_t idx = (((size_t)t * MT + mt) * KS + ks) * 64 + l;
+          f64[idx] = v;
+          f32[idx] = (float)v;
+        }
+    for (int a = 0; a < NSH; ++a) kd[(size_t)t * NSH + a] = K[a * NSH + a];
+  }
+  std::vector<int32_t> tid((size_t)3 * g->maxnel, 0);
+  for (int d = 0; d < 3; ++d)
+    for (int64_t e = 0; e < s.nel[d]; ++e) tid[(size_t)d * g->maxnel + e] = s.tid1[d][e];
+  hipStream_t st = ctx->stream;
+  if ((rc = dalloc(&g->d_kfrag64, nfrag)) || (rc = dalloc(&g->d_kfrag32, nfrag)) ||
+      (rc = dalloc(&g->d_kdiag, kd.size())) || (rc = dalloc(&g->d_tid1, tid.size())) ||
+      (rc = dalloc(&g->d_typeof, s.type_of.size())) || (rc = dalloc(&g->d_tiles, s.tiles.size()))) {
+    icc_grid_destroy(g);
+    return rc;
+  }
+  hipError_t e;
+  if ((e = hipMemcpyAsync(g->d_kfrag64, f64.data(), nfrag * 8, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(g->d_kfrag32, f32.data(), nfrag * 4, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(g->d_kdiag, kd.data(), kd.size() * 8, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(g->d_tid1, tid.data(), tid.size() * 4, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(g->d_typeof, s.type_of.data(), s.type_of.size() * 4, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(g->d_tiles, s.tiles.data(), s.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st)) ||
+      (e = hipStreamSynchronize(st))) {
+    icc_grid_destroy(g);
+    return hip_fail(e, "icc_grid_create upload");
+  }
+  *out = g;
+  return ICC_OK;
+}
+
+ICC_API int icc_grid_destroy(icc_grid* g) {
+  if (!g) return ICC_OK;
+  hipFree(g->d_kfrag64);
+  hipFree(g->d_kfrag32);
+  hipFree(g->d_kdiag);
+  hipFree(g->d_tid1);
+  hipFree(g->d_typeof);
+  hipFree(g->d_tiles);
+  hipFree(g->d_mats1);
+  delete g;
+  return ICC_OK;
+}
+
+ICC_API int icc_grid_sizes(const icc_grid* g, int64_t out[11]) {
+  if (!g || !out) return ICC_EINVAL;
+  const auto& s = g->s;
+  out[0] = s.nel_tot;
+  out[1] = s.ndof_sc;
+  out[2] = s.ndof;
+  out[3] = g->ntypes;
+  out[4] = NSH;
+  for (int d = 0; d < 3; ++d) {
+    out[5 + d] = s.ncp[d];
+    out[8 + d] = s.nel[d];
+  }
+  return ICC_OK;
+}
+
+ICC_API int icc_grid_connectivity(const icc_grid* g, int64_t* out) {
+  if (!g || !out) return ICC_EINVAL;
+  const auto& s = g->s;
+  int64_t e = 0;
+  for (int64_t k = 0; k < s.nel[2]; ++k)
+    for (int64_t j = 0; j < s.nel[1]; ++j)
+      for (int64_t i = 0; i < s.nel[0]; ++i, ++e) {
+        int64_t* o = out + e * NSH;
+        for (int c = 0; c < 3; ++c)
+          for (int a = 0; a < NSC; ++a) {
+            int64_t ax = a % 3, ay = (a / 3) % 3, az = a / 9;
+            o[c * NSC + a] = c * s.ndof_sc + (i + ax) + s.ncp[0] * ((j + ay) + s.ncp[1] * (k + az));
+          }
+      }
+  return ICC_OK;
+}
+
+ICC_API int icc_grid_type_id(const icc_grid* g, int64_t* out) {
+  if (!g || !out) return ICC_EINVAL;
+  const auto& s = g->s;
+  int64_t e = 0;
+  for (int64_t k = 0; k < s.nel[2]; ++k)
+    for (int64_t j = 0; j < s.nel[1]; ++j)
+      for (int64_t i = 0; i < s.nel[0]; ++i, ++e) out[e] = element_type(s, i, j, k);
+  return ICC_OK;
+}
+
+ICC_API int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out) {
+  if (!g || !out || type < 0 || type >= g->ntypes) return ICC_EINVAL;
+  std::memcpy(out, g->s.kmat[type].data(), sizeof(double) * NSH * NSH);
+  return ICC_OK;
+}
+
+// ---------------------------------------------------------------------------
+namespace icc {
+static Work* work_of(icc_op* op);
+}
+
+ICC_API int icc_op_create(icc_grid* g, icc_op** out) {
+  if (!g || !out) return ICC_EINVAL;
+  auto* op = new icc_op();
+  op->g = g;
+  const auto& s = g->s;
+  int rc;
+  if ((rc = dalloc(&op->d_scale, s.nel_tot)) || (rc = dalloc(&op->d_scale32, s.nel_tot))) {
+    icc_op_destroy(op);
+    return rc;
+  }
+  hipStream_t st = g->ctx->stream;
+  hipError_t e;
+  if ((e = launch_fill<double>(op->d_scale, 1.0, s.nel_tot, st)) ||
+      (e = launch_fill<float>(op->d_scale32, 1.0f, s.nel_tot, st))) {
+    icc_op_destroy(op);
+    return hip_fail(e, "icc_op_create");
+  }
+  *out = op;
+  return ICC_OK;
+}
+
+namespace icc {
+static void free_work(Work& w) {
+  hipFree(w.x);
+  hipFree(w.r);
+  hipFree(w.p);
+  hipFree(w.ap);
+  hipFree(w.dinv);
+  hipFree(w.tmp);
+  hipFree(w.f);
+  hipFree(w.x32);
+  hipFree(w.r32);
+  hipFree(w.p32);
+  hipFree(w.ap32);
+  hipFree(w.dinv32);
+  hipFree(w.parts);
+  hipFree(w.pap);
+  hipFree(w.st);
+  if (w.st_host) hipHostFree(w.st_host);
+  w = Work();
+}
+// The workspace hangs off the op through a side table keyed by pointer.
+struct WorkEntry {
+  icc_op* op;
+  Work* w;
+};
+static std::vector<WorkEntry> g_work;
+static Work* work_of(icc_op* op) {
+  for (auto& e : g_work)
+    if (e.op == op) return e.w;
+  Work* w = new Work();
+  g_work.push_back({op, w});
+  return w;
+}
+static void drop_work(icc_op* op) {
+  for (size_t i = 0; i < g_work.size(); ++i)
+    if (g_work[i].op == op) {
+      free_work(*g_work[i].w);
+      delete g_work[i].w;
+      g_work.erase(g_work.begin() + (long)i);
+      return;
+    }
+}
+
+static int csr_blocks(const icc_op* op) { return op->s_nnz > 0 ? vec_blocks(op->g->s.ndof) : 0; }
+
+// Lazily allocates the solver workspace of op.
+static int ensure_work(icc_op* op, Work*& out) {
+  Work* w = work_of(op);
+  const int64_t n = op->g->s.ndof;
+  if (w->n != n) {
+    free_work(*w);
+    w->n = n;
+    int rc;
+    size_t npap = (size_t)op->g->ntiles + 1024;
+    if ((rc = dalloc(&w->x, n)) || (rc = dalloc(&w->r, n)) || (rc = dalloc(&w->p, n)) || (rc = dalloc(&w->ap, n)) ||
+        (rc = dalloc(&w->dinv, n)) || (rc = dalloc(&w->tmp, n)) || (rc = dalloc(&w->f, n)) ||
+        (rc = dalloc(&w->x32, n)) || (rc = dalloc(&w->r32, n)) || (rc = dalloc(&w->p32, n)) ||
+        (rc = dalloc(&w->ap32, n)) || (rc = dalloc(&w->dinv32, n)) || (rc = dalloc(&w->parts, 2048)) ||
+        (rc = dalloc(&w->pap, npap)) || (rc = dalloc(&w->st, 1))) {
+      free_work(*w);
+      return rc;
+    }
+    if (hipHostMalloc((void**)&w->st_host, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+      free_work(*w);
+      set_error("hipHostMalloc failed");
+      return ICC_ENOMEM;
+    }
+    PcgState z;
+    std::memset(&z, 0, sizeof(z));
+    ICC_HIP(hipMemcpyAsync(w->st, &z, sizeof(z), hipMemcpyHostToDevice, op->g->ctx->stream));
+    ICC_HIP(hipStreamSynchronize(op->g->ctx->stream));
+  }
+  out = w;
+  return ICC_OK;
+}
+}  // namespace icc
+
+ICC_API int icc_op_destroy(icc_op* op) {
+  if (!op) return ICC_OK;
+  drop_work(op);
+  hipFree(op->d_scale);
+  hipFree(op->d_scale32);
+  hipFree(op->d_free);
+  hipFree(op->d_s_indptr);
+  hipFree(op->d_s_indices);
+  hipFree(op->d_s_data);
+  hipFree(op->d_s_data32);
+  hipFree(op->d_s_diag);
+  delete op;
+  return ICC_OK;
+}
+
+ICC_API int icc_op_set_scale_dev(icc_op* op, const double* s_dev) {
+  if (!op || !s_dev) return ICC_EINVAL;
+  const int64_t nel = op->g->s.nel_tot;
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(op->d_scale, s_dev, nel * 8, hipMemcpyDeviceToDevice, st));
+  ICC_HIP(launch_to_f32(op->d_scale, op->d_scale32, nel, st));
+  return ICC_OK;
+}
+
+ICC_API int icc_op_set_scale(icc_op* op, const double* s) {
+  if (!op || !s) return ICC_EINVAL;
+  const int64_t nel = op->g->s.nel_tot;
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(op->d_scale, s, nel * 8, hipMemcpyHostToDevice, st));
+  ICC_HIP(launch_to_f32(op->d_scale, op->d_scale32, nel, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  return ICC_OK;
+}
+
+ICC_API int icc_op_set_free_mask(icc_op* op, const uint8_t* fm) {
+  if (!op) return ICC_EINVAL;
+  const int64_t n = op->g->s.ndof;
+  if (!fm) {
+    hipFree(op->d_free);
+    op->d_free = nullptr;
+    return ICC_OK;
+  }
+  if (!op->d_free) ICC_TRY(dalloc(&op->d_free, n));
+  ICC_HIP(hipMemcpyAsync(op->d_free, fm, n, hipMemcpyHostToDevice, op->g->ctx->stream));
+  ICC_HIP(hipStreamSynchronize(op->g->ctx->stream));
+  return ICC_OK;
+}
+
+ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, const int64_t* indices,
+                              const double* data) {
+  if (!op) return ICC_EINVAL;
+  const int64_t n = op->g->s.ndof;
+  hipFree(op->d_s_indptr);
+  hipFree(op->d_s_indices);
+  hipFree(op->d_s_data);
+  hipFree(op->d_s_data32);
+  hipFree(op->d_s_diag);
+  op->d_s_indptr = nullptr;
+  op->d_s_indices = nullptr;
+  op->d_s_data = nullptr;
+  op->d_s_data32 = nullptr;
+  op->d_s_diag = nullptr;
+  op->s_nnz = 0;
+  if (nrows == 0 || !indptr) return ICC_OK;
+  if (nrows != n) {
+    set_error("icc_op_set_sparse: nrows must equal ndof");
+    return ICC_EINVAL;
+  }
+  const int64_t nnz = indptr[n];
+  std::vector<int32_t> ip(n + 1), ix(nnz);
+  std::vector<float> d32(nnz);
+  std::vector<double> diag(n, 0.0);
+  for (int64_t i = 0; i <= n; ++i) ip[i] = (int32_t)indptr[i];
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+      if (indices[k] < 0 || indices[k] >= n) {
+        set_error("icc_op_set_sparse: column index out of range");
+        return ICC_EINVAL;
+      }
+      ix[k] = (int32_t)indices[k];
+      d32[k] = (float)data[k];
+      if (indices[k] == i) diag[i] += data[k];
+    }
+  ICC_TRY(dalloc(&op->d_s_indptr, n + 1));
+  ICC_TRY(dalloc(&op->d_s_indices, nnz));
+  ICC_TRY(dalloc(&op->d_s_data, nnz));
+  ICC_TRY(dalloc(&op->d_s_data32, nnz));
+  ICC_TRY(dalloc(&op->d_s_diag, n));
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(op->d_s_indptr, ip.data(), (n + 1) * 4, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_s_indices, ix.data(), nnz * 4, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_s_data, data, nnz * 8, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_s_data32, d32.data(), nnz * 4, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_s_diag, diag.data(), n * 8, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  op->s_nnz = nnz;
+  return ICC_OK;
+}
+
+// ---------------------------------------------------------------------------
+namespace icc {
+// y = A x on the op stream; y zero-initialised here unless the sparse term
+// overwrites it first.  pap != null: per-tile / per-row partials of x.Ax.
+template <typename T>
+static hipError_t apply_t(icc_op* op, const T* x, T* y, double* pap) {
+  icc_grid* g = op->g;
+  hipStream_t st = g->ctx->stream;
+  hipError_t e;
+  const int64_t n = g->s.ndof;
+  if (op->s_nnz > 0) {
+    if constexpr (std::is_same<T, double>::value)
+      e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, x, y, op->d_free,
+                                          pap ? pap + g->ntiles : nullptr, st);
+    else
+      e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, x, y, op->d_free,
+                                        pap ? pap + g->ntiles : nullptr, st);
+  } else {
+    e = launch_fill<T>(y, T(0), n, st);
+  }
+  if (e != hipSuccess) return e;
+  if constexpr (std::is_same<T, double>::value)
+    return launch_apply_f64(g, op, x, y, pap, nullptr, st);
+  else
+    return launch_apply_f32(g, op, x, y, pap, nullptr, st);
+}
+}  // namespace icc
+
+ICC_API int icc_apply_dev(icc_op* op, int prec, const void* x, void* y) {
+  if (!op || !x || !y) return ICC_EINVAL;
+  hipError_t e;
+  if (prec == ICC_F64)
+    e = apply_t<double>(op, (const double*)x, (double*)y, nullptr);
+  else if (prec == ICC_F32)
+    e = apply_t<float>(op, (const float*)x, (float*)y, nullptr);
+  else {
+    set_error("icc_apply: unknown precision");
+    return ICC_EINVAL;
+  }
+  if (e != hipSuccess) return hip_fail(e, "icc_apply_dev");
+  return ICC_OK;
+}
+
+ICC_API int icc_apply(icc_op* op, int prec, const double* x, double* y) {
+  if (!op || !x || !y) return ICC_EINVAL;
+  Work* w;
+  ICC_TRY(ensure_work(op, w));
+  const int64_t n = op->g->s.ndof;
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(w->tmp, x, n * 8, hipMemcpyHostToDevice, st));
+  if (prec == ICC_F64) {
+    ICC_HIP(apply_t<double>(op, w->tmp, w->ap, nullptr));
+    ICC_HIP(hipMemcpyAsync(y, w->ap, n * 8, hipMemcpyDeviceToHost, st));
+  } else if (prec == ICC_F32) {
+    ICC_HIP(launch_to_f32(w->tmp, w->x32, n, st));
+    ICC_HIP(apply_t<float>(op, w->x32, w->ap32, nullptr));
+    std::vector<float> h(n);
+    ICC_HIP(hipMemcpyAsync(h.data(), w->ap32, n * 4, hipMemcpyDeviceToHost, st));
+    ICC_HIP(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < n; ++i) y[i] = h[i];
+    return ICC_OK;
+  } else {
+    set_error("icc_apply: unknown precision");
+    return ICC_EINVAL;
+  }
+  ICC_HIP(hipStreamSynchronize(st));
+  return ICC_OK;
+}
+
+ICC_API int icc_diag_dev(icc_op* op, double* d) {
+  if (!op || !d) return ICC_EINVAL;
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(launch_diag_f64(op->g, op->d_scale, d, st));
+  if (op->s_nnz > 0) {
+    // d += diag(S): reuse the fill-free path via a tiny axpy through ir_update (scale 1)
+    ICC_HIP(launch_ir_update<double>(op->g->s.ndof, d, op->d_s_diag, 1.0, st));
+  }
+  return ICC_OK;
+}
+
+ICC_API int icc_diag(icc_op* op, double* d) {
+  if (!op || !d) return ICC_EINVAL;
+  Work* w;
+  ICC_TRY(ensure_work(op, w));
+  hipStream_t st = op->g->ctx->stream;
+  ICC_TRY(icc_diag_dev(op, w->tmp));
+  ICC_HIP(hipMemcpyAsync(d, w->tmp, op->g->s.ndof * 8, hipMemcpyDeviceToHost, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  return ICC_OK;
+}
+
+// ---------------------------------------------------------------------------
+namespace icc {
+static int read_state(icc_op* op, Work* w, PcgState& out) {
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(w->st_host, w->st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  out = *w->st_host;
+  return ICC_OK;
+}
+
+static int write_tol(icc_op* op, Work* w, double tol) {
+  hipStream_t st = op->g->ctx->stream;
+  // the state is reset by the init kernel; tol is written beforehand
+  ICC_HIP(hipMemcpyAsync(&w->st->tol, &tol, sizeof(double), hipMemcpyHostToDevice, st));
+  return ICC_OK;
+}
+
+// Jacobi PCG in precision T on device vectors of the workspace:
+// solves A x = bscale * b, x from 0; dinv already formed.
+template <typename T, typename TB>
+static int pcg_core(icc_op* op, Work* w, const TB* b, double bscale, T* x, T* r, T* p, T* ap, const T* dinv,
+                    double tol, int64_t maxit, int64_t* iters, double* res, int* status) {
+  icc_grid* g = op->g;
+  hipStream_t st = g->ctx->stream;
+  const int64_t n = g->s.ndof;
+  ICC_TRY(write_tol(op, w, tol));
+  ICC_HIP((launch_pcg_init<T, TB>(n, b, bscale, dinv, x, r, p, ap, w->st, w->parts, st)));
+  const int npap = g->ntiles + csr_blocks(op);
+  PcgState hs;
+  int64_t it = 0;
+  int batch = 8;
+  *status = 0;
+  while (it < maxit) {
+    int64_t stop = std::min<int64_t>(maxit, it + batch);
+    for (; it < stop; ++it) {
+      if constexpr (std::is_same<T, double>::value) {
+        // the sparse term writes ap (= S p) before the tiles accumulate on top
+        if (op->s_nnz > 0)
+          ICC_HIP((launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, p, ap,
+                                                    op->d_free, w->pap + g->ntiles, st)));
+        ICC_HIP(launch_apply_f64(g, op, p, ap, w->pap, &w->st->done, st));
+      } else {
+        if (op->s_nnz > 0)
+          ICC_HIP((launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, p, ap,
+                                                  op->d_free, w->pap + g->ntiles, st)));
+        ICC_HIP(launch_apply_f32(g, op, p, ap, w->pap, &w->st->done, st));
+      }
+      ICC_HIP(launch_pap_finalize(w->pap, npap, w->st, st));
+      ICC_HIP(launch_pcg_update<T>(n, (int)it, w->st, p, ap, x, r, dinv, w->parts, st));
+    }
+    ICC_TRY(read_state(op, w, hs));
+    if (hs.done) break;
+    if (batch < 64) batch *= 2;
+  }
+  if (!hs.done) ICC_TRY(read_state(op, w, hs));
+  *iters = hs.iters;
+  *res = hs.res;
+  *status = hs.done;
+  return ICC_OK;
+}
+}  // namespace icc
+
+ICC_API int icc_pcg_dev(icc_op* op, const double* b, const double* diag, double tol, int64_t maxit, double* x,
+                        int64_t* iters, double* res) {
+  if (!op || !b || !diag || !x || !iters || !res) return ICC_EINVAL;
+  Work* w;
+  ICC_TRY(ensure_work(op, w));
+  const int64_t n = op->g->s.ndof;
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(launch_inv_diag<double>(diag, w->dinv, n, 0.0, 0, st));
+  int status = 0;
+  // the loop accumulates into workspace vectors; x is the caller's output
+  ICC_TRY((pcg_core<double, double>(op, w, b, 1.0, x, w->r, w->p, w->ap, w->dinv, tol, maxit, iters, res, &status)));
+  if (status == 1) return ICC_OK;
+  if (status == 2) {
+    set_error("icc_pcg: breakdown (p.Ap <= 0)");
+    return ICC_EBREAKDOWN;
+  }
+  set_error("icc_pcg: iteration limit reached");
+  return ICC_ENOCONV;
+}
+
+ICC_API int icc_pcg(icc_op* op, const double* b, const double* diag, double tol, int64_t maxit, double* x,
+                    int64_t* iters, double* res) {
+  if (!op || !b || !diag || !x || !iters || !res) return ICC_EINVAL;
+  Work* w;
+  ICC_TRY(ensure_work(op, w));
+  const int64_t n = op->g->s.ndof;
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(w->f, b, n * 8, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(w->tmp, diag, n * 8, hipMemcpyHostToDevice, st));
+  int rc = icc_pcg_dev(op, w->f, w->tmp, tol, maxit, w->x, iters, res);
+  if (rc != ICC_OK && rc != ICC_EBREAKDOWN && rc != ICC_ENOCONV) return rc;
+  ICC_HIP(hipMemcpyAsync(x, w->x, n * 8, hipMemcpyDeviceToHost, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol, double inner_tol, int64_t max_outer,
+                             int64_t max_inner, double* u, double* stats) {
+  if (!op || !f || !u) return ICC_EINVAL;
+  if (inner != ICC_F64 && inner != ICC_F32) {
+    set_error("icc_solve_ir: unknown inner precision");
+    return ICC_EINVAL;
+  }
+  Work* w;
+  ICC_TRY(ensure_work(op, w));
+  icc_grid* g = op->g;
+  const int64_t n = g->s.ndof;
+  hipStream_t st = g->ctx->stream;
+  // Jacobi from the FP64 diagonal, clamped like solve_state (topopt_iga.rs:147)
+  ICC_TRY(icc_diag_dev(op, w->tmp));
+  ICC_HIP(launch_inv_diag<double>(w->tmp, w->dinv, n, 1e-12, 1, st));
+  ICC_HIP(launch_inv_diag<float>(w->tmp, w->dinv32, n, 1e-12, 1, st));
+  double st_outer = 0, st_inner = 0, st_res = 1, st_app64 = 0, st_appin = 0;
+  int rc_final = ICC_ENOCONV;
+  PcgState hs;
+  for (int64_t outer = 0; outer <= max_outer; ++outer) {
+    // r = M (f - A u) in FP64, compensated norms
+    ICC_HIP(apply_t<double>(op, u, w->ap, nullptr));
+    st_app64 += 1;
+    ICC_HIP(launch_residual(n, f, w->ap, op->d_free, w->r, w->st, w->parts, st));
+    ICC_TRY(read_state(op, w, hs));
+    const double nr = std::sqrt(hs.rr), nf = std::sqrt(hs.norm_b);
+    st_res = nf > 0 ? nr / nf : nr;
+    if (st_res <= tol) {
+      rc_final = ICC_OK;
+      break;
+    }
+    if (outer == max_outer) break;
+    st_outer += 1;
+    int64_t its = 0;
+    double res = 0;
+    int status = 0;
+    if (inner == ICC_F32) {
+      ICC_TRY((pcg_core<float, double>(op, w, w->r, 1.0 / nr, w->x32, w->r32, w->p32, w->ap32, w->dinv32, inner_tol,
+                                       max_inner, &its, &res, &status)));
+      ICC_HIP(launch_ir_update<float>(n, u, w->x32, nr, st));
+    } else {
+      ICC_TRY((pcg_core<double, double>(op, w, w->r, 1.0 / nr, w->x, w->tmp, w->p, w->ap, w->dinv, inner_tol,
+                                        max_inner, &its, &res, &status)));
+      ICC_HIP(launch_ir_update<double>(n, u, w->x, nr, st));
+    }
+    st_inner += (double)its;
+    st_appin += (double)its;
+    if (status == 2 && its <= 1) {
+      set_error("icc_solve_ir: inner breakdown");
+      rc_final = ICC_EBREAKDOWN;
+      break;
+    }
+  }
+  if (stats) {
+    stats[0] = st_outer;
+    stats[1] = st_inner;
+    stats[2] = st_res;
+    stats[3] = st_app64;
+    stats[4] = st_appin;
+    stats[5] = stats[6] = stats[7] = 0.0;
+  }
+  if (rc_final == ICC_ENOCONV) set_error("icc_solve_ir: outer iteration limit reached");
+  return rc_final;
+}
+
+ICC_API int icc_solve_ir(icc_op* op, int inner, const double* f, double tol, double inner_tol, int64_t max_outer,
+                         int64_t max_inner, double* u, double* stats) {
+  if (!op || !f || !u) return ICC_EINVAL;
+  const int64_t n = op->g->s.ndof;
+  hipStream_t st = op->g->ctx->stream;
+  double *df = nullptr, *du = nullptr;
+  ICC_TRY(dalloc(&df, n));
+  if (dalloc(&du, n) != ICC_OK) {
+    hipFree(df);
+    return ICC_ENOMEM;
+  }
+  int rc = ICC_OK;
+  if (hipMemcpyAsync(df, f, n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(du, u, n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+    rc = ICC_EHIP;
+  } else {
+    rc = icc_solve_ir_dev(op, inner, df, tol, inner_tol, max_outer, max_inner, du, stats);
+    if (rc == ICC_OK || rc == ICC_ENOCONV || rc == ICC_EBREAKDOWN) {
+      if (hipMemcpyAsync(u, du, n * 8, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        rc = ICC_EHIP;
+    }
+  }
+  hipFree(df);
+  hipFree(du);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+ICC_API int icc_energies_dev(icc_op* op, const double* u, double* ee) {
+  if (!op || !u || !ee) return ICC_EINVAL;
+  ICC_HIP(launch_energies(op->g, u, ee, nullptr, 0.0, 0.0, nullptr, nullptr, op->g->ctx->stream));
+  return ICC_OK;
+}
+
+ICC_API int icc_sens_dev(icc_op* op, const double* u, const double* x, double penal, double emin, const double* w,
+                         double* dc) {
+  if (!op || !u || !x || !dc) return ICC_EINVAL;
+  ICC_HIP(launch_energies(op->g, u, nullptr, x, penal, emin, w, dc, op->g->ctx->stream));
+  return ICC_OK;
+}
+
+namespace icc {
+// host wrapper: uploads `nin` host arrays of given lengths, runs fn, downloads one output
+static int host_call(icc_ctx* ctx, std::vector<std::pair<const double*, int64_t>> ins, int64_t nout, double* out,
+                     const std::function<int(std::vector<double*>&, double*)>& fn) {
+  hipStream_t st = ctx->stream;
+  std::vector<double*> dins;
+  double* dout = nullptr;
+  int rc = ICC_OK;
+  for (auto& in : ins) {
+    double* d = nullptr;
+    if (in.first) {
+      if ((rc = dalloc(&d, in.second)) != ICC_OK) break;
+      if (hipMemcpyAsync(d, in.first, in.second * 8, hipMemcpyHostToDevice, st) != hipSuccess) rc = ICC_EHIP;
+    }
+    dins.push_back(d);
+  }
+  if (rc == ICC_OK) rc = dalloc(&dout, nout);
+  if (rc == ICC_OK) rc = fn(dins, dout);
+  if (rc == ICC_OK) {
+    if (hipMemcpyAsync(out, dout, nout * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = ICC_EHIP;
+  } else {
+    hipStreamSynchronize(st);
+  }
+  for (double* d : dins) hipFree(d);
+  hipFree(dout);
+  return rc;
+}
+}  // namespace icc
+
+ICC_API int icc_energies(icc_op* op, const double* u, double* ee) {
+  if (!op || !u || !ee) return ICC_EINVAL;
+  const auto& s = op->g->s;
+  return host_call(op->g->ctx, {{u, s.ndof}}, s.nel_tot, ee,
+                   [&](std::vector<double*>& in, double* out) { return icc_energies_dev(op, in[0], out); });
+}
+
+ICC_API int icc_sens(icc_op* op, const double* u, const double* x, double penal, double emin, const double* w,
+                     double* dc) {
+  if (!op || !u || !x || !dc) return ICC_EINVAL;
+  const auto& s = op->g->s;
+  return host_call(op->g->ctx, {{u, s.ndof}, {x, s.nel_tot}, {w, s.nel_tot}}, s.nel_tot, dc,
+                   [&](std::vector<double*>& in, double* out) {
+                     return icc_sens_dev(op, in[0], in[1], penal, emin, in[2], out);
+                   });
+}
+
+// ---------------------------------------------------------------------------
+ICC_API int icc_filter_create(icc_grid* g, double rmin, icc_filter** out) {
+  if (!g || !out || !(rmin > 0)) return ICC_EINVAL;
+  auto* f = new icc_filter();
+  f->g = g;
+  const auto& s = g->s;
+  double h[3];
+  for (int d = 0; d < 3; ++d) h[d] = s.len[d] / (double)s.nel[d];
+  f->r = rmin * (h[0] + h[1] + h[2]) / 3.0;
+  for (int d = 0; d < 3; ++d) f->rad[d] = (int)std::ceil(f->r / h[d]) + 1;
+  int rc = dalloc(&f->d_hs, s.nel_tot);
+  if (rc != ICC_OK) {
+    delete f;
+    return rc;
+  }
+  hipError_t e = launch_filter(f, 3, nullptr, nullptr, f->d_hs, g->ctx->stream);
+  if (e != hipSuccess) {
+    icc_filter_destroy(f);
+    return hip_fail(e, "icc_filter_create");
+  }
+  *out = f;
+  return ICC_OK;
+}
+
+ICC_API int icc_filter_destroy(icc_filter* f) {
+  if (!f) return ICC_OK;
+  hipFree(f->d_hs);
+  hipFree(f->d_tmp);
+  delete f;
+  return ICC_OK;
+}
+
+ICC_API int icc_filter_sens_dev(icc_filter* f, const double* x, const double* dc, double* out) {
+  if (!f || !x || !dc || !out) return ICC_EINVAL;
+  ICC_HIP(launch_filter(f, 0, x, dc, out, f->g->ctx->stream));
+  return ICC_OK;
+}
+ICC_API int icc_filter_fwd_dev(icc_filter* f, const double* x, double* out) {
+  if (!f || !x || !out) return ICC_EINVAL;
+  ICC_HIP(launch_filter(f, 1, nullptr, x, out, f->g->ctx->stream));
+  return ICC_OK;
+}
+ICC_API int icc_filter_bwd_dev(icc_filter* f, const double* gr, double* out) {
+  if (!f || !gr || !out) return ICC_EINVAL;
+  ICC_HIP(launch_filter(f, 2, nullptr, gr, out, f->g->ctx->stream));
+  return ICC_OK;
+}
+ICC_API int icc_filter_sens(icc_filter* f, const double* x, const double* dc, double* out) {
+  if (!f || !x || !dc || !out) return ICC_EINVAL;
+  int64_t n = f->g->s.nel_tot;
+  return host_call(f->g->ctx, {{x, n}, {dc, n}}, n, out,
+                   [&](std::vector<double*>& in, double* o) { return icc_filter_sens_dev(f, in[0], in[1], o); });
+}
+ICC_API int icc_filter_fwd(icc_filter* f, const double* x, double* out) {
+  if (!f || !x || !out) return ICC_EINVAL;
+  int64_t n = f->g->s.nel_tot;
+  return host_call(f->g->ctx, {{x, n}}, n, out,
+                   [&](std::vector<double*>& in, double* o) { return icc_filter_fwd_dev(f, in[0], o); });
+}
+ICC_API int icc_filter_bwd(icc_filter* f, const double* gr, double* out) {
+  if (!f || !gr || !out) return ICC_EINVAL;
+  int64_t n = f->g->s.nel_tot;
+  return host_call(f->g->ctx, {{gr, n}}, n, out,
+                   [&](std::vector<double*>& in, double* o) { return icc_filter_bwd_dev(f, in[0], o); });
+}
+
+// ---------------------------------------------------------------------------
+ICC_API int icc_wq_diag(icc_grid* g, const double* x, double penal, double emin, double* d) {
+  (void)g;
+  (void)x;
+  (void)penal;
+  (void)emin;
+  (void)d;
+  set_error("icc_wq_diag: not implemented yet");
+  return ICC_ENOTSUP;
+}
+ICC_API int icc_wq_diag_dev(icc_grid* g, const double* x, double penal, double emin, double* d) {
+  return icc_wq_diag(g, x, penal, emin, d);
+}
+
+ICC_API int icc_classify_cylinders(icc_ctx* ctx, const double box_lo[3], const double box_hi[3], const double* cyl,
+                                   int64_t ncyl, const double grid_bounds[6], const int64_t res[3],
+                                   const int64_t sub[3], int8_t* status, double* weights) {
+  (void)ctx;
+  (void)box_lo;
+  (void)box_hi;
+  (void)cyl;
+  (void)ncyl;
+  (void)grid_bounds;
+  (void)res;
+  (void)sub;
+  (void)status;
+  (void)weights;
+  set_error("icc_classify_cylinders: not implemented yet");
+  return ICC_ENOTSUP;
+}
+
+ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out256) {
+  if (!ctx || !out256) return ICC_EINVAL;
+  double* d = nullptr;
+  ICC_TRY(dalloc(&d, 256));
+  hipError_t e = launch_layout_probe(d, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out256, d, 256 * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "icc_debug_mfma_layout");
+  return ICC_OK;
+}

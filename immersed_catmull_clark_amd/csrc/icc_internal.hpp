@@ -1,0 +1,107 @@
+// Internal types of libicc (host side).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/icc.h"
+
+namespace icc {
+
+constexpr int NSH = 81;      // 3 components x 27 control points per p = 2 element
+constexpr int NSC = 27;      // scalar functions per element
+constexpr int MT = 6;        // 16-row M tiles covering 81 test rows (96 padded)
+constexpr int KS = 21;       // 4-deep K steps covering 81 trial columns (84 padded)
+constexpr int TILE_MAX_EL = 128;   // elements per tile (8 waves x 16)
+constexpr int TILE_MAX_PTS = 640;  // control points staged per tile
+
+// Tile of a constant-type element sub-box: elements [i0, i0+nx) x [j0, j0+ny) x [k0, k0+nz).
+struct Tile {
+  int32_t i0, j0, k0, nx, ny, nz, type, pad;
+};
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define ICC_HIP(call)                                 \
+  do {                                                \
+    hipError_t e__ = (call);                          \
+    if (e__ != hipSuccess) return icc::hip_fail(e__, #call); \
+  } while (0)
+
+// Host-side product setup: p = 2 B-spline space on a box, 1D element matrices,
+// element types and the 81x81 matrices.  Independent of the test oracle.
+struct GridSetup {
+  int64_t nel[3];
+  int64_t ncp[3];
+  double lo[3], len[3];
+  int64_t nel_tot, ndof_sc, ndof;
+  double lambda, mu;
+  // per direction: 1D type of each element and the 1D matrices of each type
+  std::vector<int32_t> tid1[3];
+  std::vector<std::array<double, 36>> mats1[3];  // [k][a][b] with k = 00,10,01,11
+  // element types (first-appearance order) as (t0, t1, t2)
+  std::vector<std::array<int32_t, 3>> type_list;
+  std::vector<int32_t> type_of;  // index t0 + T0*(t1 + T1*t2) -> element type
+  std::vector<std::vector<double>> kmat;  // [ntypes][81*81]
+  std::vector<Tile> tiles;
+};
+
+int build_setup(const int64_t nel[3], const double bounds[6], double young, double poisson, GridSetup& s);
+int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
+
+}  // namespace icc
+
+struct icc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+};
+
+struct icc_grid {
+  icc_ctx* ctx = nullptr;
+  icc::GridSetup s;
+  int ntypes = 0;
+  // device data
+  double* d_kfrag64 = nullptr;  // [ntypes][MT][KS][64]
+  float* d_kfrag32 = nullptr;   // [ntypes][MT][KS][64]
+  double* d_kdiag = nullptr;    // [ntypes][81]
+  int32_t* d_tid1 = nullptr;    // [3][maxnel] 1D types
+  int32_t* d_typeof = nullptr;  // [T0*T1*T2]
+  icc::Tile* d_tiles = nullptr;
+  int ntiles = 0;
+  double* d_mats1 = nullptr;  // [3][T][36]
+  int maxnel = 0;
+  int T[3] = {0, 0, 0};
+};
+
+struct icc_op {
+  icc_grid* g = nullptr;
+  double* d_scale = nullptr;    // [nel]
+  float* d_scale32 = nullptr;   // [nel]
+  uint8_t* d_free = nullptr;    // [ndof], null = all free
+  // optional sparse add-on (CSR, int32 indices)
+  int64_t s_nnz = 0;
+  int32_t* d_s_indptr = nullptr;
+  int32_t* d_s_indices = nullptr;
+  double* d_s_data = nullptr;
+  float* d_s_data32 = nullptr;
+  double* d_s_diag = nullptr;
+  // scratch
+  double* d_partials = nullptr;  // reduction partials
+  int partials_cap = 0;
+};
+
+struct icc_filter {
+  icc_grid* g = nullptr;
+  double r = 0;
+  int rad[3] = {0, 0, 0};
+  // neighbour offsets (di, dj, dk, w) in increasing linear-index order
+  int32_t* d_off = nullptr;  // [noff][3]
+  double* d_w = nullptr;     // [noff]
+  int noff = 0;
+  double* d_hs = nullptr;  // [nel] row sums
+  double* d_tmp = nullptr;
+};

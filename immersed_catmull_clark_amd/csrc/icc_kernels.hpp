@@ -1,0 +1,68 @@
+// Kernel launchers shared between the .hip translation units and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "icc_internal.hpp"
+
+namespace icc {
+
+// Device-resident PCG / refinement scalars.
+struct PcgState {
+  double rz[2];   // r.D^-1 r by iteration parity
+  double norm_b;  // max(|b|, 1e-16); squared |f_free|^2 after residual_kernel
+  double res;     // |r| / norm_b after the last update
+  double rr;      // r.r (squared |r| after residual_kernel)
+  double pap;     // p.Ap of the current iteration
+  double tol;     // host-set relative tolerance
+  int32_t iters;
+  int32_t done;   // 0 running, 1 converged, 2 breakdown
+  uint32_t ticket;
+  int32_t pad;
+};
+
+// k_apply.hip
+hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x, double* y, double* pap,
+                            const int32_t* done, hipStream_t st);
+hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
+                            const int32_t* done, hipStream_t st);
+hipError_t launch_energies(const icc_grid* g, const double* u, double* ee, const double* x, double penal,
+                           double emin, const double* w, double* dc, hipStream_t st);
+hipError_t launch_diag_f64(const icc_grid* g, const double* scale, double* d, hipStream_t st);
+hipError_t launch_diag_f32(const icc_grid* g, const float* scale, float* d, hipStream_t st);
+hipError_t launch_layout_probe(double* out, hipStream_t st);
+
+// k_vec.hip
+int vec_blocks(int64_t n);
+template <typename T>
+hipError_t launch_fill(T* y, T v, int64_t n, hipStream_t st);
+template <typename T>
+hipError_t launch_inv_diag(const double* d, T* dinv, int64_t n, double clamp_min, int do_clamp, hipStream_t st);
+template <typename T, typename TB>
+hipError_t launch_pcg_init(int64_t n, const TB* b, double bscale, const T* dinv, T* x, T* r, T* p, T* ap,
+                           PcgState* st, double* parts, hipStream_t s);
+hipError_t launch_pap_finalize(const double* pap_parts, int npap, PcgState* st, hipStream_t s);
+template <typename T>
+hipError_t launch_pcg_update(int64_t n, int it, PcgState* st, T* p, T* ap, T* x, T* r, const T* dinv, double* parts,
+                             hipStream_t s);
+hipError_t launch_residual(int64_t n, const double* f, const double* ap, const uint8_t* freem, double* r,
+                           PcgState* st, double* parts, hipStream_t s);
+template <typename T>
+hipError_t launch_ir_update(int64_t n, double* u, const T* d, double scale, hipStream_t s);
+template <typename T, typename TD>
+hipError_t launch_csr_spmv(int64_t n, const int32_t* indptr, const int32_t* indices, const TD* data, const T* x, T* y,
+                           const uint8_t* freem, double* parts, hipStream_t s);
+
+hipError_t launch_clamp_min(int64_t n, const double* a, double lo, double* b, hipStream_t st);
+hipError_t launch_simp_scale(int64_t n, const double* x, double penal, double emin, const double* w,
+                             const uint8_t* active, double* s, hipStream_t st);
+hipError_t launch_masked_add(int64_t n, const double* u0, const double* du, const uint8_t* freem, double* u,
+                             hipStream_t st);
+
+// k_misc.hip
+hipError_t launch_filter(const icc_filter* f, int mode, const double* x, const double* v, double* out,
+                         hipStream_t st);
+hipError_t launch_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
+
+}  // namespace icc

@@ -1,0 +1,424 @@
+// Jacobi-PCG vector kernels, FP64 and FP32, with device-resident scalars.
+//
+// One PCG iteration = apply (p -> ap, fused per-tile partials of p.Ap) +
+// pcg_update1 (alpha, x += alpha p, r -= alpha ap, partials of r.r and r.D^-1 r)
+// + pcg_update2 (convergence test, beta, p = D^-1 r + beta p, ap = 0).  Every
+// kernel checks the device `done` flag first, so the host can enqueue
+// iterations in batches and read the state back once per batch.
+//
+// Reductions are deterministic: each block writes one partial, and the last
+// block to finish (atomic ticket after a release fence) sums the partials in
+// index order.  Partial sums are always accumulated in FP64.  The FP64
+// residual norms of the refinement loop use compensated Dot2 accumulation
+// (TwoProd by fma, TwoSum; Ogita, Rump and Oishi 2005).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "icc_internal.hpp"
+#include "icc_kernels.hpp"
+
+namespace icc {
+
+constexpr int VB = 256;  // threads per block of the vector kernels
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+  __syncthreads();
+  return s;  // valid on thread 0
+}
+
+// Last-block reduction: returns true on every thread of the block that
+// finished last; that block then sums `parts` (ordered) with sum_parts.
+__device__ __forceinline__ bool last_block(unsigned* ticket) {
+  __shared__ bool is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned prev = atomicAdd(ticket, 1u);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last;
+}
+
+__device__ __forceinline__ double sum_parts(const double* parts, int n, double* sh) {
+  double v = 0.0;
+  // fixed assignment of partials to threads, then a fixed tree
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += __builtin_nontemporal_load(parts + i);
+  return block_sum(v, sh);
+}
+
+template <typename T>
+__global__ __launch_bounds__(VB) void fill_kernel(T* __restrict__ y, T v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = v;
+}
+
+// inv_diag = |d| > 1e-12 ? 1/d : 1 (solver.rs:24-27); optionally clamps d first
+template <typename T>
+__global__ __launch_bounds__(VB) void inv_diag_kernel(const double* __restrict__ d, T* __restrict__ dinv, int64_t n,
+                                                      double clamp_min, int do_clamp) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = d[i];
+    if (do_clamp) v = fmax(v, clamp_min);
+    dinv[i] = (T)(fabs(v) > 1e-12 ? 1.0 / v : 1.0);
+  }
+}
+
+// x = 0, r = scale * b (converted to T), z = dinv r, p = z, ap = 0; partials of r.z and b.b
+template <typename T, typename TB>
+__global__ __launch_bounds__(VB) void pcg_init1(int64_t n, const TB* __restrict__ b, double bscale,
+                                                const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r,
+                                                T* __restrict__ p, T* __restrict__ ap, PcgState* st,
+                                                double* __restrict__ parts) {
+  __shared__ double sh[16];
+  double rz = 0.0, bb = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T ri = (T)((double)b[i] * bscale);
+    T zi = dinv[i] * ri;
+    x[i] = T(0);
+    r[i] = ri;
+    p[i] = zi;
+    ap[i] = T(0);
+    rz += (double)ri * (double)zi;
+    bb += (double)ri * (double)ri;
+  }
+  double s1 = block_sum(rz, sh);
+  double s2 = block_sum(bb, sh);
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x] = s1;
+    parts[gridDim.x + blockIdx.x] = s2;
+  }
+  if (last_block(&st->ticket)) {
+    double rz_t = sum_parts(parts, gridDim.x, sh);
+    double bb_t = sum_parts(parts + gridDim.x, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      st->rz[0] = rz_t;
+      st->rz[1] = rz_t;
+      st->norm_b = fmax(sqrt(bb_t), 1e-16);
+      st->iters = 0;
+      st->done = 0;
+      st->res = 1.0;
+      st->ticket = 0;
+    }
+  }
+}
+
+// Sums the apply's p.Ap partials into st->pap (one block).
+__global__ __launch_bounds__(VB) void pap_finalize(const double* __restrict__ pap_parts, int npap, PcgState* st) {
+  __shared__ double sh[16];
+  if (st->done) return;
+  double s = sum_parts(pap_parts, npap, sh);
+  if (threadIdx.x == 0) st->pap = s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(VB) void pcg_update1(int64_t n, int it, PcgState* st, const T* __restrict__ p,
+                                                  const T* __restrict__ ap, T* __restrict__ x, T* __restrict__ r,
+                                                  const T* __restrict__ dinv, double* __restrict__ parts) {
+  __shared__ double sh[16];
+  if (st->done) return;
+  const double pap = st->pap;
+  if (!(pap > 0.0)) {  // breakdown (solver.rs:47-49): stop before updating x
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->iters = it + 1;
+      st->done = 2;
+    }
+    return;
+  }
+  const T alpha = (T)(st->rz[it & 1] / pap);
+  double rr = 0.0, rz = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T xi = x[i] + alpha * p[i];
+    T ri = r[i] - alpha * ap[i];
+    x[i] = xi;
+    r[i] = ri;
+    rr += (double)ri * (double)ri;
+    rz += (double)ri * (double)(dinv[i] * ri);
+  }
+  double s1 = block_sum(rr, sh);
+  double s2 = block_sum(rz, sh);
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x] = s1;
+    parts[gridDim.x + blockIdx.x] = s2;
+  }
+  if (last_block(&st->ticket)) {
+    double rr_t = sum_parts(parts, gridDim.x, sh);
+    double rz_t = sum_parts(parts + gridDim.x, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      st->iters = it + 1;
+      st->rr = rr_t;
+      st->rz[(it + 1) & 1] = rz_t;
+      st->res = sqrt(rr_t) / st->norm_b;
+      if (st->res < st->tol) st->done = 1;
+      st->ticket = 0;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(VB) void pcg_update2(int64_t n, int it, const PcgState* st, T* __restrict__ p,
+                                                  const T* __restrict__ r, const T* __restrict__ dinv,
+                                                  T* __restrict__ ap) {
+  if (st->done) return;
+  const T beta = (T)(st->rz[(it + 1) & 1] / st->rz[it & 1]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    p[i] = dinv[i] * r[i] + beta * p[i];
+    ap[i] = T(0);
+  }
+}
+
+// ---- refinement helpers ------------------------------------------------------
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated block sum of (hi, lo) pairs: thread 0 gets hi + lo.
+__device__ double block_sum_comp(double hi, double lo, double* sh) {
+  __shared__ double shl[16];
+  for (int off = 32; off > 0; off >>= 1) {
+    double oh = __shfl_down(hi, off, 64), ol = __shfl_down(lo, off, 64);
+    double s, e;
+    two_sum(hi, oh, s, e);
+    hi = s;
+    lo = lo + ol + e;
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    sh[w] = hi;
+    shl[w] = lo;
+  }
+  __syncthreads();
+  double out = 0.0;
+  if (threadIdx.x == 0) {
+    double h = 0.0, lo2 = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      double s, e;
+      two_sum(h, sh[i], s, e);
+      h = s;
+      lo2 += shl[i] + e;
+    }
+    out = h + lo2;
+  }
+  __syncthreads();
+  return out;
+}
+
+// r = free ? f - ap : 0 ; |r|^2 and |f_free|^2 by Dot2; results in st->rr, st->norm_b (squared norms)
+__global__ __launch_bounds__(VB) void residual_kernel(int64_t n, const double* __restrict__ f,
+                                                      const double* __restrict__ ap, const uint8_t* __restrict__ freem,
+                                                      double* __restrict__ r, PcgState* st, double* __restrict__ parts) {
+  __shared__ double sh[16];
+  double h1 = 0.0, l1 = 0.0, h2 = 0.0, l2 = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    bool fr = freem == nullptr || freem[i] != 0;
+    double fi = fr ? f[i] : 0.0;
+    double ri = fr ? fi - ap[i] : 0.0;
+    r[i] = ri;
+    double p = ri * ri, pe = fma(ri, ri, -p), s, e;
+    two_sum(h1, p, s, e);
+    h1 = s;
+    l1 += e + pe;
+    p = fi * fi;
+    pe = fma(fi, fi, -p);
+    two_sum(h2, p, s, e);
+    h2 = s;
+    l2 += e + pe;
+  }
+  double s1 = block_sum_comp(h1, l1, sh);
+  double s2 = block_sum_comp(h2, l2, sh);
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x] = s1;
+    parts[gridDim.x + blockIdx.x] = s2;
+  }
+  if (last_block(&st->ticket)) {
+    // compensated sum of the block partials in index order
+    double h = 0.0, lo = 0.0, hf = 0.0, lof = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+      double s, e;
+      two_sum(h, __builtin_nontemporal_load(parts + i), s, e);
+      h = s;
+      lo += e;
+      two_sum(hf, __builtin_nontemporal_load(parts + gridDim.x + i), s, e);
+      hf = s;
+      lof += e;
+    }
+    double t1 = block_sum_comp(h, lo, sh);
+    double t2 = block_sum_comp(hf, lof, sh);
+    if (threadIdx.x == 0) {
+      st->rr = t1;
+      st->norm_b = t2;
+      st->ticket = 0;
+    }
+  }
+}
+
+// u += s * d  (d in T)
+template <typename T>
+__global__ __launch_bounds__(VB) void ir_update_kernel(int64_t n, double* __restrict__ u, const T* __restrict__ d,
+                                                       double s) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    u[i] += s * (double)d[i];
+}
+
+// ---- CSR add-on: y = S x (overwrites y), per-row partials of x.(S x) -----------
+template <typename T, typename TD>
+__global__ __launch_bounds__(VB) void csr_spmv_kernel(int64_t n, const int32_t* __restrict__ indptr,
+                                                      const int32_t* __restrict__ indices, const TD* __restrict__ data,
+                                                      const T* __restrict__ x, T* __restrict__ y,
+                                                      const uint8_t* __restrict__ freem, double* __restrict__ parts) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    bool fr = freem == nullptr || freem[i] != 0;
+    T s = T(0);
+    if (fr) {
+      for (int k = indptr[i]; k < indptr[i + 1]; ++k) {
+        int c = indices[k];
+        T xv = x[c];
+        if (freem != nullptr && freem[c] == 0) xv = T(0);
+        s += (T)data[k] * xv;
+      }
+      acc += (double)x[i] * (double)s;
+    }
+    y[i] = s;
+  }
+  if (parts) {
+    double b = block_sum(acc, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = b;
+  }
+}
+
+// s_e = emin + (1 - emin) x_e^penal  [* w_e, or emin where w_e is inactive]
+// (topopt_iga.rs:132; topopt_immersed.rs:166 when w != null)
+__global__ __launch_bounds__(VB) void simp_scale_kernel(int64_t n, const double* __restrict__ x, double penal,
+                                                        double emin, const double* __restrict__ w,
+                                                        const uint8_t* __restrict__ active, double* __restrict__ s) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double v = emin + (1.0 - emin) * pow(x[i], penal);
+    if (w != nullptr) v = (active == nullptr || active[i]) ? w[i] * v : emin;
+    s[i] = v;
+  }
+}
+
+// u = free ? u0 + du : 0   (topopt_iga.rs:160)
+__global__ __launch_bounds__(VB) void masked_add_kernel(int64_t n, const double* __restrict__ u0,
+                                                        const double* __restrict__ du,
+                                                        const uint8_t* __restrict__ freem, double* __restrict__ u) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    u[i] = (freem == nullptr || freem[i] != 0) ? u0[i] + du[i] : 0.0;
+}
+
+__global__ __launch_bounds__(VB) void clamp_min_kernel(int64_t n, const double* __restrict__ a, double lo,
+                                                       double* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = fmax(a[i], lo);
+}
+
+// ---- launchers ------------------------------------------------------------------
+hipError_t launch_clamp_min(int64_t n, const double* a, double lo, double* b, hipStream_t st) {
+  hipLaunchKernelGGL(clamp_min_kernel, dim3(vec_blocks(n)), dim3(VB), 0, st, n, a, lo, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_simp_scale(int64_t n, const double* x, double penal, double emin, const double* w,
+                             const uint8_t* active, double* s, hipStream_t st) {
+  hipLaunchKernelGGL(simp_scale_kernel, dim3(vec_blocks(n)), dim3(VB), 0, st, n, x, penal, emin, w, active, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_masked_add(int64_t n, const double* u0, const double* du, const uint8_t* freem, double* u,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(masked_add_kernel, dim3(vec_blocks(n)), dim3(VB), 0, st, n, u0, du, freem, u);
+  return hipGetLastError();
+}
+
+int vec_blocks(int64_t n) {
+  int64_t nb = (n + VB * 4 - 1) / (VB * 4);
+  if (nb < 1) nb = 1;
+  if (nb > 1024) nb = 1024;
+  return (int)nb;
+}
+
+template <typename T>
+hipError_t launch_fill(T* y, T v, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL((fill_kernel<T>), dim3(vec_blocks(n)), dim3(VB), 0, st, y, v, n);
+  return hipGetLastError();
+}
+template hipError_t launch_fill<double>(double*, double, int64_t, hipStream_t);
+template hipError_t launch_fill<float>(float*, float, int64_t, hipStream_t);
+
+template <typename T>
+hipError_t launch_inv_diag(const double* d, T* dinv, int64_t n, double clamp_min, int do_clamp, hipStream_t st) {
+  hipLaunchKernelGGL((inv_diag_kernel<T>), dim3(vec_blocks(n)), dim3(VB), 0, st, d, dinv, n, clamp_min, do_clamp);
+  return hipGetLastError();
+}
+template hipError_t launch_inv_diag<double>(const double*, double*, int64_t, double, int, hipStream_t);
+template hipError_t launch_inv_diag<float>(const double*, float*, int64_t, double, int, hipStream_t);
+
+template <typename T, typename TB>
+hipError_t launch_pcg_init(int64_t n, const TB* b, double bscale, const T* dinv, T* x, T* r, T* p, T* ap,
+                           PcgState* st, double* parts, hipStream_t s) {
+  hipLaunchKernelGGL((pcg_init1<T, TB>), dim3(vec_blocks(n)), dim3(VB), 0, s, n, b, bscale, dinv, x, r, p, ap, st,
+                     parts);
+  return hipGetLastError();
+}
+template hipError_t launch_pcg_init<double, double>(int64_t, const double*, double, const double*, double*, double*,
+                                                    double*, double*, PcgState*, double*, hipStream_t);
+template hipError_t launch_pcg_init<float, double>(int64_t, const double*, double, const float*, float*, float*,
+                                                   float*, float*, PcgState*, double*, hipStream_t);
+
+hipError_t launch_pap_finalize(const double* pap_parts, int npap, PcgState* st, hipStream_t s) {
+  hipLaunchKernelGGL(pap_finalize, dim3(1), dim3(VB), 0, s, pap_parts, npap, st);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pcg_update(int64_t n, int it, PcgState* st, T* p, T* ap, T* x, T* r, const T* dinv, double* parts,
+                             hipStream_t s) {
+  int nb = vec_blocks(n);
+  hipLaunchKernelGGL((pcg_update1<T>), dim3(nb), dim3(VB), 0, s, n, it, st, p, ap, x, r, dinv, parts);
+  hipLaunchKernelGGL((pcg_update2<T>), dim3(nb), dim3(VB), 0, s, n, it, st, p, r, dinv, ap);
+  return hipGetLastError();
+}
+template hipError_t launch_pcg_update<double>(int64_t, int, PcgState*, double*, double*, double*, double*,
+                                              const double*, double*, hipStream_t);
+template hipError_t launch_pcg_update<float>(int64_t, int, PcgState*, float*, float*, float*, float*, const float*,
+                                             double*, hipStream_t);
+
+hipError_t launch_residual(int64_t n, const double* f, const double* ap, const uint8_t* freem, double* r,
+                           PcgState* st, double* parts, hipStream_t s) {
+  hipLaunchKernelGGL(residual_kernel, dim3(vec_blocks(n)), dim3(VB), 0, s, n, f, ap, freem, r, st, parts);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ir_update(int64_t n, double* u, const T* d, double scale, hipStream_t s) {
+  hipLaunchKernelGGL((ir_update_kernel<T>), dim3(vec_blocks(n)), dim3(VB), 0, s, n, u, d, scale);
+  return hipGetLastError();
+}
+template hipError_t launch_ir_update<double>(int64_t, double*, const double*, double, hipStream_t);
+template hipError_t launch_ir_update<float>(int64_t, double*, const float*, double, hipStream_t);
+
+template <typename T, typename TD>
+hipError_t launch_csr_spmv(int64_t n, const int32_t* indptr, const int32_t* indices, const TD* data, const T* x, T* y,
+                           const uint8_t* freem, double* parts, hipStream_t s) {
+  hipLaunchKernelGGL((csr_spmv_kernel<T, TD>), dim3(vec_blocks(n)), dim3(VB), 0, s, n, indptr, indices, data, x, y,
+                     freem, parts);
+  return hipGetLastError();
+}
+template hipError_t launch_csr_spmv<double, double>(int64_t, const int32_t*, const int32_t*, const double*,
+                                                    const double*, double*, const uint8_t*, double*, hipStream_t);
+template hipError_t launch_csr_spmv<float, float>(int64_t, const int32_t*, const int32_t*, const float*, const float*,
+                                                  float*, const uint8_t*, double*, hipStream_t);
+
+}  // namespace icc

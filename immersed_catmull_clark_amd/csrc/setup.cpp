@@ -1,0 +1,337 @@
+// Host-side setup of the p = 2 grid: open uniform knots, Gauss data, the 1D
+// element matrices per direction, the element types and their 81x81 matrices,
+// and the constant-type tiling the apply kernels walk.
+//
+// Semantics follow the reference (rust/src/iga/{quadrature,space,elasticity}.rs):
+// types are the distinct combinations of per-direction 1D types, numbered in
+// order of first appearance (elasticity.rs:85-98); 1D types are grouped by a
+// tolerance rather than the reference's rounded key (see build_setup);
+// K = lambda G_ij + mu G_ji + mu delta_ij sum_k G_kk (elasticity.rs:125-145).
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <string>
+
+#include "icc_internal.hpp"
+
+namespace icc {
+
+namespace {
+
+// Gauss-Legendre nodes/weights on [-1, 1] by Newton on P_n (ascending).
+void gauss(int n, double* x, double* w) {
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; ++k) {
+      double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    double d = n * (z * p1 - p0) / (z * z - 1.0);
+    if (d != 0.0) dp = d;
+    x[i] = -z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+// Quadratic B-spline values / derivatives on the open uniform knot vector of
+// `nel` elements at parametric xi inside element e (local functions 0..2).
+void quad_basis(const std::vector<double>& knots, int64_t e, double xi, double v[3], double dv[3]) {
+  // span s = e + 2 (knots has 3 repeated zeros)
+  const int64_t s = e + 2;
+  // degree 1 functions s-1 and s on the span [knots[s], knots[s+1])
+  double b1[2] = {0.0, 0.0};
+  const double d1 = knots[s + 1] - knots[s];
+  if (d1 > 0.0) {
+    b1[0] = (knots[s + 1] - xi) / d1 * 1.0;
+    b1[1] = (xi - knots[s]) / d1 * 1.0;
+  }
+  // degree 2: functions s-2, s-1, s
+  for (int r = 0; r < 3; ++r) {
+    int64_t i = s + r - 2;
+    double val = 0.0;
+    if (r > 0) {
+      double d = knots[i + 2] - knots[i];
+      if (d > 0.0) val += (xi - knots[i]) / d * b1[r - 1];
+    }
+    if (r < 2) {
+      double d = knots[i + 3] - knots[i + 1];
+      if (d > 0.0) val += (knots[i + 3] - xi) / d * b1[r];
+    }
+    v[r] = val;
+  }
+  for (int r = 0; r < 3; ++r) {
+    int64_t i = s + r - 2;
+    double val = 0.0;
+    if (r > 0) {
+      double d = knots[i + 2] - knots[i];
+      if (d > 0.0) val += b1[r - 1] / d;
+    }
+    if (r < 2) {
+      double d = knots[i + 3] - knots[i + 1];
+      if (d > 0.0) val -= b1[r] / d;
+    }
+    dv[r] = 2.0 * val;
+  }
+}
+
+std::vector<double> open_knots_p2(int64_t nel) {
+  std::vector<double> k(3, 0.0);
+  for (int64_t i = 1; i < nel; ++i) {
+    double b = (double)i / (double)nel;
+    k.push_back(b);
+  }
+  for (int i = 0; i < 3; ++i) k.push_back(1.0);
+  return k;
+}
+
+// 1D element matrices of one direction: m[k][a][b], k = 00, 10, 01, 11,
+// first index = test, "1" = physical derivative.
+void element_mats_1d(int64_t nel, double length, std::vector<std::array<double, 36>>& per_elem) {
+  std::vector<double> knots = open_knots_p2(nel);
+  double xg[3], wg[3];
+  gauss(3, xg, wg);
+  per_elem.assign(nel, {});
+  for (int64_t e = 0; e < nel; ++e) {
+    double a = knots[e + 2], b = knots[e + 3];
+    double h = b - a;
+    auto& m = per_elem[e];
+    m.fill(0.0);
+    for (int q = 0; q < 3; ++q) {
+      double xi = a + 0.5 * (xg[q] + 1.0) * h;
+      double w = 0.5 * wg[q] * h * length;
+      double v[3], dv[3];
+      quad_basis(knots, e, xi, v, dv);
+      for (int i = 0; i < 3; ++i) {
+        double ni = v[i], di = dv[i] / length;
+        for (int j = 0; j < 3; ++j) {
+          double nj = v[j], dj = dv[j] / length;
+          m[0 * 9 + i * 3 + j] += w * ni * nj;
+          m[1 * 9 + i * 3 + j] += w * di * nj;
+          m[2 * 9 + i * 3 + j] += w * ni * dj;
+          m[3 * 9 + i * 3 + j] += w * di * dj;
+        }
+      }
+    }
+  }
+}
+
+// Picks a tile shape for a constant-type sub-box of extents (ex, ey, ez):
+// as many elements as possible (<= TILE_MAX_EL), staged points within
+// TILE_MAX_PTS, ties broken by fewer staged points per element.
+void tile_shape(int64_t ex, int64_t ey, int64_t ez, int& tx, int& ty, int& tz) {
+  static const int cand[] = {1, 2, 4, 8, 16, 32};
+  double best_score = -1.0;
+  tx = ty = tz = 1;
+  for (int a : cand) {
+    if (a > 2 * ex && a != 1) continue;
+    for (int b : cand) {
+      if (b > 2 * ey && b != 1) continue;
+      for (int c : cand) {
+        if (c > 2 * ez && c != 1) continue;
+        int ax = (int)std::min<int64_t>(a, ex), by = (int)std::min<int64_t>(b, ey), cz = (int)std::min<int64_t>(c, ez);
+        int nelem = ax * by * cz;
+        int pts = (ax + 2) * (by + 2) * (cz + 2);
+        if (nelem > TILE_MAX_EL || pts > TILE_MAX_PTS) continue;
+        // number of tiles and total staged points for the whole sub-box
+        int64_t ntx = (ex + ax - 1) / ax, nty = (ey + by - 1) / by, ntz = (ez + cz - 1) / cz;
+        double ntiles = (double)(ntx * nty * ntz);
+        double fill = (double)(ex * ey * ez) / (ntiles * TILE_MAX_EL);
+        double ppe = (double)pts / nelem;
+        double score = fill * 100.0 - ppe;  // prefer full tiles, then few points per element
+        if (score > best_score) {
+          best_score = score;
+          tx = ax;
+          ty = by;
+          tz = cz;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k) {
+  int T0 = (int)s.mats1[0].size(), T1 = (int)s.mats1[1].size();
+  return s.type_of[s.tid1[0][i] + T0 * (s.tid1[1][j] + T1 * s.tid1[2][k])];
+}
+
+int build_setup(const int64_t nel[3], const double bounds[6], double young, double poisson, GridSetup& s) {
+  for (int d = 0; d < 3; ++d) {
+    if (nel[d] < 1) {
+      set_error("icc_grid_create: nel must be >= 1 in every direction");
+      return ICC_EINVAL;
+    }
+    if (!(bounds[2 * d + 1] > bounds[2 * d])) {
+      set_error("icc_grid_create: empty bounds");
+      return ICC_EINVAL;
+    }
+    s.nel[d] = nel[d];
+    s.ncp[d] = nel[d] + 2;
+    s.lo[d] = bounds[2 * d];
+    s.len[d] = bounds[2 * d + 1] - bounds[2 * d];
+  }
+  s.nel_tot = s.nel[0] * s.nel[1] * s.nel[2];
+  s.ndof_sc = s.ncp[0] * s.ncp[1] * s.ncp[2];
+  s.ndof = 3 * s.ndof_sc;
+  if (s.ndof >= (int64_t)1 << 31) {
+    set_error("icc_grid_create: grid too large for 32-bit DOF indices");
+    return ICC_EINVAL;
+  }
+  s.lambda = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson));
+  s.mu = young / (2.0 * (1.0 + poisson));
+
+  // 1D types per direction.  The reference groups by the rounded key
+  // round(v / scale * 1e12) (elasticity.rs:73); rounding noise of the knot
+  // spacing can straddle that key at some sizes (nel = 32 or 160 on a length-2
+  // side) and split one mathematical type in two.  Here an element joins the
+  // first type whose matrices agree to 1e-9 * scale, so such noise never splits
+  // a run; every element's matrix still equals the reference's to round-off.
+  for (int d = 0; d < 3; ++d) {
+    std::vector<std::array<double, 36>> per;
+    element_mats_1d(s.nel[d], s.len[d], per);
+    double scale = 0.0;
+    for (auto& m : per)
+      for (double v : m) scale = std::max(scale, std::fabs(v));
+    const double tol = 1e-9 * scale;
+    s.tid1[d].clear();
+    s.mats1[d].clear();
+    for (auto& m : per) {
+      int found = -1;
+      for (size_t t = 0; t < s.mats1[d].size() && found < 0; ++t) {
+        double diff = 0.0;
+        for (int q = 0; q < 36; ++q) diff = std::max(diff, std::fabs(m[q] - s.mats1[d][t][q]));
+        if (diff <= tol) found = (int)t;
+      }
+      if (found < 0) {
+        found = (int)s.mats1[d].size();
+        s.mats1[d].push_back(m);
+      }
+      s.tid1[d].push_back(found);
+    }
+  }
+  int T0 = (int)s.mats1[0].size(), T1 = (int)s.mats1[1].size(), T2 = (int)s.mats1[2].size();
+  s.type_of.assign((size_t)T0 * T1 * T2, -1);
+  s.type_list.clear();
+  for (int64_t k = 0; k < s.nel[2]; ++k)
+    for (int64_t j = 0; j < s.nel[1]; ++j)
+      for (int64_t i = 0; i < s.nel[0]; ++i) {
+        int32_t t0 = s.tid1[0][i], t1 = s.tid1[1][j], t2 = s.tid1[2][k];
+        int32_t& slot = s.type_of[t0 + T0 * (t1 + T1 * t2)];
+        if (slot < 0) {
+          slot = (int32_t)s.type_list.size();
+          s.type_list.push_back({t0, t1, t2});
+        }
+      }
+
+  // 81x81 matrices: G_ij = (x) over d of m_d^{k(d,i,j)}, direction 0 fastest
+  s.kmat.clear();
+  for (auto& key : s.type_list) {
+    const double* m[3] = {s.mats1[0][key[0]].data(), s.mats1[1][key[1]].data(), s.mats1[2][key[2]].data()};
+    auto G = [&](int i, int j, int a, int b) {
+      double v = 1.0;
+      int ad[3] = {a % 3, (a / 3) % 3, a / 9}, bd[3] = {b % 3, (b / 3) % 3, b / 9};
+      for (int d = 0; d < 3; ++d) {
+        int kk = (d == i && d == j) ? 3 : (d == i) ? 1 : (d == j) ? 2 : 0;
+        v = m[d][kk * 9 + ad[d] * 3 + bd[d]] * v;
+      }
+      return v;
+    };
+    std::vector<double> K(NSH * NSH);
+    for (int ci = 0; ci < 3; ++ci)
+      for (int cj = 0; cj < 3; ++cj)
+        for (int a = 0; a < NSC; ++a)
+          for (int b = 0; b < NSC; ++b) {
+            double v = s.lambda * G(ci, cj, a, b) + s.mu * G(cj, ci, a, b);
+            if (ci == cj) {
+              double lap = 0.0;
+              for (int kk = 0; kk < 3; ++kk) lap += G(kk, kk, a, b);
+              v += s.mu * lap;
+            }
+            K[(ci * NSC + a) * NSH + cj * NSC + b] = v;
+          }
+    s.kmat.push_back(std::move(K));
+  }
+
+  // Constant-type sub-boxes: runs of equal 1D type along each direction.
+  std::vector<std::pair<int64_t, int64_t>> runs[3];
+  for (int d = 0; d < 3; ++d) {
+    int64_t st = 0;
+    for (int64_t e = 1; e <= s.nel[d]; ++e)
+      if (e == s.nel[d] || s.tid1[d][e] != s.tid1[d][st]) {
+        runs[d].push_back({st, e});
+        st = e;
+      }
+  }
+  s.tiles.clear();
+  for (auto& rz : runs[2])
+    for (auto& ry : runs[1])
+      for (auto& rx : runs[0]) {
+        int64_t ex = rx.second - rx.first, ey = ry.second - ry.first, ez = rz.second - rz.first;
+        int tx, ty, tz;
+        tile_shape(ex, ey, ez, tx, ty, tz);
+        int32_t type = element_type(s, rx.first, ry.first, rz.first);
+        for (int64_t k0 = rz.first; k0 < rz.second; k0 += tz)
+          for (int64_t j0 = ry.first; j0 < ry.second; j0 += ty)
+            for (int64_t i0 = rx.first; i0 < rx.second; i0 += tx) {
+              Tile t;
+              t.i0 = (int32_t)i0;
+              t.j0 = (int32_t)j0;
+              t.k0 = (int32_t)k0;
+              t.nx = (int32_t)std::min<int64_t>(tx, rx.second - i0);
+              t.ny = (int32_t)std::min<int64_t>(ty, ry.second - j0);
+              t.nz = (int32_t)std::min<int64_t>(tz, rz.second - k0);
+              t.type = type;
+              t.pad = 0;
+              s.tiles.push_back(t);
+            }
+      }
+  return ICC_OK;
+}
+
+// Consistent traction on the face x = x_max: F(A, c) = t_c * int N_A dA, the
+// face integral taken exactly with 3-point Gauss per face element (the trace
+// of the p = 2 basis is biquadratic).
+void traction_xmax(const GridSetup& s, const double t[3], std::vector<double>& f) {
+  f.assign(s.ndof, 0.0);
+  std::vector<double> ky = open_knots_p2(s.nel[1]), kz = open_knots_p2(s.nel[2]);
+  double xg[3], wg[3];
+  gauss(3, xg, wg);
+  const int64_t i = s.ncp[0] - 1;
+  for (int64_t ez = 0; ez < s.nel[2]; ++ez)
+    for (int64_t ey = 0; ey < s.nel[1]; ++ey) {
+      const double ay = ky[ey + 2], hy = ky[ey + 3] - ay, az = kz[ez + 2], hz = kz[ez + 3] - az;
+      for (int qz = 0; qz < 3; ++qz)
+        for (int qy = 0; qy < 3; ++qy) {
+          const double yi = ay + 0.5 * (xg[qy] + 1.0) * hy, zi = az + 0.5 * (xg[qz] + 1.0) * hz;
+          const double w = (0.5 * wg[qy] * hy * s.len[1]) * (0.5 * wg[qz] * hz * s.len[2]);
+          double vy[3], dvy[3], vz[3], dvz[3];
+          quad_basis(ky, ey, yi, vy, dvy);
+          quad_basis(kz, ez, zi, vz, dvz);
+          for (int bz = 0; bz < 3; ++bz)
+            for (int by = 0; by < 3; ++by) {
+              const int64_t g = i + s.ncp[0] * ((ey + by) + s.ncp[1] * (ez + bz));
+              const double wn = w * vy[by] * vz[bz];
+              for (int c = 0; c < 3; ++c) f[c * s.ndof_sc + g] += wn * t[c];
+            }
+        }
+    }
+}
+
+}  // namespace icc

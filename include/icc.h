@@ -1,0 +1,204 @@
+/*
+ * icc.h: C ABI of libicc, the MI355X (gfx950) matrix-free elasticity hot path
+ * for quadratic (p = 2) trivariate B-spline background grids.
+ *
+ * Conventions
+ *  - Every call returns an int status (ICC_OK = 0).  icc_last_error() gives a
+ *    thread-local message for the last failure.
+ *  - Plain pointers and sizes only.  Functions without a _dev suffix take HOST
+ *    buffers owned by the caller (copied in and out).  _dev functions take
+ *    DEVICE pointers (hipMalloc'd or torch CUDA tensors) and are ordered on the
+ *    context's HIP stream (icc_ctx_stream); they do not synchronise.
+ *  - One context = one device + one HIP stream, driven from one host thread.
+ *  - Vectors are component-blocked like the reference: [u0(:); u1(:); u2(:)],
+ *    scalar control points lexicographic with direction 0 fastest.
+ *  - Reference citations are file:line into dshguilherme/immersed_catmull_clark.
+ */
+#ifndef ICC_H
+#define ICC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum icc_status {
+  ICC_OK = 0,
+  ICC_EINVAL = 1,      /* bad argument (sizes, null pointers, unsupported degree) */
+  ICC_ENOMEM = 2,      /* device or host allocation failed */
+  ICC_EHIP = 3,        /* HIP runtime error */
+  ICC_EBREAKDOWN = 4,  /* PCG breakdown: p.Ap <= 0 (solver.rs:47-49 stops silently; we report it) */
+  ICC_ENOCONV = 5,     /* iteration limit reached before the tolerance */
+  ICC_ENOTSUP = 6      /* valid request this build does not implement */
+};
+
+/* Arithmetic of the K.u contraction and of the inner solve. */
+enum icc_prec {
+  ICC_F64 = 0, /* FP64 MFMA (v_mfma_f64_16x16x4), FP64 vectors */
+  ICC_F32 = 1  /* FP32 MFMA (v_mfma_f32_16x16x4), FP32 vectors and accumulation */
+};
+
+typedef struct icc_ctx icc_ctx;
+typedef struct icc_grid icc_grid;
+typedef struct icc_op icc_op;
+typedef struct icc_filter icc_filter;
+
+const char* icc_last_error(void);
+const char* icc_version(void);
+
+/* ---- context --------------------------------------------------------------- */
+int icc_ctx_create(int device, icc_ctx** out);
+int icc_ctx_destroy(icc_ctx* ctx);
+int icc_ctx_sync(icc_ctx* ctx);
+/* the hipStream_t every _dev call is ordered on */
+void* icc_ctx_stream(icc_ctx* ctx);
+
+/* ---- grid: B-spline space + exact element matrices ------------------------ */
+/* Replaces SpaceBox::new (rust/src/iga/space.rs:107-160) for dim = 3,
+ * degree = 2, plus lame + elasticity_element_matrices
+ * (rust/src/iga/elasticity.rs:30-148,173-177).  bounds = [lo0,hi0,lo1,hi1,lo2,hi2]. */
+int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bounds[6], int64_t degree, double young,
+                    double poisson, icc_grid** out);
+int icc_grid_destroy(icc_grid* g);
+/* out[0..10] = nel, ndof_sc, ndof, ntypes, nsh, ncp0, ncp1, ncp2, nel0, nel1, nel2 */
+int icc_grid_sizes(const icc_grid* g, int64_t out[11]);
+/* connectivity[e*81 + c*27 + a] (space.rs:133-155), bit-exact */
+int icc_grid_connectivity(const icc_grid* g, int64_t* out);
+/* type_id[e] in first-appearance order (elasticity.rs:85-98) */
+int icc_grid_type_id(const icc_grid* g, int64_t* out);
+/* row-major 81x81 element matrix of a type (test x trial) */
+int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out);
+
+/* ---- operator: y = M * sum_e s_e C_e^T K_t(e) C_e * (M x) ------------------ */
+/* The matvec closure of topopt_iga.rs:151-159 / immersed.rs:558-566 over the
+ * assembled K of sparse.rs:160-175; semantic spec eval_matvec_3d
+ * (src/fastformation/topopt_iga_3d.m:252-260). */
+int icc_op_create(icc_grid* g, icc_op** out);
+int icc_op_destroy(icc_op* op);
+/* s[nel]: SIMP / volume-fraction stiffness scale (topopt_iga.rs:132, immersed.rs:484) */
+int icc_op_set_scale(icc_op* op, const double* s);
+int icc_op_set_scale_dev(icc_op* op, const double* s_dev);
+/* free[ndof]: 1 = free DOF, 0 = Dirichlet (NULL = all free) */
+int icc_op_set_free_mask(icc_op* op, const uint8_t* free_mask);
+/* adds a sparse term S (CSR, ndof x ndof, FP64) to the operator: stabilisation
+ * or penalty surface terms (topopt_immersed.rs:174-183, immersed.rs:542-547) */
+int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, const int64_t* indices, const double* data);
+
+/* y = A x.  Host pointers, FP64 in and out; prec selects the contraction. */
+int icc_apply(icc_op* op, int prec, const double* x, double* y);
+/* Device pointers: x,y are double* for ICC_F64 and float* for ICC_F32. */
+int icc_apply_dev(icc_op* op, int prec, const void* x_dev, void* y_dev);
+
+/* Jacobi diagonal d_A = sum_{e ni A} s_e K_t[a,a] (+ diag S), unclamped
+ * (sparse.rs:70-78 on the assembled K; topopt_iga_3d.m:119-125). */
+int icc_diag(icc_op* op, double* d);
+int icc_diag_dev(icc_op* op, double* d_dev);
+
+/* ---- solvers ------------------------------------------------------------------ */
+/* PcgSolver::solve (rust/src/solver.rs:14-76) with the operator as the
+ * closure: x0 = 0, Jacobi from diag (|d| > 1e-12 else 1), stop at
+ * |r|/max(|b|,1e-16) < tol or maxit.  FP64.  Returns ICC_OK on convergence,
+ * ICC_EBREAKDOWN / ICC_ENOCONV otherwise (x, iters, res filled in every case). */
+int icc_pcg(icc_op* op, const double* b, const double* diag, double tol, int64_t maxit, double* x, int64_t* iters,
+            double* res);
+int icc_pcg_dev(icc_op* op, const double* b_dev, const double* diag_dev, double tol, int64_t maxit, double* x_dev,
+                int64_t* iters, double* res);
+
+/* Mixed-precision iterative refinement: FP64 outer residual r = f - A u with
+ * compensated dot products, FP64 update u += d, inner Jacobi PCG in `inner`
+ * precision solving A d = r to relative `inner_tol`.  Stops when the TRUE
+ * relative residual |f - A u| / |f| <= tol.  stats (may be NULL), 8 doubles:
+ * outer steps, total inner iterations, final true residual, FP64 applies,
+ * inner applies, 0, 0, 0.  u_dev in/out (initial guess). */
+int icc_solve_ir_dev(icc_op* op, int inner, const double* f_dev, double tol, double inner_tol, int64_t max_outer,
+                     int64_t max_inner, double* u_dev, double* stats);
+int icc_solve_ir(icc_op* op, int inner, const double* f, double tol, double inner_tol, int64_t max_outer,
+                 int64_t max_inner, double* u, double* stats);
+
+/* ---- sensitivities --------------------------------------------------------- */
+/* ee_e = u_e^T K_t(e) u_e, unscaled (topopt_iga.rs:165-184) */
+int icc_energies(icc_op* op, const double* u, double* ee);
+int icc_energies_dev(icc_op* op, const double* u_dev, double* ee_dev);
+/* dc_e = -penal (1 - emin) x_e^(penal-1) ee_e [* w_e if w != NULL]
+ * (topopt_iga.rs:199, topopt_immersed.rs:212), fused with the energies. */
+int icc_sens_dev(icc_op* op, const double* u_dev, const double* x_dev, double penal, double emin, const double* w_dev,
+                 double* dc_dev);
+int icc_sens(icc_op* op, const double* u, const double* x, double penal, double emin, const double* w, double* dc);
+
+/* ---- filters -------------------------------------------------------------- */
+/* Cartesian filter on element centres, r = rmin * mean(h), weight max(r - d, 0)
+ * with d^2 <= r^2 (topopt_iga.rs:100-126).  active may be NULL (all active). */
+int icc_filter_create(icc_grid* g, double rmin, icc_filter** out);
+int icc_filter_destroy(icc_filter* f);
+/* sensitivity filter (topopt_iga.rs:200-205) */
+int icc_filter_sens_dev(icc_filter* f, const double* x_dev, const double* dc_dev, double* out_dev);
+/* density filter forward x~ = H x / Hs and backward g_x = H^T (g / Hs) */
+int icc_filter_fwd_dev(icc_filter* f, const double* x_dev, double* out_dev);
+int icc_filter_bwd_dev(icc_filter* f, const double* g_dev, double* out_dev);
+int icc_filter_sens(icc_filter* f, const double* x, const double* dc, double* out);
+int icc_filter_fwd(icc_filter* f, const double* x, double* out);
+int icc_filter_bwd(icc_filter* f, const double* g, double* out);
+
+/* ---- weighted-quadrature diagonal ------------------------------------------ */
+/* Diagonal of the WQ stiffness (rust/src/wq.rs:329-514, interior layout,
+ * element densities: SIMP factor emin + x^penal (1 - emin) sampled at the WQ
+ * points, wq.rs:336-358).  x = NULL: uniform density. */
+int icc_wq_diag(icc_grid* g, const double* x, double penal, double emin, double* d);
+int icc_wq_diag_dev(icc_grid* g, const double* x_dev, double penal, double emin, double* d_dev);
+
+/* ---- immersed classification --------------------------------------------- */
+/* classify_background_cells + element_weights (rust/src/immersed.rs:88-155)
+ * with the analytic box-minus-z-cylinders predicate: inside iff sdf(q) < 0.
+ * cyl = [cx, cy, r] * ncyl; grid_bounds[6]; status INSIDE 1 / OUTSIDE 0 / CUT -1. */
+int icc_classify_cylinders(icc_ctx* ctx, const double box_lo[3], const double box_hi[3], const double* cyl,
+                           int64_t ncyl, const double grid_bounds[6], const int64_t res[3], const int64_t sub[3],
+                           int8_t* status, double* weights);
+
+/* ---- structured SIMP cantilever driver ---------------------------------------- */
+/* Mirrors CantileverTopOpt (rust/src/topopt_iga.rs:19-235): clamp x = 0,
+ * SIMP scale emin + (1 - emin) x^penal, warm-started Jacobi PCG with the
+ * rescaled tolerance, element energies, filter, OC bisection (move 0.2),
+ * stop when change < 0.01 after 15 iterations. */
+typedef struct icc_topopt icc_topopt;
+typedef struct icc_topopt_config {
+  int64_t nel[3];
+  double size[3];
+  double volfrac, penal, rmin;
+  int64_t max_iter;
+  double young, poisson, emin, move_limit, pcg_tol;
+  int32_t load;   /* 0: reference nodal load (topopt_iga.rs:79-98); 1: unit downward traction on x = L */
+  int32_t filter; /* 0: sensitivity filter (topopt_iga.rs:200-205); 1: density filter with adjoint backward */
+  int32_t solver; /* 0: FP64 Jacobi PCG; 1: FP32-inner iterative refinement to the same true residual */
+  int32_t pad;
+} icc_topopt_config;
+/* CantileverTopOptConfig::default (topopt_iga.rs:35-51) */
+void icc_topopt_config_default(icc_topopt_config* cfg);
+int icc_topopt_create(icc_ctx* ctx, const icc_topopt_config* cfg, icc_topopt** out);
+int icc_topopt_destroy(icc_topopt* t);
+/* out = nel, ndof */
+int icc_topopt_sizes(const icc_topopt* t, int64_t out[2]);
+int icc_topopt_force(const icc_topopt* t, double* force, uint8_t* free_mask);
+/* solve_state (topopt_iga.rs:137-162): x = densities [nel], u0 warm start [ndof] */
+int icc_topopt_solve_state(icc_topopt* t, const double* x, const double* u0, double* u, int64_t* iters, double* res);
+/* element_energies (topopt_iga.rs:165-184) */
+int icc_topopt_element_energies(icc_topopt* t, const double* u, double* ee);
+/* compliance c = F.u of the design x and its gradient dc/dx through the
+ * filter of the config (density filter: H^T (dc/dx~ / Hs)); u_inout = warm start
+ * in, state out.  Tolerance: cfg.pcg_tol. */
+int icc_topopt_compliance(icc_topopt* t, const double* x, double* u_inout, double* c, double* dcdx);
+typedef void (*icc_report_fn)(int64_t iter, double compliance, double volume, double change, void* user);
+/* run (topopt_iga.rs:187-234); arrays sized max_iter (history) and nel (density) */
+int icc_topopt_run(icc_topopt* t, icc_report_fn report, void* user, double* compliance, double* change,
+                   int64_t* pcg_iterations, double* density, int64_t* n_iter);
+
+/* ---- diagnostics ------------------------------------------------------------- */
+/* runs one v_mfma_f64_16x16x4 with A[i][k] = i + 16k, B[k][j] = (k == j%4) and
+ * writes the 64x4 accumulator registers: used by tests to pin the lane layout */
+int icc_debug_mfma_layout(icc_ctx* ctx, double* out256);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICC_H */

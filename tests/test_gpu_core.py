@@ -1,0 +1,215 @@
+"""GPU parity of the HIP hot path against the CPU oracle, through the C ABI.
+
+Tolerances: FP64 paths 1e-13 relative (operator) / 1e-8 (solves, as the
+north-star bar); FP32 contraction 2e-6 relative on the operator.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from conftest import load_fixture
+
+pytestmark = pytest.mark.gpu
+
+icc = pytest.importorskip("immersed_catmull_clark_amd")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = icc.Context(0)
+    yield c
+    c.close()
+
+
+def _oracle_space(bounds, nel, young=1.0, poisson=0.3):
+    sp = O.Space(bounds, nel, 2)
+    lam, mu = O.lame(young, poisson)
+    types, tid = sp.elem_mats(lam, mu)
+    return sp, types, tid
+
+
+GRIDS = [
+    ([[0.5, 2.5], [-1, 0.2], [0.1, 0.4]], [4, 3, 3]),   # iga_kernel.json case 5
+    ([[0, 1.2], [0, 0.6], [0, 0.3]], [8, 4, 2]),        # topopt fixture grid
+    ([[0, 2.0], [0, 1.0], [0, 1.0]], [16, 8, 8]),       # config 0 grid
+    ([[0, 1.0], [0, 1.0], [0, 1.0]], [1, 1, 1]),
+    ([[0, 1.0], [0, 0.5], [0, 2.0]], [2, 1, 5]),
+    ([[0, 3.0], [0, 1.0], [0, 1.0]], [13, 7, 3]),
+    ([[0, 2.0], [0, 1.0], [0, 1.0]], [32, 3, 3]),      # the reference's key splits a type at nel = 32
+]
+
+
+def test_mfma_layout(ctx):
+    out = ctx.mfma_layout()
+    lane = np.arange(64)[:, None]
+    r = np.arange(4)[None, :]
+    expect = 4 * (lane // 16) + r + 16 * ((lane % 16) % 4)
+    assert np.array_equal(out, expect.astype(np.float64)), out
+
+
+@pytest.mark.parametrize("bounds,nel", GRIDS)
+def test_grid_matches_oracle(ctx, bounds, nel):
+    g = icc.Grid(ctx, nel, bounds)
+    sp, types, tid = _oracle_space(bounds, nel)
+    assert g.ndof == sp.ndof and g.nel_tot == sp.nel
+    assert np.array_equal(g.connectivity(), sp.connectivity())
+    gt = g.type_id()
+    if g.ntypes == sp.ntypes:
+        assert np.array_equal(gt, tid)
+    else:
+        # rounding noise split a type in the reference's key; the product merged it
+        assert g.ntypes < sp.ntypes
+    # every element's matrix agrees with the reference's matrix of that element
+    mats = [g.element_matrix(t) for t in range(g.ntypes)]
+    for e in range(g.nel_tot):
+        K, R = mats[gt[e]], types[tid[e]]
+        assert np.max(np.abs(K - R)) <= 1e-13 * np.abs(R).max()
+
+
+def test_apply_fixture_probes(ctx):
+    d = load_fixture("iga_kernel.json")
+    case = d["cases"][4]
+    assert case["degree"] == 2 and len(case["nsub"]) == 3
+    g = icc.Grid(ctx, case["nsub"], case["bounds"], young=1.0, poisson=0.3)
+    op = icc.Operator(g)
+    probes = np.asarray(case["probes"])
+    for j, v in enumerate(O.probe_vectors(g.ndof)):
+        y = op.apply(v)
+        assert O.rel_err(y, probes[:, j]) < 1e-13
+
+
+@pytest.mark.parametrize("bounds,nel", GRIDS)
+def test_apply_scaled_masked_matches_oracle(ctx, bounds, nel):
+    rng = np.random.default_rng(sum(nel))
+    g = icc.Grid(ctx, nel, bounds)
+    sp, _, _ = _oracle_space(bounds, nel)
+    s = rng.uniform(0.01, 1.0, g.nel_tot) ** 3
+    free = (rng.uniform(size=g.ndof) > 0.2).astype(np.uint8)
+    op = icc.Operator(g, scale=s, free=free)
+    sp.assemble(s)
+    x = rng.standard_normal(g.ndof)
+    ref = free * sp.matvec(free * x)
+    y = op.apply(x)
+    assert O.rel_err(y, ref) < 1e-13
+    y32 = op.apply(x, prec=icc.ICC_F32)
+    assert O.rel_err(y32, ref) < 2e-6
+    # Jacobi diagonal of the unmasked operator
+    assert O.rel_err(op.diag(), sp.diag()) < 1e-14
+
+
+def test_energies_and_sensitivities(ctx):
+    bounds, nel = [[0, 2.0], [0, 1.0], [0, 1.0]], [16, 8, 8]
+    g = icc.Grid(ctx, nel, bounds)
+    sp, _, _ = _oracle_space(bounds, nel)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(g.ndof)
+    x = rng.uniform(0.01, 1.0, g.nel_tot)
+    op = icc.Operator(g)
+    ee = op.energies(u)
+    ref = sp.energies(u)
+    assert O.rel_err(ee, ref) < 1e-13
+    dc = op.sens(u, x, 3.0, 1e-9)
+    dref = -3.0 * (1.0 - 1e-9) * x ** 2 * ref
+    assert O.rel_err(dc, dref) < 1e-13
+    w = rng.uniform(0, 1, g.nel_tot)
+    dcw = op.sens(u, x, 3.0, 1e-4, w=w)
+    assert O.rel_err(dcw, -w * (3.0 * (1 - 1e-4) * x ** 2 * ref)) < 1e-13
+
+
+@pytest.mark.parametrize("nel,rmin", [([8, 4, 2], 1.5), ([16, 8, 8], 1.5), ([13, 7, 3], 2.3)])
+def test_filters_match_oracle(ctx, nel, rmin):
+    bounds = [[0, 1.2], [0, 0.6], [0, 0.3]]
+    g = icc.Grid(ctx, nel, bounds)
+    sp = O.Space(bounds, nel, 2)
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0.001, 1.0, g.nel_tot)
+    dc = -rng.uniform(0.0, 5.0, g.nel_tot)
+    f = icc.Filter(g, rmin)
+    assert O.rel_err(f.sens(x, dc), sp.filter_apply(rmin, 0, x, dc)) < 1e-15
+    assert O.rel_err(f.fwd(x), sp.filter_apply(rmin, 1, None, x)) < 1e-15
+    assert O.rel_err(f.bwd(dc), sp.filter_apply(rmin, 2, None, dc)) < 1e-15
+
+
+def _cantilever_problem(nel, seed, size=(2.0, 1.0, 1.0), rho=(0.01, 1.0), emin=1e-9):
+    """Clamp x = 0, consistent downward unit traction on x = L; SIMP scale."""
+    bounds = [[0, size[0]], [0, size[1]], [0, size[2]]]
+    sp, _, _ = _oracle_space(bounds, nel)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(rho[0], rho[1], sp.nel)
+    s = emin + (1 - emin) * x ** 3
+    ncp = sp.ndof_dir
+    i = np.arange(sp.ndof_sc) % ncp[0]
+    free = np.ones(sp.ndof, dtype=np.uint8)
+    for c in range(3):
+        free[c * sp.ndof_sc:(c + 1) * sp.ndof_sc][i == 0] = 0
+    f = sp.traction_xmax([0.0, -1.0, 0.0]) * free
+    return bounds, sp, s, free, f
+
+
+def test_pcg_matches_oracle(ctx):
+    nel = [8, 4, 4]
+    bounds, sp, s, free, f = _cantilever_problem(nel, 11, rho=(0.3, 1.0), emin=1e-3)
+    sp.assemble(s)
+    diag = np.maximum(sp.diag(), 1e-12)
+    x_ref, it_ref, res_ref = sp.pcg(f, diag, free, 1e-10, 20 * sp.ndof)
+    g = icc.Grid(ctx, nel, bounds)
+    op = icc.Operator(g, scale=s, free=free)
+    d = op.diag()
+    assert O.rel_err(d, sp.diag()) < 1e-14
+    x, it, res, rc = op.pcg(f, np.maximum(d, 1e-12), 1e-10, 20 * g.ndof)
+    assert rc == icc.ICC_OK
+    assert res < 1e-10
+    assert abs(it - it_ref) <= max(2, it_ref // 50), (it, it_ref)
+    assert O.rel_err(x, x_ref) < 1e-8
+
+
+def test_pcg_reports_iteration_limit(ctx):
+    nel = [8, 4, 4]
+    bounds, sp, s, free, f = _cantilever_problem(nel, 12)
+    g = icc.Grid(ctx, nel, bounds)
+    op = icc.Operator(g, scale=s, free=free)
+    x, it, res, rc = op.pcg(f, np.maximum(op.diag(), 1e-12), 1e-14, 5)
+    assert rc == icc.ICC_ENOCONV and it == 5 and res > 1e-14
+
+
+@pytest.mark.parametrize("inner", ["f64", "f32"])
+def test_iterative_refinement_true_residual(ctx, inner):
+    nel = [16, 8, 8]
+    bounds, sp, s, free, f = _cantilever_problem(nel, 13)
+    sp.assemble(s)
+    g = icc.Grid(ctx, nel, bounds)
+    op = icc.Operator(g, scale=s, free=free)
+    prec = icc.ICC_F64 if inner == "f64" else icc.ICC_F32
+    u, st, rc = op.solve_ir(f, tol=1e-8, inner=prec, inner_tol=1e-3, max_outer=60)
+    assert rc == icc.ICC_OK, st
+    # the true residual, evaluated by the oracle's assembled FP64 operator
+    r = free * (f - sp.matvec(free * u))
+    assert np.linalg.norm(r) / np.linalg.norm(free * f) <= 1.01e-8
+    assert st["true_res"] <= 1e-8
+
+
+def test_apply_properties_config1_size(ctx):
+    """Size-independent properties at the 96x48x48 grid: linearity, symmetry,
+    rigid-body null space, FP32 vs FP64."""
+    nel = [96, 48, 48]
+    bounds = [[0, 2.0], [0, 1.0], [0, 1.0]]
+    g = icc.Grid(ctx, nel, bounds)
+    assert g.ndof == 735000 and g.ntypes == 27
+    rng = np.random.default_rng(3)
+    s = 1e-9 + (1 - 1e-9) * rng.uniform(0.01, 1.0, g.nel_tot) ** 3
+    op = icc.Operator(g, scale=s)
+    x = rng.standard_normal(g.ndof)
+    y = rng.standard_normal(g.ndof)
+    ax, ay = op.apply(x), op.apply(y)
+    axy = op.apply(2.0 * x - 0.5 * y)
+    assert O.rel_err(axy, 2.0 * ax - 0.5 * ay) < 1e-12
+    assert abs(x @ ay - y @ ax) < 1e-11 * abs(x @ ay)
+    ax32 = op.apply(x, prec=icc.ICC_F32)
+    assert O.rel_err(ax32, ax) < 2e-6
+    # rigid translation in y: K t = 0
+    t = np.zeros(g.ndof)
+    t[g.ndof_sc:2 * g.ndof_sc] = 1.0
+    assert np.linalg.norm(op.apply(t)) < 1e-12 * np.linalg.norm(ax) * np.linalg.norm(t) / np.linalg.norm(x)

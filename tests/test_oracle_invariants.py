@@ -1,0 +1,98 @@
+"""Invariants that pin the oracle beyond the fixtures (the mathematics fixes
+them): rigid-body null space and symmetry of K (iga_kernel.rs:50-92), the
+structured filter rows equal the O(nel^2) scan of topopt_iga.rs:109-125, the
+consistent traction integrates to t * area, the density filter is adjoint to
+its backward, the WQ operator equals Gauss at uniform density
+(wq_vs_matlab.rs:35-51)."""
+import numpy as np
+
+import oracle_py as O
+
+
+def test_rigid_body_modes_and_symmetry():
+    sp = O.Space([[0, 1.2], [0, 0.6], [0, 0.3]], [3, 2, 2], 2)
+    lam, mu = O.lame(1.0, 0.3)
+    types, _ = sp.elem_mats(lam, mu)
+    for K in types:
+        assert np.max(np.abs(K - K.T)) < 1e-13 * np.abs(K).max()
+    sp.assemble(None)
+    g = sp.greville()
+    fro = sp.frobenius()
+    modes = []
+    for c in range(3):
+        m = np.zeros(sp.ndof)
+        m[c * sp.ndof_sc:(c + 1) * sp.ndof_sc] = 1.0
+        modes.append(m)
+    for a, b in [(0, 1), (1, 2), (0, 2)]:
+        m = np.zeros(sp.ndof)
+        m[a * sp.ndof_sc:(a + 1) * sp.ndof_sc] = -g[:, b]
+        m[b * sp.ndof_sc:(b + 1) * sp.ndof_sc] = g[:, a]
+        modes.append(m)
+    for m in modes:
+        assert np.linalg.norm(sp.matvec(m)) / (fro * np.linalg.norm(m)) < 1e-13
+
+
+def test_assembly_paths_agree():
+    sp = O.Space([[0, 1.2], [0, 0.6], [0, 0.3]], [4, 3, 3], 2)
+    lam, mu = O.lame(1.0, 0.3)
+    sp.elem_mats(lam, mu)
+    rng = np.random.default_rng(0)
+    s = rng.uniform(0.01, 1.0, sp.nel)
+    v = rng.standard_normal(sp.ndof)
+    sp.assemble(s)
+    y1 = sp.matvec(v)
+    sp.assemble(s, pattern=True)
+    y2 = sp.matvec(v)
+    assert O.rel_err(y1, y2) < 1e-14
+
+
+def test_structured_filter_equals_bruteforce():
+    for nel, rmin in [([8, 4, 2], 1.5), ([7, 5, 3], 2.3), ([6, 6, 6], 1.0)]:
+        sp = O.Space([[0, 1.2], [0, 0.6], [0, 0.3]], nel, 2)
+        assert sp.filter_rows_match(rmin)
+
+
+def test_density_filter_adjoint():
+    sp = O.Space([[0, 2.0], [0, 1.0], [0, 1.0]], [8, 4, 4], 2)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(size=sp.nel)
+    g = rng.standard_normal(sp.nel)
+    fx = sp.filter_apply(1.5, 1, None, x)
+    bg = sp.filter_apply(1.5, 2, None, g)
+    assert abs(fx @ g - x @ bg) < 1e-12 * abs(fx @ g)
+
+
+def test_traction_total_force():
+    sp = O.Space([[0, 2.0], [0, 1.0], [0, 1.0]], [6, 3, 4], 2)
+    f = sp.traction_xmax([0.0, -1.0, 0.0])
+    nsc = sp.ndof_sc
+    assert abs(f[nsc:2 * nsc].sum() - (-1.0 * 1.0 * 1.0)) < 1e-13
+    assert abs(f[:nsc].sum()) < 1e-15 and abs(f[2 * nsc:].sum()) < 1e-15
+
+
+def test_wq_equals_gauss_uniform_density():
+    sp = O.Space([[0, 1.2], [0, 0.6], [0, 0.3]], [4, 3, 2], 2)
+    lam, mu = O.lame(1.0, 0.3)
+    sp.elem_mats(lam, mu)
+    v = O.probe_vectors(sp.ndof, 1)[0]
+    sp.assemble(None)
+    yg = sp.matvec(v)
+    dg = sp.diag()
+    for layout in (0, 1):
+        sp.assemble_wq(1.0, 0.3, 0, None, 3.0, 1e-3, layout=layout)
+        assert O.rel_err(sp.matvec(v), yg) < 1e-12
+        assert O.rel_err(sp.diag(), dg) < 1e-12
+
+
+def test_sensitivity_identity():
+    """dc = -p x^(p-1)(1-emin) u_e^T K_e u_e (testIgaKernel.m:130-142) equals the
+    quadrature form fast_sensitivities (topopt2d.rs:13-48)."""
+    sp = O.Space([[0, 1.2], [0, 0.6], [0, 0.3]], [4, 3, 2], 2)
+    lam, mu = O.lame(1.0, 0.3)
+    sp.elem_mats(lam, mu)
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(sp.ndof)
+    x = rng.uniform(0.1, 1.0, sp.nel)
+    ee = sp.energies(u)
+    dc = -3.0 * x ** 2 * (1 - 1e-3) * ee
+    assert O.rel_err(sp.fast_sens(u, x, False, 3.0, 1e-3, 1.0, 0.3), dc) < 1e-11
