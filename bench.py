@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the matrix-free p = 2 elasticity hot path on one MI355X.
+
+Workload: the structured SIMP cantilever on a 96x48x48 p = 2 B-spline grid
+(221,184 elements, 98x50x50x3 = 735,000 DOFs) over the box [0,2]x[0,1]x[0,1],
+E0 = 1, nu = 0.3, densities i.i.d. U[0.01, 1] (seeded), penalty 3,
+Emin = 1e-9, x = 0 clamped, unit downward traction on x = 2.
+
+A step is one masked K.u apply (FP64 MFMA contraction) over the whole grid
+with x and y resident in HBM; value = applies per second.  The same JSON
+line carries the FP32 apply rate, the PCG time-to-solution (true relative
+residual 1e-8, FP64 operator) of plain FP64 PCG and of FP32-inner
+iterative refinement with the errors of the latter against the former, the
+roofline of the FP64 apply kernel, and the CPU baseline (the oracle's
+restatement of the reference's rayon CSR SpMV, on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Ku applies/s + PCG time-to-1e-8 at 735k DOF: mixed-prec vs FP64, 1 GPU"
+NEL = (96, 48, 48)
+BOUNDS = ((0.0, 2.0), (0.0, 1.0), (0.0, 1.0))
+SEED = 20261203
+PEAK_F64 = 78.6e12   # MI355X dense FP64 MFMA, FLOP/s
+PEAK_F32 = 157.3e12  # MI355X dense FP32, FLOP/s
+PEAK_HBM = 8.0e12    # B/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def problem(np):
+    rng = np.random.default_rng(SEED)
+    ncp = [n + 2 for n in NEL]
+    nel = NEL[0] * NEL[1] * NEL[2]
+    rho = rng.uniform(0.01, 1.0, nel)
+    scale = 1e-9 + (1.0 - 1e-9) * rho ** 3
+    nsc = ncp[0] * ncp[1] * ncp[2]
+    i = np.arange(nsc) % ncp[0]
+    free = np.ones(3 * nsc, dtype=np.uint8)
+    for c in range(3):
+        free[c * nsc:(c + 1) * nsc][i == 0] = 0
+    x = rng.standard_normal(3 * nsc)
+    x /= np.linalg.norm(x)
+    return rho, scale, free, x
+
+
+def traffic_from_profile():
+    """HBM bytes per FP64 apply launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)."""
+    path = os.path.join(ROOT, "profiles", "apply_f64_96x48x48_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def time_applies(torch, icc, ctx, op, x, y, prec, steps, warmup, accumulate=False):
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    fn = op.apply_accumulate if accumulate else (lambda xx, yy, prec: op.apply(xx, prec=prec, out=yy))
+    for _ in range(warmup):
+        fn(x, y, prec=prec)
+    ctx.sync()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn(x, y, prec=prec)
+    e1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps  # ms
+
+
+def cpu_baseline(np, scale, x, budget_s):
+    """The oracle's restatement of the reference CPU K.u (row-parallel CSR
+    SpMV, sparse.rs:60-68, OpenMP for rayon) at the full 735k-DOF size."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    sp = O.Space([list(b) for b in BOUNDS], list(NEL), 2)
+    lam, mu = O.lame(1.0, 0.3)
+    sp.elem_mats(lam, mu)
+    t0 = time.time()
+    nnz = sp.assemble_structured(scale)
+    t_asm = time.time() - t0
+    y = sp.matvec(x)  # warm-up
+    reps = 0
+    t0 = time.time()
+    while True:
+        y = sp.matvec(x)
+        reps += 1
+        if time.time() - t0 > budget_s:
+            break
+    dt = (time.time() - t0) / reps
+    cores = int(O.lib().orc_num_threads())
+    return dict(value=1.0 / dt, unit="applies/s", cores=cores, kind="port",
+                sample=f"{reps} CSR SpMVs (nnz {nnz}) of the 96x48x48 operator, row-parallel over {cores} "
+                       f"OpenMP threads; CSR built in {t_asm:.1f}s (not timed)"), y
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-pcg", action="store_true", help="skip the PCG time-to-solution legs")
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU SpMV timing")
+    ap.add_argument("--apply-only", action="store_true", help="time only the FP64 apply (for profilers)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import immersed_catmull_clark_amd as icc
+
+    log("bench: setting up the 96x48x48 grid")
+    ctx = icc.Context(0)
+    grid = icc.Grid(ctx, NEL, BOUNDS)
+    rho, scale, free, xh = problem(np)
+    op = icc.Operator(grid, scale=scale, free=free)
+    nel = grid.nel_tot
+    ndof = grid.ndof
+    x = torch.from_numpy(xh).cuda()
+    y = torch.zeros(ndof, dtype=torch.float64, device="cuda")
+    x32 = x.float()
+    y32 = torch.zeros(ndof, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    flops = 2.0 * 81 * 81 * nel
+    log("bench: FP64 applies")
+    t0 = time.time()
+    ms_f64 = time_applies(torch, icc, ctx, op, x, y, icc.ICC_F64, args.steps, args.warmup)
+    wall = time.time() - t0
+    ms_kernel = time_applies(torch, icc, ctx, op, x, y, icc.ICC_F64, args.steps, args.warmup, accumulate=True)
+    out = {
+        "metric": METRIC, "value": 1e3 / ms_f64, "unit": "applies/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_f64, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded densities U[0.01,1], seeded unit-norm x)",
+        "config": {"workload": "SIMP cantilever K.u apply, 96x48x48 p=2 (735,000 DOF, 221,184 elements)",
+                   "grid": list(NEL), "ndof": ndof, "emin": 1e-9, "penal": 3, "parallelism": "single GPU"},
+    }
+    alg_bytes = 8 * (2 * ndof + nel) + ndof / 8
+    ach = flops / (ms_kernel * 1e-3)
+    out["roofline"] = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_F64 / 1e12, "unit": "TFLOP/s",
+                       "frac": ach / PEAK_F64, "traffic": traffic_from_profile(),
+                       "kernel": "apply_tiles<double>", "kernel_ms": ms_kernel,
+                       "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": alg_bytes}
+    if args.apply_only:
+        print(json.dumps(out), flush=True)
+        return
+
+    log("bench: FP32 applies")
+    ms_f32 = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F32, args.steps, args.warmup)
+    ms_k32 = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F32, args.steps, args.warmup, accumulate=True)
+    yh64 = op.apply(xh)
+    yh32 = op.apply(xh, prec=icc.ICC_F32)
+    out["apply_f32"] = {"applies_per_s": 1e3 / ms_f32, "ms": ms_f32, "kernel_ms": ms_k32,
+                        "tflops": flops / (ms_k32 * 1e-3) / 1e12, "frac_of_fp32_peak": flops / (ms_k32 * 1e-3) / PEAK_F32,
+                        "rel_err_vs_f64": float(np.linalg.norm(yh32 - yh64) / np.linalg.norm(yh64))}
+
+    if not args.no_pcg:
+        # traction load on x = 2 (consistent: F = t int N dA), via the driver's builder
+        cfg = icc.CantileverTopOptConfig(nel=NEL, size=(2.0, 1.0, 1.0), load=1, emin=1e-9)
+        t = icc.CantileverTopOpt(ctx, cfg)
+        f = t.force * free
+        t.close()
+        fd = torch.from_numpy(f).cuda()
+        res = {}
+        for name, inner, itol in (("fp64", icc.ICC_F64, 1e-8), ("mixed_fp32", icc.ICC_F32, args.inner_tol)):
+            log(f"bench: PCG {name}")
+            u = torch.zeros(ndof, dtype=torch.float64, device="cuda")
+            stats = np.zeros(8)
+            torch.cuda.synchronize()
+            ctx.sync()
+            t0 = time.time()
+            rc = icc.lib().icc_solve_ir_dev(op.h, inner, fd.data_ptr(), 1e-8, itol, 200, 20 * ndof, u.data_ptr(),
+                                            stats.ctypes.data)
+            ctx.sync()
+            dt = time.time() - t0
+            res[name] = dict(time_s=dt, status=int(rc), outer=int(stats[0]), pcg_iterations=int(stats[1]),
+                             true_rel_residual=float(stats[2]), u=u.cpu().numpy())
+        uf, um = res["fp64"]["u"], res["mixed_fp32"]["u"]
+        cf, cm = float(f @ uf), float(f @ um)
+        dcf = op.sens(uf, rho, 3.0, 1e-9)
+        dcm = op.sens(um, rho, 3.0, 1e-9)
+        for k in res:
+            res[k].pop("u")
+        res["mixed_fp32"].update(
+            u_rel_err_vs_fp64=float(np.linalg.norm(um - uf) / np.linalg.norm(uf)),
+            compliance_rel_err_vs_fp64=abs(cm - cf) / abs(cf),
+            dc_rel_err_vs_fp64=float(np.linalg.norm(dcm - dcf) / np.linalg.norm(dcf)))
+        res["speedup_mixed_vs_fp64"] = res["fp64"]["time_s"] / res["mixed_fp32"]["time_s"]
+        res["inner_tol"] = args.inner_tol
+        out["pcg_time_to_1e-8"] = res
+
+    if not args.no_cpu:
+        log("bench: CPU baseline")
+        cb, ycpu = cpu_baseline(np, scale, (free * xh), args.cpu_budget)
+        yg = op.apply(xh)
+        cb["rel_diff_vs_gpu_f64"] = float(np.linalg.norm(free * ycpu - yg) / np.linalg.norm(yg))
+        out["cpu_baseline"] = cb
+    out["wall_s_f64_apply_loop"] = wall
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

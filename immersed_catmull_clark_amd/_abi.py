@@ -41,6 +41,7 @@ SIGNATURES = [
     ("icc_op_set_sparse", cint, [vp, i64, vp, vp, vp]),
     ("icc_apply", cint, [vp, cint, vp, vp]),
     ("icc_apply_dev", cint, [vp, cint, vp, vp]),
+    ("icc_apply_accumulate_dev", cint, [vp, cint, vp, vp]),
     ("icc_diag", cint, [vp, vp]),
     ("icc_diag_dev", cint, [vp, vp]),
     ("icc_pcg", cint, [vp, vp, vp, dbl, i64, vp, vp, vp]),

@@ -154,6 +154,11 @@ class Operator:
         check(lib().icc_apply(self.h, prec, _p(_f64(x)), _p(y)))
         return y
 
+    def apply_accumulate(self, x, out, prec=ICC_F64):
+        """out += A x on device tensors (tile kernel only, no sparse term)."""
+        check(lib().icc_apply_accumulate_dev(self.h, prec, _p(x), _p(out)))
+        return out
+
     def diag(self, out=None):
         if out is not None and _is_dev(out):
             check(lib().icc_diag_dev(self.h, _p(out)))

@@ -189,6 +189,7 @@ ICC_API int icc_grid_destroy(icc_grid* g) {
   hipFree(g->d_typeof);
   hipFree(g->d_tiles);
   hipFree(g->d_mats1);
+  hipFree(g->d_wq);
   delete g;
   return ICC_OK;
 }
@@ -483,6 +484,25 @@ ICC_API int icc_apply_dev(icc_op* op, int prec, const void* x, void* y) {
     return ICC_EINVAL;
   }
   if (e != hipSuccess) return hip_fail(e, "icc_apply_dev");
+  return ICC_OK;
+}
+
+ICC_API int icc_apply_accumulate_dev(icc_op* op, int prec, const void* x, void* y) {
+  if (!op || !x || !y) return ICC_EINVAL;
+  if (op->s_nnz > 0) {
+    set_error("icc_apply_accumulate_dev: operators with a sparse term are not supported");
+    return ICC_ENOTSUP;
+  }
+  hipError_t e;
+  if (prec == ICC_F64)
+    e = launch_apply_f64(op->g, op, (const double*)x, (double*)y, nullptr, nullptr, op->g->ctx->stream);
+  else if (prec == ICC_F32)
+    e = launch_apply_f32(op->g, op, (const float*)x, (float*)y, nullptr, nullptr, op->g->ctx->stream);
+  else {
+    set_error("icc_apply_accumulate_dev: unknown precision");
+    return ICC_EINVAL;
+  }
+  if (e != hipSuccess) return hip_fail(e, "icc_apply_accumulate_dev");
   return ICC_OK;
 }
 
@@ -854,34 +874,69 @@ ICC_API int icc_filter_bwd(icc_filter* f, const double* gr, double* out) {
 }
 
 // ---------------------------------------------------------------------------
-ICC_API int icc_wq_diag(icc_grid* g, const double* x, double penal, double emin, double* d) {
-  (void)g;
-  (void)x;
-  (void)penal;
-  (void)emin;
-  (void)d;
-  set_error("icc_wq_diag: not implemented yet");
-  return ICC_ENOTSUP;
-}
 ICC_API int icc_wq_diag_dev(icc_grid* g, const double* x, double penal, double emin, double* d) {
-  return icc_wq_diag(g, x, penal, emin, d);
+  if (!g || !d) return ICC_EINVAL;
+  hipStream_t st = g->ctx->stream;
+  if (!g->d_wq) {
+    std::vector<double> tab;
+    for (int dd = 0; dd < 3; ++dd) {
+      std::vector<double> a0, a1;
+      wq_tables_1d(g->s.nel[dd], a0, a1);
+      tab.insert(tab.end(), a0.begin(), a0.end());
+      tab.insert(tab.end(), a1.begin(), a1.end());
+    }
+    ICC_TRY(dalloc(&g->d_wq, tab.size()));
+    ICC_HIP(hipMemcpyAsync(g->d_wq, tab.data(), tab.size() * 8, hipMemcpyHostToDevice, st));
+    ICC_HIP(hipStreamSynchronize(st));
+  }
+  ICC_HIP(launch_wq_diag(g, g->d_wq, x, penal, emin, d, st));
+  return ICC_OK;
+}
+
+ICC_API int icc_wq_diag(icc_grid* g, const double* x, double penal, double emin, double* d) {
+  if (!g || !d) return ICC_EINVAL;
+  const auto& s = g->s;
+  return host_call(g->ctx, {{x, s.nel_tot}}, s.ndof, d,
+                   [&](std::vector<double*>& in, double* out) { return icc_wq_diag_dev(g, in[0], penal, emin, out); });
 }
 
 ICC_API int icc_classify_cylinders(icc_ctx* ctx, const double box_lo[3], const double box_hi[3], const double* cyl,
                                    int64_t ncyl, const double grid_bounds[6], const int64_t res[3],
                                    const int64_t sub[3], int8_t* status, double* weights) {
-  (void)ctx;
-  (void)box_lo;
-  (void)box_hi;
-  (void)cyl;
-  (void)ncyl;
-  (void)grid_bounds;
-  (void)res;
-  (void)sub;
-  (void)status;
-  (void)weights;
-  set_error("icc_classify_cylinders: not implemented yet");
-  return ICC_ENOTSUP;
+  if (!ctx || !box_lo || !box_hi || (ncyl > 0 && !cyl) || !grid_bounds || !res || !sub || !status || !weights)
+    return ICC_EINVAL;
+  for (int d = 0; d < 3; ++d)
+    if (res[d] < 1 || sub[d] < 1 || !(grid_bounds[2 * d + 1] > grid_bounds[2 * d])) {
+      set_error("icc_classify_cylinders: bad resolution or bounds");
+      return ICC_EINVAL;
+    }
+  hipStream_t st = ctx->stream;
+  const int64_t nc = res[0] * res[1] * res[2], nv = (res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+  double *d_cyl = nullptr, *d_w = nullptr;
+  uint8_t* d_in = nullptr;
+  int8_t* d_st = nullptr;
+  int rc = ICC_OK;
+  if ((rc = dalloc(&d_cyl, 3 * std::max<int64_t>(ncyl, 1))) || (rc = dalloc(&d_w, nc)) || (rc = dalloc(&d_in, nv)) ||
+      (rc = dalloc(&d_st, nc))) {
+    hipFree(d_cyl);
+    hipFree(d_w);
+    hipFree(d_in);
+    hipFree(d_st);
+    return rc;
+  }
+  hipError_t e = hipSuccess;
+  if (ncyl > 0) e = hipMemcpyAsync(d_cyl, cyl, 3 * ncyl * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = launch_classify_cylinders(box_lo, box_hi, d_cyl, ncyl, grid_bounds, res, sub, d_in, d_st, d_w, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(status, d_st, nc, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(weights, d_w, nc * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d_cyl);
+  hipFree(d_w);
+  hipFree(d_in);
+  hipFree(d_st);
+  if (e != hipSuccess) return hip_fail(e, "icc_classify_cylinders");
+  return ICC_OK;
 }
 
 ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out256) {

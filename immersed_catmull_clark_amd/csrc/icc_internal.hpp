@@ -73,6 +73,7 @@ struct icc_grid {
   icc::Tile* d_tiles = nullptr;
   int ntiles = 0;
   double* d_mats1 = nullptr;  // [3][T][36]
+  double* d_wq = nullptr;     // WQ tables [A0x, A1x, A0y, A1y, A0z, A1z], built on first use
   int maxnel = 0;
   int T[3] = {0, 0, 0};
 };

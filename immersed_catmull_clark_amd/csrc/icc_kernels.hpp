@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "icc_internal.hpp"
 
@@ -64,5 +65,15 @@ hipError_t launch_masked_add(int64_t n, const double* u0, const double* du, cons
 hipError_t launch_filter(const icc_filter* f, int mode, const double* x, const double* v, double* out,
                          hipStream_t st);
 hipError_t launch_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
+hipError_t launch_wq_diag(const icc_grid* g, const double* tab, const double* x, double penal, double emin,
+                          double* d, hipStream_t st);
+
+// k_immersed.hip
+hipError_t launch_classify_cylinders(const double box_lo[3], const double box_hi[3], const double* cyl_dev,
+                                     int64_t ncyl, const double gb[6], const int64_t res[3], const int64_t sub[3],
+                                     uint8_t* inside_dev, int8_t* status_dev, double* w_dev, hipStream_t st);
+
+// wq.cpp
+void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1);
 
 }  // namespace icc

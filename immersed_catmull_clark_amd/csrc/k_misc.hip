@@ -67,6 +67,48 @@ __global__ __launch_bounds__(256) void filter_kernel(int n0, int n1, int n2, dou
   }
 }
 
+// WQ diagonal (see wq.cpp): gather over the <= 27 elements of each control
+// point with the per-direction tables A0 = sum W00 B, A1 = sum W11 B'.
+// tab layout: [A0x, A1x, A0y, A1y, A0z, A1z], each [ncp_d * 3].
+__global__ __launch_bounds__(256) void wq_diag_kernel(const double* __restrict__ tab, int ncp0, int ncp1, int ncp2,
+                                                      int nel0, int nel1, int nel2, const double* __restrict__ xdens,
+                                                      double penal, double emin, double cf0, double cf1, double cf2,
+                                                      double lam, double mu, double* __restrict__ d) {
+  const int nsc = ncp0 * ncp1 * ncp2;
+  const double* A0x = tab;
+  const double* A1x = A0x + 3 * ncp0;
+  const double* A0y = A1x + 3 * ncp0;
+  const double* A1y = A0y + 3 * ncp1;
+  const double* A0z = A1y + 3 * ncp1;
+  const double* A1z = A0z + 3 * ncp2;
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < nsc; P += gridDim.x * blockDim.x) {
+    const int pi = P % ncp0, pj = (P / ncp0) % ncp1, pk = P / (ncp0 * ncp1);
+    const int fi = pi - 2 < 0 ? 0 : pi - 2, fj = pj - 2 < 0 ? 0 : pj - 2, fk = pk - 2 < 0 ? 0 : pk - 2;
+    const int li = pi < nel0 - 1 ? pi : nel0 - 1, lj = pj < nel1 - 1 ? pj : nel1 - 1, lk = pk < nel2 - 1 ? pk : nel2 - 1;
+    double T0 = 0.0, T1 = 0.0, T2 = 0.0;
+    for (int ek = fk; ek <= lk; ++ek) {
+      const double a0z = A0z[pk * 3 + ek - fk], a1z = A1z[pk * 3 + ek - fk];
+      for (int ej = fj; ej <= lj; ++ej) {
+        const double a0y = A0y[pj * 3 + ej - fj], a1y = A1y[pj * 3 + ej - fj];
+        for (int ei = fi; ei <= li; ++ei) {
+          const double a0x = A0x[pi * 3 + ei - fi], a1x = A1x[pi * 3 + ei - fi];
+          double s = 1.0;
+          if (xdens != nullptr) s = emin + pow(xdens[ei + nel0 * (ej + nel1 * ek)], penal) * (1.0 - emin);
+          T0 += s * (a1x * a0y * a0z);
+          T1 += s * (a0x * a1y * a0z);
+          T2 += s * (a0x * a0y * a1z);
+        }
+      }
+    }
+    const double T[3] = {T0, T1, T2}, cf[3] = {cf0, cf1, cf2};
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+      for (int k = 0; k < 3; ++k) v += cf[k] * (mu + (c == k ? lam + mu : 0.0)) * T[k];
+      d[c * nsc + P] = v;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
@@ -81,6 +123,18 @@ hipError_t launch_filter(const icc_filter* f, int mode, const double* x, const d
   const double h0 = s.len[0] / (double)s.nel[0], h1 = s.len[1] / (double)s.nel[1], h2 = s.len[2] / (double)s.nel[2];
   hipLaunchKernelGGL(filter_kernel, dim3(nb), dim3(256), 0, st, (int)s.nel[0], (int)s.nel[1], (int)s.nel[2], h0, h1,
                      h2, f->r, f->rad[0], f->rad[1], f->rad[2], mode, x, v, f->d_hs, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_wq_diag(const icc_grid* g, const double* tab, const double* x, double penal, double emin,
+                          double* d, hipStream_t st) {
+  const auto& s = g->s;
+  const double det = s.len[0] * s.len[1] * s.len[2];
+  int nb = (int)((s.ndof_sc + 255) / 256);
+  if (nb > 16384) nb = 16384;
+  hipLaunchKernelGGL(wq_diag_kernel, dim3(nb), dim3(256), 0, st, tab, (int)s.ncp[0], (int)s.ncp[1], (int)s.ncp[2],
+                     (int)s.nel[0], (int)s.nel[1], (int)s.nel[2], x, penal, emin, det / (s.len[0] * s.len[0]),
+                     det / (s.len[1] * s.len[1]), det / (s.len[2] * s.len[2]), s.lambda, s.mu, d);
   return hipGetLastError();
 }
 

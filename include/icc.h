@@ -90,6 +90,8 @@ int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, const in
 int icc_apply(icc_op* op, int prec, const double* x, double* y);
 /* Device pointers: x,y are double* for ICC_F64 and float* for ICC_F32. */
 int icc_apply_dev(icc_op* op, int prec, const void* x_dev, void* y_dev);
+/* y += A x without zeroing y first (one kernel launch; no sparse term). */
+int icc_apply_accumulate_dev(icc_op* op, int prec, const void* x_dev, void* y_dev);
 
 /* Jacobi diagonal d_A = sum_{e ni A} s_e K_t[a,a] (+ diag S), unclamped
  * (sparse.rs:70-78 on the assembled K; topopt_iga_3d.m:119-125). */

@@ -1,6 +1,10 @@
 // Oracle C API for ctypes (tests/, smoke check, bench cpu_baseline only).
 // Test infrastructure: the product never links this library.
+#include <algorithm>
 #include <cmath>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -401,4 +405,78 @@ ORC_API i64 orc_space_assemble_wq(void* p, double young, double poisson, i64 den
   h->csr.reset(new CsrMatrix(
       wq_stiffness_layout(h->sp, young, poisson, (int)density_kind, x, penal, emin, (int)layout)));
   return (i64)h->csr->nnz();
+}
+
+// ---- CPU baseline support -------------------------------------------------------
+// Builds the same CSR matrix as AssemblyPattern::new + assemble
+// (sparse.rs:128-175) for 3D p = 2 grids without the nel*81^2 position map:
+// rows enumerate the 3 x 5^3 neighbour columns in increasing index, and each
+// entry sums s_e K_e[a][b] over the elements holding both points in
+// increasing e, starting from 0.0 -- the order assemble() uses, so the values
+// are bitwise equal.  Rows are built in parallel.
+ORC_API i64 orc_space_assemble_structured(void* p, const double* scale) {
+  auto* h = (SpaceH*)p;
+  const SpaceBox& s = h->sp;
+  if (!h->have_em || s.dim != 3 || s.nsh != 81) return -1;
+  const long n0 = (long)s.ndof_dir[0], n1 = (long)s.ndof_dir[1], n2 = (long)s.ndof_dir[2];
+  const long e0 = (long)s.nel_dir[0], e1 = (long)s.nel_dir[1], e2 = (long)s.nel_dir[2];
+  const long nsc = (long)s.ndof_sc, n = 3 * nsc;
+  auto* m = new CsrMatrix();
+  m->nrows = m->ncols = (usize)n;
+  m->indptr.assign((usize)n + 1, 0);
+  auto span = [](long p_, long np) { return std::min(p_ + 2, np - 1) - std::max(p_ - 2, 0L) + 1; };
+  for (long r = 0; r < n; ++r) {
+    long P = r % nsc;
+    long pi = P % n0, pj = (P / n0) % n1, pk = P / (n0 * n1);
+    m->indptr[(usize)r + 1] = (usize)(3 * span(pi, n0) * span(pj, n1) * span(pk, n2));
+  }
+  for (long r = 0; r < n; ++r) m->indptr[(usize)r + 1] += m->indptr[(usize)r];
+  m->indices.resize(m->indptr[(usize)n]);
+  m->data.resize(m->indptr[(usize)n]);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long r = 0; r < n; ++r) {
+    const long c = r / nsc, P = r % nsc;
+    const long pi = P % n0, pj = (P / n0) % n1, pk = P / (n0 * n1);
+    usize k = m->indptr[(usize)r];
+    for (long c2 = 0; c2 < 3; ++c2)
+      for (long qk = std::max(pk - 2, 0L); qk <= std::min(pk + 2, n2 - 1); ++qk)
+        for (long qj = std::max(pj - 2, 0L); qj <= std::min(pj + 2, n1 - 1); ++qj)
+          for (long qi = std::max(pi - 2, 0L); qi <= std::min(pi + 2, n0 - 1); ++qi, ++k) {
+            m->indices[k] = (usize)(c2 * nsc + qi + n0 * (qj + n1 * qk));
+            double v = 0.0;
+            for (long ek = std::max(std::max(pk, qk) - 2, 0L); ek <= std::min(std::min(pk, qk), e2 - 1); ++ek)
+              for (long ej = std::max(std::max(pj, qj) - 2, 0L); ej <= std::min(std::min(pj, qj), e1 - 1); ++ej)
+                for (long ei = std::max(std::max(pi, qi) - 2, 0L); ei <= std::min(std::min(pi, qi), e0 - 1); ++ei) {
+                  const long e = ei + e0 * (ej + e1 * ek);
+                  const long a = (pi - ei) + 3 * ((pj - ej) + 3 * (pk - ek));
+                  const long b = (qi - ei) + 3 * ((qj - ej) + 3 * (qk - ek));
+                  const double* ke = h->em.of_element((usize)e);
+                  v += (scale ? scale[e] : 1.0) * ke[(c * 27 + a) * 81 + (c2 * 27 + b)];
+                }
+            m->data[k] = v;
+          }
+  }
+  h->csr.reset(m);
+  return (i64)m->nnz();
+}
+
+// Compares the cached CSR with a fresh AssemblyPattern assembly: returns the
+// number of differing entries (pattern or bits); -1 on pattern mismatch.
+ORC_API i64 orc_compare_with_pattern(void* p, const double* scale) {
+  auto* h = (SpaceH*)p;
+  AssemblyPattern pat(h->sp.ndof, h->sp.connectivity, h->sp.nsh, h->sp.nel);
+  Vec s(scale, scale + h->sp.nel);
+  CsrMatrix ref = pat.assemble(h->em, s);
+  if (ref.indptr != h->csr->indptr || ref.indices != h->csr->indices) return -1;
+  i64 diff = 0;
+  for (usize k = 0; k < ref.data.size(); ++k) diff += (ref.data[k] != h->csr->data[k]);
+  return diff;
+}
+
+ORC_API i64 orc_num_threads(void) {
+#ifdef _OPENMP
+  return (i64)omp_get_max_threads();
+#else
+  return 1;
+#endif
 }

@@ -75,6 +75,12 @@ def lib():
         L.orc_wq_rule.argtypes = [i64, i64, i64, i64, i64, vp]
         L.orc_space_assemble_wq.restype = i64
         L.orc_space_assemble_wq.argtypes = [vp, dbl, dbl, i64, vp, dbl, dbl, i64]
+        L.orc_space_assemble_structured.restype = i64
+        L.orc_space_assemble_structured.argtypes = [vp, vp]
+        L.orc_compare_with_pattern.restype = i64
+        L.orc_compare_with_pattern.argtypes = [vp, vp]
+        L.orc_num_threads.restype = i64
+        L.orc_num_threads.argtypes = []
         _lib = L
     return _lib
 
@@ -138,6 +144,12 @@ class Space:
         if pattern:
             return int(lib().orc_space_assemble_pattern(self.h, _p(s)))
         return int(lib().orc_space_assemble(self.h, _p(s)))
+
+    def assemble_structured(self, scale):
+        return int(lib().orc_space_assemble_structured(self.h, _p(np.ascontiguousarray(scale, dtype=np.float64))))
+
+    def compare_with_pattern(self, scale):
+        return int(lib().orc_compare_with_pattern(self.h, _p(np.ascontiguousarray(scale, dtype=np.float64))))
 
     def assemble_wq(self, young, poisson, density_kind, x, penal, emin, layout=0):
         xx = None if x is None else np.ascontiguousarray(x, dtype=np.float64)

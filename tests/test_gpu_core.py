@@ -183,7 +183,7 @@ def test_iterative_refinement_true_residual(ctx, inner):
     g = icc.Grid(ctx, nel, bounds)
     op = icc.Operator(g, scale=s, free=free)
     prec = icc.ICC_F64 if inner == "f64" else icc.ICC_F32
-    u, st, rc = op.solve_ir(f, tol=1e-8, inner=prec, inner_tol=1e-3, max_outer=60)
+    u, st, rc = op.solve_ir(f, tol=1e-8, inner=prec, inner_tol=1e-3, max_outer=60, max_inner=20000)
     assert rc == icc.ICC_OK, st
     # the true residual, evaluated by the oracle's assembled FP64 operator
     r = free * (f - sp.matvec(free * u))
@@ -213,3 +213,39 @@ def test_apply_properties_config1_size(ctx):
     t = np.zeros(g.ndof)
     t[g.ndof_sc:2 * g.ndof_sc] = 1.0
     assert np.linalg.norm(op.apply(t)) < 1e-12 * np.linalg.norm(ax) * np.linalg.norm(t) / np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("nel", [[4, 3, 2], [8, 5, 3], [12, 7, 6]])
+def test_wq_diag_matches_oracle(ctx, nel):
+    """Diagonal of the WQ stiffness (wq.rs:329-514, interior layout, element
+    densities) vs the oracle's assembled WQ matrix; at uniform density it is
+    the Gauss diagonal (wq_vs_matlab.rs:35-51)."""
+    bounds = [[0, 1.2], [0, 0.6], [0, 0.3]]
+    g = icc.Grid(ctx, nel, bounds)
+    sp, _, _ = _oracle_space(bounds, nel)
+    x = np.random.default_rng(9).uniform(0.1, 1.0, g.nel_tot)
+    sp.assemble_wq(1.0, 0.3, 1, x, 3.0, 1e-3, layout=0)
+    assert O.rel_err(g.wq_diag(x, 3.0, 1e-3), sp.diag()) < 1e-12
+    sp.assemble(None)
+    assert O.rel_err(g.wq_diag(None), sp.diag()) < 1e-12
+
+
+def _cylinders(seed, n=3):
+    rng = np.random.default_rng(seed)
+    return np.array([[rng.uniform(0.3, 1.7), rng.uniform(0.25, 0.75), rng.uniform(0.08, 0.15)] for _ in range(n)])
+
+
+@pytest.mark.parametrize("res,padded", [([32, 16, 16], False), ([24, 14, 9], True), ([128, 64, 64], False)])
+def test_classification_bit_exact(ctx, res, padded):
+    """Cell status (INSIDE/OUTSIDE/CUT) and 4^3 sub-cell volume fractions
+    (immersed.rs:88-155) with the box-minus-cylinders predicate: bit-exact
+    with the oracle; the active list (w > 0) follows."""
+    lo, hi = [0.0, 0.0, 0.0], [2.0, 1.0, 1.0]
+    gb = [[-0.1, 2.1], [-0.05, 1.05], [-0.05, 1.05]] if padded else [[0.0, 2.0], [0.0, 1.0], [0.0, 1.0]]
+    cyl = _cylinders(sum(res))
+    st_ref, w_ref = O.classify_cylinders(lo, hi, cyl, gb, res)
+    st, w = icc.classify_cylinders(ctx, lo, hi, cyl, gb, res)
+    assert np.array_equal(st, st_ref)
+    assert np.array_equal(w, w_ref)
+    assert np.array_equal(np.flatnonzero(w > 0), np.flatnonzero(w_ref > 0))
+    assert set(np.unique(st)) <= {-1, 0, 1} and np.all((w > 0) | (st == 0))

@@ -96,3 +96,15 @@ def test_sensitivity_identity():
     ee = sp.energies(u)
     dc = -3.0 * x ** 2 * (1 - 1e-3) * ee
     assert O.rel_err(sp.fast_sens(u, x, False, 3.0, 1e-3, 1.0, 0.3), dc) < 1e-11
+
+
+def test_structured_csr_equals_assembly_pattern():
+    """The CPU-baseline CSR builder equals AssemblyPattern (sparse.rs:128-175)
+    bit for bit, pattern and values."""
+    for nel in ([4, 3, 3], [7, 5, 2], [1, 2, 3]):
+        sp = O.Space([[0, 2.0], [0, 1.0], [0, 1.0]], nel, 2)
+        lam, mu = O.lame(1.0, 0.3)
+        sp.elem_mats(lam, mu)
+        s = np.random.default_rng(3).uniform(0.01, 1.0, sp.nel) ** 3
+        assert sp.assemble_structured(s) > 0
+        assert sp.compare_with_pattern(s) == 0
