@@ -55,9 +55,10 @@ class Context:
             pass
 
     def mfma_layout(self):
-        out = np.zeros(256)
+        """(f64, f32) accumulator contents [lane, reg] of the layout probe."""
+        out = np.zeros(512)
         check(lib().icc_debug_mfma_layout(self.h, _p(out)))
-        return out.reshape(64, 4)
+        return out[:256].reshape(64, 4), out[256:].reshape(64, 4)
 
 
 class Grid:

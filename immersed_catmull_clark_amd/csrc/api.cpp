@@ -939,12 +939,12 @@ ICC_API int icc_classify_cylinders(icc_ctx* ctx, const double box_lo[3], const d
   return ICC_OK;
 }
 
-ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out256) {
-  if (!ctx || !out256) return ICC_EINVAL;
+ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out512) {
+  if (!ctx || !out512) return ICC_EINVAL;
   double* d = nullptr;
-  ICC_TRY(dalloc(&d, 256));
+  ICC_TRY(dalloc(&d, 512));
   hipError_t e = launch_layout_probe(d, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(out256, d, 256 * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out512, d, 512 * 8, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "icc_debug_mfma_layout");

@@ -16,9 +16,10 @@
 // zero on entry).  The element energy p_e^T K p_e, needed for p.Ap, is
 // accumulated on the fly from the same accumulators.
 //
-// MFMA lane layouts (v_mfma_{f64,f32}_16x16x4): lane l holds A[l%16][l/16],
-// B[l/16][l%16], and D[4(l/16)+r][l%16] in accumulator register r.
-// icc_debug_mfma_layout pins this on the device.
+// MFMA lane layouts (v_mfma_{f64,f32}_16x16x4): lane l holds A[l%16][l/16] and
+// B[l/16][l%16]; accumulator register r of lane l holds D[i][l%16] with
+// i = 4 r + l/16 for f64 and i = 4 (l/16) + r for f32 (Mfma<T>::row).
+// icc_debug_mfma_layout pins both on the device.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -38,6 +39,8 @@ struct Mfma<double> {
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // row (within the 16-row tile) held by accumulator register r of row group kq = lane / 16
+  static __device__ __forceinline__ int row(int kq, int r) { return 4 * r + kq; }
 };
 template <>
 struct Mfma<float> {
@@ -45,6 +48,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int kq, int r) { return 4 * kq + r; }
 };
 
 // Local offset of trial / test index b (0..80) inside a tile staged as
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(384) void apply_tiles(const Tile* __restrict__ tile
   bool rvalid[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    int row = 16 * wave + 4 * kq + r;
+    int row = 16 * wave + Mfma<T>::row(kq, r);
     rvalid[r] = row < NSH;
     roff[r] = rvalid[r] ? local_offset(row, px, py, npts) : 0;
   }
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(384) void energies_tiles(const Tile* __restrict__ t
   bool rvalid[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    int row = 16 * wave + 4 * kq + r;
+    int row = 16 * wave + Mfma<double>::row(kq, r);
     rvalid[r] = row < NSH;
     roff[r] = rvalid[r] ? local_offset(row, px, py, npts) : 0;
   }
@@ -284,13 +288,21 @@ __global__ void diag_gather(const double* __restrict__ kdiag, const int32_t* __r
   }
 }
 
+// A[i][k] = i + 16 k, B[k][j] = (k == j % 4), so D[i][j] = i + 16 (j % 4):
+// out[l * 4 + r] (f64) and out[256 + l * 4 + r] (f32) identify the row and
+// column each accumulator register holds.
 __global__ void mfma_layout_probe(double* out) {
   const int l = threadIdx.x;
-  const double a = (double)((l % 16) + 16 * (l / 16));   // A[i][k] = i + 16 k
-  const double b = ((l / 16) == ((l % 16) % 4)) ? 1.0 : 0.0;  // B[k][j] = (k == j % 4)
+  const double a = (double)((l % 16) + 16 * (l / 16));
+  const double b = ((l / 16) == ((l % 16) % 4)) ? 1.0 : 0.0;
   d4_t acc = {0, 0, 0, 0};
   acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-  for (int r = 0; r < 4; ++r) out[l * 4 + r] = acc[r];
+  f4_t accf = {0, 0, 0, 0};
+  accf = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, accf, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) {
+    out[l * 4 + r] = acc[r];
+    out[256 + l * 4 + r] = (double)accf[r];
+  }
 }
 
 // ---- launchers ---------------------------------------------------------------

@@ -196,9 +196,10 @@ int icc_topopt_run(icc_topopt* t, icc_report_fn report, void* user, double* comp
                    int64_t* pcg_iterations, double* density, int64_t* n_iter);
 
 /* ---- diagnostics ------------------------------------------------------------- */
-/* runs one v_mfma_f64_16x16x4 with A[i][k] = i + 16k, B[k][j] = (k == j%4) and
- * writes the 64x4 accumulator registers: used by tests to pin the lane layout */
-int icc_debug_mfma_layout(icc_ctx* ctx, double* out256);
+/* runs one v_mfma_f64_16x16x4 and one v_mfma_f32_16x16x4 with
+ * A[i][k] = i + 16k, B[k][j] = (k == j%4) and writes the 64x4 accumulator
+ * registers of each (f64 first, then f32): pins the lane layouts in tests */
+int icc_debug_mfma_layout(icc_ctx* ctx, double* out512);
 
 #ifdef __cplusplus
 }

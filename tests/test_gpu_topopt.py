@@ -1,5 +1,7 @@
 """GPU parity of the CantileverTopOpt driver (rust/src/topopt_iga.rs) and of
 the adjoint sensitivities against finite differences."""
+import math
+
 import numpy as np
 import pytest
 
@@ -74,11 +76,13 @@ def test_adjoint_vs_finite_differences(ctx, filt):
 
     def c_hat(xd, u0):
         # 2 F.u - u^T K(x) u: equals c at the solution, error quadratic in the
-        # solve residual (F.u alone is first order in it)
+        # solve residual (F.u alone is first order in it); summed exactly
+        # (fsum) so round-off does not dominate the h = 1e-6 difference
         _, _, uu = t.compliance(xd, u0=u0, grad=False)
         xt = sp.filter_apply(cfg.rmin, 1, None, xd) if filt == 1 else xd
         s = cfg.emin + (1 - cfg.emin) * xt ** cfg.penal
-        return 2.0 * (t.force @ uu) - s @ t.element_energies(uu)
+        terms = np.concatenate([2.0 * t.force * uu, -(s * t.element_energies(uu))])
+        return math.fsum(terms)
 
     assert abs(c_hat(x, u) - c) < 1e-10 * c
     hstep = 1e-6
