@@ -137,7 +137,11 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
   g->ntiles = (int)s.tiles.size();
   for (int d = 0; d < 3; ++d) g->T[d] = (int)s.mats1[d].size();
   g->maxnel = (int)std::max(s.nel[0], std::max(s.nel[1], s.nel[2]));
-  // MFMA A fragments: lane l of (type, mt, ks) holds K[16 mt + l%16][4 ks + l/16]
+  // MFMA A fragments: lane l of (type, mt, ks) holds A[i = l%16][k = l/16] =
+  // K[row][4 ks + l/16], where row is the K row the accumulator slot i of M
+  // tile mt is assigned (perm_row); slot i belongs to register r and row group
+  // kq with i = 4 r + kq for v_mfma_f64_16x16x4 and i = 4 kq + r for
+  // v_mfma_f32_16x16x4.
   const size_t nfrag = (size_t)g->ntypes * MT * KS * 64;
   std::vector<double> f64(nfrag, 0.0);
   std::vector<float> f32(nfrag, 0.0f);
@@ -147,11 +151,11 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
     for (int mt = 0; mt < MT; ++mt)
       for (int ks = 0; ks < KS; ++ks)
         for (int l = 0; l < 64; ++l) {
-          int row = 16 * mt + (l % 16), colk = 4 * ks + (l / 16);
-          double v = (row < NSH && colk < NSH) ? K[row * NSH + colk] : 0.0;
+          const int i = l % 16, colk = 4 * ks + (l / 16);
+          const int row64 = perm_row(mt, i % 4, i / 4), row32 = perm_row(mt, i / 4, i % 4);
           size_t idx = (((size_t)t * MT + mt) * KS + ks) * 64 + l;
-          f64[idx] = v;
-          f32[idx] = (float)v;
+          f64[idx] = (row64 >= 0 && colk < NSH) ? K[row64 * NSH + colk] : 0.0;
+          f32[idx] = (float)((row32 >= 0 && colk < NSH) ? K[row32 * NSH + colk] : 0.0);
         }
     for (int a = 0; a < NSH; ++a) kd[(size_t)t * NSH + a] = K[a * NSH + a];
   }

@@ -18,9 +18,37 @@ constexpr int TILE_MAX_EL = 128;   // elements per tile (8 waves x 16)
 constexpr int TILE_MAX_PTS = 640;  // control points staged per tile
 
 // Tile of a constant-type element sub-box: elements [i0, i0+nx) x [j0, j0+ny) x [k0, k0+nz).
+// lines = 1: the tile is processed as x-lines of nx <= 16 elements (one MFMA
+// N group per line); lines = 0: as groups of 16 consecutive tile elements.
 struct Tile {
-  int32_t i0, j0, k0, nx, ny, nz, type, pad;
+  int32_t i0, j0, k0, nx, ny, nz, type, lines;
 };
+
+#if defined(__HIP__) || defined(__HIPCC__)
+#define ICC_HD __host__ __device__
+#else
+#define ICC_HD
+#endif
+
+// Row of K held by accumulator register r (0..3) of row group kq (0..3) in
+// M tile w (0..5), or -1 for padding.  Rows are permuted so that registers
+// 0..2 hold the ax = 0, 1, 2 rows of one (component, ay, az) triplet
+// t = 4 w + kq (t < 24); register 3 holds ax = w of triplet 24 + kq for
+// w, kq < 3.  Within one accumulator register (one ds_add instruction) the
+// four row groups then hold four different triplets.
+ICC_HD inline int perm_row(int w, int kq, int r) {
+  int t, ax;
+  if (r < 3) {
+    t = 4 * w + kq;
+    ax = r;
+  } else {
+    if (w >= 3 || kq >= 3) return -1;
+    t = 24 + kq;
+    ax = w;
+  }
+  const int c = t / 9, rem = t - 9 * c, ay = rem % 3, az = rem / 3;
+  return c * 27 + ax + 3 * ay + 9 * az;
+}
 
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);

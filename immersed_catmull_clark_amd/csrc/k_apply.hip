@@ -80,8 +80,28 @@ __device__ __forceinline__ void stage_tile(const Tile& t, const T* __restrict__ 
   }
 }
 
+// Shift-by-k within 16-lane rows (lane c gets lane c - k, 0 for c < k).
+template <int K>
+__device__ __forceinline__ float row_shr(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x110 + K, 0xf, 0xf, true));
+}
+template <int K>
+__device__ __forceinline__ double row_shr(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x110 + K, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + K, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
 // y += M * sum_e s_e K_t x_e (M x) over one tile per workgroup; optional
 // per-tile partial of sum_e s_e x_e^T K x_e into pap[blockIdx.x].
+//
+// Each wave runs two element groups at a time as independent MFMA chains.
+// In line mode a group is one x-line of the tile (lane column = x offset);
+// accumulator registers 0..2 of a lane hold the ax = 0, 1, 2 rows of one
+// (component, ay, az) triplet (perm_row), so the three contributions each
+// control point of the line receives are summed in registers with two DPP
+// row shifts before a single ds_add per point.
 template <typename T>
 __global__ __launch_bounds__(384) void apply_tiles(const Tile* __restrict__ tiles, const T* __restrict__ kfrag,
                                                    const T* __restrict__ x, T* __restrict__ y,
@@ -90,6 +110,7 @@ __global__ __launch_bounds__(384) void apply_tiles(const Tile* __restrict__ tile
                                                    double* __restrict__ pap, const int32_t* __restrict__ done) {
   __shared__ T xs[3 * TILE_MAX_PTS];
   __shared__ T ys[3 * TILE_MAX_PTS];
+  __shared__ T ss[TILE_MAX_EL];
   __shared__ double red[6];
   if (done != nullptr && *done != 0) return;  // solver already finished
   const Tile t = tiles[blockIdx.x];
@@ -97,6 +118,10 @@ __global__ __launch_bounds__(384) void apply_tiles(const Tile* __restrict__ tile
   const int npts = px * py * pz;
   const int nel = t.nx * t.ny * t.nz;
   stage_tile<T>(t, x, freem, ncp0, ncp1, ndof_sc, xs, ys, px, py, npts);
+  for (int el = threadIdx.x; el < nel; el += blockDim.x) {
+    const int ex = el % t.nx, rr = el / t.nx, ey = rr % t.ny, ez = rr / t.ny;
+    ss[el] = scale[(t.i0 + ex) + nel0 * ((t.j0 + ey) + nel1 * (t.k0 + ez))];
+  }
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -114,40 +139,79 @@ __global__ __launch_bounds__(384) void apply_tiles(const Tile* __restrict__ tile
     boff[ks] = b < NSH ? local_offset(b, px, py, npts) : 0;  // A is zero there
   }
   int roff[4];
-  bool rvalid[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    int row = 16 * wave + Mfma<T>::row(kq, r);
-    rvalid[r] = row < NSH;
-    roff[r] = rvalid[r] ? local_offset(row, px, py, npts) : 0;
+    const int row = perm_row(wave, kq, r);
+    roff[r] = row >= 0 ? local_offset(row, px, py, npts) : 0;
   }
+  const bool r3 = perm_row(wave, kq, 3) >= 0;
   __syncthreads();
 
   double pap_acc = 0.0;
-  for (int g0 = 0; g0 < nel; g0 += 16) {
-    const int el = g0 + col;
-    const bool valid = el < nel;
-    int ex = 0, ey = 0, ez = 0;
-    if (valid) {
-      ex = el % t.nx;
-      int rr = el / t.nx;
-      ey = rr % t.ny;
-      ez = rr / t.ny;
-    }
-    const int ebase = ex + px * (ey + py * ez);
-    typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
+  const bool lines = t.lines != 0;
+  const int ngroups = lines ? t.ny * t.nz : (nel + 15) / 16;
+  for (int g0 = 0; g0 < ngroups; g0 += 2) {
+    int eb[2], el[2];
+    bool valid[2];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) acc = Mfma<T>::run(a[ks], xs[ebase + boff[ks]], acc);
-    if (valid) {
-      const int ge = (t.i0 + ex) + nel0 * ((t.j0 + ey) + nel1 * (t.k0 + ez));
-      const T s = scale[ge];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (rvalid[r]) {
-          const T v = acc[r] * s;
-          unsafeAtomicAdd(&ys[ebase + roff[r]], v);
-          pap_acc += (double)xs[ebase + roff[r]] * (double)v;
+    for (int q = 0; q < 2; ++q) {
+      const int g = g0 + q;
+      if (lines) {
+        const int ey = g % t.ny, ez = g / t.ny;
+        valid[q] = g < ngroups && col < t.nx;
+        el[q] = col + t.nx * g;
+        eb[q] = valid[q] ? col + px * (ey + py * ez) : 0;
+      } else {
+        const int e = 16 * g + col;
+        valid[q] = g < ngroups && e < nel;
+        int ex = 0, ey = 0, ez = 0;
+        if (valid[q]) {
+          ex = e % t.nx;
+          const int rr = e / t.nx;
+          ey = rr % t.ny;
+          ez = rr / t.ny;
         }
+        el[q] = e;
+        eb[q] = ex + px * (ey + py * ez);
+      }
+    }
+    typename Mfma<T>::acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      acc0 = Mfma<T>::run(a[ks], xs[eb[0] + boff[ks]], acc0);
+      acc1 = Mfma<T>::run(a[ks], xs[eb[1] + boff[ks]], acc1);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const typename Mfma<T>::acc_t& acc = q == 0 ? acc0 : acc1;
+      const T s = valid[q] ? ss[el[q]] : T(0);
+      const T v0 = acc[0] * s, v1 = acc[1] * s, v2 = acc[2] * s, v3 = acc[3] * s;
+      if (pap != nullptr && valid[q]) {
+        pap_acc += (double)xs[eb[q] + roff[0]] * (double)v0 + (double)xs[eb[q] + roff[1]] * (double)v1 +
+                   (double)xs[eb[q] + roff[2]] * (double)v2;
+        if (r3) pap_acc += (double)xs[eb[q] + roff[3]] * (double)v3;
+      }
+      if (lines) {
+        // point x = col of the line receives v0[col] + v1[col-1] + v2[col-2];
+        // the DPP shifts run on all lanes, before any lane-dependent branch
+        const T comb = v0 + row_shr<1>(v1) + row_shr<2>(v2);
+        const T e16 = v1 + row_shr<1>(v2);  // at lane 15: point 16 = v1[15] + v2[14]
+        const int g = g0 + q;
+        if (g < ngroups) {
+          const int ey = g % t.ny, ez = g / t.ny;
+          const int lb = px * (ey + py * ez) + roff[0];  // point x = 0 of this line, ax = 0 row
+          if (col <= t.nx + 1) unsafeAtomicAdd(&ys[lb + col], comb);
+          if (col == 15) {  // points 16 and 17 exist only for lines of 15 and 16 elements
+            if (t.nx >= 15) unsafeAtomicAdd(&ys[lb + 16], e16);
+            if (t.nx >= 16) unsafeAtomicAdd(&ys[lb + 17], v2);
+          }
+          if (r3 && valid[q]) unsafeAtomicAdd(&ys[eb[q] + roff[3]], v3);
+        }
+      } else if (valid[q]) {
+        unsafeAtomicAdd(&ys[eb[q] + roff[0]], v0);
+        unsafeAtomicAdd(&ys[eb[q] + roff[1]], v1);
+        unsafeAtomicAdd(&ys[eb[q] + roff[2]], v2);
+        if (r3) unsafeAtomicAdd(&ys[eb[q] + roff[3]], v3);
       }
     }
   }
@@ -207,8 +271,8 @@ __global__ __launch_bounds__(384) void energies_tiles(const Tile* __restrict__ t
   bool rvalid[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    int row = 16 * wave + Mfma<double>::row(kq, r);
-    rvalid[r] = row < NSH;
+    int row = perm_row(wave, kq, r);
+    rvalid[r] = row >= 0;
     roff[r] = rvalid[r] ? local_offset(row, px, py, npts) : 0;
   }
   __syncthreads();

@@ -284,8 +284,28 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
     for (auto& ry : runs[1])
       for (auto& rx : runs[0]) {
         int64_t ex = rx.second - rx.first, ey = ry.second - ry.first, ez = rz.second - rz.first;
-        int tx, ty, tz;
-        tile_shape(ex, ey, ez, tx, ty, tz);
+        int tx, ty, tz, lines = 0;
+        if (ex >= 8) {
+          // x-lines of <= 16 elements, spread evenly; up to 8 lines per tile
+          const int64_t ntx = (ex + 15) / 16;
+          tx = (int)((ex + ntx - 1) / ntx);
+          lines = 1;
+          int best = -1;
+          ty = tz = 1;
+          static const int yz[][2] = {{2, 4}, {4, 2}, {1, 8}, {8, 1}, {2, 2}, {1, 4}, {4, 1}, {1, 2}, {2, 1}, {1, 1}};
+          for (auto& c : yz) {
+            const int by = (int)std::min<int64_t>(c[0], ey), bz = (int)std::min<int64_t>(c[1], ez);
+            const int pts = (tx + 2) * (by + 2) * (bz + 2);
+            const int score = by * bz * 1000 - pts;
+            if (pts <= TILE_MAX_PTS && by * bz * tx <= TILE_MAX_EL && score > best) {
+              best = score;
+              ty = by;
+              tz = bz;
+            }
+          }
+        } else {
+          tile_shape(ex, ey, ez, tx, ty, tz);
+        }
         int32_t type = element_type(s, rx.first, ry.first, rz.first);
         for (int64_t k0 = rz.first; k0 < rz.second; k0 += tz)
           for (int64_t j0 = ry.first; j0 < ry.second; j0 += ty)
@@ -298,7 +318,7 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
               t.ny = (int32_t)std::min<int64_t>(ty, ry.second - j0);
               t.nz = (int32_t)std::min<int64_t>(tz, rz.second - k0);
               t.type = type;
-              t.pad = 0;
+              t.lines = lines;
               s.tiles.push_back(t);
             }
       }
