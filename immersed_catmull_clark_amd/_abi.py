@@ -48,6 +48,8 @@ SIGNATURES = [
     ("icc_pcg_dev", cint, [vp, vp, vp, dbl, i64, vp, vp, vp]),
     ("icc_solve_ir_dev", cint, [vp, cint, vp, dbl, dbl, i64, i64, vp, vp]),
     ("icc_solve_ir", cint, [vp, cint, vp, dbl, dbl, i64, i64, vp, vp]),
+    ("icc_solve_mixed_dev", cint, [vp, vp, dbl, dbl, i64, vp, vp]),
+    ("icc_solve_mixed", cint, [vp, vp, dbl, dbl, i64, vp, vp]),
     ("icc_energies", cint, [vp, vp, vp]),
     ("icc_energies_dev", cint, [vp, vp, vp]),
     ("icc_sens_dev", cint, [vp, vp, vp, dbl, dbl, vp, vp]),

@@ -190,6 +190,18 @@ class Operator:
                   fp64_applies=int(stats[3]), inner_applies=int(stats[4]))
         return u, st, rc
 
+    def solve_mixed(self, f, tol=1e-8, delta=0.1, maxit=1000000, u0=None):
+        """FP32 PCG with reliable updates against the FP64 operator: the FP32
+        residual is replaced by the FP64 true residual each time it drops by
+        `delta`.  Returns (u, stats dict, status)."""
+        u = np.zeros(self.ndof) if u0 is None else _f64(u0).copy()
+        stats = np.zeros(8)
+        rc = lib().icc_solve_mixed(self.h, _p(_f64(f)), float(tol), float(delta), int(maxit), _p(u), _p(stats))
+        check(rc, allowed=(ICC_OK,) + ICC_BREAKDOWN_CODES)
+        st = dict(updates=int(stats[0]), inner_iters=int(stats[1]), true_res=float(stats[2]),
+                  fp64_applies=int(stats[3]), inner_applies=int(stats[4]))
+        return u, st, rc
+
     def energies(self, u):
         ee = np.zeros(self.grid.nel_tot)
         check(lib().icc_energies(self.h, _p(_f64(u)), _p(ee)))

@@ -48,10 +48,10 @@ struct Work {
   int64_t n = 0;
   double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *dinv = nullptr, *tmp = nullptr, *f = nullptr;
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *dinv32 = nullptr;
-  double* parts = nullptr;  // 2 * 1024 vector partials
+  double* parts = nullptr;  // 2 * VEC_MAX_BLOCKS vector partials
   double* pap = nullptr;    // ntiles + csr blocks
   PcgState* st = nullptr;
-  PcgState* st_host = nullptr;  // pinned
+  PcgState* st_host = nullptr;  // pinned, 2 entries (batch lookahead)
 };
 
 static Work* work_of(icc_op* op);
@@ -325,16 +325,16 @@ static int ensure_work(icc_op* op, Work*& out) {
     free_work(*w);
     w->n = n;
     int rc;
-    size_t npap = (size_t)op->g->ntiles + 1024;
+    size_t npap = (size_t)op->g->ntiles + VEC_MAX_BLOCKS;
     if ((rc = dalloc(&w->x, n)) || (rc = dalloc(&w->r, n)) || (rc = dalloc(&w->p, n)) || (rc = dalloc(&w->ap, n)) ||
         (rc = dalloc(&w->dinv, n)) || (rc = dalloc(&w->tmp, n)) || (rc = dalloc(&w->f, n)) ||
         (rc = dalloc(&w->x32, n)) || (rc = dalloc(&w->r32, n)) || (rc = dalloc(&w->p32, n)) ||
-        (rc = dalloc(&w->ap32, n)) || (rc = dalloc(&w->dinv32, n)) || (rc = dalloc(&w->parts, 2048)) ||
+        (rc = dalloc(&w->ap32, n)) || (rc = dalloc(&w->dinv32, n)) || (rc = dalloc(&w->parts, 2 * VEC_MAX_BLOCKS)) ||
         (rc = dalloc(&w->pap, npap)) || (rc = dalloc(&w->st, 1))) {
       free_work(*w);
       return rc;
     }
-    if (hipHostMalloc((void**)&w->st_host, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&w->st_host, 2 * sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
       free_work(*w);
       set_error("hipHostMalloc failed");
       return ICC_ENOMEM;
@@ -575,44 +575,110 @@ static int write_tol(icc_op* op, Work* w, double tol) {
   return ICC_OK;
 }
 
-// Jacobi PCG in precision T on device vectors of the workspace:
-// solves A x = bscale * b, x from 0; dinv already formed.
+// One PCG iteration in precision T enqueued on the op stream (sparse term
+// first, then the tiles, then the two vector kernels).
+template <typename T>
+static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv) {
+  icc_grid* g = op->g;
+  hipStream_t st = g->ctx->stream;
+  const int64_t n = g->s.ndof;
+  const int npap = g->ntiles + csr_blocks(op);
+  hipError_t e = hipSuccess;
+  if constexpr (std::is_same<T, double>::value) {
+    if (op->s_nnz > 0)
+      e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, p, ap, op->d_free,
+                                          w->pap + g->ntiles, st);
+    if (e == hipSuccess) e = launch_apply_f64(g, op, p, ap, w->pap, w->st, st);
+  } else {
+    if (op->s_nnz > 0)
+      e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, p, ap, op->d_free,
+                                        w->pap + g->ntiles, st);
+    if (e == hipSuccess) e = launch_apply_f32(g, op, p, ap, w->pap, w->st, st);
+  }
+  if (e == hipSuccess) e = launch_pcg_update<T>(n, w->st, w->pap, npap, p, ap, x, r, dinv, w->parts, st);
+  return e;
+}
+
+// A batch of PCG iterations captured once as a HIP graph.
+struct IterGraph {
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  ~IterGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+template <typename T>
+static int capture_batch(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, int batch, IterGraph& G) {
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < batch && e == hipSuccess; ++i) e = enqueue_iteration<T>(op, w, x, r, p, ap, dinv);
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(st, &graph);
+  if (e != hipSuccess) return hip_fail(e, "PCG batch capture");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+  G.graph = graph;
+  ICC_HIP(hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0));
+  return ICC_OK;
+}
+
+// Replays the batch graph until the device state says done; one batch of
+// lookahead keeps the GPU busy while the host reads the previous state.
+static int run_until_done(icc_op* op, Work* w, IterGraph& G, PcgState& out) {
+  hipStream_t st = op->g->ctx->stream;
+  hipEvent_t ev[2];
+  ICC_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  ICC_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int rc = ICC_OK;
+  hipError_t e = hipSuccess;
+  for (long k = 0;; ++k) {
+    if ((e = hipGraphLaunch(G.exec, st)) != hipSuccess) break;
+    if ((e = hipMemcpyAsync(&w->st_host[k & 1], w->st, sizeof(PcgState), hipMemcpyDeviceToHost, st)) != hipSuccess)
+      break;
+    if ((e = hipEventRecord(ev[k & 1], st)) != hipSuccess) break;
+    if (k == 0) continue;
+    if ((e = hipEventSynchronize(ev[(k - 1) & 1])) != hipSuccess) break;
+    if (w->st_host[(k - 1) & 1].done) break;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipMemcpy(&out, w->st, sizeof(PcgState), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = hip_fail(e, "PCG batch replay");
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return rc;
+}
+
+static int set_limits(icc_op* op, Work* w, double tol, int64_t maxit) {
+  hipStream_t st = op->g->ctx->stream;
+  int32_t mi = (int32_t)std::min<int64_t>(maxit, 2000000000);
+  ICC_HIP(hipMemcpyAsync(&w->st->tol, &tol, sizeof(double), hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(&w->st->maxit, &mi, sizeof(int32_t), hipMemcpyHostToDevice, st));
+  return ICC_OK;
+}
+
+constexpr int PCG_BATCH = 16;
+
+// Jacobi PCG in precision T on device vectors: solves A x = bscale * b from
+// x = 0 with dinv already formed.  status: 1 converged, 2 breakdown, 3 limit.
 template <typename T, typename TB>
 static int pcg_core(icc_op* op, Work* w, const TB* b, double bscale, T* x, T* r, T* p, T* ap, const T* dinv,
                     double tol, int64_t maxit, int64_t* iters, double* res, int* status) {
   icc_grid* g = op->g;
   hipStream_t st = g->ctx->stream;
   const int64_t n = g->s.ndof;
-  ICC_TRY(write_tol(op, w, tol));
+  ICC_TRY(set_limits(op, w, tol, maxit));
   ICC_HIP((launch_pcg_init<T, TB>(n, b, bscale, dinv, x, r, p, ap, w->st, w->parts, st)));
-  const int npap = g->ntiles + csr_blocks(op);
+  IterGraph G;
+  ICC_TRY(capture_batch<T>(op, w, x, r, p, ap, dinv, PCG_BATCH, G));
   PcgState hs;
-  int64_t it = 0;
-  int batch = 8;
-  *status = 0;
-  while (it < maxit) {
-    int64_t stop = std::min<int64_t>(maxit, it + batch);
-    for (; it < stop; ++it) {
-      if constexpr (std::is_same<T, double>::value) {
-        // the sparse term writes ap (= S p) before the tiles accumulate on top
-        if (op->s_nnz > 0)
-          ICC_HIP((launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, p, ap,
-                                                    op->d_free, w->pap + g->ntiles, st)));
-        ICC_HIP(launch_apply_f64(g, op, p, ap, w->pap, &w->st->done, st));
-      } else {
-        if (op->s_nnz > 0)
-          ICC_HIP((launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, p, ap,
-                                                  op->d_free, w->pap + g->ntiles, st)));
-        ICC_HIP(launch_apply_f32(g, op, p, ap, w->pap, &w->st->done, st));
-      }
-      ICC_HIP(launch_pap_finalize(w->pap, npap, w->st, st));
-      ICC_HIP(launch_pcg_update<T>(n, (int)it, w->st, p, ap, x, r, dinv, w->parts, st));
-    }
-    ICC_TRY(read_state(op, w, hs));
-    if (hs.done) break;
-    if (batch < 64) batch *= 2;
+  if (maxit > 0) {
+    ICC_TRY(run_until_done(op, w, G, hs));
+  } else {
+    ICC_HIP(hipMemcpy(&hs, w->st, sizeof(PcgState), hipMemcpyDeviceToHost));
+    hs.done = 3;
   }
-  if (!hs.done) ICC_TRY(read_state(op, w, hs));
   *iters = hs.iters;
   *res = hs.res;
   *status = hs.done;
@@ -637,7 +703,7 @@ ICC_API int icc_pcg_dev(icc_op* op, const double* b, const double* diag, double 
     return ICC_EBREAKDOWN;
   }
   set_error("icc_pcg: iteration limit reached");
-  return ICC_ENOCONV;
+  return ICC_ENOCONV;  // status 3
 }
 
 ICC_API int icc_pcg(icc_op* op, const double* b, const double* diag, double tol, int64_t maxit, double* x,
@@ -722,9 +788,90 @@ ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol,
   return rc_final;
 }
 
-ICC_API int icc_solve_ir(icc_op* op, int inner, const double* f, double tol, double inner_tol, int64_t max_outer,
-                         int64_t max_inner, double* u, double* stats) {
-  if (!op || !f || !u) return ICC_EINVAL;
+// Mixed precision with reliable updates: Jacobi PCG iterates in FP32 (FP32
+// MFMA apply, FP32 vectors, FP64 reductions); whenever the FP32 residual has
+// dropped by `delta` since the last update, the FP32 correction is added to
+// the FP64 solution, the true residual f - A u is recomputed with the FP64
+// operator and compensated norms, and the FP32 residual is replaced by it
+// while the search direction is kept (no restart).  Stops when the true
+// relative residual <= tol.  stats: updates, FP32 iterations, true residual,
+// FP64 applies, FP32 applies.
+ICC_API int icc_solve_mixed_dev(icc_op* op, const double* f, double tol, double delta, int64_t maxit, double* u,
+                                double* stats) {
+  if (!op || !f || !u || !(delta > 0.0 && delta < 1.0)) return ICC_EINVAL;
+  Work* w;
+  ICC_TRY(ensure_work(op, w));
+  icc_grid* g = op->g;
+  const int64_t n = g->s.ndof;
+  hipStream_t st = g->ctx->stream;
+  ICC_TRY(icc_diag_dev(op, w->tmp));
+  ICC_HIP(launch_inv_diag<float>(w->tmp, w->dinv32, n, 1e-12, 1, st));
+  double n_upd = 0, n_app64 = 0, res_true = 1.0;
+  int rc_final = ICC_ENOCONV;
+  PcgState hs;
+  // true residual of the initial guess
+  ICC_HIP(apply_t<double>(op, u, w->ap, nullptr));
+  n_app64 += 1;
+  ICC_HIP(launch_residual(n, f, w->ap, op->d_free, w->r, w->st, w->parts, st));
+  ICC_TRY(read_state(op, w, hs));
+  const double nf = std::sqrt(hs.norm_b);
+  res_true = nf > 0 ? std::sqrt(hs.rr) / nf : std::sqrt(hs.rr);
+  int64_t iters = 0;
+  if (res_true <= tol) {
+    rc_final = ICC_OK;
+  } else {
+    // FP32 recurrence from r (x32 = 0, p = D^-1 r), stopping at |r| < delta |r_0|
+    ICC_TRY(set_limits(op, w, delta, maxit));
+    ICC_HIP((launch_pcg_init<float, double>(n, w->r, 1.0, w->dinv32, w->x32, w->r32, w->p32, w->ap32, w->st, w->parts,
+                                             st)));
+    IterGraph G;
+    ICC_TRY(capture_batch<float>(op, w, w->x32, w->r32, w->p32, w->ap32, w->dinv32, PCG_BATCH, G));
+    for (;;) {
+      ICC_TRY(run_until_done(op, w, G, hs));
+      iters = hs.iters;
+      // fold the FP32 correction into u and recompute the true residual
+      ICC_HIP(launch_ir_update<float>(n, u, w->x32, 1.0, st));
+      ICC_HIP(apply_t<double>(op, u, w->ap, nullptr));
+      n_app64 += 1;
+      n_upd += 1;
+      ICC_HIP(launch_residual(n, f, w->ap, op->d_free, w->r, w->st, w->parts, st));
+      PcgState hr;
+      ICC_TRY(read_state(op, w, hr));
+      res_true = nf > 0 ? std::sqrt(hr.rr) / nf : std::sqrt(hr.rr);
+      if (res_true <= tol) {
+        rc_final = ICC_OK;
+        break;
+      }
+      if (hs.done == 3) break;  // iteration limit
+      if (hs.done == 2) {       // breakdown of the FP32 recurrence
+        rc_final = ICC_EBREAKDOWN;
+        set_error("icc_solve_mixed: breakdown of the FP32 recurrence");
+        break;
+      }
+      // replace r32 by the true residual, x32 = 0, and finish the pending
+      // direction update p = D^-1 r + beta p with the replaced residual
+      ICC_HIP(launch_reliable_replace(n, w->r, w->dinv32, w->r32, w->x32, w->st, w->parts, st));
+      ICC_HIP(launch_pdir_update(n, w->st, w->p32, w->r32, w->dinv32, w->ap32, st));
+    }
+  }
+  if (stats) {
+    stats[0] = n_upd;
+    stats[1] = (double)iters;
+    stats[2] = res_true;
+    stats[3] = n_app64;
+    stats[4] = (double)iters;
+    stats[5] = stats[6] = stats[7] = 0.0;
+  }
+  if (rc_final == ICC_ENOCONV) set_error("icc_solve_mixed: iteration limit reached");
+  return rc_final;
+}
+
+namespace icc {
+// Host-pointer wrapper of a device solve: copies f and the initial guess u to
+// temporaries, runs `solve` on them and copies u back when the solve ended
+// with a result (converged, iteration limit or breakdown).
+template <typename F>
+static int host_solve(icc_op* op, const double* f, double* u, F&& solve) {
   const int64_t n = op->g->s.ndof;
   hipStream_t st = op->g->ctx->stream;
   double *df = nullptr, *du = nullptr;
@@ -736,17 +883,39 @@ ICC_API int icc_solve_ir(icc_op* op, int inner, const double* f, double tol, dou
   int rc = ICC_OK;
   if (hipMemcpyAsync(df, f, n * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(du, u, n * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+    set_error("host_solve: copy to device failed");
     rc = ICC_EHIP;
   } else {
-    rc = icc_solve_ir_dev(op, inner, df, tol, inner_tol, max_outer, max_inner, du, stats);
+    rc = solve(df, du);
     if (rc == ICC_OK || rc == ICC_ENOCONV || rc == ICC_EBREAKDOWN) {
-      if (hipMemcpyAsync(u, du, n * 8, hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      if (hipMemcpyAsync(u, du, n * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
+        set_error("host_solve: copy to host failed");
         rc = ICC_EHIP;
+      }
     }
   }
+  (void)hipStreamSynchronize(st);
   hipFree(df);
   hipFree(du);
   return rc;
+}
+}  // namespace icc
+
+ICC_API int icc_solve_ir(icc_op* op, int inner, const double* f, double tol, double inner_tol, int64_t max_outer,
+                         int64_t max_inner, double* u, double* stats) {
+  if (!op || !f || !u) return ICC_EINVAL;
+  return host_solve(op, f, u, [&](const double* df, double* du) {
+    return icc_solve_ir_dev(op, inner, df, tol, inner_tol, max_outer, max_inner, du, stats);
+  });
+}
+
+ICC_API int icc_solve_mixed(icc_op* op, const double* f, double tol, double delta, int64_t maxit, double* u,
+                            double* stats) {
+  if (!op || !f || !u) return ICC_EINVAL;
+  return host_solve(op, f, u, [&](const double* df, double* du) {
+    return icc_solve_mixed_dev(op, df, tol, delta, maxit, du, stats);
+  });
 }
 
 // ---------------------------------------------------------------------------

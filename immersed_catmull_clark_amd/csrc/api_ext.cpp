@@ -195,7 +195,7 @@ ICC_API int icc_topopt_create(icc_ctx* ctx, const icc_topopt_config* cfg, icc_to
   if ((rc = dalloc(&t->x, ne)) || (rc = dalloc(&t->xt, ne)) || (rc = dalloc(&t->s, ne)) || (rc = dalloc(&t->u, n)) ||
       (rc = dalloc(&t->u0, n)) || (rc = dalloc(&t->b, n)) || (rc = dalloc(&t->du, n)) || (rc = dalloc(&t->dg, n)) ||
       (rc = dalloc(&t->f, n)) || (rc = dalloc(&t->dc, ne)) || (rc = dalloc(&t->tmp, n)) ||
-      (rc = dalloc(&t->ones, ne)) || (rc = dalloc(&t->dv, ne)) || (rc = dalloc(&t->parts, 2048)) ||
+      (rc = dalloc(&t->ones, ne)) || (rc = dalloc(&t->dv, ne)) || (rc = dalloc(&t->parts, 2 * VEC_MAX_BLOCKS)) ||
       (rc = dalloc(&t->fr_d, n)) || (rc = dalloc(&t->st, 1))) {
     icc_topopt_destroy(t);
     return rc;

@@ -17,17 +17,22 @@ struct PcgState {
   double rr;      // r.r (squared |r| after residual_kernel)
   double pap;     // p.Ap of the current iteration
   double tol;     // host-set relative tolerance
-  int32_t iters;
-  int32_t done;   // 0 running, 1 converged, 2 breakdown
+  int32_t iters;   // completed iterations (written by pcg_update2)
+  int32_t done;    // 0 running, 1 converged, 2 breakdown, 3 iteration limit
   uint32_t ticket;
+  int32_t maxit;   // host-set iteration limit
+  int32_t it_cur;  // iteration in progress (written by pcg_update1)
   int32_t pad;
 };
 
+constexpr int VEC_MAX_BLOCKS = 4096;
+
 // k_apply.hip
+// pst != null: the launch does nothing when pst->done is set
 hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x, double* y, double* pap,
-                            const int32_t* done, hipStream_t st);
+                            const PcgState* pst, hipStream_t st);
 hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
-                            const int32_t* done, hipStream_t st);
+                            const PcgState* pst, hipStream_t st);
 hipError_t launch_energies(const icc_grid* g, const double* u, double* ee, const double* x, double penal,
                            double emin, const double* w, double* dc, hipStream_t st);
 hipError_t launch_diag_f64(const icc_grid* g, const double* scale, double* d, hipStream_t st);
@@ -43,10 +48,13 @@ hipError_t launch_inv_diag(const double* d, T* dinv, int64_t n, double clamp_min
 template <typename T, typename TB>
 hipError_t launch_pcg_init(int64_t n, const TB* b, double bscale, const T* dinv, T* x, T* r, T* p, T* ap,
                            PcgState* st, double* parts, hipStream_t s);
-hipError_t launch_pap_finalize(const double* pap_parts, int npap, PcgState* st, hipStream_t s);
 template <typename T>
-hipError_t launch_pcg_update(int64_t n, int it, PcgState* st, T* p, T* ap, T* x, T* r, const T* dinv, double* parts,
-                             hipStream_t s);
+hipError_t launch_pcg_update(int64_t n, PcgState* st, const double* pap_parts, int npap, T* p, T* ap, T* x, T* r,
+                             const T* dinv, double* parts, hipStream_t s);
+hipError_t launch_pdir_update(int64_t n, const PcgState* st, float* p, const float* r, const float* dinv, float* ap,
+                              hipStream_t s);
+hipError_t launch_reliable_replace(int64_t n, const double* r64, const float* dinv, float* r, float* x, PcgState* st,
+                                   double* parts, hipStream_t s);
 hipError_t launch_residual(int64_t n, const double* f, const double* ap, const uint8_t* freem, double* r,
                            PcgState* st, double* parts, hipStream_t s);
 template <typename T>

@@ -25,6 +25,7 @@
 #include <cstdint>
 
 #include "icc_internal.hpp"
+#include "icc_kernels.hpp"
 
 namespace icc {
 
@@ -107,12 +108,12 @@ __global__ __launch_bounds__(384) void apply_tiles(const Tile* __restrict__ tile
                                                    const T* __restrict__ x, T* __restrict__ y,
                                                    const T* __restrict__ scale, const uint8_t* __restrict__ freem,
                                                    int ncp0, int ncp1, int ndof_sc, int nel0, int nel1,
-                                                   double* __restrict__ pap, const int32_t* __restrict__ done) {
+                                                   double* __restrict__ pap, const PcgState* st) {
   __shared__ T xs[3 * TILE_MAX_PTS];
   __shared__ T ys[3 * TILE_MAX_PTS];
   __shared__ T ss[TILE_MAX_EL];
   __shared__ double red[6];
-  if (done != nullptr && *done != 0) return;  // solver already finished
+  if (st != nullptr && st->done != 0) return;  // solver already finished
   const Tile t = tiles[blockIdx.x];
   const int px = t.nx + 2, py = t.ny + 2, pz = t.nz + 2;
   const int npts = px * py * pz;
@@ -371,20 +372,20 @@ __global__ void mfma_layout_probe(double* out) {
 
 // ---- launchers ---------------------------------------------------------------
 hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x, double* y, double* pap,
-                            const int32_t* done, hipStream_t st) {
+                            const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
   hipLaunchKernelGGL((apply_tiles<double>), dim3(g->ntiles), dim3(384), 0, st, g->d_tiles, g->d_kfrag64, x, y,
                      op->d_scale, op->d_free, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0],
-                     (int)s.nel[1], pap, done);
+                     (int)s.nel[1], pap, pst);
   return hipGetLastError();
 }
 
 hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
-                            const int32_t* done, hipStream_t st) {
+                            const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
   hipLaunchKernelGGL((apply_tiles<float>), dim3(g->ntiles), dim3(384), 0, st, g->d_tiles, g->d_kfrag32, x, y,
                      op->d_scale32, op->d_free, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0],
-                     (int)s.nel[1], pap, done);
+                     (int)s.nel[1], pap, pst);
   return hipGetLastError();
 }
 

@@ -1,16 +1,20 @@
 // Jacobi-PCG vector kernels, FP64 and FP32, with device-resident scalars.
 //
-// One PCG iteration = apply (p -> ap, fused per-tile partials of p.Ap) +
-// pcg_update1 (alpha, x += alpha p, r -= alpha ap, partials of r.r and r.D^-1 r)
-// + pcg_update2 (convergence test, beta, p = D^-1 r + beta p, ap = 0).  Every
-// kernel checks the device `done` flag first, so the host can enqueue
-// iterations in batches and read the state back once per batch.
+// One PCG iteration = apply (p -> ap, per-tile partials of p.Ap) + pcg_update1
+// (alpha, x += alpha p, r -= alpha ap, partials of r.r and r.D^-1 r) +
+// pcg_update2 (convergence test, beta, p = D^-1 r + beta p, ap = 0).  The
+// kernels take the iteration index from the device state and return at once
+// when its `done` flag is set, so a batch of iterations is captured once as a
+// HIP graph and replayed until the solver stops.
 //
-// Reductions are deterministic: each block writes one partial, and the last
-// block to finish (atomic ticket after a release fence) sums the partials in
-// index order.  Partial sums are always accumulated in FP64.  The FP64
-// residual norms of the refinement loop use compensated Dot2 accumulation
-// (TwoProd by fma, TwoSum; Ogita, Rump and Oishi 2005).
+// Reductions are deterministic and fence-free inside the iteration: each
+// block writes one partial and every block of the consuming kernel re-sums
+// all partials in the same fixed order (kernel boundaries publish them).  A
+// device-scope fence per block costs an L2 write-back on the multi-XCD part,
+// so the last-block pattern is kept for the once-per-solve kernels only.
+// Partial sums are accumulated in FP64; the FP64 residual norms of the
+// refinement loops use compensated Dot2 accumulation (TwoProd by fma, TwoSum;
+// Ogita, Rump and Oishi 2005).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -111,24 +115,41 @@ __global__ __launch_bounds__(VB) void pcg_init1(int64_t n, const TB* __restrict_
   }
 }
 
-// Sums the apply's p.Ap partials into st->pap (one block).
-__global__ __launch_bounds__(VB) void pap_finalize(const double* __restrict__ pap_parts, int npap, PcgState* st) {
-  __shared__ double sh[16];
-  if (st->done) return;
-  double s = sum_parts(pap_parts, npap, sh);
-  if (threadIdx.x == 0) st->pap = s;
+// Sum of n partials in a fixed order (each thread a fixed strided subset, then
+// a fixed tree), so every block of a kernel obtains the identical value.
+__device__ __forceinline__ double sum_parts_all(const double* __restrict__ parts, int n, double* sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += parts[i];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+  return s;  // valid on every thread
 }
 
+// alpha = rz / p.Ap (p.Ap summed from the apply's partials), x += alpha p,
+// r -= alpha Ap, per-block partials of r.r and r.D^-1 r.  Only block 0 writes
+// the state (it_cur, breakdown / limit flags).
 template <typename T>
-__global__ __launch_bounds__(VB) void pcg_update1(int64_t n, int it, PcgState* st, const T* __restrict__ p,
-                                                  const T* __restrict__ ap, T* __restrict__ x, T* __restrict__ r,
-                                                  const T* __restrict__ dinv, double* __restrict__ parts) {
+__global__ __launch_bounds__(VB) void pcg_update1(int64_t n, PcgState* st, const double* __restrict__ pap_parts,
+                                                  int npap, const T* __restrict__ p, const T* __restrict__ ap,
+                                                  T* __restrict__ x, T* __restrict__ r, const T* __restrict__ dinv,
+                                                  double* __restrict__ parts) {
   __shared__ double sh[16];
   if (st->done) return;
-  const double pap = st->pap;
+  const int it = st->iters;
+  if (it >= st->maxit) {  // iteration limit
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
+    return;
+  }
+  const double pap = sum_parts_all(pap_parts, npap, sh);
   if (!(pap > 0.0)) {  // breakdown (solver.rs:47-49): stop before updating x
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->iters = it + 1;
+      st->pap = pap;
+      st->it_cur = it;
       st->done = 2;
     }
     return;
@@ -148,30 +169,88 @@ __global__ __launch_bounds__(VB) void pcg_update1(int64_t n, int it, PcgState* s
   if (threadIdx.x == 0) {
     parts[blockIdx.x] = s1;
     parts[gridDim.x + blockIdx.x] = s2;
-  }
-  if (last_block(&st->ticket)) {
-    double rr_t = sum_parts(parts, gridDim.x, sh);
-    double rz_t = sum_parts(parts + gridDim.x, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      st->iters = it + 1;
-      st->rr = rr_t;
-      st->rz[(it + 1) & 1] = rz_t;
-      st->res = sqrt(rr_t) / st->norm_b;
-      if (st->res < st->tol) st->done = 1;
-      st->ticket = 0;
+    if (blockIdx.x == 0) {
+      st->pap = pap;
+      st->it_cur = it;
     }
   }
 }
 
+// Convergence test on |r| / norm_b (solver.rs:57-61), then
+// beta = rz_new / rz_old, p = D^-1 r + beta p, ap = 0.  Block 0 writes the
+// state (iters, rz_new, res, done).
 template <typename T>
-__global__ __launch_bounds__(VB) void pcg_update2(int64_t n, int it, const PcgState* st, T* __restrict__ p,
-                                                  const T* __restrict__ r, const T* __restrict__ dinv,
-                                                  T* __restrict__ ap) {
+__global__ __launch_bounds__(VB) void pcg_update2(int64_t n, PcgState* st, const double* __restrict__ parts,
+                                                  T* __restrict__ p, const T* __restrict__ r,
+                                                  const T* __restrict__ dinv, T* __restrict__ ap) {
+  __shared__ double sh[16];
   if (st->done) return;
-  const T beta = (T)(st->rz[(it + 1) & 1] / st->rz[it & 1]);
+  const int it = st->it_cur;
+  const double rr = sum_parts_all(parts, gridDim.x, sh);
+  const double rz = sum_parts_all(parts + gridDim.x, gridDim.x, sh);
+  const double res = sqrt(rr) / st->norm_b;
+  const bool conv = res < st->tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rr = rr;
+    st->res = res;
+    st->rz[(it + 1) & 1] = rz;
+    st->iters = it + 1;
+    if (conv) st->done = 1;
+  }
+  if (conv) return;
+  const T beta = (T)(rz / st->rz[it & 1]);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     p[i] = dinv[i] * r[i] + beta * p[i];
     ap[i] = T(0);
+  }
+}
+
+// p = D^-1 r + beta p, beta = rz[iters & 1] / rz[(iters - 1) & 1], ap = 0:
+// the direction update the mixed solver completes after a reliable update.
+__global__ __launch_bounds__(VB) void pdir_update_kernel(int64_t n, const PcgState* st, float* __restrict__ p,
+                                                         const float* __restrict__ r, const float* __restrict__ dinv,
+                                                         float* __restrict__ ap) {
+  const int it = st->iters;
+  const float beta = (float)(st->rz[it & 1] / st->rz[(it - 1) & 1]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    p[i] = dinv[i] * r[i] + beta * p[i];
+    ap[i] = 0.0f;
+  }
+}
+
+// Reliable update of the mixed solver: r32 = (float) r64, x32 = 0, and the
+// recurrence scalars restart from the replaced residual:
+// rz[iters & 1] = r.D^-1 r, norm_b = |r|, done = 0.
+__global__ __launch_bounds__(VB) void reliable_replace_kernel(int64_t n, const double* __restrict__ r64,
+                                                              const float* __restrict__ dinv, float* __restrict__ r,
+                                                              float* __restrict__ x, PcgState* st,
+                                                              double* __restrict__ parts) {
+  __shared__ double sh[16];
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float ri = (float)r64[i];
+    r[i] = ri;
+    x[i] = 0.0f;
+    rz += (double)ri * (double)(dinv[i] * ri);
+    rr += (double)ri * (double)ri;
+  }
+  double s1 = block_sum(rz, sh);
+  double s2 = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    parts[blockIdx.x] = s1;
+    parts[gridDim.x + blockIdx.x] = s2;
+  }
+  if (last_block(&st->ticket)) {
+    double rz_t = sum_parts(parts, gridDim.x, sh);
+    double rr_t = sum_parts(parts + gridDim.x, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+      st->rz[st->iters & 1] = rz_t;
+      st->norm_b = fmax(sqrt(rr_t), 1e-300);
+      st->rr = rr_t;
+      st->res = 1.0;
+      st->done = 0;
+      st->ticket = 0;
+    }
   }
 }
 
@@ -342,10 +421,12 @@ hipError_t launch_masked_add(int64_t n, const double* u0, const double* du, cons
   return hipGetLastError();
 }
 
+// Blocks of the vector kernels: two elements per thread, at most VEC_MAX_BLOCKS
+// (the partial arrays hold 2 * VEC_MAX_BLOCKS values).
 int vec_blocks(int64_t n) {
-  int64_t nb = (n + VB * 4 - 1) / (VB * 4);
+  int64_t nb = (n + VB * 2 - 1) / (VB * 2);
   if (nb < 1) nb = 1;
-  if (nb > 1024) nb = 1024;
+  if (nb > VEC_MAX_BLOCKS) nb = VEC_MAX_BLOCKS;
   return (int)nb;
 }
 
@@ -377,23 +458,30 @@ template hipError_t launch_pcg_init<double, double>(int64_t, const double*, doub
 template hipError_t launch_pcg_init<float, double>(int64_t, const double*, double, const float*, float*, float*,
                                                    float*, float*, PcgState*, double*, hipStream_t);
 
-hipError_t launch_pap_finalize(const double* pap_parts, int npap, PcgState* st, hipStream_t s) {
-  hipLaunchKernelGGL(pap_finalize, dim3(1), dim3(VB), 0, s, pap_parts, npap, st);
+template <typename T>
+hipError_t launch_pcg_update(int64_t n, PcgState* st, const double* pap_parts, int npap, T* p, T* ap, T* x, T* r,
+                             const T* dinv, double* parts, hipStream_t s) {
+  int nb = vec_blocks(n);
+  hipLaunchKernelGGL((pcg_update1<T>), dim3(nb), dim3(VB), 0, s, n, st, pap_parts, npap, p, ap, x, r, dinv, parts);
+  hipLaunchKernelGGL((pcg_update2<T>), dim3(nb), dim3(VB), 0, s, n, st, parts, p, r, dinv, ap);
+  return hipGetLastError();
+}
+template hipError_t launch_pcg_update<double>(int64_t, PcgState*, const double*, int, double*, double*, double*,
+                                              double*, const double*, double*, hipStream_t);
+template hipError_t launch_pcg_update<float>(int64_t, PcgState*, const double*, int, float*, float*, float*, float*,
+                                             const float*, double*, hipStream_t);
+
+hipError_t launch_pdir_update(int64_t n, const PcgState* st, float* p, const float* r, const float* dinv, float* ap,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(pdir_update_kernel, dim3(vec_blocks(n)), dim3(VB), 0, s, n, st, p, r, dinv, ap);
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_pcg_update(int64_t n, int it, PcgState* st, T* p, T* ap, T* x, T* r, const T* dinv, double* parts,
-                             hipStream_t s) {
-  int nb = vec_blocks(n);
-  hipLaunchKernelGGL((pcg_update1<T>), dim3(nb), dim3(VB), 0, s, n, it, st, p, ap, x, r, dinv, parts);
-  hipLaunchKernelGGL((pcg_update2<T>), dim3(nb), dim3(VB), 0, s, n, it, st, p, r, dinv, ap);
+hipError_t launch_reliable_replace(int64_t n, const double* r64, const float* dinv, float* r, float* x, PcgState* st,
+                                   double* parts, hipStream_t s) {
+  hipLaunchKernelGGL(reliable_replace_kernel, dim3(vec_blocks(n)), dim3(VB), 0, s, n, r64, dinv, r, x, st, parts);
   return hipGetLastError();
 }
-template hipError_t launch_pcg_update<double>(int64_t, int, PcgState*, double*, double*, double*, double*,
-                                              const double*, double*, hipStream_t);
-template hipError_t launch_pcg_update<float>(int64_t, int, PcgState*, float*, float*, float*, float*, const float*,
-                                             double*, hipStream_t);
 
 hipError_t launch_residual(int64_t n, const double* f, const double* ap, const uint8_t* freem, double* r,
                            PcgState* st, double* parts, hipStream_t s) {

@@ -119,6 +119,17 @@ int icc_solve_ir_dev(icc_op* op, int inner, const double* f_dev, double tol, dou
 int icc_solve_ir(icc_op* op, int inner, const double* f, double tol, double inner_tol, int64_t max_outer,
                  int64_t max_inner, double* u, double* stats);
 
+/* Mixed precision with reliable updates: FP32 Jacobi PCG (FP32 MFMA apply,
+ * FP32 vectors, FP64 reductions) whose residual is replaced by the FP64 true
+ * residual f - A u (compensated norms) and whose correction is folded into the
+ * FP64 solution each time the FP32 residual drops by `delta`; the search
+ * direction is kept across updates.  Stops when the TRUE relative residual
+ * <= tol.  stats (may be NULL), 8 doubles: reliable updates, FP32 iterations,
+ * final true residual, FP64 applies, FP32 applies, 0, 0, 0. */
+int icc_solve_mixed_dev(icc_op* op, const double* f_dev, double tol, double delta, int64_t maxit, double* u_dev,
+                        double* stats);
+int icc_solve_mixed(icc_op* op, const double* f, double tol, double delta, int64_t maxit, double* u, double* stats);
+
 /* ---- sensitivities --------------------------------------------------------- */
 /* ee_e = u_e^T K_t(e) u_e, unscaled (topopt_iga.rs:165-184) */
 int icc_energies(icc_op* op, const double* u, double* ee);

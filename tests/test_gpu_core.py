@@ -194,6 +194,38 @@ def test_iterative_refinement_true_residual(ctx, inner):
     assert st["true_res"] <= 1e-8
 
 
+@pytest.mark.parametrize("delta", [0.1, 1e-3])
+def test_mixed_reliable_updates_true_residual(ctx, delta):
+    """FP32 recurrence with FP64 residual replacement: the true residual of the
+    returned u, recomputed by the oracle's assembled FP64 operator, meets 1e-8,
+    and u agrees with the FP64 PCG solution to the conditioning of the solve."""
+    nel = [16, 8, 8]
+    bounds, sp, s, free, f = _cantilever_problem(nel, 14)
+    sp.assemble(s)
+    g = icc.Grid(ctx, nel, bounds)
+    op = icc.Operator(g, scale=s, free=free)
+    u, st, rc = op.solve_mixed(f, tol=1e-8, delta=delta)
+    assert rc == icc.ICC_OK, st
+    r = free * (f - sp.matvec(free * u))
+    assert np.linalg.norm(r) / np.linalg.norm(free * f) <= 1.01e-8
+    assert st["true_res"] <= 1e-8 and st["updates"] >= 1
+    u64, it, res, rc64 = op.pcg(f, np.maximum(op.diag(), 1e-12), 1e-10, 20 * op.ndof)
+    assert rc64 == icc.ICC_OK
+    assert O.rel_err(u, u64) < 1e-5
+    # a converged start returns at once with no FP32 iterations
+    u2, st2, rc2 = op.solve_mixed(f, tol=1e-6, delta=delta, u0=u)
+    assert rc2 == icc.ICC_OK and st2["inner_iters"] == 0 and st2["updates"] == 0
+
+
+def test_mixed_iteration_limit(ctx):
+    nel = [8, 4, 4]
+    bounds, sp, s, free, f = _cantilever_problem(nel, 15)
+    g = icc.Grid(ctx, nel, bounds)
+    op = icc.Operator(g, scale=s, free=free)
+    u, st, rc = op.solve_mixed(f, tol=1e-14, delta=0.5, maxit=7)
+    assert rc == icc.ICC_ENOCONV and st["inner_iters"] == 7
+
+
 def test_apply_properties_config1_size(ctx):
     """Size-independent properties at the 96x48x48 grid: linearity, symmetry,
     rigid-body null space, FP32 vs FP64."""

@@ -65,7 +65,10 @@ def test_solve_state_matches_oracle(ctx):
 @pytest.mark.parametrize("filt", [0, 1])
 def test_adjoint_vs_finite_differences(ctx, filt):
     """dc/dx from the hand-written sensitivity kernel (+ density filter
-    backward) vs central differences of c(x), h = 1e-6, each solve to 1e-13."""
+    backward) vs central differences of c(x), h = 1e-4, each solve to 1e-13.
+    With c ~ 137 the difference quotient at h = 1e-6 carries round-off of a
+    few 1e-6 relative even with exact direct solves; at h = 1e-4 the
+    truncation and round-off errors are both below 1e-7."""
     cfg = icc.CantileverTopOptConfig(nel=(12, 6, 6), size=(2.0, 1.0, 1.0), emin=1e-9, rmin=1.5, pcg_tol=1e-13,
                                      load=1, filter=filt)
     t = icc.CantileverTopOpt(ctx, cfg)
@@ -85,7 +88,7 @@ def test_adjoint_vs_finite_differences(ctx, filt):
         return math.fsum(terms)
 
     assert abs(c_hat(x, u) - c) < 1e-10 * c
-    hstep = 1e-6
+    hstep = 1e-4
     for e in rng.choice(t.nel, 8, replace=False):
         xp, xm = x.copy(), x.copy()
         xp[e] += hstep
