@@ -9,10 +9,16 @@ Emin = 1e-9, x = 0 clamped, unit downward traction on x = 2.
 A step is one masked K.u apply (FP64 MFMA contraction) over the whole grid
 with x and y resident in HBM; value = applies per second.  The same JSON
 line carries the FP32 apply rate, the PCG time-to-solution (true relative
-residual 1e-8, FP64 operator) of plain FP64 PCG and of FP32-inner
-iterative refinement with the errors of the latter against the former, the
+residual 1e-8, FP64 operator) of plain FP64 PCG and of the mixed solver
+(FP32 PCG with FP64 reliable updates) with the errors of the latter against
+the former, the
 roofline of the FP64 apply kernel, and the CPU baseline (the oracle's
 restatement of the reference's rayon CSR SpMV, on this host's cores).
+
+Profiling modes: --apply-only times only the apply in --apply-prec;
+--copy-calibration runs a plain device copy of 735,000 doubles 20 times, the
+known-size kernel against which the FETCH_SIZE / WRITE_SIZE counters are
+scaled.
 """
 import argparse
 import json
@@ -119,12 +125,25 @@ def main():
     ap.add_argument("--no-pcg", action="store_true", help="skip the PCG time-to-solution legs")
     ap.add_argument("--delta", type=float, default=0.1, help="FP32 residual drop between reliable updates")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU SpMV timing")
-    ap.add_argument("--apply-only", action="store_true", help="time only the FP64 apply (for profilers)")
+    ap.add_argument("--apply-only", action="store_true", help="time only the apply (for profilers)")
+    ap.add_argument("--apply-prec", choices=("f64", "f32"), default="f64", help="precision of --apply-only")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain device copy of 735,000 doubles 20 times and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this implementation runs on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = 735000
+        src = torch.randn(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(20):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 20}), flush=True)
+        return
 
     import numpy as np
     import torch
@@ -144,6 +163,11 @@ def main():
     torch.cuda.synchronize()
 
     flops = 2.0 * 81 * 81 * nel
+    if args.apply_only and args.apply_prec == "f32":
+        ms32 = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F32, args.steps, args.warmup, accumulate=True)
+        print(json.dumps({"kernel": "apply_tiles<float>", "kernel_ms": ms32,
+                          "tflops": flops / (ms32 * 1e-3) / 1e12}), flush=True)
+        return
     log("bench: FP64 applies")
     t0 = time.time()
     ms_f64 = time_applies(torch, icc, ctx, op, x, y, icc.ICC_F64, args.steps, args.warmup)

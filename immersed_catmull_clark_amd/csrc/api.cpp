@@ -355,6 +355,7 @@ ICC_API int icc_op_destroy(icc_op* op) {
   hipFree(op->d_scale);
   hipFree(op->d_scale32);
   hipFree(op->d_free);
+  hipFree(op->d_tmask);
   hipFree(op->d_s_indptr);
   hipFree(op->d_s_indices);
   hipFree(op->d_s_data);
@@ -388,11 +389,32 @@ ICC_API int icc_op_set_free_mask(icc_op* op, const uint8_t* fm) {
   const int64_t n = op->g->s.ndof;
   if (!fm) {
     hipFree(op->d_free);
+    hipFree(op->d_tmask);
     op->d_free = nullptr;
+    op->d_tmask = nullptr;
     return ICC_OK;
   }
+  // per tile: does any staged point carry a fixed DOF?  Only such tiles read
+  // the mask inside the apply kernel.
+  const auto& s = op->g->s;
+  std::vector<uint8_t> tm(s.tiles.size(), 0);
+  for (size_t ti = 0; ti < s.tiles.size(); ++ti) {
+    const Tile& t = s.tiles[ti];
+    bool fixed = false;
+    for (int c = 0; c < 3 && !fixed; ++c)
+      for (int64_t k = t.k0; k < t.k0 + t.nz + 2 && !fixed; ++k)
+        for (int64_t j = t.j0; j < t.j0 + t.ny + 2 && !fixed; ++j)
+          for (int64_t i = t.i0; i < t.i0 + t.nx + 2; ++i)
+            if (!fm[c * s.ndof_sc + i + s.ncp[0] * (j + s.ncp[1] * k)]) {
+              fixed = true;
+              break;
+            }
+    tm[ti] = fixed ? 1 : 0;
+  }
   if (!op->d_free) ICC_TRY(dalloc(&op->d_free, n));
+  if (!op->d_tmask) ICC_TRY(dalloc(&op->d_tmask, (int64_t)tm.size()));
   ICC_HIP(hipMemcpyAsync(op->d_free, fm, n, hipMemcpyHostToDevice, op->g->ctx->stream));
+  ICC_HIP(hipMemcpyAsync(op->d_tmask, tm.data(), tm.size(), hipMemcpyHostToDevice, op->g->ctx->stream));
   ICC_HIP(hipStreamSynchronize(op->g->ctx->stream));
   return ICC_OK;
 }

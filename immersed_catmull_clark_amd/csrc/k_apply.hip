@@ -94,8 +94,91 @@ __device__ __forceinline__ double row_shr(double v) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
-// y += M * sum_e s_e K_t x_e (M x) over one tile per workgroup; optional
-// per-tile partial of sum_e s_e x_e^T K x_e into pap[blockIdx.x].
+// Exact quotient n / d for n < 2^12 and 3 <= d < 2^12 with m = magic(d):
+// floor(n * m / 2^32), m = ceil(2^32 / d) (the error n (m - 2^32/d) / 2^32
+// stays below 1/d).
+__host__ __device__ __forceinline__ uint32_t magic_div(uint32_t d) { return 0xffffffffu / d + 1u; }
+__device__ __forceinline__ int qdiv(int n, uint32_t m) { return (int)__umulhi((uint32_t)n, m); }
+
+// Geometry of a staged tile: points [c][lz][ly][lx] of extents px, py, pz.
+struct TileGeom {
+  int px, py, npts, n3, nel;
+  uint32_t m_npts, m_px, m_py, m_nx, m_ny;
+  __device__ __forceinline__ void init(const Tile& t) {
+    px = t.nx + 2;
+    py = t.ny + 2;
+    npts = px * py * (t.nz + 2);
+    n3 = 3 * npts;
+    nel = t.nx * t.ny * t.nz;
+    m_npts = magic_div((uint32_t)npts);
+    m_px = magic_div((uint32_t)px);
+    m_py = magic_div((uint32_t)py);
+    m_nx = t.nx >= 3 ? magic_div((uint32_t)t.nx) : 0u;
+    m_ny = t.ny >= 3 ? magic_div((uint32_t)t.ny) : 0u;
+  }
+  // global DOF of staged slot idx < n3
+  __device__ __forceinline__ int dof(const Tile& t, int idx, int ncp0, int ncp1, int ndof_sc) const {
+    const int c = qdiv(idx, m_npts);
+    const int p = idx - c * npts;
+    const int r = qdiv(p, m_px);
+    const int lx = p - r * px;
+    const int lz = qdiv(r, m_py);
+    const int ly = r - lz * py;
+    return (t.i0 + lx) + ncp0 * ((t.j0 + ly) + ncp1 * (t.k0 + lz)) + c * ndof_sc;
+  }
+  // global element of tile element el < nel
+  __device__ __forceinline__ int elem(const Tile& t, int el, int nel0, int nel1) const {
+    const int r = t.nx >= 3 ? qdiv(el, m_nx) : el / t.nx;
+    const int ex = el - r * t.nx;
+    const int ez = t.ny >= 3 ? qdiv(r, m_ny) : r / t.ny;
+    const int ey = r - ez * t.ny;
+    return (t.i0 + ex) + nel0 * ((t.j0 + ey) + nel1 * (t.k0 + ez));
+  }
+};
+
+constexpr int APPLY_THREADS = 384;
+constexpr int BCH = 4;                        // K steps per B-operand chunk
+constexpr int NBCH = (KS + BCH - 1) / BCH;    // chunks per element group
+// offset table of a tile: [kq][ks] trial offsets (4 x KS), then [wave][kq][r]
+// output-row offsets (6 x 4 x 4)
+constexpr int OTAB = 4 * KS + MT * 16;
+
+// Fills the offset table of a tile (threads 0 .. OTAB-1).  Entries for the
+// padding trial columns and rows point at offset 0 (their A entries are zero
+// and padding rows are never stored).
+__device__ __forceinline__ void offset_table(const TileGeom& G, int* tab) {
+  const int i = threadIdx.x;
+  if (i < 4 * KS) {
+    const int kq = i / KS, ks = i - KS * kq, b = 4 * ks + kq;
+    tab[i] = b < NSH ? local_offset(b, G.px, G.py, G.npts) : 0;
+  } else if (i < OTAB) {
+    const int j = i - 4 * KS, w = j >> 4, kq = (j >> 2) & 3, r = j & 3;
+    const int row = perm_row(w, kq, r);
+    tab[i] = row >= 0 ? local_offset(row, G.px, G.py, G.npts) : 0;
+  }
+}
+constexpr int NPF = (3 * TILE_MAX_PTS + APPLY_THREADS - 1) / APPLY_THREADS;  // staged slots per thread
+
+// Tiles [first, last) of workgroup b among G: contiguous chunks, the chunks of
+// the workgroups that share an XCD (b mod 8) adjacent, so that neighbouring
+// tiles and their shared halo points meet in one L2.
+__device__ __forceinline__ void tile_range(int ntiles, int& first, int& last) {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int c = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
+  first = (int)(((long long)c * ntiles) / G);
+  last = (int)(((long long)(c + 1) * ntiles) / G);
+}
+
+// y += M * sum_e s_e K_t x_e (M x); optional per-tile partials of
+// sum_e s_e x_e^T K x_e into pap[tile].
+//
+// Persistent workgroups, each walking a contiguous chunk of tiles with a
+// two-stage software pipeline: while the MFMA loop runs on tile i (LDS
+// buffer b), the masked control points, free flags and scales of tile i+1
+// are already in flight into registers; after the loop they are written to
+// buffer b^1, and tile i's output points are flushed from LDS to y with
+// global atomic adds (y must hold the sum to accumulate into).  The A
+// fragments are reloaded only when the element type changes.
 //
 // Each wave runs two element groups at a time as independent MFMA chains.
 // In line mode a group is one x-line of the tile (lane column = x offset);
@@ -104,139 +187,215 @@ __device__ __forceinline__ double row_shr(double v) {
 // control point of the line receives are summed in registers with two DPP
 // row shifts before a single ds_add per point.
 template <typename T>
-__global__ __launch_bounds__(384) void apply_tiles(const Tile* __restrict__ tiles, const T* __restrict__ kfrag,
-                                                   const T* __restrict__ x, T* __restrict__ y,
-                                                   const T* __restrict__ scale, const uint8_t* __restrict__ freem,
-                                                   int ncp0, int ncp1, int ndof_sc, int nel0, int nel1,
-                                                   double* __restrict__ pap, const PcgState* st) {
-  __shared__ T xs[3 * TILE_MAX_PTS];
-  __shared__ T ys[3 * TILE_MAX_PTS];
-  __shared__ T ss[TILE_MAX_EL];
+__global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void apply_tiles(
+    const Tile* __restrict__ tiles, int ntiles, const T* __restrict__ kfrag, const T* __restrict__ x,
+    T* __restrict__ y, const T* __restrict__ scale, const uint8_t* __restrict__ freem,
+    const uint8_t* __restrict__ tmask, int ncp0, int ncp1, int ndof_sc, int nel0, int nel1,
+    double* __restrict__ pap, const PcgState* st) {
+  __shared__ T xs[2][3 * TILE_MAX_PTS];
+  // output points accumulate in FP64 for both precisions: ds_add_f64 runs at
+  // full rate, while FP32 LDS atomics back up the LDS data FIFO on gfx950
+  __shared__ double ys[3 * TILE_MAX_PTS];
+  __shared__ T ss[2][TILE_MAX_EL];
+  __shared__ int otab[2][OTAB];  // per tile: LDS offsets of the B operands and of the output rows
   __shared__ double red[6];
   if (st != nullptr && st->done != 0) return;  // solver already finished
-  const Tile t = tiles[blockIdx.x];
-  const int px = t.nx + 2, py = t.ny + 2, pz = t.nz + 2;
-  const int npts = px * py * pz;
-  const int nel = t.nx * t.ny * t.nz;
-  stage_tile<T>(t, x, freem, ncp0, ncp1, ndof_sc, xs, ys, px, py, npts);
-  for (int el = threadIdx.x; el < nel; el += blockDim.x) {
-    const int ex = el % t.nx, rr = el / t.nx, ey = rr % t.ny, ez = rr / t.ny;
-    ss[el] = scale[(t.i0 + ex) + nel0 * ((t.j0 + ey) + nel1 * (t.k0 + ez))];
-  }
+  int first, last;
+  tile_range(ntiles, first, last);
+  if (first >= last) return;
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int col = lane & 15;
   const int kq = lane >> 4;
 
-  // A fragments of this wave's M tile, and the trial offsets of this lane
-  T a[KS];
-  int boff[KS];
-  const T* kf = kfrag + ((size_t)(t.type * MT + wave) * KS) * 64 + lane;
-#pragma unroll
-  for (int ks = 0; ks < KS; ++ks) {
-    a[ks] = kf[ks * 64];
-    int b = 4 * ks + kq;
-    boff[ks] = b < NSH ? local_offset(b, px, py, npts) : 0;  // A is zero there
+  // prologue: stage the first tile, clear the output buffer
+  Tile t = tiles[first];
+  TileGeom G;
+  G.init(t);
+  {
+    const bool masked = tmask != nullptr && tmask[first] != 0;
+    for (int idx = threadIdx.x; idx < 3 * TILE_MAX_PTS; idx += APPLY_THREADS) {
+      ys[idx] = 0.0;
+      if (idx < G.n3) {
+        const int g = G.dof(t, idx, ncp0, ncp1, ndof_sc);
+        xs[0][idx] = (!masked || freem[g] != 0) ? x[g] : T(0);
+      }
+    }
   }
-  int roff[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = perm_row(wave, kq, r);
-    roff[r] = row >= 0 ? local_offset(row, px, py, npts) : 0;
-  }
-  const bool r3 = perm_row(wave, kq, 3) >= 0;
-  __syncthreads();
+  if (threadIdx.x < G.nel) ss[0][threadIdx.x] = scale[G.elem(t, threadIdx.x, nel0, nel1)];
+  offset_table(G, otab[0]);
 
-  double pap_acc = 0.0;
-  const bool lines = t.lines != 0;
-  const int ngroups = lines ? t.ny * t.nz : (nel + 15) / 16;
-  for (int g0 = 0; g0 < ngroups; g0 += 2) {
-    int eb[2], el[2];
-    bool valid[2];
+  T a[KS];
+  int cur_type = -1;
+  int buf = 0;
+  for (int ti = first; ti < last; ++ti) {
+    const bool has_next = ti + 1 < last;
+    const Tile tn = tiles[has_next ? ti + 1 : ti];
+    TileGeom Gn;
+    Gn.init(tn);
+    // tiles whose staged points include fixed DOFs (few) read the free mask
+    const bool mask_c = tmask != nullptr && tmask[ti] != 0;
+    const bool mask_n = tmask != nullptr && has_next && tmask[ti + 1] != 0;
+    if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile
+      const T* kf = kfrag + ((size_t)(t.type * MT + wave) * KS) * 64 + lane;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int g = g0 + q;
-      if (lines) {
-        const int ey = g % t.ny, ez = g / t.ny;
-        valid[q] = g < ngroups && col < t.nx;
-        el[q] = col + t.nx * g;
-        eb[q] = valid[q] ? col + px * (ey + py * ez) : 0;
-      } else {
-        const int e = 16 * g + col;
-        valid[q] = g < ngroups && e < nel;
-        int ex = 0, ey = 0, ez = 0;
-        if (valid[q]) {
-          ex = e % t.nx;
-          const int rr = e / t.nx;
-          ey = rr % t.ny;
-          ez = rr / t.ny;
-        }
-        el[q] = e;
-        eb[q] = ex + px * (ey + py * ez);
-      }
+      for (int ks = 0; ks < KS; ++ks) a[ks] = kf[ks * 64];
+      cur_type = t.type;
     }
-    typename Mfma<T>::acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+    // prefetch of the next tile's points (issued unconditionally; unused
+    // slots read slot 0 of the tile, which always exists)
+    T pv[NPF];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      acc0 = Mfma<T>::run(a[ks], xs[eb[0] + boff[ks]], acc0);
-      acc1 = Mfma<T>::run(a[ks], xs[eb[1] + boff[ks]], acc1);
+    for (int j = 0; j < NPF; ++j) {
+      const int idx = threadIdx.x + j * APPLY_THREADS;
+      pv[j] = x[Gn.dof(tn, idx < Gn.n3 ? idx : 0, ncp0, ncp1, ndof_sc)];
     }
+    const T psc = scale[Gn.elem(tn, threadIdx.x < Gn.nel ? threadIdx.x : 0, nel0, nel1)];
+
+    const int px = G.px, py = G.py, nel = G.nel;
+    const bool r3 = perm_row(wave, kq, 3) >= 0;
+    const T* xb = xs[buf];
+    const T* sb = ss[buf];
+    __syncthreads();  // xs[buf], ss[buf], otab[buf] staged; ys clear
+    int boff[KS];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const typename Mfma<T>::acc_t& acc = q == 0 ? acc0 : acc1;
-      const T s = valid[q] ? ss[el[q]] : T(0);
-      const T v0 = acc[0] * s, v1 = acc[1] * s, v2 = acc[2] * s, v3 = acc[3] * s;
-      if (pap != nullptr && valid[q]) {
-        pap_acc += (double)xs[eb[q] + roff[0]] * (double)v0 + (double)xs[eb[q] + roff[1]] * (double)v1 +
-                   (double)xs[eb[q] + roff[2]] * (double)v2;
-        if (r3) pap_acc += (double)xs[eb[q] + roff[3]] * (double)v3;
-      }
-      if (lines) {
-        // point x = col of the line receives v0[col] + v1[col-1] + v2[col-2];
-        // the DPP shifts run on all lanes, before any lane-dependent branch
-        const T comb = v0 + row_shr<1>(v1) + row_shr<2>(v2);
-        const T e16 = v1 + row_shr<1>(v2);  // at lane 15: point 16 = v1[15] + v2[14]
+    for (int ks = 0; ks < KS; ++ks) boff[ks] = otab[buf][kq * KS + ks];
+    int roff[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) roff[r] = otab[buf][4 * KS + (wave * 4 + kq) * 4 + r];
+
+    const bool lines = t.lines != 0;
+    const int ngroups = lines ? t.ny * t.nz : (nel + 15) / 16;
+    for (int g0 = 0; g0 < ngroups; g0 += 2) {
+      int eb[2];
+      bool valid[2];
+      T sc[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
         const int g = g0 + q;
-        if (g < ngroups) {
+        int e;
+        if (lines) {
           const int ey = g % t.ny, ez = g / t.ny;
-          const int lb = px * (ey + py * ez) + roff[0];  // point x = 0 of this line, ax = 0 row
-          if (col <= t.nx + 1) unsafeAtomicAdd(&ys[lb + col], comb);
-          if (col == 15) {  // points 16 and 17 exist only for lines of 15 and 16 elements
-            if (t.nx >= 15) unsafeAtomicAdd(&ys[lb + 16], e16);
-            if (t.nx >= 16) unsafeAtomicAdd(&ys[lb + 17], v2);
+          valid[q] = g < ngroups && col < t.nx;
+          e = col + t.nx * g;
+          eb[q] = valid[q] ? col + px * (ey + py * ez) : 0;
+        } else {
+          e = 16 * g + col;
+          valid[q] = g < ngroups && e < nel;
+          int ex = 0, ey = 0, ez = 0;
+          if (valid[q]) {
+            ex = e % t.nx;
+            const int rr = e / t.nx;
+            ey = rr % t.ny;
+            ez = rr / t.ny;
           }
-          if (r3 && valid[q]) unsafeAtomicAdd(&ys[eb[q] + roff[3]], v3);
+          eb[q] = ex + px * (ey + py * ez);
         }
-      } else if (valid[q]) {
-        unsafeAtomicAdd(&ys[eb[q] + roff[0]], v0);
-        unsafeAtomicAdd(&ys[eb[q] + roff[1]], v1);
-        unsafeAtomicAdd(&ys[eb[q] + roff[2]], v2);
-        if (r3) unsafeAtomicAdd(&ys[eb[q] + roff[3]], v3);
+        // the scale is read before the MFMA chain so that no LDS read sits
+        // between the chain and the scatter below
+        sc[q] = sb[valid[q] ? e : 0];
+        if (!valid[q]) sc[q] = T(0);
+      }
+      // B operands software-pipelined in register chunks of BCH K steps:
+      // the reads of chunk c + 1 are in flight while chunk c runs on MFMA
+      // (left to itself the scheduler issues each read right before its MFMA
+      // and exposes the full LDS latency)
+      typename Mfma<T>::acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+      T bq0[2][BCH], bq1[2][BCH];
+#pragma unroll
+      for (int i = 0; i < BCH; ++i) {
+        bq0[0][i] = xb[eb[0] + boff[i]];
+        bq1[0][i] = xb[eb[1] + boff[i]];
+      }
+#pragma unroll
+      for (int c = 0; c < NBCH; ++c) {
+        const int cur = c & 1;
+        if (c + 1 < NBCH) {
+#pragma unroll
+          for (int i = 0; i < BCH; ++i) {
+            const int ks = (c + 1) * BCH + i;
+            if (ks < KS) {
+              bq0[cur ^ 1][i] = xb[eb[0] + boff[ks]];
+              bq1[cur ^ 1][i] = xb[eb[1] + boff[ks]];
+            }
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < BCH; ++i) {
+          const int ks = c * BCH + i;
+          if (ks < KS) {
+            acc0 = Mfma<T>::run(a[ks], bq0[cur][i], acc0);
+            acc1 = Mfma<T>::run(a[ks], bq1[cur][i], acc1);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const typename Mfma<T>::acc_t& acc = q == 0 ? acc0 : acc1;
+        const T s = sc[q];
+        const T v0 = acc[0] * s, v1 = acc[1] * s, v2 = acc[2] * s, v3 = acc[3] * s;
+        if (lines) {
+          // point x = col of the line receives v0[col] + v1[col-1] + v2[col-2];
+          // the DPP shifts run on all lanes, before any lane-dependent branch
+          const T comb = v0 + row_shr<1>(v1) + row_shr<2>(v2);
+          const T e16 = v1 + row_shr<1>(v2);  // at lane 15: point 16 = v1[15] + v2[14]
+          const int g = g0 + q;
+          if (g < ngroups) {
+            const int ey = g % t.ny, ez = g / t.ny;
+            const int lb = px * (ey + py * ez) + roff[0];  // point x = 0 of this line, ax = 0 row
+            if (col <= t.nx + 1) unsafeAtomicAdd(&ys[lb + col], (double)comb);
+            if (col == 15) {  // points 16 and 17 exist only for lines of 15 and 16 elements
+              if (t.nx >= 15) unsafeAtomicAdd(&ys[lb + 16], (double)e16);
+              if (t.nx >= 16) unsafeAtomicAdd(&ys[lb + 17], (double)v2);
+            }
+            if (r3 && valid[q]) unsafeAtomicAdd(&ys[eb[q] + roff[3]], (double)v3);
+          }
+        } else if (valid[q]) {
+          unsafeAtomicAdd(&ys[eb[q] + roff[0]], (double)v0);
+          unsafeAtomicAdd(&ys[eb[q] + roff[1]], (double)v1);
+          unsafeAtomicAdd(&ys[eb[q] + roff[2]], (double)v2);
+          if (r3) unsafeAtomicAdd(&ys[eb[q] + roff[3]], (double)v3);
+        }
       }
     }
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < 3 * npts; idx += blockDim.x) {
-    int c = idx / npts;
-    int p = idx - c * npts;
-    int lx = p % px;
-    int r = p / px;
-    int ly = r % py;
-    int lz = r / py;
-    int g = (t.i0 + lx) + ncp0 * ((t.j0 + ly) + ncp1 * (t.k0 + lz)) + c * ndof_sc;
-    if (freem == nullptr || freem[g] != 0) unsafeAtomicAdd(&y[g], ys[idx]);
-  }
-  if (pap != nullptr) {
-    // wave reduction, then across the six waves in fixed order
-    for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
-    if (lane == 0) red[wave] = pap_acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < 6; ++w) s += red[w];
-      pap[blockIdx.x] = s;
+    __syncthreads();  // ys complete; xs[buf ^ 1] free
+    // flush this tile (masked) and clear ys; p.Ap partial = sum over the
+    // staged points of (M x)_i times the tile's contribution to (A x)_i, which
+    // summed over tiles is x^T M K M x; then stage the next tile
+    double pap_acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int idx = threadIdx.x + j * APPLY_THREADS;
+      if (idx < G.n3) {
+        const double v = ys[idx];
+        ys[idx] = 0.0;
+        if (pap != nullptr) pap_acc += (double)xb[idx] * v;
+        const int g = G.dof(t, idx, ncp0, ncp1, ndof_sc);
+        if (!mask_c || freem[g] != 0) unsafeAtomicAdd(&y[g], (T)v);
+      }
+      if (has_next && idx < Gn.n3)
+        xs[buf ^ 1][idx] = (!mask_n || freem[Gn.dof(tn, idx, ncp0, ncp1, ndof_sc)] != 0) ? pv[j] : T(0);
     }
+    if (has_next && threadIdx.x < Gn.nel) ss[buf ^ 1][threadIdx.x] = psc;
+    if (has_next) offset_table(Gn, otab[buf ^ 1]);
+    if (pap != nullptr) {
+      // wave sums, then the six waves in fixed order (red is free: the last
+      // reader finished before the barrier above)
+      for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
+      if (lane == 0) red[wave] = pap_acc;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < 6; ++w) sum += red[w];
+        pap[ti] = sum;
+      }
+    }
+    t = tn;
+    G = Gn;
+    buf ^= 1;
   }
 }
 
@@ -371,21 +530,33 @@ __global__ void mfma_layout_probe(double* out) {
 }
 
 // ---- launchers ---------------------------------------------------------------
+// Persistent apply grid: two workgroups per CU (the VGPR budget of the MFMA
+// loop allows twelve waves per CU), a multiple of 8 so the XCD grouping of
+// tile_range applies, never more workgroups than tiles.
+static int apply_grid(const icc_grid* g) {
+  int G = 2 * (g->ctx->num_cu > 0 ? g->ctx->num_cu : 256);
+  G -= G % 8;
+  if (G > g->ntiles) G = g->ntiles;
+  return G < 1 ? 1 : G;
+}
+
 hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x, double* y, double* pap,
                             const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
-  hipLaunchKernelGGL((apply_tiles<double>), dim3(g->ntiles), dim3(384), 0, st, g->d_tiles, g->d_kfrag64, x, y,
-                     op->d_scale, op->d_free, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0],
-                     (int)s.nel[1], pap, pst);
+  hipLaunchKernelGGL((apply_tiles<double>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->ntiles,
+                     g->d_kfrag64, x, y,
+                     op->d_scale, op->d_free, op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc,
+                     (int)s.nel[0], (int)s.nel[1], pap, pst);
   return hipGetLastError();
 }
 
 hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
                             const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
-  hipLaunchKernelGGL((apply_tiles<float>), dim3(g->ntiles), dim3(384), 0, st, g->d_tiles, g->d_kfrag32, x, y,
-                     op->d_scale32, op->d_free, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0],
-                     (int)s.nel[1], pap, pst);
+  hipLaunchKernelGGL((apply_tiles<float>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->ntiles,
+                     g->d_kfrag32, x, y,
+                     op->d_scale32, op->d_free, op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc,
+                     (int)s.nel[0], (int)s.nel[1], pap, pst);
   return hipGetLastError();
 }
 
