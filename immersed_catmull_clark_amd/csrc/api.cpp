@@ -49,7 +49,7 @@ struct Work {
   double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *dinv = nullptr, *tmp = nullptr, *f = nullptr;
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *dinv32 = nullptr;
   double* parts = nullptr;  // 2 * VEC_MAX_BLOCKS vector partials
-  double* pap = nullptr;    // ntiles + csr blocks
+  double* pap = nullptr;    // apply workgroups + csr blocks
   PcgState* st = nullptr;
   PcgState* st_host = nullptr;  // pinned, 2 entries (batch lookahead)
 };
@@ -325,7 +325,7 @@ static int ensure_work(icc_op* op, Work*& out) {
     free_work(*w);
     w->n = n;
     int rc;
-    size_t npap = (size_t)op->g->ntiles + VEC_MAX_BLOCKS;
+    size_t npap = (size_t)apply_grid(op->g) + VEC_MAX_BLOCKS;
     if ((rc = dalloc(&w->x, n)) || (rc = dalloc(&w->r, n)) || (rc = dalloc(&w->p, n)) || (rc = dalloc(&w->ap, n)) ||
         (rc = dalloc(&w->dinv, n)) || (rc = dalloc(&w->tmp, n)) || (rc = dalloc(&w->f, n)) ||
         (rc = dalloc(&w->x32, n)) || (rc = dalloc(&w->r32, n)) || (rc = dalloc(&w->p32, n)) ||
@@ -483,10 +483,10 @@ static hipError_t apply_t(icc_op* op, const T* x, T* y, double* pap) {
   if (op->s_nnz > 0) {
     if constexpr (std::is_same<T, double>::value)
       e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, x, y, op->d_free,
-                                          pap ? pap + g->ntiles : nullptr, st);
+                                          pap ? pap + apply_grid(g) : nullptr, st);
     else
       e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, x, y, op->d_free,
-                                        pap ? pap + g->ntiles : nullptr, st);
+                                        pap ? pap + apply_grid(g) : nullptr, st);
   } else {
     e = launch_fill<T>(y, T(0), n, st);
   }
@@ -604,17 +604,17 @@ static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap
   icc_grid* g = op->g;
   hipStream_t st = g->ctx->stream;
   const int64_t n = g->s.ndof;
-  const int npap = g->ntiles + csr_blocks(op);
+  const int npap = apply_grid(g) + csr_blocks(op);
   hipError_t e = hipSuccess;
   if constexpr (std::is_same<T, double>::value) {
     if (op->s_nnz > 0)
       e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, p, ap, op->d_free,
-                                          w->pap + g->ntiles, st);
+                                          w->pap + apply_grid(g), st);
     if (e == hipSuccess) e = launch_apply_f64(g, op, p, ap, w->pap, w->st, st);
   } else {
     if (op->s_nnz > 0)
       e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, p, ap, op->d_free,
-                                        w->pap + g->ntiles, st);
+                                        w->pap + apply_grid(g), st);
     if (e == hipSuccess) e = launch_apply_f32(g, op, p, ap, w->pap, w->st, st);
   }
   if (e == hipSuccess) e = launch_pcg_update<T>(n, w->st, w->pap, npap, p, ap, x, r, dinv, w->parts, st);

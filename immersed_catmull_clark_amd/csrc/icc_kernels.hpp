@@ -25,10 +25,14 @@ struct PcgState {
   int32_t pad;
 };
 
-constexpr int VEC_MAX_BLOCKS = 4096;
+// Vector kernels use at most VEC_MAX_BLOCKS workgroups: each block of a
+// reduction's consumer re-sums all partials, so few partials keep that cheap.
+constexpr int VEC_MAX_BLOCKS = 512;
 
 // k_apply.hip
 // pst != null: the launch does nothing when pst->done is set
+// workgroups of the persistent apply launch (one p.Ap partial per workgroup)
+int apply_grid(const icc_grid* g);
 hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x, double* y, double* pap,
                             const PcgState* pst, hipStream_t st);
 hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,

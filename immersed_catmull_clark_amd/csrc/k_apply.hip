@@ -169,8 +169,8 @@ __device__ __forceinline__ void tile_range(int ntiles, int& first, int& last) {
   last = (int)(((long long)(c + 1) * ntiles) / G);
 }
 
-// y += M * sum_e s_e K_t x_e (M x); optional per-tile partials of
-// sum_e s_e x_e^T K x_e into pap[tile].
+// y += M * sum_e s_e K_t x_e (M x); optional per-workgroup partials of
+// x^T M K M x into pap[blockIdx.x].
 //
 // Persistent workgroups, each walking a contiguous chunk of tiles with a
 // two-stage software pipeline: while the MFMA loop runs on tile i (LDS
@@ -202,7 +202,11 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
   if (st != nullptr && st->done != 0) return;  // solver already finished
   int first, last;
   tile_range(ntiles, first, last);
-  if (first >= last) return;
+  if (first >= last) {
+    if (pap != nullptr && threadIdx.x == 0) pap[blockIdx.x] = 0.0;
+    return;
+  }
+  double pap_blk = 0.0;  // thread 0: this workgroup's p.Ap partial, tiles in order
 
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -390,7 +394,8 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
       if (threadIdx.x == 0) {
         double sum = 0.0;
         for (int w = 0; w < 6; ++w) sum += red[w];
-        pap[ti] = sum;
+        pap_blk += sum;
+        if (!has_next) pap[blockIdx.x] = pap_blk;
       }
     }
     t = tn;
@@ -533,7 +538,7 @@ __global__ void mfma_layout_probe(double* out) {
 // Persistent apply grid: two workgroups per CU (the VGPR budget of the MFMA
 // loop allows twelve waves per CU), a multiple of 8 so the XCD grouping of
 // tile_range applies, never more workgroups than tiles.
-static int apply_grid(const icc_grid* g) {
+int apply_grid(const icc_grid* g) {
   int G = 2 * (g->ctx->num_cu > 0 ? g->ctx->num_cu : 256);
   G -= G % 8;
   if (G > g->ntiles) G = g->ntiles;

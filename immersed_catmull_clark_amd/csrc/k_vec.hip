@@ -421,10 +421,10 @@ hipError_t launch_masked_add(int64_t n, const double* u0, const double* du, cons
   return hipGetLastError();
 }
 
-// Blocks of the vector kernels: two elements per thread, at most VEC_MAX_BLOCKS
-// (the partial arrays hold 2 * VEC_MAX_BLOCKS values).
+// Blocks of the vector kernels: at least four elements per thread, at most
+// VEC_MAX_BLOCKS (the partial arrays hold 2 * VEC_MAX_BLOCKS values).
 int vec_blocks(int64_t n) {
-  int64_t nb = (n + VB * 2 - 1) / (VB * 2);
+  int64_t nb = (n + VB * 4 - 1) / (VB * 4);
   if (nb < 1) nb = 1;
   if (nb > VEC_MAX_BLOCKS) nb = VEC_MAX_BLOCKS;
   return (int)nb;
