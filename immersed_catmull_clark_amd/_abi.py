@@ -75,6 +75,14 @@ SIGNATURES = [
     ("icc_topopt_element_energies", cint, [vp, vp, vp]),
     ("icc_topopt_compliance", cint, [vp, vp, vp, vp, vp]),
     ("icc_topopt_run", cint, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    ("icc_immersed_config_default", None, [vp]),
+    ("icc_immersed_create", cint, [vp, vp, vp, i64, vpp]),
+    ("icc_immersed_destroy", cint, [vp]),
+    ("icc_immersed_sizes", cint, [vp, vp, vp]),
+    ("icc_immersed_problem", cint, [vp, vp, vp, vp, vp, vp, vp]),
+    ("icc_immersed_apply", cint, [vp, vp, vp]),
+    ("icc_immersed_solve", cint, [vp, vp, vp]),
+    ("icc_immersed_solve_dev", cint, [vp, vp, vp]),
 ]
 
 REPORT_FN = C.CFUNCTYPE(None, i64, dbl, dbl, dbl, vp)
@@ -86,6 +94,15 @@ class TopOptConfigC(C.Structure):
                 ("max_iter", i64), ("young", dbl), ("poisson", dbl), ("emin", dbl), ("move_limit", dbl),
                 ("pcg_tol", dbl), ("load", C.c_int32), ("filter", C.c_int32), ("solver", C.c_int32),
                 ("pad", C.c_int32)]
+
+
+class ImmersedConfigC(C.Structure):
+    """icc_immersed_config (include/icc.h)."""
+    _fields_ = [("box_lo", dbl * 3), ("box_hi", dbl * 3), ("res", i64 * 3), ("sub", i64 * 3), ("young", dbl),
+                ("poisson", dbl), ("emin", dbl), ("active_min", dbl), ("penalty_factor", dbl),
+                ("traction", dbl * 3), ("pcg_tol", dbl), ("pcg_maxit", i64), ("mode", C.c_int32),
+                ("solver", C.c_int32)]
+
 
 _lib = None
 

@@ -33,6 +33,7 @@ SOURCES = [
     ("wq.cpp", None),
     ("api.cpp", None),
     ("api_ext.cpp", None),
+    ("api_imm.cpp", None),
 ]
 
 

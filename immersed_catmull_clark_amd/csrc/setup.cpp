@@ -354,4 +354,42 @@ void traction_xmax(const GridSetup& s, const double t[3], std::vector<double>& f
     }
 }
 
+// Penalty face term on x = x_min: beta * int N_A N_B dA for each component,
+// over the face cells (ey, ez) with face_on[ey + nel1 * ez] set, 3-point Gauss
+// per face direction.  On x = x_min only the first control-point layer has a
+// nonzero trace (open knots), so rows and columns are DOFs of that layer.
+void penalty_xmin(const GridSetup& s, double beta, const std::vector<uint8_t>& face_on, std::vector<int64_t>& rows,
+                  std::vector<int64_t>& cols, std::vector<double>& vals) {
+  std::vector<double> ky = open_knots_p2(s.nel[1]), kz = open_knots_p2(s.nel[2]);
+  double xg[3], wg[3];
+  gauss(3, xg, wg);
+  for (int64_t ez = 0; ez < s.nel[2]; ++ez)
+    for (int64_t ey = 0; ey < s.nel[1]; ++ey) {
+      if (!face_on[ey + s.nel[1] * ez]) continue;
+      const double ay = ky[ey + 2], hy = ky[ey + 3] - ay, az = kz[ez + 2], hz = kz[ez + 3] - az;
+      for (int qz = 0; qz < 3; ++qz)
+        for (int qy = 0; qy < 3; ++qy) {
+          const double yi = ay + 0.5 * (xg[qy] + 1.0) * hy, zi = az + 0.5 * (xg[qz] + 1.0) * hz;
+          const double w = (0.5 * wg[qy] * hy * s.len[1]) * (0.5 * wg[qz] * hz * s.len[2]);
+          double vy[3], dvy[3], vz[3], dvz[3];
+          quad_basis(ky, ey, yi, vy, dvy);
+          quad_basis(kz, ez, zi, vz, dvz);
+          for (int az2 = 0; az2 < 3; ++az2)
+            for (int ay2 = 0; ay2 < 3; ++ay2) {
+              const int64_t ga = s.ncp[0] * ((ey + ay2) + s.ncp[1] * (ez + az2));
+              for (int bz = 0; bz < 3; ++bz)
+                for (int by = 0; by < 3; ++by) {
+                  const int64_t gb = s.ncp[0] * ((ey + by) + s.ncp[1] * (ez + bz));
+                  const double v = beta * w * (vy[ay2] * vz[az2]) * (vy[by] * vz[bz]);
+                  for (int c = 0; c < 3; ++c) {
+                    rows.push_back(c * s.ndof_sc + ga);
+                    cols.push_back(c * s.ndof_sc + gb);
+                    vals.push_back(v);
+                  }
+                }
+            }
+        }
+    }
+}
+
 }  // namespace icc

@@ -206,6 +206,58 @@ typedef void (*icc_report_fn)(int64_t iter, double compliance, double volume, do
 int icc_topopt_run(icc_topopt* t, icc_report_fn report, void* user, double* compliance, double* change,
                    int64_t* pcg_iterations, double* density, int64_t* n_iter);
 
+/* ---- immersed solve with an analytic geometry ---------------------------------- */
+/* build_immersed_problem + solve_immersed_problem (rust/src/immersed.rs:466-570)
+ * for a box minus cylinders parallel to z, on a p = 2 background grid aligned to
+ * the box:
+ *  - classification and 4^3-style sub-cell volume fractions as
+ *    icc_classify_cylinders (immersed.rs:79-155);
+ *  - mode 0: scale emin + (1 - emin) w on every cell (immersed.rs:484);
+ *    mode 1: scale w on cells with w >= active_min, other cells dropped
+ *    (active set as topopt_immersed.rs:84);
+ *  - DOFs whose support holds no cell of nonzero scale are fixed;
+ *  - Dirichlet u = 0 on the face x = box_lo[0] by the penalty
+ *    beta int u.v dA, beta = penalty_factor * young / h_min, over the face cells
+ *    whose volume cell has nonzero scale (immersed_bc.rs:161-190);
+ *  - consistent traction `traction` on the face x = box_hi[0];
+ *  - Jacobi PCG on the free DOFs, diagonal clamped at 1e-6 (immersed.rs:555),
+ *    stop at |r| / |b| < pcg_tol (solver 0), or the mixed solver of
+ *    icc_solve_mixed_dev to a true relative residual <= pcg_tol (solver 1). */
+typedef struct icc_immersed icc_immersed;
+typedef struct icc_immersed_config {
+  double box_lo[3], box_hi[3];
+  int64_t res[3]; /* background elements per direction */
+  int64_t sub[3]; /* sub-cells per cell and direction for the volume fractions */
+  double young, poisson, emin, active_min, penalty_factor;
+  double traction[3];
+  double pcg_tol;
+  int64_t pcg_maxit; /* 0: 20 * ndof */
+  int32_t mode;      /* 0: reference scale on all cells; 1: drop cells with w < active_min */
+  int32_t solver;    /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates */
+} icc_immersed_config;
+/* box [0,2]x[0,1]x[0,1], 32x16x16 elements, 4^3 sub-cells, E = 1, nu = 0.3,
+ * emin = 1e-4, active_min = 0.05, penalty factor 1e3, traction (0,-1,0),
+ * pcg_tol 1e-8, mode 1, solver 0 */
+void icc_immersed_config_default(icc_immersed_config* cfg);
+/* cyl = [cx, cy, r] * ncyl */
+int icc_immersed_create(icc_ctx* ctx, const icc_immersed_config* cfg, const double* cyl, int64_t ncyl,
+                        icc_immersed** out);
+int icc_immersed_destroy(icc_immersed* h);
+/* out = nel, ndof, active cells, free DOFs, penalty nonzeros; *beta = penalty coefficient */
+int icc_immersed_sizes(const icc_immersed* h, int64_t out[5], double* beta);
+/* status [nel] (1 inside, 0 outside, -1 cut), weights [nel], active [nel], scale [nel],
+ * free mask [ndof], force [ndof] (masked); any pointer may be NULL */
+int icc_immersed_problem(const icc_immersed* h, int8_t* status, double* weights, uint8_t* active, double* scale,
+                         uint8_t* free_mask, double* force);
+/* y = M (K_s + P) M x on host vectors (the solver's operator) */
+int icc_immersed_apply(icc_immersed* h, const double* x, double* y);
+/* u [ndof] from zero; stats (8 doubles, may be NULL): iterations, recursive
+ * relative residual (solver 0) or reliable updates (solver 1), true relative
+ * residual, compliance F.u, FP64 applies, 0, 0, 0.  Returns ICC_ENOCONV /
+ * ICC_EBREAKDOWN like icc_pcg with u filled. */
+int icc_immersed_solve(icc_immersed* h, double* u, double* stats);
+int icc_immersed_solve_dev(icc_immersed* h, double* u_dev, double* stats);
+
 /* ---- diagnostics ------------------------------------------------------------- */
 /* runs one v_mfma_f64_16x16x4 and one v_mfma_f32_16x16x4 with
  * A[i][k] = i + 16k, B[k][j] = (k == j%4) and writes the 64x4 accumulator

@@ -364,20 +364,7 @@ ORC_API i64 orc_immersed_topopt_run(const double* nodes, i64 nv, const i64* tris
 ORC_API void orc_classify_cylinders(const double* box_lo, const double* box_hi, const double* cyl, i64 ncyl,
                                     const double* grid_bounds, const i64* res, const i64* sub, int8_t* status,
                                     double* weights) {
-  InsideFn inside = [&](const std::array<double, 3>& q) {
-    double sdf_box = -1e300;
-    for (int d = 0; d < 3; ++d) {
-      double a = box_lo[d] - q[d], b = q[d] - box_hi[d];
-      sdf_box = std::fmax(sdf_box, std::fmax(a, b));
-    }
-    double sdf = sdf_box;
-    for (i64 c = 0; c < ncyl; ++c) {
-      double dx = q[0] - cyl[3 * c], dy = q[1] - cyl[3 * c + 1];
-      double dcyl = cyl[3 * c + 2] - std::sqrt(dx * dx + dy * dy);  // > 0 inside the hole
-      sdf = std::fmax(sdf, dcyl);
-    }
-    return sdf < 0.0;
-  };
+  InsideFn inside = box_minus_cylinders(box_lo, box_hi, std::vector<double>(cyl, cyl + 3 * ncyl));
   std::array<std::array<double, 2>, 3> gb;
   for (int d = 0; d < 3; ++d) gb[d] = {grid_bounds[2 * d], grid_bounds[2 * d + 1]};
   Vec w;
@@ -386,6 +373,58 @@ ORC_API void orc_classify_cylinders(const double* box_lo, const double* box_hi, 
                   w, st);
   std::memcpy(status, st.data(), st.size());
   std::memcpy(weights, w.data(), w.size() * sizeof(double));
+}
+
+// ---- analytic immersed problem (handle) ----
+struct SdfH {
+  SdfImmersed pb;
+};
+// par: young, poisson, emin, active_min, penalty_factor, tx, ty, tz
+ORC_API void* orc_sdf_new(const double* box_lo, const double* box_hi, const double* cyl, i64 ncyl, const i64* res,
+                          const i64* sub, const double* par, int mode) {
+  auto* h = new SdfH();
+  h->pb = build_sdf_immersed(box_lo, box_hi, std::vector<double>(cyl, cyl + 3 * ncyl),
+                             {(usize)res[0], (usize)res[1], (usize)res[2]}, {(usize)sub[0], (usize)sub[1], (usize)sub[2]},
+                             par[0], par[1], par[2], par[3], par[4], par + 5, mode);
+  return h;
+}
+ORC_API void orc_sdf_free(void* p) { delete (SdfH*)p; }
+// out: nel, ndof, nnz, beta bits are returned separately
+ORC_API double orc_sdf_sizes(void* p, i64* out) {
+  const SdfImmersed& pb = ((SdfH*)p)->pb;
+  out[0] = (i64)pb.sp.nel;
+  out[1] = (i64)pb.sp.ndof;
+  out[2] = (i64)pb.op.nnz();
+  return pb.beta;
+}
+ORC_API void orc_sdf_get(void* p, int8_t* status, double* w, uint8_t* active, double* scale, uint8_t* fr,
+                         double* force) {
+  const SdfImmersed& pb = ((SdfH*)p)->pb;
+  std::memcpy(status, pb.status.data(), pb.status.size());
+  std::memcpy(w, pb.weights.data(), pb.weights.size() * 8);
+  std::memcpy(active, pb.active.data(), pb.active.size());
+  std::memcpy(scale, pb.scale.data(), pb.scale.size() * 8);
+  std::memcpy(fr, pb.free_.data(), pb.free_.size());
+  std::memcpy(force, pb.force.data(), pb.force.size() * 8);
+}
+// y = M op M x
+ORC_API void orc_sdf_matvec(void* p, const double* x, double* y) {
+  const SdfImmersed& pb = ((SdfH*)p)->pb;
+  const usize n = pb.sp.ndof;
+  Vec xm(n);
+  for (usize i = 0; i < n; ++i) xm[i] = pb.free_[i] ? x[i] : 0.0;
+  pb.op.matvec(xm.data(), y);
+  for (usize i = 0; i < n; ++i)
+    if (!pb.free_[i]) y[i] = 0.0;
+}
+// stats: iterations, residual, compliance
+ORC_API void orc_sdf_solve(void* p, double tol, i64 maxit, double* u, double* stats) {
+  const SdfImmersed& pb = ((SdfH*)p)->pb;
+  ImmersedSolution sol = solve_sdf_immersed(pb, tol, (usize)maxit);
+  std::memcpy(u, sol.u.data(), sol.u.size() * 8);
+  stats[0] = (double)sol.iterations;
+  stats[1] = sol.residual;
+  stats[2] = sol.compliance;
 }
 
 // ---- weighted quadrature ---------------------------------------------------

@@ -288,6 +288,147 @@ ImmersedSolution solve_immersed_problem(const ImmersedProblem& pb, double tol, u
   return sol;
 }
 
+InsideFn box_minus_cylinders(const double lo[3], const double hi[3], const std::vector<double>& cyl) {
+  std::array<double, 3> l = {lo[0], lo[1], lo[2]}, h = {hi[0], hi[1], hi[2]};
+  return [l, h, cyl](const std::array<double, 3>& q) {
+    double sdf = -1e300;
+    for (int d = 0; d < 3; ++d) sdf = std::fmax(sdf, std::fmax(l[d] - q[d], q[d] - h[d]));
+    for (usize c = 0; 3 * c + 2 < cyl.size(); ++c) {
+      double dx = q[0] - cyl[3 * c], dy = q[1] - cyl[3 * c + 1];
+      sdf = std::fmax(sdf, cyl[3 * c + 2] - std::sqrt(dx * dx + dy * dy));  // > 0 inside a hole
+    }
+    return sdf < 0.0;
+  };
+}
+
+// The immersed problem of build_immersed_problem (immersed.rs:466-538) for the
+// analytic geometry, on a background grid aligned to the box:
+// * weights / status: element_weights with the box-minus-cylinders predicate;
+// * scale per mode (see oracle.hpp);
+// * fixed DOFs: those whose support holds no cell with s_e > 0 (the clamp is
+//   replaced by the penalty below, so no face is clamped);
+// * penalty Dirichlet u = 0 on the face x = lo: beta int u.v dA over the face
+//   cells whose volume cell has s_e > 0 (dirichlet_penalty_fn,
+//   immersed_bc.rs:161-183, with g = 0; beta = factor * E / h_min as
+//   surface_boundary_terms, immersed_bc.rs:186-190);
+// * consistent traction t on the face x = hi.
+SdfImmersed build_sdf_immersed(const double lo[3], const double hi[3], const std::vector<double>& cyl,
+                               const std::array<usize, 3>& res, const std::array<usize, 3>& sub, double young,
+                               double poisson, double emin, double active_min, double penalty_factor,
+                               const double traction[3], int mode) {
+  SdfImmersed pb;
+  std::array<std::array<double, 2>, 3> gb;
+  for (int d = 0; d < 3; ++d) gb[d] = {lo[d], hi[d]};
+  pb.sp = SpaceBox::make({gb[0], gb[1], gb[2]}, {res[0], res[1], res[2]}, 2);
+  const SpaceBox& sp = pb.sp;
+  element_weights(box_minus_cylinders(lo, hi, cyl), gb, res, sub, pb.weights, pb.status);
+  pb.scale.assign(sp.nel, 0.0);
+  pb.active.assign(sp.nel, 0);
+  for (usize e = 0; e < sp.nel; ++e) {
+    if (mode == 0) {
+      pb.scale[e] = emin + (1.0 - emin) * pb.weights[e];
+      pb.active[e] = 1;
+    } else if (pb.weights[e] >= active_min) {
+      pb.scale[e] = pb.weights[e];
+      pb.active[e] = 1;
+    }
+  }
+  double lambda, mu;
+  lame(young, poisson, lambda, mu);
+  ElementMatrices em = elasticity_element_matrices(sp, lambda, mu);
+  std::vector<usize> rows, cols;
+  Vec vals;
+  const usize nsh = sp.nsh, nsc = sp.ndof_sc;
+  pb.free_.assign(sp.ndof, 0);
+  for (usize e = 0; e < sp.nel; ++e) {
+    if (!(pb.scale[e] > 0.0)) continue;
+    const double* ke = em.of_element(e);
+    const usize* ce = &sp.connectivity[e * nsh];
+    for (usize a = 0; a < nsh; ++a) {
+      pb.free_[ce[a]] = 1;
+      for (usize b = 0; b < nsh; ++b) {
+        rows.push_back(ce[a]);
+        cols.push_back(ce[b]);
+        vals.push_back(pb.scale[e] * ke[a * nsh + b]);
+      }
+    }
+  }
+  double hmin = 1e300;
+  for (int d = 0; d < 3; ++d) hmin = std::fmin(hmin, (hi[d] - lo[d]) / (double)res[d]);
+  pb.beta = penalty_factor * young / hmin;
+  const Space1d& uy = sp.univ[1];
+  const Space1d& uz = sp.univ[2];
+  const usize nqy = uy.nquad, nqz = uz.nquad, ny = uy.nsh(), nz = uz.nsh();
+  for (usize ez = 0; ez < uz.nel; ++ez)
+    for (usize ey = 0; ey < uy.nel; ++ey) {
+      const usize e0 = sp.nel_dir[0] * (ey + sp.nel_dir[1] * ez);
+      if (!(pb.scale[e0] > 0.0)) continue;
+      for (usize qz = 0; qz < nqz; ++qz)
+        for (usize qy = 0; qy < nqy; ++qy) {
+          const double w = uy.qw[ey * nqy + qy] * sp.lengths[1] * uz.qw[ez * nqz + qz] * sp.lengths[2];
+          const double* Ny = &uy.shape[(ey * nqy + qy) * ny];
+          const double* Nz = &uz.shape[(ez * nqz + qz) * nz];
+          for (usize az = 0; az < nz; ++az)
+            for (usize ay = 0; ay < ny; ++ay) {
+              const usize ga = sp.ndof_dir[0] * (uy.connectivity[ey * ny + ay] + sp.ndof_dir[1] * uz.connectivity[ez * nz + az]);
+              for (usize bz = 0; bz < nz; ++bz)
+                for (usize by = 0; by < ny; ++by) {
+                  const usize gbb = sp.ndof_dir[0] * (uy.connectivity[ey * ny + by] + sp.ndof_dir[1] * uz.connectivity[ez * nz + bz]);
+                  const double v = pb.beta * w * (Ny[ay] * Nz[az]) * (Ny[by] * Nz[bz]);
+                  for (usize c = 0; c < 3; ++c) {
+                    rows.push_back(c * nsc + ga);
+                    cols.push_back(c * nsc + gbb);
+                    vals.push_back(v);
+                  }
+                }
+            }
+        }
+    }
+  pb.op = CsrMatrix::from_triplets(sp.ndof, sp.ndof, rows, cols, vals);
+  pb.force.assign(sp.ndof, 0.0);
+  const usize il = sp.ndof_dir[0] - 1;
+  for (usize ez = 0; ez < uz.nel; ++ez)
+    for (usize ey = 0; ey < uy.nel; ++ey)
+      for (usize qz = 0; qz < nqz; ++qz)
+        for (usize qy = 0; qy < nqy; ++qy) {
+          const double w = uy.qw[ey * nqy + qy] * sp.lengths[1] * uz.qw[ez * nqz + qz] * sp.lengths[2];
+          for (usize az = 0; az < nz; ++az)
+            for (usize ay = 0; ay < ny; ++ay) {
+              const double n = uy.shape[(ey * nqy + qy) * ny + ay] * uz.shape[(ez * nqz + qz) * nz + az];
+              const usize g = il + sp.ndof_dir[0] * (uy.connectivity[ey * ny + ay] + sp.ndof_dir[1] * uz.connectivity[ez * nz + az]);
+              for (usize c = 0; c < 3; ++c) pb.force[c * nsc + g] += w * n * traction[c];
+            }
+        }
+  for (usize i = 0; i < sp.ndof; ++i)
+    if (!pb.free_[i]) pb.force[i] = 0.0;
+  return pb;
+}
+
+// solve_immersed_problem (immersed.rs:551-570) on the analytic problem
+ImmersedSolution solve_sdf_immersed(const SdfImmersed& pb, double tol, usize maxit) {
+  const usize n = pb.sp.ndof;
+  const auto& fr = pb.free_;
+  Vec b(n);
+  for (usize i = 0; i < n; ++i) b[i] = fr[i] ? pb.force[i] : 0.0;
+  Vec diag = pb.op.diagonal();
+  for (double& d : diag) d = std::fmax(d, 1e-6);
+  Vec pm(n, 0.0), tmp(n, 0.0);
+  PcgResult r = pcg_solve(maxit, tol, n, b, diag, [&](const double* p, double* ap) {
+    for (usize i = 0; i < n; ++i) pm[i] = fr[i] ? p[i] : 0.0;
+    pb.op.matvec(pm.data(), tmp.data());
+    for (usize i = 0; i < n; ++i) ap[i] = fr[i] ? tmp[i] : 0.0;
+  });
+  ImmersedSolution sol;
+  sol.u.resize(n);
+  for (usize i = 0; i < n; ++i) sol.u[i] = fr[i] ? r.x[i] : 0.0;
+  double c = 0.0;
+  for (usize i = 0; i < n; ++i) c += pb.force[i] * sol.u[i];
+  sol.compliance = c;
+  sol.iterations = r.iters;
+  sol.residual = r.res;
+  return sol;
+}
+
 // topopt_immersed.rs:78-153 (new) and 155-247 (run)
 ImmersedTopOptHistory immersed_topopt_run(const TriangleMesh3D& brep, const ImmersedTopOptCfg& cfg) {
   auto gb = padded_bounds(brep, 0.05);

@@ -235,6 +235,27 @@ struct ImmersedTopOptHistory {
 };
 ImmersedTopOptHistory immersed_topopt_run(const TriangleMesh3D& brep, const ImmersedTopOptCfg& cfg);
 
+// ---- immersed solve with an analytic geometry (box minus z-cylinders) -----
+// Box [lo, hi] minus cylinders parallel to z, cyl = (cx, cy, r) triples;
+// inside iff the signed distance is < 0 (strict).
+InsideFn box_minus_cylinders(const double lo[3], const double hi[3], const std::vector<double>& cyl);
+struct SdfImmersed {
+  SpaceBox sp;  // background grid aligned to the box
+  std::vector<int8_t> status;
+  Vec weights, scale;
+  std::vector<uint8_t> active, free_;
+  CsrMatrix op;  // sum_e s_e K_e + penalty face term
+  Vec force;
+  double beta = 0.0;
+};
+// mode 0: s_e = emin + (1 - emin) w_e on every cell (immersed.rs:484);
+// mode 1: s_e = w_e on cells with w_e >= active_min, other cells dropped.
+SdfImmersed build_sdf_immersed(const double lo[3], const double hi[3], const std::vector<double>& cyl,
+                               const std::array<usize, 3>& res, const std::array<usize, 3>& sub, double young,
+                               double poisson, double emin, double active_min, double penalty_factor,
+                               const double traction[3], int mode);
+ImmersedSolution solve_sdf_immersed(const SdfImmersed& pb, double tol, usize maxit);
+
 // ---- rust/src/wq.rs -------------------------------------------------------
 struct WqRules1d {
   Vec points;

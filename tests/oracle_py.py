@@ -58,6 +58,14 @@ def lib():
         L.orc_topopt_new.restype = vp
         L.orc_topopt_new.argtypes = [vp, vp, vp]
         L.orc_topopt_free.argtypes = [vp]
+        L.orc_sdf_new.restype = vp
+        L.orc_sdf_new.argtypes = [vp, vp, vp, i64, vp, vp, vp, C.c_int]
+        L.orc_sdf_free.argtypes = [vp]
+        L.orc_sdf_sizes.restype = dbl
+        L.orc_sdf_sizes.argtypes = [vp, vp]
+        L.orc_sdf_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_sdf_matvec.argtypes = [vp, vp, vp]
+        L.orc_sdf_solve.argtypes = [vp, dbl, i64, vp, vp]
         L.orc_topopt_force_free.argtypes = [vp, vp, vp]
         L.orc_topopt_solve_state.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_immersed_new.restype = vp
@@ -318,6 +326,55 @@ def classify_cylinders(box_lo, box_hi, cyl, grid_bounds, res, sub=(4, 4, 4)):
     lib().orc_classify_cylinders(_p(box_lo), _p(box_hi), _p(cyl), len(cyl) // 3, _p(gb), _p(r), _p(s), _p(status),
                                  _p(w))
     return status, w
+
+
+class SdfImmersed:
+    """Oracle restatement of the immersed problem for a box minus z-cylinders
+    (build_immersed_problem / solve_immersed_problem, immersed.rs:466-570)."""
+
+    def __init__(self, box_lo, box_hi, cyl, res, sub=(4, 4, 4), young=1.0, poisson=0.3, emin=1e-4, active_min=0.05,
+                 penalty_factor=1e3, traction=(0.0, -1.0, 0.0), mode=1):
+        lo = np.ascontiguousarray(box_lo, dtype=np.float64)
+        hi = np.ascontiguousarray(box_hi, dtype=np.float64)
+        cy = np.ascontiguousarray(np.asarray(cyl, dtype=np.float64).ravel())
+        r = np.ascontiguousarray(res, dtype=np.int64)
+        sb = np.ascontiguousarray(sub, dtype=np.int64)
+        par = np.ascontiguousarray([young, poisson, emin, active_min, penalty_factor] + list(traction), dtype=np.float64)
+        self.h = lib().orc_sdf_new(_p(lo), _p(hi), _p(cy), len(cy) // 3, _p(r), _p(sb), _p(par), int(mode))
+        sz = np.zeros(3, dtype=np.int64)
+        self.beta = float(lib().orc_sdf_sizes(self.h, _p(sz)))
+        self.nel, self.ndof, self.nnz = (int(v) for v in sz)
+        self.status = np.zeros(self.nel, dtype=np.int8)
+        self.weights = np.zeros(self.nel)
+        self.active = np.zeros(self.nel, dtype=np.uint8)
+        self.scale = np.zeros(self.nel)
+        self.free = np.zeros(self.ndof, dtype=np.uint8)
+        self.force = np.zeros(self.ndof)
+        lib().orc_sdf_get(self.h, _p(self.status), _p(self.weights), _p(self.active), _p(self.scale), _p(self.free),
+                          _p(self.force))
+
+    def __del__(self):
+        try:
+            lib().orc_sdf_free(self.h)
+        except Exception:
+            pass
+
+    def matvec(self, x):
+        y = np.zeros(self.ndof)
+        lib().orc_sdf_matvec(self.h, _p(np.ascontiguousarray(x, dtype=np.float64)), _p(y))
+        return y
+
+    def solve(self, tol, maxit):
+        u = np.zeros(self.ndof)
+        st = np.zeros(3)
+        lib().orc_sdf_solve(self.h, tol, int(maxit), _p(u), _p(st))
+        return u, int(st[0]), float(st[1]), float(st[2])
+
+
+def seeded_cylinders(seed, n=3, r=(0.08, 0.15), cx=(0.4, 1.6), cy=(0.3, 0.7)):
+    """Cylinder holes (cx, cy, r) parallel to z, axes away from the box faces."""
+    rng = np.random.default_rng(seed)
+    return np.column_stack([rng.uniform(*cx, n), rng.uniform(*cy, n), rng.uniform(*r, n)])
 
 
 def wq_rule(nel, p, layout, which, i=0):

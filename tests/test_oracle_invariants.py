@@ -108,3 +108,45 @@ def test_structured_csr_equals_assembly_pattern():
         s = np.random.default_rng(3).uniform(0.01, 1.0, sp.nel) ** 3
         assert sp.assemble_structured(s) > 0
         assert sp.compare_with_pattern(s) == 0
+
+
+def test_sdf_immersed_problem_invariants():
+    """The analytic immersed problem of the oracle: traction total = t * face
+    area, penalty normalisation beta * area on a rigid translation (K 1 = 0),
+    symmetry, support mask, and mode 0 = reference scale on every cell."""
+    cyl = O.seeded_cylinders(5)
+    lo, hi = [0.0, 0.0, 0.0], [2.0, 1.0, 1.0]
+    pb = O.SdfImmersed(lo, hi, cyl, [16, 8, 8], mode=1)
+    nsc = pb.ndof // 3
+    assert abs(pb.force[nsc:2 * nsc].sum() + 1.0) < 1e-13 and abs(pb.force[:nsc].sum()) == 0.0
+    assert abs(pb.beta - 1e3 * 1.0 / 0.125) < 1e-9
+    # weights: a cylinder of radius r removes pi r^2 per unit length from the box volume
+    vol = pb.weights.sum() * (2.0 / 16) * (1.0 / 8) * (1.0 / 8)
+    exact = 2.0 - np.pi * (cyl[:, 2] ** 2).sum()
+    assert abs(vol - exact) < 0.02 * exact
+    assert np.array_equal(pb.active, (pb.weights >= 0.05).astype(np.uint8))
+    assert np.array_equal(pb.scale, np.where(pb.weights >= 0.05, pb.weights, 0.0))
+    # rigid translation in y: only the penalty acts, total = beta * area of the x = 0 face
+    t = np.zeros(pb.ndof)
+    t[nsc:2 * nsc] = 1.0
+    y = pb.matvec(t * pb.free)
+    assert abs(y[nsc:2 * nsc].sum() - pb.beta * 1.0) < 1e-9 * pb.beta
+    assert np.abs(y[:nsc]).max() < 1e-9 * pb.beta
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal(pb.ndof), rng.standard_normal(pb.ndof)
+    assert abs(a @ pb.matvec(b) - b @ pb.matvec(a)) < 1e-11 * abs(a @ pb.matvec(b))
+    # DOFs outside every active cell's support are fixed: with a hole large
+    # enough to leave unsupported control points, the mask is the union of
+    # the supports of the active cells
+    big = O.SdfImmersed(lo, hi, [[1.0, 0.5, 0.36]], [16, 8, 8], mode=1)
+    nel = (16, 8, 8)
+    act = big.active.reshape(nel[2], nel[1], nel[0])
+    sup = np.zeros((nel[2] + 2, nel[1] + 2, nel[0] + 2), dtype=bool)
+    for dz in range(3):
+        for dy in range(3):
+            for dx in range(3):
+                sup[dz:dz + nel[2], dy:dy + nel[1], dx:dx + nel[0]] |= act.astype(bool)
+    assert np.array_equal(big.free, np.tile(sup.ravel(), 3).astype(np.uint8))
+    assert 0 < big.free.sum() < big.ndof
+    pb0 = O.SdfImmersed(lo, hi, cyl, [16, 8, 8], mode=0, emin=1e-4)
+    assert np.array_equal(pb0.scale, 1e-4 + (1 - 1e-4) * pb0.weights) and pb0.free.all()
