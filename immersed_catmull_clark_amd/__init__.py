@@ -7,5 +7,6 @@ from ._abi import (ICC_F32, ICC_F64, ICC_OK, ICC_EINVAL, ICC_ENOMEM, ICC_EHIP, I
 from .core import Context, Filter, Grid, Operator, classify_cylinders  # noqa: F401
 from .topopt import CantileverTopOpt, CantileverTopOptConfig, TopOptHistory  # noqa: F401
 from .immersed import ImmersedConfig, ImmersedSdf  # noqa: F401
+from .batch import Batch  # noqa: F401
 
 __all__ = ["Context", "Grid", "Operator", "Filter", "classify_cylinders", "ICC_F64", "ICC_F32", "IccError"]

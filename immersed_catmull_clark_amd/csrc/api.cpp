@@ -600,9 +600,8 @@ static int write_tol(icc_op* op, Work* w, double tol) {
 // One PCG iteration in precision T enqueued on the op stream (sparse term
 // first, then the tiles, then the two vector kernels).
 template <typename T>
-static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv) {
+static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, hipStream_t st) {
   icc_grid* g = op->g;
-  hipStream_t st = g->ctx->stream;
   const int64_t n = g->s.ndof;
   const int npap = apply_grid(g) + csr_blocks(op);
   hipError_t e = hipSuccess;
@@ -636,7 +635,7 @@ static int capture_batch(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* 
   hipStream_t st = op->g->ctx->stream;
   ICC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
-  for (int i = 0; i < batch && e == hipSuccess; ++i) e = enqueue_iteration<T>(op, w, x, r, p, ap, dinv);
+  for (int i = 0; i < batch && e == hipSuccess; ++i) e = enqueue_iteration<T>(op, w, x, r, p, ap, dinv, st);
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(st, &graph);
   if (e != hipSuccess) return hip_fail(e, "PCG batch capture");
@@ -1143,5 +1142,306 @@ ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out512) {
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "icc_debug_mfma_layout");
+  return ICC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched instances: every unfinished instance's PCG iteration chain is one
+// branch of a HIP graph of PCG_BATCH iterations (captured by forking one
+// stream per instance off the context stream), so small instances run side by
+// side; each instance's kernels return at once when its device state is done.
+// Mixed precision: the reliable updates of the instances whose FP32 residual
+// dropped by delta run on the context stream between graph replays.
+struct icc_batch {
+  icc_ctx* ctx = nullptr;
+  std::vector<icc_grid*> g;
+  std::vector<icc_op*> op;
+  std::vector<double*> d_f, d_u;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+};
+
+ICC_API int icc_batch_destroy(icc_batch* b) {
+  if (!b) return ICC_OK;
+  for (size_t k = 0; k < b->op.size(); ++k) {
+    icc_op_destroy(b->op[k]);
+    icc_grid_destroy(b->g[k]);
+  }
+  for (double* p : b->d_f) hipFree(p);
+  for (double* p : b->d_u) hipFree(p);
+  for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  delete b;
+  return ICC_OK;
+}
+
+ICC_API int icc_batch_create(icc_ctx* ctx, int64_t n, const int64_t* nel, const double* size, double young,
+                             double poisson, icc_batch** out) {
+  if (!ctx || n < 1 || !nel || !out) return ICC_EINVAL;
+  auto* b = new icc_batch();
+  b->ctx = ctx;
+  int rc = ICC_OK;
+  for (int64_t k = 0; k < n && rc == ICC_OK; ++k) {
+    const double L[3] = {size ? size[3 * k] : 2.0, size ? size[3 * k + 1] : 1.0, size ? size[3 * k + 2] : 1.0};
+    const double bounds[6] = {0.0, L[0], 0.0, L[1], 0.0, L[2]};
+    icc_grid* g = nullptr;
+    icc_op* op = nullptr;
+    rc = icc_grid_create(ctx, nel + 3 * k, bounds, 2, young, poisson, &g);
+    if (rc == ICC_OK) rc = icc_op_create(g, &op);
+    b->g.push_back(g);
+    b->op.push_back(op);
+    double *f = nullptr, *u = nullptr;
+    if (rc == ICC_OK && (rc = dalloc(&f, g->s.ndof)) == ICC_OK) rc = dalloc(&u, g->s.ndof);
+    b->d_f.push_back(f);
+    b->d_u.push_back(u);
+    hipStream_t s = nullptr;
+    hipEvent_t e = nullptr;
+    if (rc == ICC_OK && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) rc = ICC_EHIP;
+    if (rc == ICC_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) rc = ICC_EHIP;
+    b->streams.push_back(s);
+    b->join.push_back(e);
+    if (rc == ICC_OK) {
+      // default: clamp x = 0, no load
+      const GridSetup& gs = g->s;
+      std::vector<uint8_t> fr(gs.ndof, 1);
+      for (int64_t i = 0; i < gs.ndof_sc; ++i)
+        if (i % gs.ncp[0] == 0)
+          for (int c = 0; c < 3; ++c) fr[c * gs.ndof_sc + i] = 0;
+      rc = icc_op_set_free_mask(op, fr.data());
+      if (rc == ICC_OK && hipMemsetAsync(f, 0, gs.ndof * 8, ctx->stream) != hipSuccess) rc = ICC_EHIP;
+    }
+  }
+  if (rc == ICC_OK && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) rc = ICC_EHIP;
+  if (rc == ICC_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ICC_EHIP;
+  if (rc != ICC_OK) {
+    if (rc == ICC_EHIP) set_error("icc_batch_create: HIP stream/event creation failed");
+    icc_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return ICC_OK;
+}
+
+ICC_API int icc_batch_sizes(const icc_batch* b, int64_t* ndof, int64_t* nel) {
+  if (!b) return ICC_EINVAL;
+  for (size_t k = 0; k < b->g.size(); ++k) {
+    if (ndof) ndof[k] = b->g[k]->s.ndof;
+    if (nel) nel[k] = b->g[k]->s.nel_tot;
+  }
+  return ICC_OK;
+}
+
+ICC_API int icc_batch_set(icc_batch* b, int64_t k, const double* scale, const uint8_t* free_mask, const double* force) {
+  if (!b || k < 0 || k >= (int64_t)b->op.size()) return ICC_EINVAL;
+  icc_op* op = b->op[k];
+  const GridSetup& gs = b->g[k]->s;
+  if (scale) ICC_TRY(icc_op_set_scale(op, scale));
+  std::vector<uint8_t> fr;
+  if (free_mask) {
+    fr.assign(free_mask, free_mask + gs.ndof);
+    ICC_TRY(icc_op_set_free_mask(op, free_mask));
+  } else {
+    fr.assign(gs.ndof, 1);
+    for (int64_t i = 0; i < gs.ndof_sc; ++i)
+      if (i % gs.ncp[0] == 0)
+        for (int c = 0; c < 3; ++c) fr[c * gs.ndof_sc + i] = 0;
+    ICC_TRY(icc_op_set_free_mask(op, fr.data()));
+  }
+  if (force) {
+    std::vector<double> fm(force, force + gs.ndof);
+    for (int64_t i = 0; i < gs.ndof; ++i)
+      if (!fr[i]) fm[i] = 0.0;
+    ICC_HIP(hipMemcpyAsync(b->d_f[k], fm.data(), gs.ndof * 8, hipMemcpyHostToDevice, b->ctx->stream));
+    ICC_HIP(hipStreamSynchronize(b->ctx->stream));
+  }
+  return ICC_OK;
+}
+
+namespace icc {
+// Captures PCG_BATCH iterations of every instance in `live` as parallel
+// branches off the context stream.
+template <typename T>
+static int capture_batched(icc_batch* b, const std::vector<int>& live, const std::vector<Work*>& w, IterGraph& G) {
+  hipStream_t m = b->ctx->stream;
+  ICC_HIP(hipStreamBeginCapture(m, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipEventRecord(b->fork, m);
+  for (int k : live) {
+    if (e != hipSuccess) break;
+    hipStream_t s = b->streams[k];
+    Work* wk = w[k];
+    e = hipStreamWaitEvent(s, b->fork, 0);
+    for (int i = 0; i < PCG_BATCH && e == hipSuccess; ++i) {
+      if constexpr (std::is_same<T, double>::value)
+        e = enqueue_iteration<double>(b->op[k], wk, b->d_u[k], wk->r, wk->p, wk->ap, wk->dinv, s);
+      else
+        e = enqueue_iteration<float>(b->op[k], wk, wk->x32, wk->r32, wk->p32, wk->ap32, wk->dinv32, s);
+    }
+    if (e == hipSuccess) e = hipEventRecord(b->join[k], s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(m, b->join[k], 0);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(m, &graph);
+  if (e != hipSuccess) return hip_fail(e, "batched capture");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+  G.graph = graph;
+  ICC_HIP(hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0));
+  return ICC_OK;
+}
+
+// Copies the device state of every instance in `ks` to its pinned host slot.
+static int read_states(icc_batch* b, const std::vector<int>& ks, const std::vector<Work*>& w) {
+  hipStream_t m = b->ctx->stream;
+  for (int k : ks) ICC_HIP(hipMemcpyAsync(w[k]->st_host, w[k]->st, sizeof(PcgState), hipMemcpyDeviceToHost, m));
+  ICC_HIP(hipStreamSynchronize(m));
+  return ICC_OK;
+}
+
+// FP64 true residual r = f - A u (masked) of instance k into its state (rr, norm_b squared).
+static hipError_t true_residual(icc_batch* b, int k, Work* w) {
+  hipError_t e = apply_t<double>(b->op[k], b->d_u[k], w->ap, nullptr);
+  if (e == hipSuccess)
+    e = launch_residual(b->g[k]->s.ndof, b->d_f[k], w->ap, b->op[k]->d_free, w->r, w->st, w->parts, b->ctx->stream);
+  return e;
+}
+}  // namespace icc
+
+ICC_API int icc_batch_solve(icc_batch* b, int prec, double tol, double delta, int64_t maxit, double* u,
+                            double* stats) {
+  if (!b || (prec != ICC_F64 && prec != ICC_F32) || !(tol > 0.0)) return ICC_EINVAL;
+  if (prec == ICC_F32 && !(delta > 0.0 && delta < 1.0)) return ICC_EINVAL;
+  const int nb = (int)b->op.size();
+  hipStream_t m = b->ctx->stream;
+  std::vector<Work*> w(nb, nullptr);
+  std::vector<int> status(nb, ICC_ENOCONV), live, finished(nb, 0);
+  std::vector<double> nf(nb, 1.0), res_true(nb, 1.0), updates(nb, 0.0);
+  std::vector<int64_t> iters(nb, 0);
+  for (int k = 0; k < nb; ++k) {
+    ICC_TRY(ensure_work(b->op[k], w[k]));
+    const int64_t n = b->g[k]->s.ndof;
+    ICC_TRY(icc_diag_dev(b->op[k], w[k]->tmp));
+    ICC_HIP(launch_fill<double>(b->d_u[k], 0.0, n, m));
+    if (prec == ICC_F32) {
+      ICC_HIP(launch_inv_diag<float>(w[k]->tmp, w[k]->dinv32, n, 1e-12, 1, m));
+      ICC_HIP(true_residual(b, k, w[k]));
+    } else {
+      ICC_HIP(launch_inv_diag<double>(w[k]->tmp, w[k]->dinv, n, 1e-12, 1, m));
+    }
+  }
+  const int32_t mi = (int32_t)std::min<int64_t>(maxit > 0 ? maxit : (int64_t)2000000000, 2000000000);
+  if (prec == ICC_F32) {
+    std::vector<int> all(nb);
+    for (int k = 0; k < nb; ++k) all[k] = k;
+    ICC_TRY(read_states(b, all, w));
+    for (int k = 0; k < nb; ++k) {
+      const PcgState& hs = *w[k]->st_host;
+      nf[k] = std::sqrt(hs.norm_b);
+      res_true[k] = nf[k] > 0 ? std::sqrt(hs.rr) / nf[k] : std::sqrt(hs.rr);
+      if (res_true[k] <= tol) {
+        finished[k] = 1;
+        status[k] = ICC_OK;
+        continue;
+      }
+      const int64_t n = b->g[k]->s.ndof;
+      ICC_HIP(hipMemcpyAsync(&w[k]->st->tol, &delta, sizeof(double), hipMemcpyHostToDevice, m));
+      ICC_HIP(hipMemcpyAsync(&w[k]->st->maxit, &mi, sizeof(int32_t), hipMemcpyHostToDevice, m));
+      ICC_HIP((launch_pcg_init<float, double>(n, w[k]->r, 1.0, w[k]->dinv32, w[k]->x32, w[k]->r32, w[k]->p32,
+                                              w[k]->ap32, w[k]->st, w[k]->parts, m)));
+      live.push_back(k);
+    }
+  } else {
+    for (int k = 0; k < nb; ++k) {
+      const int64_t n = b->g[k]->s.ndof;
+      ICC_HIP(hipMemcpyAsync(&w[k]->st->tol, &tol, sizeof(double), hipMemcpyHostToDevice, m));
+      ICC_HIP(hipMemcpyAsync(&w[k]->st->maxit, &mi, sizeof(int32_t), hipMemcpyHostToDevice, m));
+      ICC_HIP((launch_pcg_init<double, double>(n, b->d_f[k], 1.0, w[k]->dinv, b->d_u[k], w[k]->r, w[k]->p, w[k]->ap,
+                                               w[k]->st, w[k]->parts, m)));
+      live.push_back(k);
+    }
+  }
+  if (!live.empty()) {
+    IterGraph G;
+    if (prec == ICC_F32)
+      ICC_TRY(capture_batched<float>(b, live, w, G));
+    else
+      ICC_TRY(capture_batched<double>(b, live, w, G));
+    const int32_t done_fin = 4;  // done value that keeps a finished instance's kernels idle
+    std::vector<int> pend;
+    while (true) {
+      ICC_HIP(hipGraphLaunch(G.exec, m));
+      ICC_TRY(read_states(b, live, w));
+      pend.clear();
+      bool any_running = false;
+      for (int k : live) {
+        if (finished[k]) continue;
+        const PcgState& hs = *w[k]->st_host;
+        iters[k] = hs.iters;
+        if (hs.done == 0) {
+          any_running = true;
+          continue;
+        }
+        if (prec == ICC_F64) {
+          finished[k] = 1;
+          status[k] = hs.done == 1 ? ICC_OK : hs.done == 2 ? ICC_EBREAKDOWN : ICC_ENOCONV;
+        } else {
+          pend.push_back(k);
+        }
+      }
+      if (prec == ICC_F32 && !pend.empty()) {
+        // reliable updates: u += x32, FP64 true residual
+        std::vector<int> done_code(nb, 0);
+        for (int k : pend) {
+          done_code[k] = w[k]->st_host->done;
+          ICC_HIP(launch_ir_update<float>(b->g[k]->s.ndof, b->d_u[k], w[k]->x32, 1.0, m));
+          ICC_HIP(true_residual(b, k, w[k]));
+          updates[k] += 1.0;
+        }
+        ICC_TRY(read_states(b, pend, w));
+        for (int k : pend) {
+          const PcgState& hr = *w[k]->st_host;
+          res_true[k] = nf[k] > 0 ? std::sqrt(hr.rr) / nf[k] : std::sqrt(hr.rr);
+          if (res_true[k] <= tol || done_code[k] == 3 || done_code[k] == 2) {
+            finished[k] = 1;
+            status[k] = res_true[k] <= tol ? ICC_OK : done_code[k] == 2 ? ICC_EBREAKDOWN : ICC_ENOCONV;
+            ICC_HIP(hipMemcpyAsync(&w[k]->st->done, &done_fin, sizeof(int32_t), hipMemcpyHostToDevice, m));
+          } else {
+            const int64_t n = b->g[k]->s.ndof;
+            ICC_HIP(launch_reliable_replace(n, w[k]->r, w[k]->dinv32, w[k]->r32, w[k]->x32, w[k]->st, w[k]->parts, m));
+            ICC_HIP(launch_pdir_update(n, w[k]->st, w[k]->p32, w[k]->r32, w[k]->dinv32, w[k]->ap32, m));
+            any_running = true;
+          }
+        }
+      }
+      if (!any_running) break;
+    }
+    ICC_HIP(hipStreamSynchronize(m));
+  }
+  // true residuals of the FP64 path; copy out
+  int64_t off = 0;
+  for (int k = 0; k < nb; ++k) {
+    const int64_t n = b->g[k]->s.ndof;
+    if (prec == ICC_F64) {
+      ICC_HIP(true_residual(b, k, w[k]));
+      ICC_TRY(read_states(b, std::vector<int>{k}, w));
+      const PcgState& hr = *w[k]->st_host;
+      const double nfk = std::sqrt(hr.norm_b);
+      res_true[k] = nfk > 0 ? std::sqrt(hr.rr) / nfk : std::sqrt(hr.rr);
+    }
+    if (u) ICC_HIP(hipMemcpyAsync(u + off, b->d_u[k], n * 8, hipMemcpyDeviceToHost, m));
+    if (stats) {
+      stats[4 * k] = (double)iters[k];
+      stats[4 * k + 1] = updates[k];
+      stats[4 * k + 2] = res_true[k];
+      stats[4 * k + 3] = (double)status[k];
+    }
+    off += n;
+  }
+  ICC_HIP(hipStreamSynchronize(m));
+  for (int k = 0; k < nb; ++k)
+    if (status[k] != ICC_OK) {
+      set_error("icc_batch_solve: an instance did not converge (see stats)");
+      return status[k];
+    }
   return ICC_OK;
 }

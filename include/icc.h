@@ -258,6 +258,29 @@ int icc_immersed_apply(icc_immersed* h, const double* x, double* y);
 int icc_immersed_solve(icc_immersed* h, double* u, double* stats);
 int icc_immersed_solve_dev(icc_immersed* h, double* u_dev, double* stats);
 
+/* ---- batched instances (uneven sizes) ------------------------------------------ */
+/* n structured p = 2 instances, instance k on [0,size_k0]x[0,size_k1]x[0,size_k2]
+ * (size NULL: 2 x 1 x 1) with nel_k = nel[3k..3k+2]; default free mask clamps
+ * x = 0, default force is zero.  Every unfinished instance's PCG iteration chain
+ * is one branch of a HIP graph of 16 iterations, replayed until all converge;
+ * each instance stops on its own.  No counterpart in the reference (it solves
+ * one instance at a time). */
+typedef struct icc_batch icc_batch;
+int icc_batch_create(icc_ctx* ctx, int64_t n, const int64_t* nel, const double* size, double young, double poisson,
+                     icc_batch** out);
+int icc_batch_destroy(icc_batch* b);
+/* ndof [n], nel [n] (either may be NULL) */
+int icc_batch_sizes(const icc_batch* b, int64_t* ndof, int64_t* nel);
+/* instance k: element scale [nel_k] (NULL: keep), free mask [ndof_k] (NULL: clamp
+ * x = 0), force [ndof_k] (NULL: keep; masked on entry) */
+int icc_batch_set(icc_batch* b, int64_t k, const double* scale, const uint8_t* free_mask, const double* force);
+/* Solves every instance from u = 0.  prec ICC_F64: Jacobi PCG to |r|/|b| < tol;
+ * ICC_F32: FP32 PCG with FP64 reliable updates every `delta` drop, to a true
+ * relative residual <= tol.  u (host, may be NULL): instances concatenated.
+ * stats (may be NULL) [n][4]: iterations, reliable updates, true relative
+ * residual, status code.  Returns the first non-OK instance status, if any. */
+int icc_batch_solve(icc_batch* b, int prec, double tol, double delta, int64_t maxit, double* u, double* stats);
+
 /* ---- diagnostics ------------------------------------------------------------- */
 /* runs one v_mfma_f64_16x16x4 and one v_mfma_f32_16x16x4 with
  * A[i][k] = i + 16k, B[k][j] = (k == j%4) and writes the 64x4 accumulator
