@@ -79,6 +79,7 @@ SIGNATURES = [
     ("icc_batch_destroy", cint, [vp]),
     ("icc_batch_sizes", cint, [vp, vp, vp]),
     ("icc_batch_set", cint, [vp, i64, vp, vp, vp]),
+    ("icc_batch_set_traction", cint, [vp, i64, vp]),
     ("icc_batch_solve", cint, [vp, cint, dbl, dbl, i64, vp, vp]),
     ("icc_immersed_config_default", None, [vp]),
     ("icc_immersed_create", cint, [vp, vp, vp, i64, vpp]),

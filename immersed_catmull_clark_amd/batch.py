@@ -1,6 +1,6 @@
 """Batched structured instances of uneven sizes over libicc's graph-batched
-solver (icc_batch_*): every unfinished instance's PCG iteration chain is one
-branch of a HIP graph of 16 iterations, and each instance stops on its own."""
+solver (icc_batch_*): the PCG iterations of every unfinished instance are
+captured into one HIP graph, and each instance stops on its own."""
 import ctypes as C
 
 import numpy as np
@@ -31,6 +31,10 @@ class Batch:
         fr = None if free is None else np.ascontiguousarray(free, dtype=np.uint8)
         f = None if force is None else np.ascontiguousarray(force, dtype=np.float64)
         check(lib().icc_batch_set(self.h, int(k), _p(s), _p(fr), _p(f)))
+
+    def set_traction(self, k, t):
+        tt = np.ascontiguousarray(t, dtype=np.float64)
+        check(lib().icc_batch_set_traction(self.h, int(k), _p(tt)))
 
     def solve(self, prec=ICC_F32, tol=1e-8, delta=0.1, maxit=0):
         """Returns (list of u per instance, stats [n, 4]: iterations, reliable

@@ -1146,20 +1146,20 @@ ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out512) {
 }
 
 // ---------------------------------------------------------------------------
-// Batched instances: every unfinished instance's PCG iteration chain is one
-// branch of a HIP graph of PCG_BATCH iterations (captured by forking one
-// stream per instance off the context stream), so small instances run side by
-// side; each instance's kernels return at once when its device state is done.
-// Mixed precision: the reliable updates of the instances whose FP32 residual
-// dropped by delta run on the context stream between graph replays.
+// Batched instances: the PCG iterations of every unfinished instance are
+// captured, one instance after another, into one HIP graph of PCG_BATCH
+// iterations per instance on the context stream, replayed until every
+// instance has stopped; each instance's kernels return at once when its own
+// device state is done.  Mixed precision: the reliable updates of the
+// instances whose FP32 residual dropped by delta run on the context stream
+// between graph replays.  (Capturing the instances as concurrent graph
+// branches on forked streams gave wrong iteration counts on this platform,
+// so the chains are serialised.)
 struct icc_batch {
   icc_ctx* ctx = nullptr;
   std::vector<icc_grid*> g;
   std::vector<icc_op*> op;
   std::vector<double*> d_f, d_u;
-  std::vector<hipStream_t> streams;
-  std::vector<hipEvent_t> join;
-  hipEvent_t fork = nullptr;
 };
 
 ICC_API int icc_batch_destroy(icc_batch* b) {
@@ -1170,9 +1170,6 @@ ICC_API int icc_batch_destroy(icc_batch* b) {
   }
   for (double* p : b->d_f) hipFree(p);
   for (double* p : b->d_u) hipFree(p);
-  for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
-  for (hipEvent_t e : b->join) (void)hipEventDestroy(e);
-  if (b->fork) (void)hipEventDestroy(b->fork);
   delete b;
   return ICC_OK;
 }
@@ -1196,12 +1193,6 @@ ICC_API int icc_batch_create(icc_ctx* ctx, int64_t n, const int64_t* nel, const 
     if (rc == ICC_OK && (rc = dalloc(&f, g->s.ndof)) == ICC_OK) rc = dalloc(&u, g->s.ndof);
     b->d_f.push_back(f);
     b->d_u.push_back(u);
-    hipStream_t s = nullptr;
-    hipEvent_t e = nullptr;
-    if (rc == ICC_OK && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) rc = ICC_EHIP;
-    if (rc == ICC_OK && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) rc = ICC_EHIP;
-    b->streams.push_back(s);
-    b->join.push_back(e);
     if (rc == ICC_OK) {
       // default: clamp x = 0, no load
       const GridSetup& gs = g->s;
@@ -1213,10 +1204,9 @@ ICC_API int icc_batch_create(icc_ctx* ctx, int64_t n, const int64_t* nel, const 
       if (rc == ICC_OK && hipMemsetAsync(f, 0, gs.ndof * 8, ctx->stream) != hipSuccess) rc = ICC_EHIP;
     }
   }
-  if (rc == ICC_OK && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) rc = ICC_EHIP;
   if (rc == ICC_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ICC_EHIP;
   if (rc != ICC_OK) {
-    if (rc == ICC_EHIP) set_error("icc_batch_create: HIP stream/event creation failed");
+    if (rc == ICC_EHIP) set_error("icc_batch_create: HIP error");
     icc_batch_destroy(b);
     return rc;
   }
@@ -1260,26 +1250,39 @@ ICC_API int icc_batch_set(icc_batch* b, int64_t k, const double* scale, const ui
 }
 
 namespace icc {
-// Captures PCG_BATCH iterations of every instance in `live` as parallel
-// branches off the context stream.
+void traction_xmax(const GridSetup& s, const double t[3], std::vector<double>& f);
+}
+
+ICC_API int icc_batch_set_traction(icc_batch* b, int64_t k, const double t[3]) {
+  if (!b || !t || k < 0 || k >= (int64_t)b->op.size()) return ICC_EINVAL;
+  const GridSetup& gs = b->g[k]->s;
+  std::vector<double> f;
+  traction_xmax(gs, t, f);
+  std::vector<uint8_t> fr(gs.ndof, 1);
+  if (b->op[k]->d_free) ICC_HIP(hipMemcpy(fr.data(), b->op[k]->d_free, gs.ndof, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < gs.ndof; ++i)
+    if (!fr[i]) f[i] = 0.0;
+  ICC_HIP(hipMemcpyAsync(b->d_f[k], f.data(), gs.ndof * 8, hipMemcpyHostToDevice, b->ctx->stream));
+  ICC_HIP(hipStreamSynchronize(b->ctx->stream));
+  return ICC_OK;
+}
+
+namespace icc {
+// Captures PCG_BATCH iterations of every instance in `live`, one instance
+// after another, on the context stream.
 template <typename T>
 static int capture_batched(icc_batch* b, const std::vector<int>& live, const std::vector<Work*>& w, IterGraph& G) {
   hipStream_t m = b->ctx->stream;
   ICC_HIP(hipStreamBeginCapture(m, hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipEventRecord(b->fork, m);
+  hipError_t e = hipSuccess;
   for (int k : live) {
-    if (e != hipSuccess) break;
-    hipStream_t s = b->streams[k];
     Work* wk = w[k];
-    e = hipStreamWaitEvent(s, b->fork, 0);
     for (int i = 0; i < PCG_BATCH && e == hipSuccess; ++i) {
       if constexpr (std::is_same<T, double>::value)
-        e = enqueue_iteration<double>(b->op[k], wk, b->d_u[k], wk->r, wk->p, wk->ap, wk->dinv, s);
+        e = enqueue_iteration<double>(b->op[k], wk, b->d_u[k], wk->r, wk->p, wk->ap, wk->dinv, m);
       else
-        e = enqueue_iteration<float>(b->op[k], wk, wk->x32, wk->r32, wk->p32, wk->ap32, wk->dinv32, s);
+        e = enqueue_iteration<float>(b->op[k], wk, wk->x32, wk->r32, wk->p32, wk->ap32, wk->dinv32, m);
     }
-    if (e == hipSuccess) e = hipEventRecord(b->join[k], s);
-    if (e == hipSuccess) e = hipStreamWaitEvent(m, b->join[k], 0);
   }
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(m, &graph);

@@ -92,7 +92,7 @@ int solve_state_dev(icc_topopt* t, const double* x_dev, const double* u0_dev, do
   } else {
     ICC_HIP(launch_fill<double>(t->du, 0.0, n, st));
     double stats[8];
-    int rc = icc_solve_ir_dev(t->op, ICC_F32, t->b, tol, 1e-3, 200, 20 * n, t->du, stats);
+    int rc = icc_solve_mixed_dev(t->op, t->b, tol, 0.1, 20 * n, t->du, stats);
     if (rc != ICC_OK && rc != ICC_ENOCONV && rc != ICC_EBREAKDOWN) return rc;
     its = (int64_t)stats[1];
     res = stats[2];

@@ -183,7 +183,7 @@ typedef struct icc_topopt_config {
   double young, poisson, emin, move_limit, pcg_tol;
   int32_t load;   /* 0: reference nodal load (topopt_iga.rs:79-98); 1: unit downward traction on x = L */
   int32_t filter; /* 0: sensitivity filter (topopt_iga.rs:200-205); 1: density filter with adjoint backward */
-  int32_t solver; /* 0: FP64 Jacobi PCG; 1: FP32-inner iterative refinement to the same true residual */
+  int32_t solver; /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates to the same true residual */
   int32_t pad;
 } icc_topopt_config;
 /* CantileverTopOptConfig::default (topopt_iga.rs:35-51) */
@@ -261,10 +261,10 @@ int icc_immersed_solve_dev(icc_immersed* h, double* u_dev, double* stats);
 /* ---- batched instances (uneven sizes) ------------------------------------------ */
 /* n structured p = 2 instances, instance k on [0,size_k0]x[0,size_k1]x[0,size_k2]
  * (size NULL: 2 x 1 x 1) with nel_k = nel[3k..3k+2]; default free mask clamps
- * x = 0, default force is zero.  Every unfinished instance's PCG iteration chain
- * is one branch of a HIP graph of 16 iterations, replayed until all converge;
- * each instance stops on its own.  No counterpart in the reference (it solves
- * one instance at a time). */
+ * x = 0, default force is zero.  The PCG iterations of every unfinished
+ * instance (16 per instance) are captured into one HIP graph, replayed until
+ * all have stopped; each instance stops on its own.  No counterpart in the
+ * reference (it solves one instance at a time). */
 typedef struct icc_batch icc_batch;
 int icc_batch_create(icc_ctx* ctx, int64_t n, const int64_t* nel, const double* size, double young, double poisson,
                      icc_batch** out);
@@ -274,6 +274,8 @@ int icc_batch_sizes(const icc_batch* b, int64_t* ndof, int64_t* nel);
 /* instance k: element scale [nel_k] (NULL: keep), free mask [ndof_k] (NULL: clamp
  * x = 0), force [ndof_k] (NULL: keep; masked on entry) */
 int icc_batch_set(icc_batch* b, int64_t k, const double* scale, const uint8_t* free_mask, const double* force);
+/* instance k: force = consistent traction t on the face x = x_max, masked */
+int icc_batch_set_traction(icc_batch* b, int64_t k, const double t[3]);
 /* Solves every instance from u = 0.  prec ICC_F64: Jacobi PCG to |r|/|b| < tol;
  * ICC_F32: FP32 PCG with FP64 reliable updates every `delta` drop, to a true
  * relative residual <= tol.  u (host, may be NULL): instances concatenated.

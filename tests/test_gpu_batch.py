@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 icc = pytest.importorskip("immersed_catmull_clark_amd")
 
-NELS = [(8, 4, 4), (12, 6, 6), (16, 8, 8), (10, 4, 6)]
+NELS = [(8, 4, 4), (12, 6, 6), (16, 8, 8), (10, 4, 6), (36, 18, 18)]
 
 
 @pytest.fixture(scope="module")
@@ -57,10 +57,11 @@ def test_batch_fp64_reproduces_single_solves(ctx):
         # amplify that round-off (the stopping test may flip by one iteration)
         assert abs(int(st[k, 0]) - it) <= 1
         assert O.rel_err(us[k], x) < 1e-9
-        sp.assemble(s)
-        r = free * (f - sp.matvec(free * us[k]))
-        assert np.linalg.norm(r) / np.linalg.norm(f) < 2e-9
-        assert abs(st[k, 2] - np.linalg.norm(r) / np.linalg.norm(f)) < 1e-12
+        if sp.nel <= 2000:
+            sp.assemble(s)
+            r = free * (f - sp.matvec(free * us[k]))
+            assert np.linalg.norm(r) / np.linalg.norm(f) < 2e-9
+            assert abs(st[k, 2] - np.linalg.norm(r) / np.linalg.norm(f)) < 1e-12
 
 
 def test_batch_mixed_true_residuals(ctx):
@@ -71,7 +72,15 @@ def test_batch_mixed_true_residuals(ctx):
     us, st, rc = b.solve(prec=icc.ICC_F32, tol=1e-8, delta=0.1)
     assert rc == icc.ICC_OK, st
     for k, (sp, s, free, f) in enumerate(inst):
-        sp.assemble(s)
-        r = free * (f - sp.matvec(free * us[k]))
-        assert np.linalg.norm(r) / np.linalg.norm(f) <= 1.01e-8
-        assert st[k, 1] >= 1 and st[k, 3] == icc.ICC_OK
+        assert st[k, 1] >= 1 and st[k, 3] == icc.ICC_OK and st[k, 2] <= 1e-8
+        # the same instance alone: same number of FP32 iterations up to the
+        # round-off of the atomic scatter
+        g = icc.Grid(ctx, NELS[k], [[0, 2.0], [0, 1.0], [0, 1.0]])
+        op = icc.Operator(g, scale=s, free=free)
+        u1, st1, rc1 = op.solve_mixed(f, tol=1e-8, delta=0.1)
+        assert rc1 == icc.ICC_OK
+        assert abs(int(st[k, 0]) - st1["inner_iters"]) <= max(3, st1["inner_iters"] // 50), (st[k, 0], st1)
+        if sp.nel <= 2000:
+            sp.assemble(s)
+            r = free * (f - sp.matvec(free * us[k]))
+            assert np.linalg.norm(r) / np.linalg.norm(f) <= 1.01e-8

@@ -65,35 +65,42 @@ def test_solve_state_matches_oracle(ctx):
 @pytest.mark.parametrize("filt", [0, 1])
 def test_adjoint_vs_finite_differences(ctx, filt):
     """dc/dx from the hand-written sensitivity kernel (+ density filter
-    backward) vs central differences of c(x), h = 1e-4, each solve to 1e-13.
-    With c ~ 137 the difference quotient at h = 1e-6 carries round-off of a
-    few 1e-6 relative even with exact direct solves; at h = 1e-4 the
-    truncation and round-off errors are both below 1e-7."""
+    backward) vs central differences of c(x) with h = 1e-6 on 16 seeded
+    elements, every solve to 1e-13.  Each side's c(x') - c(x) is summed from
+    small terms, exactly for exact states (u' = u + d, K' the operator of x'):
+        c(x') - c(x) = -sum_f (s'_f - s_f) ee_f(u) + 2 d.(F - K' u) - d.K' d,
+    so the difference is not swamped by the round-off of c itself."""
     cfg = icc.CantileverTopOptConfig(nel=(12, 6, 6), size=(2.0, 1.0, 1.0), emin=1e-9, rmin=1.5, pcg_tol=1e-13,
                                      load=1, filter=filt)
     t = icc.CantileverTopOpt(ctx, cfg)
     sp = O.Space([[0, 2.0], [0, 1.0], [0, 1.0]], [12, 6, 6], 2)
+    grid = icc.Grid(ctx, (12, 6, 6), [[0, 2.0], [0, 1.0], [0, 1.0]])
+    op = icc.Operator(grid, free=t.free)
     rng = np.random.default_rng(21)
     x = rng.uniform(0.2, 1.0, t.nel)
     c, g, u = t.compliance(x)
 
-    def c_hat(xd, u0):
-        # 2 F.u - u^T K(x) u: equals c at the solution, error quadratic in the
-        # solve residual (F.u alone is first order in it); summed exactly
-        # (fsum) so round-off does not dominate the h = 1e-6 difference
-        _, _, uu = t.compliance(xd, u0=u0, grad=False)
+    def scale(xd):
         xt = sp.filter_apply(cfg.rmin, 1, None, xd) if filt == 1 else xd
-        s = cfg.emin + (1 - cfg.emin) * xt ** cfg.penal
-        terms = np.concatenate([2.0 * t.force * uu, -(s * t.element_energies(uu))])
-        return math.fsum(terms)
+        return cfg.emin + (1 - cfg.emin) * xt ** cfg.penal
 
-    assert abs(c_hat(x, u) - c) < 1e-10 * c
-    hstep = 1e-4
-    for e in rng.choice(t.nel, 8, replace=False):
+    s0, ee0 = scale(x), t.element_energies(u)
+    assert abs(math.fsum(s0 * ee0) - c) < 1e-10 * c  # u^T K u = F.u at the solution
+
+    def dcomp(xd):
+        sd = scale(xd)
+        _, _, ud = t.compliance(xd, u0=u, grad=False)
+        d = ud - u
+        op.set_scale(sd)
+        ku, kd = op.apply(u), op.apply(d)
+        return -math.fsum((sd - s0) * ee0) + 2.0 * math.fsum(d * (t.force - ku)) - math.fsum(d * kd)
+
+    hstep = 1e-6
+    for e in rng.choice(t.nel, 16, replace=False):
         xp, xm = x.copy(), x.copy()
         xp[e] += hstep
         xm[e] -= hstep
-        fd = (c_hat(xp, u) - c_hat(xm, u)) / (2 * hstep)
+        fd = (dcomp(xp) - dcomp(xm)) / (2 * hstep)
         assert abs(g[e] - fd) <= 1e-6 * abs(fd), (e, g[e], fd)
 
 
@@ -105,3 +112,17 @@ def test_traction_load_total(ctx):
     assert O.rel_err(t.force, ref) < 1e-14
     nsc = sp.ndof_sc
     assert abs(t.force[nsc:2 * nsc].sum() + 0.5) < 1e-13
+
+
+def test_mixed_solver_state_matches_fp64(ctx):
+    """solver 1 (FP32 PCG with FP64 reliable updates) reaches the state of the
+    FP64 solve: compliance and dc/dx to the tolerance's level."""
+    base = dict(nel=(16, 8, 8), size=(2.0, 1.0, 1.0), emin=1e-9, pcg_tol=1e-10, load=1, filter=1)
+    t64 = icc.CantileverTopOpt(ctx, icc.CantileverTopOptConfig(solver=0, **base))
+    t32 = icc.CantileverTopOpt(ctx, icc.CantileverTopOptConfig(solver=1, **base))
+    x = np.random.default_rng(9).uniform(0.05, 1.0, t64.nel)
+    c64, g64, u64 = t64.compliance(x)
+    c32, g32, u32 = t32.compliance(x)
+    assert abs(c32 - c64) < 1e-8 * c64
+    assert O.rel_err(g32, g64) < 1e-7
+    assert O.rel_err(u32, u64) < 1e-6
