@@ -165,6 +165,9 @@ def batch_leg(np, icc, ctx, seed=20261205):
     for k, (s, mag) in enumerate(setups):
         b.set(k, scale=s)
         b.set_traction(k, (0.0, -mag, 0.0))
+    t0 = time.time()
+    b.autotune(icc.ICC_F32, reps=3)  # tile shapes per instance size
+    t_tune = time.time() - t0
     ctx.sync()
     t0 = time.time()
     us, st, rc = b.solve(prec=icc.ICC_F32, tol=1e-8, delta=0.1)
@@ -176,6 +179,7 @@ def batch_leg(np, icc, ctx, seed=20261205):
         b1 = icc.Batch(ctx, [nel[k]])
         b1.set(0, scale=s)
         b1.set_traction(0, (0.0, -mag, 0.0))
+        b1.autotune(icc.ICC_F32, reps=3)
         ctx.sync()
         t0 = time.time()
         b1.solve(prec=icc.ICC_F32, tol=1e-8, delta=0.1)
@@ -183,7 +187,7 @@ def batch_leg(np, icc, ctx, seed=20261205):
         b1.close()
     return dict(instances=len(nel), nx=nx, ndof=ndof, total_ndof=int(sum(ndof)), time_s=dt, status=int(rc),
                 iterations=[int(v) for v in st[:, 0]], max_true_rel_residual=float(st[:, 2].max()),
-                sequential_time_s=seq, speedup_vs_sequential=seq / dt)
+                sequential_time_s=seq, speedup_vs_sequential=seq / dt, autotune_s=t_tune)
 
 
 def adjoint_leg(np, icc, ctx, oc_iters=20, nfd=32, seed=20261206):
@@ -295,7 +299,10 @@ def main():
     torch.cuda.synchronize()
 
     flops = 2.0 * 81 * 81 * nel
+    # tile shape and workgroups per CU picked by timing (FP64 apply first)
+    tuned = {"f64": op.autotune(icc.ICC_F64, reps=5)}
     if args.apply_only and args.apply_prec == "f32":
+        op.autotune(icc.ICC_F32, reps=5)
         ms32 = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F32, args.steps, args.warmup, accumulate=True)
         print(json.dumps({"kernel": "apply_tiles<float>", "kernel_ms": ms32,
                           "tflops": flops / (ms32 * 1e-3) / 1e12}), flush=True)
@@ -323,6 +330,7 @@ def main():
         return
 
     log("bench: FP32 applies")
+    tuned["f32"] = op.autotune(icc.ICC_F32, reps=5)
     ms_f32 = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F32, args.steps, args.warmup)
     ms_k32 = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F32, args.steps, args.warmup, accumulate=True)
     yh64 = op.apply(xh)
@@ -341,6 +349,7 @@ def main():
         res = {}
         for name in ("fp64", "mixed_fp32"):
             log(f"bench: PCG {name}")
+            tuned[name] = op.autotune(icc.ICC_F64 if name == "fp64" else icc.ICC_F32, reps=5)
             u = torch.zeros(ndof, dtype=torch.float64, device="cuda")
             stats = np.zeros(8)
             torch.cuda.synchronize()
@@ -388,6 +397,7 @@ def main():
         cb["rel_diff_vs_gpu_f64"] = float(np.linalg.norm(free * ycpu - yg) / np.linalg.norm(yg))
         out["cpu_baseline"] = cb
     out["wall_s_f64_apply_loop"] = wall
+    out["tiling"] = {k: {"variant": v[0], "workgroups_per_cu": v[1], "us_per_apply": v[2]} for k, v in tuned.items()}
     print(json.dumps(out), flush=True)
 
 

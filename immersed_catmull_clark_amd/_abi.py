@@ -75,11 +75,15 @@ SIGNATURES = [
     ("icc_topopt_element_energies", cint, [vp, vp, vp]),
     ("icc_topopt_compliance", cint, [vp, vp, vp, vp, vp]),
     ("icc_topopt_run", cint, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    ("icc_grid_set_tiling", cint, [vp, cint, cint]),
+    ("icc_grid_tiling", cint, [vp, vp]),
+    ("icc_grid_autotune", cint, [vp, cint, cint, vp, vp, vp]),
     ("icc_batch_create", cint, [vp, i64, vp, vp, dbl, dbl, vpp]),
     ("icc_batch_destroy", cint, [vp]),
     ("icc_batch_sizes", cint, [vp, vp, vp]),
     ("icc_batch_set", cint, [vp, i64, vp, vp, vp]),
     ("icc_batch_set_traction", cint, [vp, i64, vp]),
+    ("icc_batch_autotune", cint, [vp, cint, cint]),
     ("icc_batch_solve", cint, [vp, cint, dbl, dbl, i64, vp, vp]),
     ("icc_immersed_config_default", None, [vp]),
     ("icc_immersed_create", cint, [vp, vp, vp, i64, vpp]),

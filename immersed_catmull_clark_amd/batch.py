@@ -36,6 +36,10 @@ class Batch:
         tt = np.ascontiguousarray(t, dtype=np.float64)
         check(lib().icc_batch_set_traction(self.h, int(k), _p(tt)))
 
+    def autotune(self, prec=ICC_F32, reps=3):
+        """Tile shapes per instance, by timing each instance's apply."""
+        check(lib().icc_batch_autotune(self.h, int(prec), int(reps)))
+
     def solve(self, prec=ICC_F32, tol=1e-8, delta=0.1, maxit=0):
         """Returns (list of u per instance, stats [n, 4]: iterations, reliable
         updates, true relative residual, status; return code)."""

@@ -95,6 +95,16 @@ class Grid:
         check(lib().icc_grid_element_matrix(self.h, int(t), _p(out)))
         return out
 
+    def set_tiling(self, variant, wg_per_cu=2):
+        check(lib().icc_grid_set_tiling(self.h, int(variant), int(wg_per_cu)))
+
+    @property
+    def tiling(self):
+        """(variant, workgroups per CU, tile count)."""
+        out = np.zeros(3, dtype=np.int64)
+        check(lib().icc_grid_tiling(self.h, _p(out)))
+        return tuple(int(v) for v in out)
+
     def wq_diag(self, x=None, penal=3.0, emin=1e-3):
         d = np.zeros(self.ndof)
         xx = None if x is None else _f64(x)
@@ -189,6 +199,15 @@ class Operator:
         st = dict(outer=int(stats[0]), inner_iters=int(stats[1]), true_res=float(stats[2]),
                   fp64_applies=int(stats[3]), inner_applies=int(stats[4]))
         return u, st, rc
+
+    def autotune(self, prec=ICC_F64, reps=5):
+        """Picks the grid's tiling by timing this operator's apply; returns
+        (variant, workgroups per CU, microseconds per apply)."""
+        v = np.zeros(1, dtype=np.int64)
+        wg = np.zeros(1, dtype=np.int64)
+        us = np.zeros(1)
+        check(lib().icc_grid_autotune(self.h, int(prec), int(reps), _p(v), _p(wg), _p(us)))
+        return int(v[0]), int(wg[0]), float(us[0])
 
     def solve_mixed(self, f, tol=1e-8, delta=0.1, maxit=1000000, u0=None):
         """FP32 PCG with reliable updates against the FP64 operator: the FP32

@@ -268,6 +268,7 @@ ICC_API int icc_op_create(icc_grid* g, icc_op** out) {
     icc_op_destroy(op);
     return hip_fail(e, "icc_op_create");
   }
+  g->ops.push_back(op);
   *out = op;
   return ICC_OK;
 }
@@ -325,7 +326,7 @@ static int ensure_work(icc_op* op, Work*& out) {
     free_work(*w);
     w->n = n;
     int rc;
-    size_t npap = (size_t)apply_grid(op->g) + VEC_MAX_BLOCKS;
+    size_t npap = (size_t)2 * (op->g->ctx->num_cu > 0 ? op->g->ctx->num_cu : 256) + VEC_MAX_BLOCKS;
     if ((rc = dalloc(&w->x, n)) || (rc = dalloc(&w->r, n)) || (rc = dalloc(&w->p, n)) || (rc = dalloc(&w->ap, n)) ||
         (rc = dalloc(&w->dinv, n)) || (rc = dalloc(&w->tmp, n)) || (rc = dalloc(&w->f, n)) ||
         (rc = dalloc(&w->x32, n)) || (rc = dalloc(&w->r32, n)) || (rc = dalloc(&w->p32, n)) ||
@@ -351,6 +352,10 @@ static int ensure_work(icc_op* op, Work*& out) {
 
 ICC_API int icc_op_destroy(icc_op* op) {
   if (!op) return ICC_OK;
+  if (op->g) {
+    auto& v = op->g->ops;
+    v.erase(std::remove(v.begin(), v.end(), op), v.end());
+  }
   drop_work(op);
   hipFree(op->d_scale);
   hipFree(op->d_scale32);
@@ -384,19 +389,12 @@ ICC_API int icc_op_set_scale(icc_op* op, const double* s) {
   return ICC_OK;
 }
 
-ICC_API int icc_op_set_free_mask(icc_op* op, const uint8_t* fm) {
-  if (!op) return ICC_EINVAL;
-  const int64_t n = op->g->s.ndof;
-  if (!fm) {
-    hipFree(op->d_free);
-    hipFree(op->d_tmask);
-    op->d_free = nullptr;
-    op->d_tmask = nullptr;
-    return ICC_OK;
-  }
-  // per tile: does any staged point carry a fixed DOF?  Only such tiles read
-  // the mask inside the apply kernel.
+namespace icc {
+// Per tile: does any staged point carry a fixed DOF?  Only such tiles read
+// the mask inside the apply kernel.  Rebuilt whenever the grid is re-tiled.
+static int upload_tile_mask(icc_op* op) {
   const auto& s = op->g->s;
+  const std::vector<uint8_t>& fm = op->h_free;
   std::vector<uint8_t> tm(s.tiles.size(), 0);
   for (size_t ti = 0; ti < s.tiles.size(); ++ti) {
     const Tile& t = s.tiles[ti];
@@ -411,11 +409,58 @@ ICC_API int icc_op_set_free_mask(icc_op* op, const uint8_t* fm) {
             }
     tm[ti] = fixed ? 1 : 0;
   }
-  if (!op->d_free) ICC_TRY(dalloc(&op->d_free, n));
-  if (!op->d_tmask) ICC_TRY(dalloc(&op->d_tmask, (int64_t)tm.size()));
-  ICC_HIP(hipMemcpyAsync(op->d_free, fm, n, hipMemcpyHostToDevice, op->g->ctx->stream));
+  hipFree(op->d_tmask);
+  op->d_tmask = nullptr;
+  ICC_TRY(dalloc(&op->d_tmask, (int64_t)tm.size()));
   ICC_HIP(hipMemcpyAsync(op->d_tmask, tm.data(), tm.size(), hipMemcpyHostToDevice, op->g->ctx->stream));
   ICC_HIP(hipStreamSynchronize(op->g->ctx->stream));
+  return ICC_OK;
+}
+}  // namespace icc
+
+ICC_API int icc_op_set_free_mask(icc_op* op, const uint8_t* fm) {
+  if (!op) return ICC_EINVAL;
+  const int64_t n = op->g->s.ndof;
+  if (!fm) {
+    hipFree(op->d_free);
+    hipFree(op->d_tmask);
+    op->d_free = nullptr;
+    op->d_tmask = nullptr;
+    op->h_free.clear();
+    return ICC_OK;
+  }
+  op->h_free.assign(fm, fm + n);
+  if (!op->d_free) ICC_TRY(dalloc(&op->d_free, n));
+  ICC_HIP(hipMemcpyAsync(op->d_free, fm, n, hipMemcpyHostToDevice, op->g->ctx->stream));
+  return upload_tile_mask(op);
+}
+
+ICC_API int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu) {
+  if (!g || variant < 0 || variant >= n_tile_variants() || wg_per_cu < 1 || wg_per_cu > 2) return ICC_EINVAL;
+  ICC_HIP(hipStreamSynchronize(g->ctx->stream));
+  build_tiles(g->s, variant);
+  Tile* d = nullptr;
+  ICC_TRY(dalloc(&d, g->s.tiles.size()));
+  if (hipMemcpy(d, g->s.tiles.data(), g->s.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(d);
+    set_error("icc_grid_set_tiling: upload failed");
+    return ICC_EHIP;
+  }
+  hipFree(g->d_tiles);
+  g->d_tiles = d;
+  g->ntiles = (int)g->s.tiles.size();
+  g->tile_variant = variant;
+  g->wg_per_cu = wg_per_cu;
+  for (icc_op* op : g->ops)
+    if (op->d_free) ICC_TRY(upload_tile_mask(op));
+  return ICC_OK;
+}
+
+ICC_API int icc_grid_tiling(const icc_grid* g, int64_t out[3]) {
+  if (!g || !out) return ICC_EINVAL;
+  out[0] = g->tile_variant;
+  out[1] = g->wg_per_cu;
+  out[2] = g->ntiles;
   return ICC_OK;
 }
 
@@ -1146,6 +1191,68 @@ ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out512) {
 }
 
 // ---------------------------------------------------------------------------
+// Tiling autotuner: times `reps` accumulate-only applies of op in precision
+// prec for every tiling variant and workgroups-per-CU choice, and keeps the
+// fastest (ties to the lower variant).  The operator's vectors are scratch.
+ICC_API int icc_grid_autotune(icc_op* op, int prec, int reps, int64_t* best_variant, int64_t* best_wg,
+                              double* best_us) {
+  if (!op || (prec != ICC_F64 && prec != ICC_F32) || reps < 1) return ICC_EINVAL;
+  if (op->s_nnz > 0) {
+    set_error("icc_grid_autotune: operators with a sparse term are not supported");
+    return ICC_ENOTSUP;
+  }
+  Work* w;
+  ICC_TRY(ensure_work(op, w));
+  icc_grid* g = op->g;
+  hipStream_t st = g->ctx->stream;
+  const int64_t n = g->s.ndof;
+  ICC_HIP(launch_fill<double>(w->tmp, 1.0, n, st));
+  ICC_HIP(launch_fill<double>(w->ap, 0.0, n, st));
+  ICC_HIP(launch_fill<float>(w->x32, 1.0f, n, st));
+  ICC_HIP(launch_fill<float>(w->ap32, 0.0f, n, st));
+  hipEvent_t e0, e1;
+  ICC_HIP(hipEventCreate(&e0));
+  ICC_HIP(hipEventCreate(&e1));
+  int bv = 0, bw = 2;
+  float bt = 1e30f;
+  int rc = ICC_OK;
+  for (int v = 0; v < n_tile_variants() && rc == ICC_OK; ++v)
+    for (int wg = 1; wg <= 2 && rc == ICC_OK; ++wg) {
+      if ((rc = icc_grid_set_tiling(g, v, wg)) != ICC_OK) break;
+      hipError_t e = hipSuccess;
+      for (int r = -2; r < reps && e == hipSuccess; ++r) {
+        if (r == 0) e = hipEventRecord(e0, st);
+        if (e != hipSuccess) break;
+        if (prec == ICC_F64)
+          e = launch_apply_f64(g, op, w->tmp, w->ap, nullptr, nullptr, st);
+        else
+          e = launch_apply_f32(g, op, w->x32, w->ap32, nullptr, nullptr, st);
+      }
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "icc_grid_autotune");
+        break;
+      }
+      if (ms < bt) {
+        bt = ms;
+        bv = v;
+        bw = wg;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != ICC_OK) return rc;
+  ICC_TRY(icc_grid_set_tiling(g, bv, bw));
+  if (best_variant) *best_variant = bv;
+  if (best_wg) *best_wg = bw;
+  if (best_us) *best_us = 1e3 * (double)bt / reps;
+  return ICC_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Batched instances: the PCG iterations of every unfinished instance are
 // captured, one instance after another, into one HIP graph of PCG_BATCH
 // iterations per instance on the context stream, replayed until every
@@ -1246,6 +1353,12 @@ ICC_API int icc_batch_set(icc_batch* b, int64_t k, const double* scale, const ui
     ICC_HIP(hipMemcpyAsync(b->d_f[k], fm.data(), gs.ndof * 8, hipMemcpyHostToDevice, b->ctx->stream));
     ICC_HIP(hipStreamSynchronize(b->ctx->stream));
   }
+  return ICC_OK;
+}
+
+ICC_API int icc_batch_autotune(icc_batch* b, int prec, int reps) {
+  if (!b) return ICC_EINVAL;
+  for (size_t k = 0; k < b->op.size(); ++k) ICC_TRY(icc_grid_autotune(b->op[k], prec, reps, nullptr, nullptr, nullptr));
   return ICC_OK;
 }
 

@@ -78,6 +78,9 @@ struct GridSetup {
 };
 
 int build_setup(const int64_t nel[3], const double bounds[6], double young, double poisson, GridSetup& s);
+// (Re)builds s.tiles with tiling variant v (0 .. n_tile_variants() - 1).
+void build_tiles(GridSetup& s, int variant);
+int n_tile_variants();
 int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
 
 }  // namespace icc
@@ -104,6 +107,9 @@ struct icc_grid {
   double* d_wq = nullptr;     // WQ tables [A0x, A1x, A0y, A1y, A0z, A1z], built on first use
   int maxnel = 0;
   int T[3] = {0, 0, 0};
+  int tile_variant = 0;  // build_tiles variant of s.tiles
+  int wg_per_cu = 2;     // persistent apply workgroups per CU
+  std::vector<struct icc_op*> ops;  // operators on this grid (their tile masks follow the tiling)
 };
 
 struct icc_op {
@@ -112,6 +118,7 @@ struct icc_op {
   float* d_scale32 = nullptr;   // [nel]
   uint8_t* d_free = nullptr;    // [ndof], null = all free
   uint8_t* d_tmask = nullptr;   // [ntiles]: 1 if the tile stages a fixed DOF
+  std::vector<uint8_t> h_free;  // host copy of the free mask
   // optional sparse add-on (CSR, int32 indices)
   int64_t s_nnz = 0;
   int32_t* d_s_indptr = nullptr;

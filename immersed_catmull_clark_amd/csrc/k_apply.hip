@@ -535,11 +535,11 @@ __global__ void mfma_layout_probe(double* out) {
 }
 
 // ---- launchers ---------------------------------------------------------------
-// Persistent apply grid: two workgroups per CU (the VGPR budget of the MFMA
-// loop allows twelve waves per CU), a multiple of 8 so the XCD grouping of
-// tile_range applies, never more workgroups than tiles.
+// Persistent apply grid: wg_per_cu workgroups per CU (default two; the VGPR
+// budget of the MFMA loop allows twelve waves per CU), a multiple of 8 so the
+// XCD grouping of tile_range applies, never more workgroups than tiles.
 int apply_grid(const icc_grid* g) {
-  int G = 2 * (g->ctx->num_cu > 0 ? g->ctx->num_cu : 256);
+  int G = g->wg_per_cu * (g->ctx->num_cu > 0 ? g->ctx->num_cu : 256);
   G -= G % 8;
   if (G > g->ntiles) G = g->ntiles;
   return G < 1 ? 1 : G;

@@ -269,6 +269,27 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
     s.kmat.push_back(std::move(K));
   }
 
+  build_tiles(s, 0);
+  return ICC_OK;
+}
+
+// Tiles of constant-type sub-boxes.  Sub-boxes at least 8 elements long in x
+// are cut into x-line tiles: lines of <= 16 elements spread evenly, and up to
+// a variant-dependent number of lines per tile (y x z candidates tried in the
+// variant's order, the first one that fills the most lines within the
+// staging limits wins).  Shorter sub-boxes use tile_shape.
+int n_tile_variants() { return 6; }
+
+void build_tiles(GridSetup& s, int variant) {
+  static const int yz_orders[6][10][2] = {
+      {{2, 4}, {4, 2}, {1, 8}, {8, 1}, {2, 2}, {1, 4}, {4, 1}, {1, 2}, {2, 1}, {1, 1}},
+      {{4, 2}, {2, 4}, {8, 1}, {1, 8}, {2, 2}, {4, 1}, {1, 4}, {2, 1}, {1, 2}, {1, 1}},
+      {{1, 8}, {2, 4}, {4, 2}, {8, 1}, {1, 4}, {2, 2}, {4, 1}, {1, 2}, {2, 1}, {1, 1}},
+      {{8, 1}, {4, 2}, {2, 4}, {1, 8}, {4, 1}, {2, 2}, {1, 4}, {2, 1}, {1, 2}, {1, 1}},
+      {{2, 2}, {1, 4}, {4, 1}, {1, 2}, {2, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}},
+      {{1, 4}, {4, 1}, {2, 2}, {1, 2}, {2, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}},
+  };
+  if (variant < 0 || variant >= n_tile_variants()) variant = 0;
   // Constant-type sub-boxes: runs of equal 1D type along each direction.
   std::vector<std::pair<int64_t, int64_t>> runs[3];
   for (int d = 0; d < 3; ++d) {
@@ -292,11 +313,10 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
           lines = 1;
           int best = -1;
           ty = tz = 1;
-          static const int yz[][2] = {{2, 4}, {4, 2}, {1, 8}, {8, 1}, {2, 2}, {1, 4}, {4, 1}, {1, 2}, {2, 1}, {1, 1}};
-          for (auto& c : yz) {
+          for (auto& c : yz_orders[variant]) {
             const int by = (int)std::min<int64_t>(c[0], ey), bz = (int)std::min<int64_t>(c[1], ez);
             const int pts = (tx + 2) * (by + 2) * (bz + 2);
-            const int score = by * bz * 1000 - pts;
+            const int score = by * bz * 1000;
             if (pts <= TILE_MAX_PTS && by * bz * tx <= TILE_MAX_EL && score > best) {
               best = score;
               ty = by;
@@ -322,7 +342,6 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
               s.tiles.push_back(t);
             }
       }
-  return ICC_OK;
 }
 
 // Consistent traction on the face x = x_max: F(A, c) = t_c * int N_A dA, the

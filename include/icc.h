@@ -71,6 +71,18 @@ int icc_grid_type_id(const icc_grid* g, int64_t* out);
 /* row-major 81x81 element matrix of a type (test x trial) */
 int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out);
 
+/* ---- tiling (no counterpart in the reference) ---------------------------------- */
+/* The apply walks the grid in tiles of constant element type.  variant selects
+ * the x-line tile shape order (0 .. 5, 0 = default), wg_per_cu the persistent
+ * workgroups per CU (1 or 2).  Operators on the grid follow the new tiling. */
+int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu);
+/* out = variant, workgroups per CU, tile count */
+int icc_grid_tiling(const icc_grid* g, int64_t out[3]);
+/* Times `reps` applies of op (precision prec) for every variant and
+ * workgroups-per-CU choice, keeps the fastest and reports it (any out pointer
+ * may be NULL; best_us = time per apply).  Operators with a sparse term: ENOTSUP. */
+int icc_grid_autotune(icc_op* op, int prec, int reps, int64_t* best_variant, int64_t* best_wg, double* best_us);
+
 /* ---- operator: y = M * sum_e s_e C_e^T K_t(e) C_e * (M x) ------------------ */
 /* The matvec closure of topopt_iga.rs:151-159 / immersed.rs:558-566 over the
  * assembled K of sparse.rs:160-175; semantic spec eval_matvec_3d
@@ -276,6 +288,8 @@ int icc_batch_sizes(const icc_batch* b, int64_t* ndof, int64_t* nel);
 int icc_batch_set(icc_batch* b, int64_t k, const double* scale, const uint8_t* free_mask, const double* force);
 /* instance k: force = consistent traction t on the face x = x_max, masked */
 int icc_batch_set_traction(icc_batch* b, int64_t k, const double t[3]);
+/* icc_grid_autotune on every instance (tile shapes per instance size) */
+int icc_batch_autotune(icc_batch* b, int prec, int reps);
 /* Solves every instance from u = 0.  prec ICC_F64: Jacobi PCG to |r|/|b| < tol;
  * ICC_F32: FP32 PCG with FP64 reliable updates every `delta` drop, to a true
  * relative residual <= tol.  u (host, may be NULL): instances concatenated.

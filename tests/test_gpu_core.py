@@ -287,3 +287,24 @@ def test_classification_bit_exact(ctx, res, padded):
     assert set(np.unique(st)) <= {-1, 0, 1}
     assert np.all(w[st == 1] == 1.0) and np.all(w[st == 0] == 0.0)
     assert np.all((w >= 0.0) & (w <= 1.0))
+
+
+def test_tiling_variants_and_autotune(ctx):
+    """Every tiling variant and workgroup count gives the same masked operator
+    (the per-tile mask flags follow the re-tiling); the autotuner keeps one."""
+    nel = [20, 10, 6]
+    bounds, sp, s, free, f = _cantilever_problem(nel, 17)
+    sp.assemble(s)
+    g = icc.Grid(ctx, nel, bounds)
+    op = icc.Operator(g, scale=s, free=free)
+    x = np.random.default_rng(5).standard_normal(g.ndof)
+    ref = free * sp.matvec(free * x)
+    for v in range(6):
+        for wg in (1, 2):
+            g.set_tiling(v, wg)
+            assert g.tiling[:2] == (v, wg)
+            assert O.rel_err(op.apply(x), ref) < 1e-13, (v, wg)
+            assert O.rel_err(op.apply(x, prec=icc.ICC_F32), ref) < 2e-6, (v, wg)
+    v, wg, us = op.autotune(icc.ICC_F64, reps=3)
+    assert g.tiling[:2] == (v, wg) and us > 0
+    assert O.rel_err(op.apply(x), ref) < 1e-13
