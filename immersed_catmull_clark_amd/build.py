@@ -65,6 +65,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the topopt_3d command-line driver (links libicc.so next to it)
+    exe = os.path.join(PKG, "topopt_3d")
+    _run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "topopt_3d.cpp"), "-o", exe + ".tmp", "-L" + PKG, "-licc",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

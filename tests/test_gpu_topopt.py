@@ -126,3 +126,26 @@ def test_mixed_solver_state_matches_fp64(ctx):
     assert abs(c32 - c64) < 1e-8 * c64
     assert O.rel_err(g32, g64) < 1e-7
     assert O.rel_err(u32, u64) < 1e-6
+
+
+def test_topopt_3d_cli_matches_reference_run(ctx, tmp_path):
+    """The topopt_3d command-line driver against the crate's own run of the
+    same command (benchmarks/topopt_result.json: 24x12x6, 40 iterations):
+    same result-file keys, every compliance to 1e-6 relative and the final
+    densities to 1e-4 (the GPU solves round differently and the 40 OC
+    iterations carry that forward)."""
+    import json
+    import os
+    import subprocess
+    d = load_fixture("topopt_result.json")
+    exe = os.path.join(os.path.dirname(icc.LIB_PATH), "topopt_3d")
+    r = subprocess.run([exe, "24", "12", "6", "40"], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "saved topopt_result.json" in r.stdout
+    out = json.loads((tmp_path / "topopt_result.json").read_text())
+    assert set(out) == set(d)
+    assert (out["nelx"], out["nely"], out["nelz"]) == (24, 12, 6)
+    c, cref = np.asarray(out["compliance"]), np.asarray(d["compliance"])
+    assert len(c) == len(cref)
+    assert np.max(np.abs(c - cref) / cref) < 1e-6, np.max(np.abs(c - cref) / cref)
+    assert np.max(np.abs(np.asarray(out["densities"]) - np.asarray(d["densities"]))) < 1e-4
