@@ -8,5 +8,7 @@ from .core import Context, Filter, Grid, Operator, classify_cylinders  # noqa: F
 from .topopt import CantileverTopOpt, CantileverTopOptConfig, TopOptHistory  # noqa: F401
 from .immersed import ImmersedConfig, ImmersedSdf  # noqa: F401
 from .batch import Batch  # noqa: F401
+from .imm_topopt import (ImmersedTopOpt, ImmersedTopOptConfig, ImmersedTopOptHistory, Stabilization,  # noqa: F401
+                         TriangleMesh3D, element_weights, padded_bounds)
 
 __all__ = ["Context", "Grid", "Operator", "Filter", "classify_cylinders", "ICC_F64", "ICC_F32", "IccError"]

@@ -93,9 +93,21 @@ SIGNATURES = [
     ("icc_immersed_apply", cint, [vp, vp, vp]),
     ("icc_immersed_solve", cint, [vp, vp, vp]),
     ("icc_immersed_solve_dev", cint, [vp, vp, vp]),
+    ("icc_op_set_legacy_stabilization", cint, [vp, vp, dbl, vp]),
+    ("icc_classify_mesh", cint, [vp, vp, i64, vp, i64, vp, vp, vp, vp, vp]),
+    ("icc_mesh_padded_bounds", cint, [vp, i64, dbl, vp]),
+    ("icc_imm_topopt_config_default", None, [vp]),
+    ("icc_imm_topopt_create", cint, [vp, vp, vp, i64, vp, i64, vpp]),
+    ("icc_imm_topopt_destroy", cint, [vp]),
+    ("icc_imm_topopt_sizes", cint, [vp, vp, vp, vp]),
+    ("icc_imm_topopt_problem", cint, [vp, vp, vp, vp, vp, vp]),
+    ("icc_imm_topopt_run", cint, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    ("icc_imm_topopt_state", cint, [vp, vp]),
 ]
 
 REPORT_FN = C.CFUNCTYPE(None, i64, dbl, dbl, dbl, vp)
+IMM_REPORT_FN = C.CFUNCTYPE(None, i64, dbl, dbl, dbl, i64, vp)
+ICC_STAB_NONE, ICC_STAB_LEGACY, ICC_STAB_GHOST = range(3)
 
 
 class TopOptConfigC(C.Structure):
@@ -112,6 +124,13 @@ class ImmersedConfigC(C.Structure):
                 ("poisson", dbl), ("emin", dbl), ("active_min", dbl), ("penalty_factor", dbl),
                 ("traction", dbl * 3), ("pcg_tol", dbl), ("pcg_maxit", i64), ("mode", C.c_int32),
                 ("solver", C.c_int32)]
+
+
+class ImmTopOptConfigC(C.Structure):
+    """icc_imm_topopt_config (include/icc.h)."""
+    _fields_ = [("grid_res", i64 * 3), ("volfrac", dbl), ("penal", dbl), ("rmin", dbl), ("max_iter", i64),
+                ("young", dbl), ("poisson", dbl), ("emin", dbl), ("stabilization", C.c_int32),
+                ("solver", C.c_int32), ("gamma", dbl), ("pcg_tol", dbl), ("pcg_maxit", i64)]
 
 
 _lib = None

@@ -29,11 +29,14 @@ SOURCES = [
     ("k_vec.hip", []),
     ("k_misc.hip", ["-ffp-contract=off"]),
     ("k_immersed.hip", ["-ffp-contract=off"]),
+    ("k_faces.hip", []),
+    ("k_topopt.hip", ["-ffp-contract=off"]),
     ("setup.cpp", None),
     ("wq.cpp", None),
     ("api.cpp", None),
     ("api_ext.cpp", None),
     ("api_imm.cpp", None),
+    ("api_imm_topopt.cpp", None),
 ]
 
 

@@ -157,6 +157,14 @@ class Operator:
         dd = _f64(data)
         check(lib().icc_op_set_sparse(self.h, len(ip) - 1, _p(ip), _p(ix), _p(dd)))
 
+    def set_legacy_stabilization(self, status, gamma):
+        """Legacy cut-cell stabilisation (immersed.rs:168-222) as a matrix-free
+        face term; status None removes it.  Returns the number of faces."""
+        nf = C.c_int64(0)
+        st = None if status is None else np.ascontiguousarray(status, dtype=np.int8)
+        check(lib().icc_op_set_legacy_stabilization(self.h, _p(st), float(gamma), C.byref(nf)))
+        return int(nf.value)
+
     def apply(self, x, prec=ICC_F64, out=None):
         if _is_dev(x):
             check(lib().icc_apply_dev(self.h, prec, _p(x), _p(out)))

@@ -317,6 +317,8 @@ static void drop_work(icc_op* op) {
 }
 
 static int csr_blocks(const icc_op* op) { return op->s_nnz > 0 ? vec_blocks(op->g->s.ndof) : 0; }
+// p.Ap partial layout of one apply: [apply_grid | csr_blocks | face_blocks]
+static int npap_of(const icc_op* op) { return apply_grid(op->g) + csr_blocks(op) + face_blocks(op); }
 
 // Lazily allocates the solver workspace of op.
 static int ensure_work(icc_op* op, Work*& out) {
@@ -326,7 +328,7 @@ static int ensure_work(icc_op* op, Work*& out) {
     free_work(*w);
     w->n = n;
     int rc;
-    size_t npap = (size_t)2 * (op->g->ctx->num_cu > 0 ? op->g->ctx->num_cu : 256) + VEC_MAX_BLOCKS;
+    size_t npap = (size_t)2 * (op->g->ctx->num_cu > 0 ? op->g->ctx->num_cu : 256) + VEC_MAX_BLOCKS + FACE_MAX_BLOCKS;
     if ((rc = dalloc(&w->x, n)) || (rc = dalloc(&w->r, n)) || (rc = dalloc(&w->p, n)) || (rc = dalloc(&w->ap, n)) ||
         (rc = dalloc(&w->dinv, n)) || (rc = dalloc(&w->tmp, n)) || (rc = dalloc(&w->f, n)) ||
         (rc = dalloc(&w->x32, n)) || (rc = dalloc(&w->r32, n)) || (rc = dalloc(&w->p32, n)) ||
@@ -366,6 +368,8 @@ ICC_API int icc_op_destroy(icc_op* op) {
   hipFree(op->d_s_data);
   hipFree(op->d_s_data32);
   hipFree(op->d_s_diag);
+  hipFree(op->d_faces);
+  hipFree(op->d_f_diag);
   delete op;
   return ICC_OK;
 }
@@ -515,6 +519,79 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
   return ICC_OK;
 }
 
+// Legacy cut-cell stabilisation (rust/src/immersed.rs:168-222) as a
+// matrix-free face term: faces between face-neighbouring cells that are both
+// active (INSIDE or CUT) with at least one CUT, alpha = gamma * mean(h), and
+// per face and component the 9 springs between the control points owned by
+// one cell only (the reference's shared-point block sums to zero entry by
+// entry and is left out).  Faces are listed in increasing e1 so neighbouring
+// threads gather neighbouring control points; the diagonal (alpha at both
+// ends of every spring) is accumulated on the host.
+ICC_API int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, double gamma, int64_t* nfaces) {
+  if (!op) return ICC_EINVAL;
+  hipFree(op->d_faces);
+  hipFree(op->d_f_diag);
+  op->d_faces = nullptr;
+  op->d_f_diag = nullptr;
+  op->f_n = 0;
+  op->f_alpha = 0.0;
+  if (nfaces) *nfaces = 0;
+  if (!status) return ICC_OK;
+  const GridSetup& s = op->g->s;
+  const int64_t n0 = s.nel[0], n1 = s.nel[1], n2 = s.nel[2];
+  if (s.nel_tot * 4 > (int64_t)INT32_MAX) {
+    set_error("icc_op_set_legacy_stabilization: grid too large for the face encoding");
+    return ICC_EINVAL;
+  }
+  const double h0 = s.len[0] / (double)n0, h1 = s.len[1] / (double)n1, h2 = s.len[2] / (double)n2;
+  const double alpha = gamma * ((h0 + h1 + h2) / 3.0);
+  auto act = [&](int64_t e) { return status[e] == 1 || status[e] == -1; };
+  std::vector<int32_t> faces;
+  for (int64_t k = 0; k < n2; ++k)
+    for (int64_t j = 0; j < n1; ++j)
+      for (int64_t i = 0; i < n0; ++i) {
+        const int64_t e1 = i + n0 * (j + n1 * k);
+        if (!act(e1)) continue;
+        const bool in_range[3] = {i + 1 < n0, j + 1 < n1, k + 1 < n2};
+        const int64_t step[3] = {1, n0, n0 * n1};
+        for (int d = 0; d < 3; ++d) {
+          if (!in_range[d]) continue;
+          const int64_t e2 = e1 + step[d];
+          if (act(e2) && (status[e1] == -1 || status[e2] == -1)) faces.push_back((int32_t)(e1 * 4 + d));
+        }
+      }
+  if (faces.empty() || alpha == 0.0) return ICC_OK;
+  if ((int64_t)faces.size() * 27 > (int64_t)INT32_MAX) {
+    set_error("icc_op_set_legacy_stabilization: too many faces");
+    return ICC_EINVAL;
+  }
+  std::vector<double> dg(s.ndof, 0.0);
+  const int64_t sx = 1, sy = s.ncp[0], sz = s.ncp[0] * s.ncp[1];
+  for (int32_t code : faces) {
+    const int64_t e1 = code >> 2, dir = code & 3;
+    const int64_t i = e1 % n0, j = (e1 / n0) % n1, k = e1 / (n0 * n1);
+    const int64_t st = dir == 0 ? sx : (dir == 1 ? sy : sz), sa = dir == 0 ? sy : sx, sb = dir == 2 ? sy : sz;
+    for (int c = 0; c < 3; ++c) {
+      const int64_t base = c * s.ndof_sc + i + sy * j + sz * k;
+      for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a) {
+          dg[base + a * sa + b * sb] += alpha;
+          dg[base + 3 * st + a * sa + b * sb] += alpha;
+        }
+    }
+  }
+  ICC_TRY(dalloc(&op->d_faces, (int64_t)faces.size()));
+  ICC_TRY(dalloc(&op->d_f_diag, s.ndof));
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(op->d_faces, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_f_diag, dg.data(), s.ndof * 8, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  op->f_n = (int64_t)faces.size();
+  op->f_alpha = alpha;
+  if (nfaces) *nfaces = op->f_n;
+  return ICC_OK;
+}
+
 // ---------------------------------------------------------------------------
 namespace icc {
 // y = A x on the op stream; y zero-initialised here unless the sparse term
@@ -535,6 +612,7 @@ static hipError_t apply_t(icc_op* op, const T* x, T* y, double* pap) {
   } else {
     e = launch_fill<T>(y, T(0), n, st);
   }
+  if (e == hipSuccess) e = launch_faces<T>(op, x, y, pap ? pap + apply_grid(g) + csr_blocks(op) : nullptr, nullptr, st);
   if (e != hipSuccess) return e;
   if constexpr (std::is_same<T, double>::value)
     return launch_apply_f64(g, op, x, y, pap, nullptr, st);
@@ -560,7 +638,7 @@ ICC_API int icc_apply_dev(icc_op* op, int prec, const void* x, void* y) {
 
 ICC_API int icc_apply_accumulate_dev(icc_op* op, int prec, const void* x, void* y) {
   if (!op || !x || !y) return ICC_EINVAL;
-  if (op->s_nnz > 0) {
+  if (op->s_nnz > 0 || op->f_n > 0) {
     set_error("icc_apply_accumulate_dev: operators with a sparse term are not supported");
     return ICC_ENOTSUP;
   }
@@ -611,6 +689,7 @@ ICC_API int icc_diag_dev(icc_op* op, double* d) {
     // d += diag(S): reuse the fill-free path via a tiny axpy through ir_update (scale 1)
     ICC_HIP(launch_ir_update<double>(op->g->s.ndof, d, op->d_s_diag, 1.0, st));
   }
+  if (op->f_n > 0) ICC_HIP(launch_ir_update<double>(op->g->s.ndof, d, op->d_f_diag, 1.0, st));
   return ICC_OK;
 }
 
@@ -648,17 +727,19 @@ template <typename T>
 static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, hipStream_t st) {
   icc_grid* g = op->g;
   const int64_t n = g->s.ndof;
-  const int npap = apply_grid(g) + csr_blocks(op);
+  const int npap = npap_of(op);
   hipError_t e = hipSuccess;
   if constexpr (std::is_same<T, double>::value) {
     if (op->s_nnz > 0)
       e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, p, ap, op->d_free,
                                           w->pap + apply_grid(g), st);
+    if (e == hipSuccess) e = launch_faces<double>(op, p, ap, w->pap + apply_grid(g) + csr_blocks(op), w->st, st);
     if (e == hipSuccess) e = launch_apply_f64(g, op, p, ap, w->pap, w->st, st);
   } else {
     if (op->s_nnz > 0)
       e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, p, ap, op->d_free,
                                         w->pap + apply_grid(g), st);
+    if (e == hipSuccess) e = launch_faces<float>(op, p, ap, w->pap + apply_grid(g) + csr_blocks(op), w->st, st);
     if (e == hipSuccess) e = launch_apply_f32(g, op, p, ap, w->pap, w->st, st);
   }
   if (e == hipSuccess) e = launch_pcg_update<T>(n, w->st, w->pap, npap, p, ap, x, r, dinv, w->parts, st);
@@ -1197,7 +1278,7 @@ ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out512) {
 ICC_API int icc_grid_autotune(icc_op* op, int prec, int reps, int64_t* best_variant, int64_t* best_wg,
                               double* best_us) {
   if (!op || (prec != ICC_F64 && prec != ICC_F32) || reps < 1) return ICC_EINVAL;
-  if (op->s_nnz > 0) {
+  if (op->s_nnz > 0 || op->f_n > 0) {
     set_error("icc_grid_autotune: operators with a sparse term are not supported");
     return ICC_ENOTSUP;
   }

@@ -1,0 +1,464 @@
+// Immersed geometry from a triangle B-rep and the immersed SIMP topology
+// optimisation driver above the C ABI: the host side of ImmersedTopOpt
+// (rust/src/topopt_immersed.rs:78-247).
+//
+// Set-up: padded bounds, ray-parity classification and volume fractions on
+// the GPU, design domain w >= 0.05, the legacy face stabilisation as a
+// matrix-free operator term, bottom clamp, top-centre load, and the
+// design-domain filter table.  Every design iteration then runs on the
+// device: SIMP scale, warm-started Jacobi PCG (FP64, or FP32 with FP64
+// reliable updates) with the reference's rescaled tolerance, element
+// energies with the w-weighted sensitivity, the filter, the OC bisection
+// and the history scalars; the host reads a handful of scalars per
+// iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "icc_internal.hpp"
+#include "icc_kernels.hpp"
+
+#define ICC_API extern "C" __attribute__((visibility("default")))
+
+using namespace icc;
+
+extern "C" int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, double gamma, int64_t* nfaces);
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  if (hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    set_error("hipMalloc failed");
+    return ICC_ENOMEM;
+  }
+  return ICC_OK;
+}
+
+#define TRY(call)                    \
+  do {                               \
+    int rc__ = (call);               \
+    if (rc__ != ICC_OK) return rc__; \
+  } while (0)
+
+bool mesh_ok(const double* v, int64_t nv, const int64_t* t, int64_t nt) {
+  if (!v || !t || nv < 1 || nt < 1) return false;
+  for (int64_t i = 0; i < 3 * nt; ++i)
+    if (t[i] < 0 || t[i] >= nv) return false;
+  return true;
+}
+
+// Triangle records of the ray-parity predicate (immersed.rs:35-46).
+std::vector<double> triangle_records(const double* v, const int64_t* t, int64_t nt) {
+  std::vector<double> r((size_t)nt * MESH_TRI_STRIDE);
+  for (int64_t q = 0; q < nt; ++q) {
+    double* R = r.data() + q * MESH_TRI_STRIDE;
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) R[3 * c + d] = v[3 * t[3 * q + c] + d];
+    R[9] = std::fmin(std::fmin(R[1], R[4]), R[7]);
+    R[10] = std::fmax(std::fmax(R[1], R[4]), R[7]);
+    R[11] = std::fmin(std::fmin(R[2], R[5]), R[8]);
+    R[12] = std::fmax(std::fmax(R[2], R[5]), R[8]);
+  }
+  return r;
+}
+
+}  // namespace
+
+// padded_bounds (rust/src/immersed.rs:450-463)
+ICC_API int icc_mesh_padded_bounds(const double* vertices, int64_t nv, double padding, double out[6]) {
+  if (!vertices || nv < 1 || !out) return ICC_EINVAL;
+  for (int d = 0; d < 3; ++d) {
+    double lo = std::numeric_limits<double>::max(), hi = std::numeric_limits<double>::lowest();
+    for (int64_t i = 0; i < nv; ++i) {
+      lo = std::fmin(lo, vertices[3 * i + d]);
+      hi = std::fmax(hi, vertices[3 * i + d]);
+    }
+    const double pad = padding * (hi - lo);
+    out[2 * d] = lo - pad;
+    out[2 * d + 1] = hi + pad;
+  }
+  return ICC_OK;
+}
+
+ICC_API int icc_classify_mesh(icc_ctx* ctx, const double* vertices, int64_t nv, const int64_t* tris, int64_t ntri,
+                              const double grid_bounds[6], const int64_t res[3], const int64_t sub[3], int8_t* status,
+                              double* weights) {
+  if (!ctx || !grid_bounds || !res || !sub || !status || !weights) return ICC_EINVAL;
+  if (!mesh_ok(vertices, nv, tris, ntri)) {
+    set_error("icc_classify_mesh: empty mesh or triangle index out of range");
+    return ICC_EINVAL;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (res[d] < 1 || sub[d] < 1 || !(grid_bounds[2 * d + 1] > grid_bounds[2 * d])) {
+      set_error("icc_classify_mesh: bad resolution or bounds");
+      return ICC_EINVAL;
+    }
+  const std::vector<double> rec = triangle_records(vertices, tris, ntri);
+  hipStream_t st = ctx->stream;
+  const int64_t nc = res[0] * res[1] * res[2], nvx = (res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+  double *d_tri = nullptr, *d_w = nullptr;
+  uint8_t* d_in = nullptr;
+  int8_t* d_st = nullptr;
+  int rc = ICC_OK;
+  if ((rc = dalloc(&d_tri, rec.size())) || (rc = dalloc(&d_w, nc)) || (rc = dalloc(&d_in, nvx)) ||
+      (rc = dalloc(&d_st, nc))) {
+    hipFree(d_tri);
+    hipFree(d_w);
+    hipFree(d_in);
+    hipFree(d_st);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(d_tri, rec.data(), rec.size() * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_classify_mesh(d_tri, ntri, grid_bounds, res, sub, d_in, d_st, d_w, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(status, d_st, nc, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(weights, d_w, nc * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d_tri);
+  hipFree(d_w);
+  hipFree(d_in);
+  hipFree(d_st);
+  if (e != hipSuccess) return hip_fail(e, "icc_classify_mesh");
+  return ICC_OK;
+}
+
+// ---------------------------------------------------------------------------
+struct icc_imm_topopt {
+  icc_ctx* ctx = nullptr;
+  icc_imm_topopt_config cfg;
+  icc_grid* g = nullptr;
+  icc_op* op = nullptr;
+  double gb[6];
+  int64_t nel = 0, ndof = 0, n_active = 0, n_faces = 0, n_free = 0;
+  double cad_volume = 0.0, nf = 0.0;
+  std::vector<int8_t> status;
+  std::vector<double> w, force;
+  std::vector<uint8_t> active, fr;
+  int noff = 0;
+  // device
+  double *x = nullptr, *xold = nullptr, *xnew = nullptr, *wd = nullptr, *s = nullptr, *u = nullptr, *b = nullptr,
+         *du = nullptr, *dg = nullptr, *f = nullptr, *dcraw = nullptr, *dc = nullptr, *tmp = nullptr, *hs = nullptr,
+         *ow = nullptr, *parts = nullptr, *pmax = nullptr, *sparts = nullptr;
+  int32_t* off = nullptr;
+  uint8_t *act_d = nullptr, *fr_d = nullptr;
+  PcgState* pst = nullptr;
+  OcState* ocs = nullptr;
+  PcgState* pst_h = nullptr;
+  OcState* ocs_h = nullptr;
+};
+
+ICC_API void icc_imm_topopt_config_default(icc_imm_topopt_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->grid_res[0] = 28;
+  c->grid_res[1] = 20;
+  c->grid_res[2] = 16;
+  c->volfrac = 0.35;
+  c->penal = 3.0;
+  c->rmin = 1.8;
+  c->max_iter = 30;
+  c->young = 1.0;
+  c->poisson = 0.3;
+  c->emin = 1e-4;
+  c->stabilization = ICC_STAB_LEGACY;
+  c->gamma = 0.05;
+  c->pcg_tol = 1e-8;
+  c->pcg_maxit = 20000;
+  c->solver = 0;
+}
+
+ICC_API int icc_imm_topopt_destroy(icc_imm_topopt* t) {
+  if (!t) return ICC_OK;
+  icc_op_destroy(t->op);
+  icc_grid_destroy(t->g);
+  double* bufs[] = {t->x,  t->xold, t->xnew, t->wd, t->s,  t->u,  t->b,     t->du,    t->dg,
+                    t->f,  t->dcraw, t->dc,  t->tmp, t->hs, t->ow, t->parts, t->pmax, t->sparts};
+  for (double* p : bufs) hipFree(p);
+  hipFree(t->off);
+  hipFree(t->act_d);
+  hipFree(t->fr_d);
+  hipFree(t->pst);
+  hipFree(t->ocs);
+  if (t->pst_h) hipHostFree(t->pst_h);
+  if (t->ocs_h) hipHostFree(t->ocs_h);
+  delete t;
+  return ICC_OK;
+}
+
+ICC_API int icc_imm_topopt_create(icc_ctx* ctx, const icc_imm_topopt_config* cfg, const double* vertices, int64_t nv,
+                                  const int64_t* tris, int64_t ntri, icc_imm_topopt** out) {
+  if (!ctx || !cfg || !out) return ICC_EINVAL;
+  if (!mesh_ok(vertices, nv, tris, ntri)) {
+    set_error("icc_imm_topopt_create: empty mesh or triangle index out of range");
+    return ICC_EINVAL;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (cfg->grid_res[d] < 1) {
+      set_error("icc_imm_topopt_create: grid_res must be >= 1");
+      return ICC_EINVAL;
+    }
+  if (cfg->stabilization == ICC_STAB_GHOST) {
+    set_error("icc_imm_topopt_create: the consistent ghost penalty is not implemented; use ICC_STAB_LEGACY");
+    return ICC_ENOTSUP;
+  }
+  if (cfg->stabilization != ICC_STAB_NONE && cfg->stabilization != ICC_STAB_LEGACY) return ICC_EINVAL;
+  if (cfg->solver != 0 && cfg->solver != 1) return ICC_EINVAL;
+  auto* t = new icc_imm_topopt();
+  t->ctx = ctx;
+  t->cfg = *cfg;
+  int rc = icc_mesh_padded_bounds(vertices, nv, 0.05, t->gb);
+  if (rc == ICC_OK) rc = icc_grid_create(ctx, cfg->grid_res, t->gb, 2, cfg->young, cfg->poisson, &t->g);
+  if (rc == ICC_OK) rc = icc_op_create(t->g, &t->op);
+  if (rc != ICC_OK) {
+    icc_imm_topopt_destroy(t);
+    return rc;
+  }
+  const GridSetup& s = t->g->s;
+  const int64_t ne = s.nel_tot, n = s.ndof;
+  t->nel = ne;
+  t->ndof = n;
+  t->status.resize(ne);
+  t->w.resize(ne);
+  const int64_t sub[3] = {4, 4, 4};
+  rc = icc_classify_mesh(ctx, vertices, nv, tris, ntri, t->gb, cfg->grid_res, sub, t->status.data(), t->w.data());
+  if (rc != ICC_OK) {
+    icc_imm_topopt_destroy(t);
+    return rc;
+  }
+  // design domain and CAD volume (topopt_immersed.rs:84-85)
+  t->active.assign(ne, 0);
+  for (int64_t e = 0; e < ne; ++e) {
+    t->active[e] = t->w[e] >= 0.05 ? 1 : 0;
+    if (t->active[e]) {
+      t->cad_volume += t->w[e];
+      t->n_active++;
+    }
+  }
+  if (cfg->stabilization == ICC_STAB_LEGACY) {
+    rc = icc_op_set_legacy_stabilization(t->op, t->status.data(), cfg->gamma, &t->n_faces);
+    if (rc != ICC_OK) {
+      icc_imm_topopt_destroy(t);
+      return rc;
+    }
+  }
+  // bottom clamp, load on the top face centre (topopt_immersed.rs:95-114)
+  t->fr.assign(n, 1);
+  t->force.assign(n, 0.0);
+  const int64_t nd0 = s.ncp[0], nd1 = s.ncp[1], nd2 = s.ncp[2];
+  const int64_t mid_x = (int64_t)std::round((double)nd0 / 2.0), mid_y = (int64_t)std::round((double)nd1 / 2.0);
+  std::vector<int64_t> load;
+  for (int64_t i = 0; i < s.ndof_sc; ++i) {
+    const int64_t s0 = i % nd0, s1 = (i / nd0) % nd1, s2 = i / (nd0 * nd1);
+    if (s2 == 0)
+      for (int c = 0; c < 3; ++c) t->fr[c * s.ndof_sc + i] = 0;
+    if (s2 == nd2 - 1 && std::llabs(s0 + 1 - mid_x) <= 2 && std::llabs(s1 + 1 - mid_y) <= 2) load.push_back(i);
+  }
+  for (int64_t i : load) t->force[2 * s.ndof_sc + i] = -1.0 / (double)load.size();
+  for (int64_t i = 0; i < n; ++i) {
+    t->n_free += t->fr[i];
+    if (t->fr[i]) t->nf += t->force[i] * t->force[i];
+  }
+  t->nf = std::sqrt(t->nf);
+  // filter offsets on the design domain (topopt_immersed.rs:117-131)
+  const double h[3] = {s.len[0] / (double)s.nel[0], s.len[1] / (double)s.nel[1], s.len[2] / (double)s.nel[2]};
+  const double r = cfg->rmin * (h[0] + h[1] + h[2]) / 3.0;
+  int64_t rmax[3];
+  for (int d = 0; d < 3; ++d) rmax[d] = (int64_t)std::ceil(r / h[d]);
+  std::vector<int32_t> offs;
+  std::vector<double> ows;
+  for (int64_t dk = -rmax[2]; dk <= rmax[2]; ++dk)
+    for (int64_t dj = -rmax[1]; dj <= rmax[1]; ++dj)
+      for (int64_t di = -rmax[0]; di <= rmax[0]; ++di) {
+        const double a = (double)di * h[0], b = (double)dj * h[1], c = (double)dk * h[2];
+        const double dist = std::sqrt(a * a + b * b + c * c);
+        if (dist <= r) {
+          offs.insert(offs.end(), {(int32_t)di, (int32_t)dj, (int32_t)dk});
+          ows.push_back(r - dist);
+        }
+      }
+  t->noff = (int)ows.size();
+  const int G = topopt_blocks(ne);
+  if ((rc = dalloc(&t->x, ne)) || (rc = dalloc(&t->xold, ne)) || (rc = dalloc(&t->xnew, ne)) ||
+      (rc = dalloc(&t->wd, ne)) || (rc = dalloc(&t->s, ne)) || (rc = dalloc(&t->u, n)) || (rc = dalloc(&t->b, n)) ||
+      (rc = dalloc(&t->du, n)) || (rc = dalloc(&t->dg, n)) || (rc = dalloc(&t->f, n)) || (rc = dalloc(&t->dcraw, ne)) ||
+      (rc = dalloc(&t->dc, ne)) || (rc = dalloc(&t->tmp, n)) || (rc = dalloc(&t->hs, ne)) ||
+      (rc = dalloc(&t->ow, ows.size())) || (rc = dalloc(&t->off, offs.size())) ||
+      (rc = dalloc(&t->parts, 2 * (size_t)VEC_MAX_BLOCKS)) || (rc = dalloc(&t->pmax, G)) ||
+      (rc = dalloc(&t->sparts, 3 * (size_t)G)) || (rc = dalloc(&t->act_d, ne)) || (rc = dalloc(&t->fr_d, n)) ||
+      (rc = dalloc(&t->pst, 1)) || (rc = dalloc(&t->ocs, 2))) {
+    icc_imm_topopt_destroy(t);
+    return rc;
+  }
+  if (hipHostMalloc((void**)&t->pst_h, sizeof(PcgState), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&t->ocs_h, sizeof(OcState), hipHostMallocDefault) != hipSuccess) {
+    icc_imm_topopt_destroy(t);
+    set_error("hipHostMalloc failed");
+    return ICC_ENOMEM;
+  }
+  hipStream_t st = ctx->stream;
+  PcgState z;
+  std::memset(&z, 0, sizeof(z));
+  hipError_t e;
+  if ((e = hipMemcpyAsync(t->pst, &z, sizeof(z), hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(t->wd, t->w.data(), ne * 8, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(t->act_d, t->active.data(), ne, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(t->f, t->force.data(), n * 8, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(t->fr_d, t->fr.data(), n, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(t->ow, ows.data(), ows.size() * 8, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(t->off, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, st)) ||
+      (e = launch_imm_filter(1, s.nel, t->off, t->ow, t->noff, t->act_d, nullptr, nullptr, nullptr, t->hs, st)) ||
+      (e = hipStreamSynchronize(st))) {
+    icc_imm_topopt_destroy(t);
+    return hip_fail(e, "icc_imm_topopt_create");
+  }
+  rc = icc_op_set_free_mask(t->op, t->fr.data());
+  if (rc != ICC_OK) {
+    icc_imm_topopt_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return ICC_OK;
+}
+
+ICC_API int icc_imm_topopt_sizes(const icc_imm_topopt* t, int64_t out[5], double* cad_volume, double grid_bounds[6]) {
+  if (!t) return ICC_EINVAL;
+  if (out) {
+    out[0] = t->nel;
+    out[1] = t->ndof;
+    out[2] = t->n_active;
+    out[3] = t->n_faces;
+    out[4] = t->n_free;
+  }
+  if (cad_volume) *cad_volume = t->cad_volume;
+  if (grid_bounds) std::memcpy(grid_bounds, t->gb, sizeof(t->gb));
+  return ICC_OK;
+}
+
+ICC_API int icc_imm_topopt_problem(const icc_imm_topopt* t, int8_t* status, double* weights, uint8_t* active,
+                                   double* force, uint8_t* free_mask) {
+  if (!t) return ICC_EINVAL;
+  if (status) std::memcpy(status, t->status.data(), t->nel);
+  if (weights) std::memcpy(weights, t->w.data(), t->nel * 8);
+  if (active) std::memcpy(active, t->active.data(), t->nel);
+  if (force) std::memcpy(force, t->force.data(), t->ndof * 8);
+  if (free_mask) std::memcpy(free_mask, t->fr.data(), t->ndof);
+  return ICC_OK;
+}
+
+namespace {
+
+// One design iteration (topopt_immersed.rs:163-243) on the device.
+int imm_iteration(icc_imm_topopt* t, double* c_out, double* vol_out, double* ch_out, int64_t* its_out) {
+  const icc_imm_topopt_config& cf = t->cfg;
+  hipStream_t st = t->ctx->stream;
+  const int64_t n = t->ndof, ne = t->nel;
+  const GridSetup& s = t->g->s;
+  ICC_HIP(hipMemcpyAsync(t->xold, t->x, ne * 8, hipMemcpyDeviceToDevice, st));
+  // scale = active ? w (emin + (1 - emin) x^p) : emin
+  ICC_HIP(launch_simp_scale(ne, t->x, cf.penal, cf.emin, t->wd, t->act_d, t->s, st));
+  TRY(icc_op_set_scale_dev(t->op, t->s));
+  // b = free ? F - (K + S) u : 0 and |b|
+  TRY(icc_apply_dev(t->op, ICC_F64, t->u, t->tmp));
+  ICC_HIP(launch_residual(n, t->f, t->tmp, t->fr_d, t->b, t->pst, t->parts, st));
+  ICC_HIP(hipMemcpyAsync(t->pst_h, t->pst, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  const double rb = std::sqrt(t->pst_h->rr);
+  // diag = max(diag K + diag S, 1e-6)
+  TRY(icc_diag_dev(t->op, t->tmp));
+  ICC_HIP(launch_clamp_min(n, t->tmp, 1e-6, t->dg, st));
+  const double tol = rb > 0.0 ? std::fmin(cf.pcg_tol * t->nf / rb, 0.5) : 1.0;
+  int64_t its = 0;
+  if (cf.solver == 0) {
+    double res = 0.0;
+    int rc = icc_pcg_dev(t->op, t->b, t->dg, tol, cf.pcg_maxit, t->du, &its, &res);
+    if (rc != ICC_OK && rc != ICC_EBREAKDOWN && rc != ICC_ENOCONV) return rc;
+  } else {
+    ICC_HIP(launch_fill<double>(t->du, 0.0, n, st));
+    double stats[8];
+    int rc = icc_solve_mixed_dev(t->op, t->b, tol, 0.1, cf.pcg_maxit, t->du, stats);
+    if (rc != ICC_OK && rc != ICC_ENOCONV && rc != ICC_EBREAKDOWN) return rc;
+    its = (int64_t)stats[1];
+  }
+  ICC_HIP(launch_masked_add(n, t->u, t->du, t->fr_d, t->tmp, st));
+  std::swap(t->u, t->tmp);
+  // dc_raw = -w p (1 - emin) x^(p-1) ee, then the design-domain filter
+  TRY(icc_sens_dev(t->op, t->u, t->x, cf.penal, cf.emin, t->wd, t->dcraw));
+  ICC_HIP(launch_imm_filter(0, s.nel, t->off, t->ow, t->noff, t->act_d, t->x, t->dcraw, t->hs, t->dc, st));
+  // OC bisection with the CAD-weighted volume target
+  const double target = cf.volfrac * t->cad_volume;
+  ICC_HIP(launch_oc_init(ne, t->act_d, t->dc, t->xold, t->xnew, t->pmax, st));
+  constexpr int CHUNK = 8, MAX_STEPS = 4096;
+  int k = 0;
+  for (;;) {
+    for (int q = 0; q < CHUNK; ++q, ++k)
+      ICC_HIP(launch_oc_step(k, ne, t->act_d, t->wd, t->dc, t->xold, t->xnew, target, 0.2, t->pmax, t->parts, t->ocs,
+                             st));
+    ICC_HIP(hipMemcpyAsync(t->ocs_h, t->ocs + (k & 1), sizeof(OcState), hipMemcpyDeviceToHost, st));
+    ICC_HIP(hipStreamSynchronize(st));
+    if (t->ocs_h->done) break;
+    if (k >= MAX_STEPS) {
+      set_error("icc_imm_topopt_run: OC bisection did not terminate");
+      return ICC_ENOCONV;
+    }
+  }
+  std::swap(t->x, t->xnew);
+  // compliance F.u, change and volume
+  const int G = topopt_blocks(ne);
+  ICC_HIP(launch_topopt_stats(n, t->f, t->u, ne, t->act_d, t->wd, t->x, t->xold, t->sparts, st));
+  std::vector<double> sp(3 * (size_t)G);
+  ICC_HIP(hipMemcpyAsync(sp.data(), t->sparts, sp.size() * 8, hipMemcpyDeviceToHost, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  double c = 0.0, ch = 0.0, vol = 0.0;
+  for (int i = 0; i < G; ++i) {
+    c += sp[i];
+    ch = std::fmax(ch, sp[G + i]);
+    vol += sp[2 * G + i];
+  }
+  *c_out = c;
+  *vol_out = vol / t->cad_volume;
+  *ch_out = ch;
+  *its_out = its;
+  return ICC_OK;
+}
+
+}  // namespace
+
+ICC_API int icc_imm_topopt_run(icc_imm_topopt* t, icc_imm_report_fn report, void* user, double* compliance,
+                               double* volume, double* change, int64_t* pcg_iterations, double* density) {
+  if (!t) return ICC_EINVAL;
+  const icc_imm_topopt_config& cf = t->cfg;
+  hipStream_t st = t->ctx->stream;
+  if (t->n_active == 0) {
+    set_error("icc_imm_topopt_run: empty design domain");
+    return ICC_EINVAL;
+  }
+  ICC_HIP(launch_fill_active(t->nel, t->act_d, cf.volfrac, t->x, st));
+  ICC_HIP(launch_fill<double>(t->u, 0.0, t->ndof, st));
+  for (int64_t iter = 1; iter <= cf.max_iter; ++iter) {
+    double c, vol, ch;
+    int64_t its;
+    TRY(imm_iteration(t, &c, &vol, &ch, &its));
+    if (compliance) compliance[iter - 1] = c;
+    if (volume) volume[iter - 1] = vol;
+    if (change) change[iter - 1] = ch;
+    if (pcg_iterations) pcg_iterations[iter - 1] = its;
+    if (report) report(iter, c, vol, ch, its, user);
+  }
+  if (density) {
+    ICC_HIP(hipMemcpyAsync(density, t->x, t->nel * 8, hipMemcpyDeviceToHost, st));
+    ICC_HIP(hipStreamSynchronize(st));
+  }
+  return ICC_OK;
+}
+
+ICC_API int icc_imm_topopt_state(icc_imm_topopt* t, double* u) {
+  if (!t || !u) return ICC_EINVAL;
+  ICC_HIP(hipMemcpyAsync(u, t->u, t->ndof * 8, hipMemcpyDeviceToHost, t->ctx->stream));
+  ICC_HIP(hipStreamSynchronize(t->ctx->stream));
+  return ICC_OK;
+}

@@ -126,6 +126,11 @@ struct icc_op {
   double* d_s_data = nullptr;
   float* d_s_data32 = nullptr;
   double* d_s_diag = nullptr;
+  // optional matrix-free cut-cell face stabilisation (k_faces.hip)
+  int64_t f_n = 0;              // faces
+  int32_t* d_faces = nullptr;   // [f_n]: e1 * 4 + dir, neighbour e2 = e1 + unit step along dir
+  double f_alpha = 0.0;         // penalty alpha = gamma * mean(h)
+  double* d_f_diag = nullptr;   // [ndof] diagonal of the face term
   // scratch
   double* d_partials = nullptr;  // reduction partials
   int partials_cap = 0;

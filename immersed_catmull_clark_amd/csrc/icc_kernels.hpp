@@ -80,10 +80,40 @@ hipError_t launch_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
 hipError_t launch_wq_diag(const icc_grid* g, const double* tab, const double* x, double penal, double emin,
                           double* d, hipStream_t st);
 
+// k_faces.hip: matrix-free face stabilisation, at most FACE_MAX_BLOCKS p.Ap partials
+constexpr int FACE_MAX_BLOCKS = 256;
+int face_blocks(const icc_op* op);
+template <typename T>
+hipError_t launch_faces(const icc_op* op, const T* x, T* y, double* pap, const PcgState* pst, hipStream_t st);
+
+// k_topopt.hip: immersed SIMP design update (filter on the design domain, OC bisection)
+struct OcState {
+  double l1, l2;
+  int32_t done;   // 1 once the bisection interval test fails
+  int32_t steps;  // bisection steps evaluated
+};
+int topopt_blocks(int64_t nel);  // blocks (= partials) of the OC and stats kernels
+hipError_t launch_imm_filter(int mode, const int64_t nel[3], const int32_t* off, const double* ow, int noff,
+                             const uint8_t* act, const double* x, const double* v, const double* hs, double* out,
+                             hipStream_t st);
+hipError_t launch_oc_init(int64_t nel, const uint8_t* act, const double* dc, const double* xold, double* xnew,
+                          double* parts_max, hipStream_t st);
+hipError_t launch_oc_step(int k, int64_t nel, const uint8_t* act, const double* w, const double* dc,
+                          const double* xold, double* xnew, double target, double move, const double* parts_max,
+                          double* parts, OcState* st, hipStream_t s);
+hipError_t launch_topopt_stats(int64_t n, const double* f, const double* u, int64_t nel, const uint8_t* act,
+                               const double* w, const double* x, const double* xold, double* parts, hipStream_t st);
+hipError_t launch_fill_active(int64_t nel, const uint8_t* act, double v, double* x, hipStream_t st);
+
 // k_immersed.hip
 hipError_t launch_classify_cylinders(const double box_lo[3], const double box_hi[3], const double* cyl_dev,
                                      int64_t ncyl, const double gb[6], const int64_t res[3], const int64_t sub[3],
                                      uint8_t* inside_dev, int8_t* status_dev, double* w_dev, hipStream_t st);
+// triangle records for the ray-parity predicate: a, b, c (xyz) and the y-z box
+constexpr int MESH_TRI_STRIDE = 13;
+hipError_t launch_classify_mesh(const double* tri_dev, int64_t ntri, const double gb[6], const int64_t res[3],
+                                const int64_t sub[3], uint8_t* inside_dev, int8_t* status_dev, double* w_dev,
+                                hipStream_t st);
 
 // wq.cpp
 void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1);

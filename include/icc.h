@@ -97,6 +97,17 @@ int icc_op_set_free_mask(icc_op* op, const uint8_t* free_mask);
 /* adds a sparse term S (CSR, ndof x ndof, FP64) to the operator: stabilisation
  * or penalty surface terms (topopt_immersed.rs:174-183, immersed.rs:542-547) */
 int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, const int64_t* indices, const double* data);
+/* Adds the legacy cut-cell stabilisation of legacy_stabilization
+ * (rust/src/immersed.rs:168-222) as a matrix-free face term: for each face
+ * between two active cells (status INSIDE 1 or CUT -1) with at least one CUT,
+ * and each component, springs alpha [[1,-1],[-1,1]] between the 9 pairs of
+ * control points owned by one of the two cells only, alpha = gamma * mean(h).
+ * The reference's block on the 18 shared points touches diagonal entries
+ * only and sums to zero in each (round-off), so it is not applied.  The
+ * operator's mask applies on both sides (topopt_immersed.rs:174-183).
+ * status[nel]; NULL removes the term.  *nfaces (may be NULL) receives the
+ * number of stabilised faces. */
+int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, double gamma, int64_t* nfaces);
 
 /* y = A x.  Host pointers, FP64 in and out; prec selects the contraction. */
 int icc_apply(icc_op* op, int prec, const double* x, double* y);
@@ -180,6 +191,16 @@ int icc_wq_diag_dev(icc_grid* g, const double* x_dev, double penal, double emin,
 int icc_classify_cylinders(icc_ctx* ctx, const double box_lo[3], const double box_hi[3], const double* cyl,
                            int64_t ncyl, const double grid_bounds[6], const int64_t res[3], const int64_t sub[3],
                            int8_t* status, double* weights);
+/* The same with the ray-parity predicate of ParityRayCaster
+ * (rust/src/immersed.rs:29-76) for a closed triangle mesh: vertices [nv][3],
+ * tris [ntri][3] (vertex indices).  Status and weights are bit-identical to
+ * the scalar CPU evaluation. */
+int icc_classify_mesh(icc_ctx* ctx, const double* vertices, int64_t nv, const int64_t* tris, int64_t ntri,
+                      const double grid_bounds[6], const int64_t res[3], const int64_t sub[3], int8_t* status,
+                      double* weights);
+/* padded_bounds (rust/src/immersed.rs:450-463): the vertices' bounding box
+ * grown by padding * extent on each side; out = [x0, x1, y0, y1, z0, z1]. */
+int icc_mesh_padded_bounds(const double* vertices, int64_t nv, double padding, double out[6]);
 
 /* ---- structured SIMP cantilever driver ---------------------------------------- */
 /* Mirrors CantileverTopOpt (rust/src/topopt_iga.rs:19-235): clamp x = 0,
@@ -269,6 +290,54 @@ int icc_immersed_apply(icc_immersed* h, const double* x, double* y);
  * ICC_EBREAKDOWN like icc_pcg with u filled. */
 int icc_immersed_solve(icc_immersed* h, double* u, double* stats);
 int icc_immersed_solve_dev(icc_immersed* h, double* u_dev, double* stats);
+
+/* ---- immersed SIMP topology optimisation in a triangle B-rep ------------------ */
+/* Mirrors ImmersedTopOpt::{new, run} (rust/src/topopt_immersed.rs:21-247):
+ * p = 2 background grid on padded_bounds(brep, 0.05) with grid_res cells,
+ * ray-parity classification with 4^3 sub-cells, design domain w >= 0.05,
+ * CAD volume sum w over it; stiffness scale w (emin + (1 - emin) x^penal) on
+ * the design domain and emin elsewhere, plus the stabilisation; bottom face
+ * clamped, load -1/count on z at the top-face control points with
+ * |i - round(n/2)| <= 2 (1-based); filter radius rmin * mean(h) over the
+ * design domain; per iteration a warm-started Jacobi PCG (diagonal clamped at
+ * 1e-6, relative tolerance min(pcg_tol |F| / |F - K u|, 0.5)), compliance F.u,
+ * w-weighted sensitivities, the filter and the OC bisection (move 0.2,
+ * bounds [0.001, 1], target volfrac * CAD volume).  Runs exactly max_iter
+ * iterations like the reference. */
+enum { ICC_STAB_NONE = 0, ICC_STAB_LEGACY = 1, ICC_STAB_GHOST = 2 };
+typedef struct icc_imm_topopt icc_imm_topopt;
+typedef struct icc_imm_topopt_config {
+  int64_t grid_res[3];
+  double volfrac, penal, rmin; /* rmin in units of the mean cell size */
+  int64_t max_iter;
+  double young, poisson, emin;
+  int32_t stabilization; /* ICC_STAB_NONE, ICC_STAB_LEGACY (ICC_STAB_GHOST: ICC_ENOTSUP) */
+  int32_t solver;        /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates */
+  double gamma;          /* stabilisation parameter */
+  double pcg_tol;
+  int64_t pcg_maxit;
+} icc_imm_topopt_config;
+/* grid 28x20x16, volfrac 0.35, penal 3, rmin 1.8, 30 iterations, E = 1,
+ * nu = 0.3, emin 1e-4, legacy stabilisation gamma 0.05, pcg_tol 1e-8,
+ * pcg_maxit 20000, solver 0 */
+void icc_imm_topopt_config_default(icc_imm_topopt_config* cfg);
+int icc_imm_topopt_create(icc_ctx* ctx, const icc_imm_topopt_config* cfg, const double* vertices, int64_t nv,
+                          const int64_t* tris, int64_t ntri, icc_imm_topopt** out);
+int icc_imm_topopt_destroy(icc_imm_topopt* t);
+/* out = nel, ndof, design cells, stabilised faces, free DOFs; CAD volume; grid bounds (any may be NULL) */
+int icc_imm_topopt_sizes(const icc_imm_topopt* t, int64_t out[5], double* cad_volume, double grid_bounds[6]);
+/* status, weights, active [nel]; force, free mask [ndof] (any may be NULL) */
+int icc_imm_topopt_problem(const icc_imm_topopt* t, int8_t* status, double* weights, uint8_t* active, double* force,
+                           uint8_t* free_mask);
+typedef void (*icc_imm_report_fn)(int64_t iter, double compliance, double volume, double change, int64_t pcg_iters,
+                                  void* user);
+/* Runs max_iter iterations from x = volfrac on the design domain, u = 0.
+ * History arrays [max_iter] and density [nel] may be NULL; report (may be
+ * NULL) is called after every iteration like the reference's callback. */
+int icc_imm_topopt_run(icc_imm_topopt* t, icc_imm_report_fn report, void* user, double* compliance, double* volume,
+                       double* change, int64_t* pcg_iterations, double* density);
+/* displacement of the last iteration [ndof] */
+int icc_imm_topopt_state(icc_imm_topopt* t, double* u);
 
 /* ---- batched instances (uneven sizes) ------------------------------------------ */
 /* n structured p = 2 instances, instance k on [0,size_k0]x[0,size_k1]x[0,size_k2]

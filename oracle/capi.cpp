@@ -334,7 +334,8 @@ ORC_API void orc_immersed_solve(void* p, double tol, i64 maxit, double* u, doubl
 
 // ipar = res[3], max_iter, stab_kind ; par = volfrac, penal, rmin, young, poisson, emin, gamma, pcg_tol
 ORC_API i64 orc_immersed_topopt_run(const double* nodes, i64 nv, const i64* tris, i64 nt, const i64* ipar,
-                                    const double* par, double* compliance, double* volume, double* density) {
+                                    const double* par, double* compliance, double* volume, double* density,
+                                    double* change, i64* pcg_iters) {
   ImmersedTopOptCfg c;
   for (int d = 0; d < 3; ++d) c.grid_res[d] = (usize)ipar[d];
   c.max_iter = (usize)ipar[3];
@@ -351,6 +352,8 @@ ORC_API i64 orc_immersed_topopt_run(const double* nodes, i64 nv, const i64* tris
   for (usize i = 0; i < h.compliance.size(); ++i) {
     compliance[i] = h.compliance[i];
     volume[i] = h.volume[i];
+    if (change) change[i] = h.change[i];
+    if (pcg_iters) pcg_iters[i] = (i64)h.pcg_iterations[i];
   }
   std::memcpy(density, h.density.data(), h.density.size() * sizeof(double));
   return (i64)h.compliance.size();
@@ -373,6 +376,31 @@ ORC_API void orc_classify_cylinders(const double* box_lo, const double* box_hi, 
                   w, st);
   std::memcpy(status, st.data(), st.size());
   std::memcpy(weights, w.data(), w.size() * sizeof(double));
+}
+
+// element_weights with the ParityRayCaster predicate (immersed.rs:121-155),
+// and padded_bounds (immersed.rs:450-463)
+ORC_API void orc_classify_mesh(const double* nodes, i64 nv, const i64* tris, i64 nt, const double* grid_bounds,
+                               const i64* res, const i64* sub, int8_t* status, double* weights) {
+  TriangleMesh3D m = make_mesh(nodes, nv, tris, nt);
+  ParityRayCaster rc(m);
+  std::array<std::array<double, 2>, 3> gb;
+  for (int d = 0; d < 3; ++d) gb[d] = {grid_bounds[2 * d], grid_bounds[2 * d + 1]};
+  Vec w;
+  std::vector<int8_t> st;
+  element_weights([&](const std::array<double, 3>& q) { return rc.is_inside(q); }, gb,
+                  {(usize)res[0], (usize)res[1], (usize)res[2]}, {(usize)sub[0], (usize)sub[1], (usize)sub[2]}, w, st);
+  std::memcpy(status, st.data(), st.size());
+  std::memcpy(weights, w.data(), w.size() * sizeof(double));
+}
+ORC_API void orc_padded_bounds(const double* nodes, i64 nv, double padding, double* out) {
+  TriangleMesh3D m;
+  for (i64 i = 0; i < nv; ++i) m.vertices.push_back({nodes[3 * i], nodes[3 * i + 1], nodes[3 * i + 2]});
+  auto b = padded_bounds(m, padding);
+  for (int d = 0; d < 3; ++d) {
+    out[2 * d] = b[d][0];
+    out[2 * d + 1] = b[d][1];
+  }
 }
 
 // ---- analytic immersed problem (handle) ----

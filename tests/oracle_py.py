@@ -77,8 +77,10 @@ def lib():
         L.orc_immersed_matvec.argtypes = [vp, vp, vp]
         L.orc_immersed_solve.argtypes = [vp, dbl, i64, vp, vp, vp, vp]
         L.orc_immersed_topopt_run.restype = i64
-        L.orc_immersed_topopt_run.argtypes = [vp, i64, vp, i64, vp, vp, vp, vp, vp]
+        L.orc_immersed_topopt_run.argtypes = [vp, i64, vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.orc_classify_cylinders.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp]
+        L.orc_classify_mesh.argtypes = [vp, i64, vp, i64, vp, vp, vp, vp, vp]
+        L.orc_padded_bounds.argtypes = [vp, i64, dbl, vp]
         L.orc_wq_rule.restype = i64
         L.orc_wq_rule.argtypes = [i64, i64, i64, i64, i64, vp]
         L.orc_space_assemble_wq.restype = i64
@@ -300,7 +302,7 @@ class Immersed:
 
 
 def immersed_topopt_run(nodes, tris, res, max_iter, volfrac, penal, rmin, gamma, young=1.0, poisson=0.3, emin=1e-4,
-                        pcg_tol=1e-10, stab_kind=1):
+                        pcg_tol=1e-10, stab_kind=1, full=False):
     nodes = np.ascontiguousarray(nodes, dtype=np.float64)
     tris = np.ascontiguousarray(tris, dtype=np.int64)
     ipar = np.array([res[0], res[1], res[2], max_iter, stab_kind], dtype=np.int64)
@@ -308,9 +310,34 @@ def immersed_topopt_run(nodes, tris, res, max_iter, volfrac, penal, rmin, gamma,
     comp = np.zeros(max_iter)
     vol = np.zeros(max_iter)
     dens = np.zeros(int(np.prod(res)))
+    change = np.zeros(max_iter)
+    its = np.zeros(max_iter, dtype=np.int64)
     n = lib().orc_immersed_topopt_run(_p(nodes), len(nodes), _p(tris), len(tris), _p(ipar), _p(par), _p(comp),
-                                      _p(vol), _p(dens))
+                                      _p(vol), _p(dens), _p(change), _p(its))
+    if full:
+        return comp[:n], vol[:n], dens, change[:n], its[:n]
     return comp[:n], vol[:n], dens
+
+
+def classify_mesh(nodes, tris, grid_bounds, res, sub=(4, 4, 4)):
+    """(status, weights) by ray parity (immersed.rs:29-155)."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    tris = np.ascontiguousarray(tris, dtype=np.int64)
+    gb = np.ascontiguousarray(np.asarray(grid_bounds, dtype=np.float64).ravel())
+    r = np.asarray(res, dtype=np.int64)
+    s = np.asarray(sub, dtype=np.int64)
+    nel = int(np.prod(res))
+    status = np.zeros(nel, dtype=np.int8)
+    w = np.zeros(nel)
+    lib().orc_classify_mesh(_p(nodes), len(nodes), _p(tris), len(tris), _p(gb), _p(r), _p(s), _p(status), _p(w))
+    return status, w
+
+
+def padded_bounds(nodes, padding):
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    out = np.zeros(6)
+    lib().orc_padded_bounds(_p(nodes), len(nodes), float(padding), _p(out))
+    return out
 
 
 def classify_cylinders(box_lo, box_hi, cyl, grid_bounds, res, sub=(4, 4, 4)):

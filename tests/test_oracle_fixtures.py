@@ -77,6 +77,34 @@ def test_immersed_fixture():
     assert abs(c - d["compliance"]) / d["compliance"] < 1e-9
 
 
+def test_oracle_mesh_classifier_fixture():
+    """The standalone ray-parity classifier and padded_bounds of the oracle
+    (used as the GPU classifier's bitwise reference) against the fixture."""
+    d, nodes, tris = _tilted_box()
+    gb = O.padded_bounds(nodes, 0.05)
+    assert np.all(np.abs(gb - np.asarray(d["grid_bounds"]).ravel()) < 1e-14)
+    st, w = O.classify_mesh(nodes, tris, gb, d["grid_res"])
+    assert np.array_equal(st, np.asarray(d["status"], dtype=np.int8))
+    assert np.max(np.abs(w - np.asarray(d["weights"]))) < 1e-7
+
+
+def test_oracle_immersed_topopt_history_fields():
+    """change and PCG iteration history of the oracle's immersed SIMP loop:
+    change = max |x_k - x_(k-1)| over the design domain (first iteration from
+    volfrac), a positive iteration count per solve."""
+    d = load_fixture("imm_topopt_tilted_box.json")
+    nodes = np.asarray(d["nodes"], dtype=np.float64)
+    tris = np.asarray(d["elements"], dtype=np.int64)
+    comp, vol, dens, change, its = O.immersed_topopt_run(nodes, tris, d["grid_res"], 2, d["volfrac"], d["penal"],
+                                                         d["rmin"], d["gamma_gp"], pcg_tol=1e-10, full=True)
+    _, _, dens1 = O.immersed_topopt_run(nodes, tris, d["grid_res"], 1, d["volfrac"], d["penal"], d["rmin"],
+                                        d["gamma_gp"], pcg_tol=1e-10)
+    act = dens1 > 0
+    assert abs(change[0] - np.max(np.abs(dens1[act] - d["volfrac"]))) == 0.0
+    assert abs(change[1] - np.max(np.abs(dens[act] - dens1[act]))) == 0.0
+    assert np.all(its > 0)
+
+
 def test_immersed_topopt_fixture():
     """rust/tests/imm_topopt_vs_matlab.rs:18-49 (MATLAB used FP32 PCG, hence
     1e-4 / 1e-3)."""
