@@ -165,6 +165,15 @@ class Operator:
         check(lib().icc_op_set_legacy_stabilization(self.h, _p(st), float(gamma), C.byref(nf)))
         return int(nf.value)
 
+    def set_ghost_penalty(self, status, gamma, young=None):
+        """Consistent ghost penalty (immersed.rs:234-370) as a matrix-free
+        face term; status None removes it.  Returns the number of faces."""
+        nf = C.c_int64(0)
+        st = None if status is None else np.ascontiguousarray(status, dtype=np.int8)
+        yg = self.grid.young if young is None else float(young)
+        check(lib().icc_op_set_ghost_penalty(self.h, _p(st), float(gamma), yg, C.byref(nf)))
+        return int(nf.value)
+
     def apply(self, x, prec=ICC_F64, out=None):
         if _is_dev(x):
             check(lib().icc_apply_dev(self.h, prec, _p(x), _p(out)))

@@ -370,6 +370,7 @@ ICC_API int icc_op_destroy(icc_op* op) {
   hipFree(op->d_s_diag);
   hipFree(op->d_faces);
   hipFree(op->d_f_diag);
+  hipFree(op->d_ftab);
   delete op;
   return ICC_OK;
 }
@@ -519,34 +520,30 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
   return ICC_OK;
 }
 
-// Legacy cut-cell stabilisation (rust/src/immersed.rs:168-222) as a
-// matrix-free face term: faces between face-neighbouring cells that are both
-// active (INSIDE or CUT) with at least one CUT, alpha = gamma * mean(h), and
-// per face and component the 9 springs between the control points owned by
-// one cell only (the reference's shared-point block sums to zero entry by
-// entry and is left out).  Faces are listed in increasing e1 so neighbouring
-// threads gather neighbouring control points; the diagonal (alpha at both
-// ends of every spring) is accumulated on the host.
-ICC_API int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, double gamma, int64_t* nfaces) {
-  if (!op) return ICC_EINVAL;
+namespace icc {
+static void clear_faces(icc_op* op) {
   hipFree(op->d_faces);
   hipFree(op->d_f_diag);
+  hipFree(op->d_ftab);
   op->d_faces = nullptr;
   op->d_f_diag = nullptr;
+  op->d_ftab = nullptr;
   op->f_n = 0;
+  op->f_kind = 0;
   op->f_alpha = 0.0;
-  if (nfaces) *nfaces = 0;
-  if (!status) return ICC_OK;
-  const GridSetup& s = op->g->s;
+}
+
+// Faces between face-neighbouring cells that are both active (INSIDE or CUT)
+// with at least one CUT (immersed.rs:175-187), as e1 * 4 + dir in increasing
+// e1, so neighbouring threads gather neighbouring control points.
+static int stab_faces(const GridSetup& s, const int8_t* status, std::vector<int32_t>& faces) {
   const int64_t n0 = s.nel[0], n1 = s.nel[1], n2 = s.nel[2];
   if (s.nel_tot * 4 > (int64_t)INT32_MAX) {
-    set_error("icc_op_set_legacy_stabilization: grid too large for the face encoding");
+    set_error("face stabilisation: grid too large for the face encoding");
     return ICC_EINVAL;
   }
-  const double h0 = s.len[0] / (double)n0, h1 = s.len[1] / (double)n1, h2 = s.len[2] / (double)n2;
-  const double alpha = gamma * ((h0 + h1 + h2) / 3.0);
   auto act = [&](int64_t e) { return status[e] == 1 || status[e] == -1; };
-  std::vector<int32_t> faces;
+  faces.clear();
   for (int64_t k = 0; k < n2; ++k)
     for (int64_t j = 0; j < n1; ++j)
       for (int64_t i = 0; i < n0; ++i) {
@@ -560,19 +557,63 @@ ICC_API int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, do
           if (act(e2) && (status[e1] == -1 || status[e2] == -1)) faces.push_back((int32_t)(e1 * 4 + d));
         }
       }
-  if (faces.empty() || alpha == 0.0) return ICC_OK;
   if ((int64_t)faces.size() * 27 > (int64_t)INT32_MAX) {
-    set_error("icc_op_set_legacy_stabilization: too many faces");
+    set_error("face stabilisation: too many faces");
     return ICC_EINVAL;
   }
+  return ICC_OK;
+}
+
+static int upload_faces(icc_op* op, int kind, double coef, const std::vector<int32_t>& faces,
+                        const std::vector<double>& dg) {
+  ICC_TRY(dalloc(&op->d_faces, (int64_t)faces.size()));
+  ICC_TRY(dalloc(&op->d_f_diag, op->g->s.ndof));
+  hipStream_t st = op->g->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(op->d_faces, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_f_diag, dg.data(), op->g->s.ndof * 8, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  op->f_n = (int64_t)faces.size();
+  op->f_kind = kind;
+  op->f_alpha = coef;
+  return ICC_OK;
+}
+
+// Strides of a face's 4 x 3 x 3 slab: along the normal, the first and the
+// second tangential direction (x < y < z order).
+static void slab_strides(const GridSetup& s, int dir, int64_t& st, int64_t& sa, int64_t& sb) {
+  const int64_t sy = s.ncp[0], sz = s.ncp[0] * s.ncp[1];
+  st = dir == 0 ? 1 : (dir == 1 ? sy : sz);
+  sa = dir == 0 ? sy : 1;
+  sb = dir == 2 ? sy : sz;
+}
+}  // namespace icc
+
+// Legacy cut-cell stabilisation (rust/src/immersed.rs:168-222) as a
+// matrix-free face term: alpha = gamma * mean(h) and, per face and
+// component, the 9 springs between the control points owned by one cell only
+// (the reference's shared-point block sums to zero entry by entry and is
+// left out).  The diagonal (alpha at both ends of every spring) is
+// accumulated on the host.
+ICC_API int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, double gamma, int64_t* nfaces) {
+  if (!op) return ICC_EINVAL;
+  clear_faces(op);
+  if (nfaces) *nfaces = 0;
+  if (!status) return ICC_OK;
+  const GridSetup& s = op->g->s;
+  std::vector<int32_t> faces;
+  ICC_TRY(stab_faces(s, status, faces));
+  const double h0 = s.len[0] / (double)s.nel[0], h1 = s.len[1] / (double)s.nel[1], h2 = s.len[2] / (double)s.nel[2];
+  const double alpha = gamma * ((h0 + h1 + h2) / 3.0);
+  if (faces.empty() || alpha == 0.0) return ICC_OK;
   std::vector<double> dg(s.ndof, 0.0);
-  const int64_t sx = 1, sy = s.ncp[0], sz = s.ncp[0] * s.ncp[1];
   for (int32_t code : faces) {
-    const int64_t e1 = code >> 2, dir = code & 3;
-    const int64_t i = e1 % n0, j = (e1 / n0) % n1, k = e1 / (n0 * n1);
-    const int64_t st = dir == 0 ? sx : (dir == 1 ? sy : sz), sa = dir == 0 ? sy : sx, sb = dir == 2 ? sy : sz;
+    const int64_t e1 = code >> 2;
+    const int dir = code & 3;
+    const int64_t i = e1 % s.nel[0], j = (e1 / s.nel[0]) % s.nel[1], k = e1 / (s.nel[0] * s.nel[1]);
+    int64_t st, sa, sb;
+    slab_strides(s, dir, st, sa, sb);
     for (int c = 0; c < 3; ++c) {
-      const int64_t base = c * s.ndof_sc + i + sy * j + sz * k;
+      const int64_t base = c * s.ndof_sc + i + s.ncp[0] * (j + s.ncp[1] * k);
       for (int b = 0; b < 3; ++b)
         for (int a = 0; a < 3; ++a) {
           dg[base + a * sa + b * sb] += alpha;
@@ -580,14 +621,68 @@ ICC_API int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, do
         }
     }
   }
-  ICC_TRY(dalloc(&op->d_faces, (int64_t)faces.size()));
-  ICC_TRY(dalloc(&op->d_f_diag, s.ndof));
-  hipStream_t st = op->g->ctx->stream;
-  ICC_HIP(hipMemcpyAsync(op->d_faces, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, st));
-  ICC_HIP(hipMemcpyAsync(op->d_f_diag, dg.data(), s.ndof * 8, hipMemcpyHostToDevice, st));
-  ICC_HIP(hipStreamSynchronize(st));
-  op->f_n = (int64_t)faces.size();
-  op->f_alpha = alpha;
+  ICC_TRY(upload_faces(op, FACE_LEGACY, alpha, faces, dg));
+  if (nfaces) *nfaces = op->f_n;
+  return ICC_OK;
+}
+
+// Consistent ghost penalty (rust/src/immersed.rs:234-370) as a matrix-free
+// face term: coef = gamma * young * mean(h)^3, per direction the jump tables
+// of the second derivative and the 1D element mass matrices (k_faces.hip
+// applies coef (j j^T) (x) M_a (x) M_b per face and component); the diagonal
+// coef j_t^2 M_a[a][a] M_b[b][b] is accumulated on the host.
+ICC_API int icc_op_set_ghost_penalty(icc_op* op, const int8_t* status, double gamma, double young, int64_t* nfaces) {
+  if (!op) return ICC_EINVAL;
+  clear_faces(op);
+  if (nfaces) *nfaces = 0;
+  if (!status) return ICC_OK;
+  const GridSetup& s = op->g->s;
+  std::vector<int32_t> faces;
+  ICC_TRY(stab_faces(s, status, faces));
+  double h[3];
+  for (int d = 0; d < 3; ++d) h[d] = s.len[d] / (double)s.nel[d];
+  const double hc = (h[0] + h[1] + h[2]) / 3.0;
+  const double coef = gamma * young * (hc * hc * hc);
+  if (faces.empty() || coef == 0.0) return ICC_OK;
+  std::vector<double> jt[3], mt[3], tab;
+  int joff[3], moff[3];
+  for (int d = 0; d < 3; ++d) {
+    ghost_tables_1d(s.nel[d], s.len[d], jt[d], mt[d]);
+    joff[d] = (int)tab.size();
+    tab.insert(tab.end(), jt[d].begin(), jt[d].end());
+  }
+  const int mass0 = (int)tab.size();
+  for (int d = 0; d < 3; ++d) {
+    moff[d] = (int)tab.size() - mass0;
+    tab.insert(tab.end(), mt[d].begin(), mt[d].end());
+  }
+  std::vector<double> dg(s.ndof, 0.0);
+  for (int32_t code : faces) {
+    const int64_t e1 = code >> 2;
+    const int dir = code & 3;
+    const int64_t e[3] = {e1 % s.nel[0], (e1 / s.nel[0]) % s.nel[1], e1 / (s.nel[0] * s.nel[1])};
+    const int ta = dir == 0 ? 1 : 0, tb = dir == 2 ? 1 : 2;
+    const double* jv = &jt[dir][4 * (e[dir] + 1)];
+    const double* Ma = &mt[ta][9 * e[ta]];
+    const double* Mb = &mt[tb][9 * e[tb]];
+    int64_t st, sa, sb;
+    slab_strides(s, dir, st, sa, sb);
+    for (int c = 0; c < 3; ++c) {
+      const int64_t base = c * s.ndof_sc + e[0] + s.ncp[0] * (e[1] + s.ncp[1] * e[2]);
+      for (int t = 0; t < 4; ++t)
+        for (int b = 0; b < 3; ++b)
+          for (int a = 0; a < 3; ++a)
+            dg[base + t * st + a * sa + b * sb] += coef * jv[t] * jv[t] * Ma[4 * a] * Mb[4 * b];
+    }
+  }
+  ICC_TRY(dalloc(&op->d_ftab, (int64_t)tab.size()));
+  ICC_HIP(hipMemcpy(op->d_ftab, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+  for (int d = 0; d < 3; ++d) {
+    op->ftab_joff[d] = joff[d];
+    op->ftab_moff[d] = moff[d];
+  }
+  op->ftab_mass = mass0;
+  ICC_TRY(upload_faces(op, FACE_GHOST, coef, faces, dg));
   if (nfaces) *nfaces = op->f_n;
   return ICC_OK;
 }

@@ -3,8 +3,8 @@
 // (rust/src/topopt_immersed.rs:78-247).
 //
 // Set-up: padded bounds, ray-parity classification and volume fractions on
-// the GPU, design domain w >= 0.05, the legacy face stabilisation as a
-// matrix-free operator term, bottom clamp, top-centre load, and the
+// the GPU, design domain w >= 0.05, the face stabilisation (consistent
+// ghost penalty or the legacy springs) as a matrix-free operator term, bottom clamp, top-centre load, and the
 // design-domain filter table.  Every design iteration then runs on the
 // device: SIMP scale, warm-started Jacobi PCG (FP64, or FP32 with FP64
 // reliable updates) with the reference's rescaled tolerance, element
@@ -26,7 +26,6 @@
 
 using namespace icc;
 
-extern "C" int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, double gamma, int64_t* nfaces);
 
 namespace {
 
@@ -165,8 +164,8 @@ ICC_API void icc_imm_topopt_config_default(icc_imm_topopt_config* c) {
   c->young = 1.0;
   c->poisson = 0.3;
   c->emin = 1e-4;
-  c->stabilization = ICC_STAB_LEGACY;
-  c->gamma = 0.05;
+  c->stabilization = ICC_STAB_GHOST;
+  c->gamma = 1e-2;
   c->pcg_tol = 1e-8;
   c->pcg_maxit = 20000;
   c->solver = 0;
@@ -202,11 +201,11 @@ ICC_API int icc_imm_topopt_create(icc_ctx* ctx, const icc_imm_topopt_config* cfg
       set_error("icc_imm_topopt_create: grid_res must be >= 1");
       return ICC_EINVAL;
     }
-  if (cfg->stabilization == ICC_STAB_GHOST) {
-    set_error("icc_imm_topopt_create: the consistent ghost penalty is not implemented; use ICC_STAB_LEGACY");
-    return ICC_ENOTSUP;
+  if (cfg->stabilization != ICC_STAB_NONE && cfg->stabilization != ICC_STAB_LEGACY &&
+      cfg->stabilization != ICC_STAB_GHOST) {
+    set_error("icc_imm_topopt_create: unknown stabilization");
+    return ICC_EINVAL;
   }
-  if (cfg->stabilization != ICC_STAB_NONE && cfg->stabilization != ICC_STAB_LEGACY) return ICC_EINVAL;
   if (cfg->solver != 0 && cfg->solver != 1) return ICC_EINVAL;
   auto* t = new icc_imm_topopt();
   t->ctx = ctx;
@@ -239,8 +238,10 @@ ICC_API int icc_imm_topopt_create(icc_ctx* ctx, const icc_imm_topopt_config* cfg
       t->n_active++;
     }
   }
-  if (cfg->stabilization == ICC_STAB_LEGACY) {
-    rc = icc_op_set_legacy_stabilization(t->op, t->status.data(), cfg->gamma, &t->n_faces);
+  if (cfg->stabilization != ICC_STAB_NONE) {
+    rc = cfg->stabilization == ICC_STAB_LEGACY
+             ? icc_op_set_legacy_stabilization(t->op, t->status.data(), cfg->gamma, &t->n_faces)
+             : icc_op_set_ghost_penalty(t->op, t->status.data(), cfg->gamma, cfg->young, &t->n_faces);
     if (rc != ICC_OK) {
       icc_imm_topopt_destroy(t);
       return rc;

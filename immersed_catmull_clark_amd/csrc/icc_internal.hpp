@@ -83,6 +83,8 @@ void build_tiles(GridSetup& s, int variant);
 int n_tile_variants();
 int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
 
+void ghost_tables_1d(int64_t nel, double length, std::vector<double>& jump, std::vector<double>& mass);
+
 }  // namespace icc
 
 struct icc_ctx {
@@ -127,10 +129,13 @@ struct icc_op {
   float* d_s_data32 = nullptr;
   double* d_s_diag = nullptr;
   // optional matrix-free cut-cell face stabilisation (k_faces.hip)
+  int f_kind = 0;               // FACE_LEGACY or FACE_GHOST
   int64_t f_n = 0;              // faces
   int32_t* d_faces = nullptr;   // [f_n]: e1 * 4 + dir, neighbour e2 = e1 + unit step along dir
-  double f_alpha = 0.0;         // penalty alpha = gamma * mean(h)
+  double f_alpha = 0.0;         // legacy: alpha = gamma * mean(h); ghost: gamma E mean(h)^3
   double* d_f_diag = nullptr;   // [ndof] diagonal of the face term
+  double* d_ftab = nullptr;     // ghost: jump tables then mass tables
+  int ftab_joff[3] = {0, 0, 0}, ftab_moff[3] = {0, 0, 0}, ftab_mass = 0;
   // scratch
   double* d_partials = nullptr;  // reduction partials
   int partials_cap = 0;

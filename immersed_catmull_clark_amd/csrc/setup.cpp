@@ -411,4 +411,35 @@ void penalty_xmin(const GridSetup& s, double beta, const std::vector<uint8_t>& f
     }
 }
 
+// Ghost-penalty tables of one direction (p = 2 open uniform knots on a
+// segment of `length`): jump[k][t] = jump of the second physical derivative
+// of function k - 1 + t across break k (elements k - 1 | k), row 0 unused;
+// mass[e][a][b] = int N_a N_b over element e (physical).  The second
+// derivative is constant on each element: (N'(end) - N'(start)) / h.
+void ghost_tables_1d(int64_t nel, double length, std::vector<double>& jump, std::vector<double>& mass) {
+  std::vector<double> knots = open_knots_p2(nel);
+  std::vector<std::array<double, 36>> em;
+  element_mats_1d(nel, length, em);
+  std::vector<double> d2(nel * 3);
+  for (int64_t e = 0; e < nel; ++e) {
+    const double a = knots[e + 2], b = knots[e + 3];
+    double v[3], da[3], db[3];
+    quad_basis(knots, e, a, v, da);
+    quad_basis(knots, e, b, v, db);
+    for (int r = 0; r < 3; ++r) d2[e * 3 + r] = (db[r] - da[r]) / (b - a);
+  }
+  const double l2 = length * length;
+  jump.assign(nel * 4, 0.0);
+  mass.assign(nel * 9, 0.0);
+  for (int64_t k = 1; k < nel; ++k)
+    for (int t = 0; t < 4; ++t) {
+      const int64_t g = k - 1 + t;
+      const double rv = (g >= k && g <= k + 2) ? d2[k * 3 + (g - k)] : 0.0;
+      const double lv = (g >= k - 1 && g <= k + 1) ? d2[(k - 1) * 3 + (g - k + 1)] : 0.0;
+      jump[k * 4 + t] = (rv - lv) / l2;
+    }
+  for (int64_t e = 0; e < nel; ++e)
+    for (int m = 0; m < 9; ++m) mass[e * 9 + m] = em[e][m];
+}
+
 }  // namespace icc

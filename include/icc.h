@@ -108,6 +108,13 @@ int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, const in
  * status[nel]; NULL removes the term.  *nfaces (may be NULL) receives the
  * number of stabilised faces. */
 int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, double gamma, int64_t* nfaces);
+/* Adds the consistent ghost penalty of ghost_penalty
+ * (rust/src/immersed.rs:234-370) as a matrix-free face term on the same
+ * faces: gamma * young * mean(h)^3 * int_F [d_n^2 u].[d_n^2 v], evaluated per
+ * face and component as coef (j j^T) (x) M_a (x) M_b with j the jump of the
+ * second normal derivative and M_a, M_b the tangential 1D element mass
+ * matrices.  Replaces any face term set before; status NULL removes it. */
+int icc_op_set_ghost_penalty(icc_op* op, const int8_t* status, double gamma, double young, int64_t* nfaces);
 
 /* y = A x.  Host pointers, FP64 in and out; prec selects the contraction. */
 int icc_apply(icc_op* op, int prec, const double* x, double* y);
@@ -311,15 +318,15 @@ typedef struct icc_imm_topopt_config {
   double volfrac, penal, rmin; /* rmin in units of the mean cell size */
   int64_t max_iter;
   double young, poisson, emin;
-  int32_t stabilization; /* ICC_STAB_NONE, ICC_STAB_LEGACY (ICC_STAB_GHOST: ICC_ENOTSUP) */
+  int32_t stabilization; /* ICC_STAB_NONE, ICC_STAB_LEGACY or ICC_STAB_GHOST */
   int32_t solver;        /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates */
   double gamma;          /* stabilisation parameter */
   double pcg_tol;
   int64_t pcg_maxit;
 } icc_imm_topopt_config;
 /* grid 28x20x16, volfrac 0.35, penal 3, rmin 1.8, 30 iterations, E = 1,
- * nu = 0.3, emin 1e-4, legacy stabilisation gamma 0.05, pcg_tol 1e-8,
- * pcg_maxit 20000, solver 0 */
+ * nu = 0.3, emin 1e-4, ghost penalty gamma 1e-2, pcg_tol 1e-8,
+ * pcg_maxit 20000, solver 0 (ImmersedTopOptConfig::default) */
 void icc_imm_topopt_config_default(icc_imm_topopt_config* cfg);
 int icc_imm_topopt_create(icc_ctx* ctx, const icc_imm_topopt_config* cfg, const double* vertices, int64_t nv,
                           const int64_t* tris, int64_t ntri, icc_imm_topopt** out);

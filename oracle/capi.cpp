@@ -403,6 +403,21 @@ ORC_API void orc_padded_bounds(const double* nodes, i64 nv, double padding, doub
   }
 }
 
+// y = G x with G the ghost-penalty matrix (immersed.rs:234-370) of the grid
+// (bounds[6], res[3], degree) and cell status; returns |G|_F.
+ORC_API double orc_ghost_penalty_matvec(const double* bounds, const i64* res, i64 degree, const int8_t* status,
+                                        double gamma, double young, const double* x, double* y) {
+  SpaceBox sp = SpaceBox::make({{bounds[0], bounds[1]}, {bounds[2], bounds[3]}, {bounds[4], bounds[5]}},
+                               {(usize)res[0], (usize)res[1], (usize)res[2]}, (usize)degree);
+  std::vector<int8_t> st(status, status + sp.nel);
+  std::vector<usize> r, c;
+  Vec v;
+  ghost_penalty(sp, st, gamma, young, r, c, v);
+  CsrMatrix g = CsrMatrix::from_triplets(sp.ndof, sp.ndof, r, c, v);
+  g.matvec(x, y);
+  return g.frobenius();
+}
+
 // ---- analytic immersed problem (handle) ----
 struct SdfH {
   SdfImmersed pb;

@@ -1,6 +1,6 @@
 // Oracle: immersed background-grid classification, volume fractions, the
-// legacy cut-cell stabilisation, the assembled immersed problem and the
-// immersed SIMP loop.  Restates rust/src/cut_cell.rs (new_box),
+// legacy cut-cell stabilisation, the consistent ghost penalty, the assembled
+// immersed problem and the immersed SIMP loop.  Restates rust/src/cut_cell.rs (new_box),
 // rust/src/immersed.rs and rust/src/topopt_immersed.rs of the reference.
 // Test infrastructure only (see oracle.hpp).
 #include <algorithm>
@@ -192,6 +192,128 @@ void legacy_stabilization(const SpaceBox& sp, const std::vector<int8_t>& status,
   }
 }
 
+// immersed.rs:234-370 ghost_penalty: gamma E h^(2p-1) sum_F int_F [d_n^p u].[d_n^p v]
+// over the same faces as the legacy term, factored into the 1D jump vector of
+// the p-th derivative (normal direction) times the 1D element mass matrices
+// (tangential directions); one block per component.
+void ghost_penalty(const SpaceBox& sp, const std::vector<int8_t>& status, double gamma, double young,
+                   std::vector<usize>& ri, std::vector<usize>& ci, Vec& vi) {
+  const usize dim = sp.dim;
+  Vec h = sp.element_size();
+  double hsum = 0.0;
+  for (double v : h) hsum += v;
+  const double h_char = hsum / (double)dim;
+  const auto& res = sp.nel_dir;
+  auto active = [&](usize e) { return status[e] == INSIDE || status[e] == CUT; };
+  auto cut = [&](usize e) { return status[e] == CUT; };
+  // mass[d][e] (nl x nl), physical
+  std::vector<std::vector<Vec>> mass(dim);
+  for (usize d = 0; d < dim; ++d) {
+    const Space1d& u = sp.univ[d];
+    const usize nl = u.nsh();
+    for (usize e = 0; e < u.nel; ++e) {
+      Vec m(nl * nl, 0.0);
+      for (usize q = 0; q < u.nquad; ++q) {
+        const double w = u.qw[e * u.nquad + q] * sp.lengths[d];
+        const usize base = (e * u.nquad + q) * nl;
+        for (usize a = 0; a < nl; ++a)
+          for (usize b = 0; b < nl; ++b) m[a * nl + b] += w * u.shape[base + a] * u.shape[base + b];
+      }
+      mass[d].push_back(m);
+    }
+  }
+  // jumps[d][k] = (indices, values) across break k (elements k-1 | k)
+  std::vector<std::vector<std::pair<std::vector<usize>, Vec>>> jumps(dim);
+  for (usize d = 0; d < dim; ++d) {
+    const Space1d& u = sp.univ[d];
+    const usize p = u.degree, nl = u.nsh();
+    const double lp = std::pow(sp.lengths[d], (double)p);
+    for (usize k = 0; k < u.nel; ++k) {
+      if (k == 0) {
+        jumps[d].push_back({});
+        continue;
+      }
+      const double x = u.breaks[k];
+      const usize lf = u.connectivity[(k - 1) * nl], rf = u.connectivity[k * nl];
+      std::vector<Vec> dl, dr;
+      basis_funs_ders(u.knots, p, x, p, (long)(lf + p), dl);
+      basis_funs_ders(u.knots, p, x, p, (long)(rf + p), dr);
+      const usize lo = std::min(lf, rf), hi = std::max(lf + p, rf + p);
+      std::vector<usize> idx;
+      Vec vals;
+      for (usize g = lo; g <= hi; ++g) {
+        const double rv = (g >= rf && g <= rf + p) ? dr[p][g - rf] : 0.0;
+        const double lv = (g >= lf && g <= lf + p) ? dl[p][g - lf] : 0.0;
+        idx.push_back(g);
+        vals.push_back((rv - lv) / lp);
+      }
+      jumps[d].push_back({idx, vals});
+    }
+  }
+  for (usize d = 0; d < dim; ++d) {
+    const usize p = sp.degree[d];
+    const double coef = gamma * young * std::pow(h_char, (double)(2 * p - 1));
+    for (usize e2 = 0; e2 < sp.nel; ++e2) {
+      auto s2 = unravel(e2, res);
+      if (s2[d] == 0) continue;
+      auto s1 = s2;
+      s1[d] -= 1;
+      usize lin1 = 0, stride = 1;
+      for (usize dd = 0; dd < dim; ++dd) {
+        lin1 += s1[dd] * stride;
+        stride *= res[dd];
+      }
+      if (!(active(lin1) && active(e2) && (cut(lin1) || cut(e2)))) continue;
+      const auto& jidx = jumps[d][s2[d]].first;
+      const auto& jval = jumps[d][s2[d]].second;
+      std::vector<std::vector<usize>> sets(dim);
+      for (usize dd = 0; dd < dim; ++dd) {
+        if (dd == d) {
+          sets[dd] = jidx;
+        } else {
+          const Space1d& u = sp.univ[dd];
+          sets[dd].assign(u.connectivity.begin() + (long)(s2[dd] * u.nsh()),
+                          u.connectivity.begin() + (long)((s2[dd] + 1) * u.nsh()));
+        }
+      }
+      std::vector<usize> sizes;
+      usize nloc = 1;
+      for (auto& st : sets) {
+        sizes.push_back(st.size());
+        nloc *= st.size();
+      }
+      std::vector<usize> gidx(nloc);
+      for (usize a = 0; a < nloc; ++a) {
+        auto sa = unravel(a, sizes);
+        usize g = 0, str = 1;
+        for (usize dd = 0; dd < dim; ++dd) {
+          g += sets[dd][sa[dd]] * str;
+          str *= sp.ndof_dir[dd];
+        }
+        gidx[a] = g;
+      }
+      for (usize a = 0; a < nloc; ++a) {
+        auto sa = unravel(a, sizes);
+        for (usize b = 0; b < nloc; ++b) {
+          auto sb = unravel(b, sizes);
+          double v = coef * jval[sa[d]] * jval[sb[d]];
+          for (usize t = 0; t < dim; ++t) {
+            if (t == d) continue;
+            const usize nl = sp.nsh_dir[t];
+            v *= mass[t][s2[t]][sa[t] * nl + sb[t]];
+          }
+          if (v == 0.0) continue;
+          for (usize c = 0; c < dim; ++c) {
+            ri.push_back(c * sp.ndof_sc + gidx[a]);
+            ci.push_back(c * sp.ndof_sc + gidx[b]);
+            vi.push_back(v);
+          }
+        }
+      }
+    }
+  }
+}
+
 // immersed.rs:450-463 padded_bounds
 std::array<std::array<double, 2>, 3> padded_bounds(const TriangleMesh3D& mesh, double padding) {
   std::array<std::array<double, 2>, 3> b;
@@ -225,6 +347,7 @@ ImmersedProblem build_immersed_problem(const TriangleMesh3D& mesh, const std::ar
   std::vector<usize> rows, cols;
   Vec vals;
   if (stab_kind == 1) legacy_stabilization(sp, pb.status, gamma, rows, cols, vals);
+  if (stab_kind == 2) ghost_penalty(sp, pb.status, gamma, young, rows, cols, vals);
   usize n_gp = vals.size();
   CsrMatrix stab = CsrMatrix::from_triplets(sp.ndof, sp.ndof, rows, cols, vals);
   pb.stabilization_frobenius = stab.frobenius();
@@ -452,6 +575,7 @@ ImmersedTopOptHistory immersed_topopt_run(const TriangleMesh3D& brep, const Imme
   std::vector<usize> r_, c_;
   Vec v_;
   if (cfg.stab_kind == 1) legacy_stabilization(sp, status, cfg.gamma, r_, c_, v_);
+  if (cfg.stab_kind == 2) ghost_penalty(sp, status, cfg.gamma, cfg.young, r_, c_, v_);
   CsrMatrix stab = CsrMatrix::from_triplets(sp.ndof, sp.ndof, r_, c_, v_);
   auto nd = sp.ndof_dir;
   std::vector<uint8_t> fr(n, 1);

@@ -193,6 +193,8 @@ void element_weights(const InsideFn& inside, const std::array<std::array<double,
                      std::vector<int8_t>& status);
 void legacy_stabilization(const SpaceBox& sp, const std::vector<int8_t>& status, double gamma,
                           std::vector<usize>& ri, std::vector<usize>& ci, Vec& vi);
+void ghost_penalty(const SpaceBox& sp, const std::vector<int8_t>& status, double gamma, double young,
+                   std::vector<usize>& ri, std::vector<usize>& ci, Vec& vi);
 std::array<std::array<double, 2>, 3> padded_bounds(const TriangleMesh3D& mesh, double padding);
 
 struct ImmersedProblem {

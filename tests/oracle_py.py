@@ -81,6 +81,8 @@ def lib():
         L.orc_classify_cylinders.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp]
         L.orc_classify_mesh.argtypes = [vp, i64, vp, i64, vp, vp, vp, vp, vp]
         L.orc_padded_bounds.argtypes = [vp, i64, dbl, vp]
+        L.orc_ghost_penalty_matvec.restype = dbl
+        L.orc_ghost_penalty_matvec.argtypes = [vp, vp, i64, vp, dbl, dbl, vp, vp]
         L.orc_wq_rule.restype = i64
         L.orc_wq_rule.argtypes = [i64, i64, i64, i64, i64, vp]
         L.orc_space_assemble_wq.restype = i64
@@ -331,6 +333,17 @@ def classify_mesh(nodes, tris, grid_bounds, res, sub=(4, 4, 4)):
     w = np.zeros(nel)
     lib().orc_classify_mesh(_p(nodes), len(nodes), _p(tris), len(tris), _p(gb), _p(r), _p(s), _p(status), _p(w))
     return status, w
+
+
+def ghost_penalty_matvec(bounds, res, degree, status, gamma, young, x):
+    """(G x, |G|_F) for the ghost-penalty matrix G (immersed.rs:234-370)."""
+    b = np.ascontiguousarray(np.asarray(bounds, dtype=np.float64).ravel())
+    r = np.asarray(res, dtype=np.int64)
+    st = np.ascontiguousarray(status, dtype=np.int8)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    fro = lib().orc_ghost_penalty_matvec(_p(b), _p(r), int(degree), _p(st), float(gamma), float(young), _p(x), _p(y))
+    return y, fro
 
 
 def padded_bounds(nodes, padding):

@@ -195,10 +195,54 @@ def test_imm_topopt_mixed_solver(ctx):
     assert np.max(np.abs(h32.density - h64.density)) < 1e-4
 
 
-def test_imm_topopt_rejects_ghost_penalty(ctx):
+@pytest.mark.parametrize("case", ["all_cut", "tilted"])
+def test_ghost_penalty_operator_matches_oracle(ctx, case):
+    """The matrix-free ghost penalty (scale 0 isolates it) vs the oracle's
+    assembled ghost_penalty matrix (immersed.rs:234-370): unmasked and masked
+    products and the diagonal."""
+    if case == "all_cut":
+        bounds, res = np.array([0.0, 1.3, -0.2, 0.7, 0.1, 0.6]), (6, 5, 4)
+        status = -np.ones(int(np.prod(res)), dtype=np.int8)
+    else:
+        _, m = _tilted()
+        res = (11, 9, 7)
+        bounds = icc.padded_bounds(m, 0.05).ravel()
+        status = icc.element_weights(ctx, m, bounds, res)[1]
+    gamma, young = 1e-2, 2.5
+    g = icc.Grid(ctx, res, bounds, young=young)
+    op = icc.Operator(g, scale=np.zeros(g.nel_tot))
+    assert op.set_ghost_penalty(status, gamma) > 0
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(g.ndof)
+    want, fro = O.ghost_penalty_matvec(bounds, res, 2, status, gamma, young, x)
+    assert fro > 0.0
+    assert O.rel_err(op.apply(x), want) < 1e-13
+    fr = (rng.random(g.ndof) > 0.2).astype(np.uint8)
+    op.set_free(fr)
+    want_m, _ = O.ghost_penalty_matvec(bounds, res, 2, status, gamma, young, fr * x)
+    assert O.rel_err(op.apply(x), fr * want_m) < 1e-13
+    sel = np.arange(0, g.ndof, 29)
+    d = op.diag()
+    for i in sel:
+        e = np.zeros(g.ndof)
+        e[i] = 1.0
+        gi, _ = O.ghost_penalty_matvec(bounds, res, 2, status, gamma, young, e)
+        assert abs(d[i] - gi[i]) <= 1e-13 * fro
+
+
+def test_imm_topopt_ghost_penalty_matches_oracle(ctx):
+    """ImmersedTopOptConfig::default's stabilisation (ghost penalty, gamma
+    1e-2) through the whole loop vs the oracle (stab_kind 2)."""
     d, m = _tilted("imm_topopt_tilted_box.json")
-    cfg = _topopt_cfg(d)
-    cfg.stabilization = icc.Stabilization.ghost_penalty(1e-2)
-    with pytest.raises(icc.IccError) as ei:
-        icc.ImmersedTopOpt(ctx, m, cfg)
-    assert ei.value.code == icc.ICC_ENOTSUP
+    res, iters = (10, 8, 6), 5
+    cfg = icc.ImmersedTopOptConfig(grid_res=res, max_iter=iters, pcg_tol=1e-10)
+    assert cfg.stabilization.kind == icc.Stabilization().kind and cfg.stabilization.gamma == 1e-2
+    opt = icc.ImmersedTopOpt(ctx, m, cfg)
+    h = opt.run()
+    comp, vol, dens, change, its = O.immersed_topopt_run(m.vertices, m.triangles, res, iters, cfg.volfrac, cfg.penal,
+                                                         cfg.rmin, 1e-2, emin=cfg.emin, pcg_tol=1e-10, stab_kind=2,
+                                                         full=True)
+    assert np.all(np.abs(np.asarray(h.compliance) - comp) / comp < 1e-8), (h.compliance, comp)
+    assert np.all(np.abs(np.asarray(h.volume) - vol) < 1e-9)
+    assert np.max(np.abs(h.density - dens)) < 1e-6
+    assert np.all(np.abs(np.asarray(h.pcg_iterations) - its) <= 2 + 0.03 * its), (h.pcg_iterations, its)

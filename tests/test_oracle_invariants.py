@@ -150,3 +150,78 @@ def test_sdf_immersed_problem_invariants():
     assert 0 < big.free.sum() < big.ndof
     pb0 = O.SdfImmersed(lo, hi, cyl, [16, 8, 8], mode=0, emin=1e-4)
     assert np.array_equal(pb0.scale, 1e-4 + (1 - 1e-4) * pb0.weights) and pb0.free.all()
+
+
+def _open_knots(nel, p):
+    return np.concatenate([np.zeros(p), np.arange(nel + 1) / nel, np.ones(p)])
+
+
+def _blossom_coeffs(nel, p, m):
+    """B-spline coefficients of xi^m (m <= p) on the open uniform knots:
+    Marsden's identity, c_i = e_m(t_(i+1), ..., t_(i+p)) / binom(p, m)."""
+    from itertools import combinations
+    from math import comb
+    t = _open_knots(nel, p)
+    out = []
+    for i in range(nel + p):
+        args = t[i + 1:i + p + 1]
+        out.append(sum(np.prod(c) for c in combinations(args, m)) / comb(p, m) if m > 0 else 1.0)
+    return np.asarray(out)
+
+
+def test_ghost_penalty_annihilates_polynomials_up_to_degree_p():
+    """immersed_cad_bc.rs:63-93: j(u, v) vanishes on global polynomials of
+    degree <= p (fields built here from Marsden's identity, not by the
+    reference's interpolation) and penalises a generic field."""
+    bounds = [0.0, 1.3, -0.2, 0.7, 0.1, 0.6]
+    res = (6, 5, 4)
+    for p in (2, 3):
+        nd = [r + p for r in res]
+        nsc = int(np.prod(nd))
+        status = -np.ones(int(np.prod(res)), dtype=np.int8)
+        rnd = ((np.arange(3 * nsc) * 7919) % 101) / 101.0
+        y, fro = O.ghost_penalty_matvec(bounds, res, p, status, 1.0, 1.0, rnd)
+        assert fro > 0.0 and np.linalg.norm(y) > 1e-3 * fro * np.linalg.norm(rnd)
+        for d in range(3):
+            for m in range(p + 1):
+                cd = _blossom_coeffs(res[d], p, m)
+                u = np.zeros(3 * nsc)
+                idx = np.arange(nsc)
+                s = [idx % nd[0], (idx // nd[0]) % nd[1], idx // (nd[0] * nd[1])]
+                u[((d + 1) % 3) * nsc + idx] = cd[s[d]]
+                y, _ = O.ghost_penalty_matvec(bounds, res, p, status, 1.0, 1.0, u)
+                assert np.linalg.norm(y) < 1e-10 * fro * np.linalg.norm(u), (p, d, m)
+
+
+def test_ghost_penalty_single_interior_face_closed_form():
+    """One x-face deep inside a uniform p = 2 grid: the block is
+    coef (j j^T) (x) M_y (x) M_z per component, with the jump of the second
+    derivative j = (-1, 3, -3, 1) / h_x^2, the uniform quadratic element mass
+    M = h/120 [[6,13,1],[13,54,13],[1,13,6]] and coef = gamma E h_mean^3."""
+    bounds = [0.0, 1.2, 0.0, 0.8, 0.0, 1.0]
+    res = (8, 8, 8)
+    h = np.array([1.2, 0.8, 1.0]) / 8
+    gamma, young = 0.37, 2.5
+    nel = 512
+    status = np.zeros(nel, dtype=np.int8)
+    lin = lambda i, j, k: i + 8 * (j + 8 * k)
+    status[lin(3, 4, 4)] = -1
+    status[lin(4, 4, 4)] = 1
+    nd = 10
+    nsc = nd ** 3
+    jv = np.array([-1.0, 3.0, -3.0, 1.0]) / h[0] ** 2
+    M = lambda hh: hh / 120.0 * np.array([[6.0, 13, 1], [13, 54, 13], [1, 13, 6]])
+    coef = gamma * young * np.mean(h) ** 3
+    # local DOFs: x index 3..6, y index 4..6, z index 4..6
+    loc = [(t, a, b) for b in range(3) for a in range(3) for t in range(4)]
+    gl = [(3 + t) + nd * ((4 + a) + nd * (4 + b)) for (t, a, b) in loc]
+    blk = np.array([[coef * jv[t1] * jv[t2] * M(h[1])[a1, a2] * M(h[2])[b1, b2] for (t2, a2, b2) in loc]
+                    for (t1, a1, b1) in loc])
+    for c in (0, 2):
+        for col in (0, 13, 35):
+            e = np.zeros(3 * nsc)
+            e[c * nsc + gl[col]] = 1.0
+            y, _ = O.ghost_penalty_matvec(bounds, res, 2, status, gamma, young, e)
+            want = np.zeros(3 * nsc)
+            want[c * nsc + np.asarray(gl)] = blk[:, col]
+            assert np.max(np.abs(y - want)) < 1e-13 * np.max(np.abs(blk)), (c, col)
