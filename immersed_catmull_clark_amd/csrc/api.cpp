@@ -371,6 +371,8 @@ ICC_API int icc_op_destroy(icc_op* op) {
   hipFree(op->d_faces);
   hipFree(op->d_f_diag);
   hipFree(op->d_ftab);
+  hipFree(op->d_fblk);
+  hipFree(op->d_fptr);
   delete op;
   return ICC_OK;
 }
@@ -525,10 +527,15 @@ static void clear_faces(icc_op* op) {
   hipFree(op->d_faces);
   hipFree(op->d_f_diag);
   hipFree(op->d_ftab);
+  hipFree(op->d_fblk);
+  hipFree(op->d_fptr);
   op->d_faces = nullptr;
   op->d_f_diag = nullptr;
   op->d_ftab = nullptr;
+  op->d_fblk = nullptr;
+  op->d_fptr = nullptr;
   op->f_n = 0;
+  op->f_nblk = 0;
   op->f_kind = 0;
   op->f_alpha = 0.0;
 }
@@ -564,15 +571,46 @@ static int stab_faces(const GridSetup& s, const int8_t* status, std::vector<int3
   return ICC_OK;
 }
 
-static int upload_faces(icc_op* op, int kind, double coef, const std::vector<int32_t>& faces,
+// Uploads the faces grouped by FACE_BLK^3 element block of e1 (block index x
+// fastest; inside a block by direction, then e1) with the block list and
+// the face range of each block.
+static int upload_faces(icc_op* op, int kind, double coef, const std::vector<int32_t>& faces_in,
                         const std::vector<double>& dg) {
+  const GridSetup& s = op->g->s;
+  const int64_t nbx = (s.nel[0] + FACE_BLK - 1) / FACE_BLK, nby = (s.nel[1] + FACE_BLK - 1) / FACE_BLK;
+  auto block_of = [&](int32_t code) {
+    const int64_t e1 = code >> 2;
+    const int64_t i = e1 % s.nel[0], j = (e1 / s.nel[0]) % s.nel[1], k = e1 / (s.nel[0] * s.nel[1]);
+    return (i / FACE_BLK) + nbx * ((j / FACE_BLK) + nby * (k / FACE_BLK));
+  };
+  std::vector<int32_t> faces(faces_in);
+  std::stable_sort(faces.begin(), faces.end(), [&](int32_t a, int32_t b) {
+    const int64_t ba = block_of(a), bb = block_of(b);
+    if (ba != bb) return ba < bb;
+    if ((a & 3) != (b & 3)) return (a & 3) < (b & 3);
+    return a < b;
+  });
+  std::vector<int32_t> blk, ptr;
+  for (size_t f = 0; f < faces.size(); ++f) {
+    const int64_t b = block_of(faces[f]);
+    if (blk.empty() || blk.back() != (int32_t)b) {
+      blk.push_back((int32_t)b);
+      ptr.push_back((int32_t)f);
+    }
+  }
+  ptr.push_back((int32_t)faces.size());
   ICC_TRY(dalloc(&op->d_faces, (int64_t)faces.size()));
-  ICC_TRY(dalloc(&op->d_f_diag, op->g->s.ndof));
+  ICC_TRY(dalloc(&op->d_fblk, (int64_t)blk.size()));
+  ICC_TRY(dalloc(&op->d_fptr, (int64_t)ptr.size()));
+  ICC_TRY(dalloc(&op->d_f_diag, s.ndof));
   hipStream_t st = op->g->ctx->stream;
   ICC_HIP(hipMemcpyAsync(op->d_faces, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, st));
-  ICC_HIP(hipMemcpyAsync(op->d_f_diag, dg.data(), op->g->s.ndof * 8, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_fblk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_fptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_f_diag, dg.data(), s.ndof * 8, hipMemcpyHostToDevice, st));
   ICC_HIP(hipStreamSynchronize(st));
   op->f_n = (int64_t)faces.size();
+  op->f_nblk = (int64_t)blk.size();
   op->f_kind = kind;
   op->f_alpha = coef;
   return ICC_OK;

@@ -131,7 +131,10 @@ struct icc_op {
   // optional matrix-free cut-cell face stabilisation (k_faces.hip)
   int f_kind = 0;               // FACE_LEGACY or FACE_GHOST
   int64_t f_n = 0;              // faces
-  int32_t* d_faces = nullptr;   // [f_n]: e1 * 4 + dir, neighbour e2 = e1 + unit step along dir
+  int32_t* d_faces = nullptr;   // [f_n]: e1 * 4 + dir, neighbour e2 = e1 + unit step along dir, by block
+  int64_t f_nblk = 0;           // element blocks holding faces
+  int32_t* d_fblk = nullptr;    // [f_nblk] block index (x fastest)
+  int32_t* d_fptr = nullptr;    // [f_nblk + 1] face range of each block
   double f_alpha = 0.0;         // legacy: alpha = gamma * mean(h); ghost: gamma E mean(h)^3
   double* d_f_diag = nullptr;   // [ndof] diagonal of the face term
   double* d_ftab = nullptr;     // ghost: jump tables then mass tables

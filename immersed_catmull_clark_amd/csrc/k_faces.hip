@@ -2,23 +2,27 @@
 // of rust/src/immersed.rs:168-222 and the consistent ghost penalty of
 // immersed.rs:234-370, applied without assembling either.
 //
-// For each face (e1, e2 = e1 + unit step along `dir`) between two active
-// cells of which at least one is cut, and each displacement component, the
-// reference adds
-//   * a block on the 18 control points the two cells share whose triplets
-//     all land on diagonal entries and sum to zero per entry (it contributes
-//     nothing beyond round-off, as the reference itself documents), and
-//   * springs of stiffness alpha = gamma * mean(h) between the 9 control
-//     points owned by e1 only (layer 0 of the 4 x 3 x 3 union along dir) and
-//     the 9 owned by e2 only (layer 3), paired in ascending global index:
-//     both lists are then ordered by the same tangential index, so each
-//     spring joins the two points of one line along dir.
-// The operator therefore applies the springs: y_0 += alpha (x_0 - x_3),
-// y_3 += alpha (x_3 - x_0) on masked values, scattered to free DOFs only.
-// One thread per (face, component, tangential point): two gathers, two
-// hardware FP atomics, and with pap != null the quadratic form
-// alpha (x_0 - x_3)^2 reduced to one partial per block, so the PCG's p.Ap
-// needs no extra pass.
+// Faces (e1, e2 = e1 + unit step along `dir`) between two active cells of
+// which at least one is cut; each face couples the 4 x 3 x 3 slab of control
+// points of its two cells (4 layers along dir), per displacement component:
+//   * legacy: springs alpha between layer 0 and layer 3 at each of the 9
+//     tangential points (the reference's block on the 18 shared points lands
+//     on diagonal entries only and sums to zero in each, so it is left out);
+//   * ghost: coef (j j^T) (x) M_a (x) M_b, j the jump of the second normal
+//     derivative of the 4 layer functions across the face's break, M_a, M_b
+//     the 1D element mass matrices of the two tangential directions:
+//     T = j . X (the jump at each tangential point), W = M_a T M_b^T,
+//     y += coef j (x) W, quadratic form coef T : W.
+//
+// Faces are grouped by 8 x 8 x 8 element blocks.  One workgroup takes one
+// block: it stages the masked x of the block's (8+3)^3 control-point box in
+// LDS, applies every face of the block with LDS double atomics into a box of
+// y (one thread per face and component), and flushes the nonzero box
+// entries to the free DOFs with one global atomic each (coalesced rows).  Global atomics drop from 18 (legacy) or 36
+// (ghost) per face and component to about one per touched DOF and block,
+// the cost that dominated the per-face formulation.  With pap != null each
+// workgroup also writes its share of x_m^T S x_m, so the PCG's p.Ap needs no
+// extra pass.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -29,139 +33,120 @@
 namespace icc {
 
 constexpr int FB = 256;
+constexpr int FBOX = FACE_BLK + 3;
+constexpr int FBOX3 = FBOX * FBOX * FBOX;
 
-template <typename T>
-__global__ __launch_bounds__(FB) void legacy_faces_kernel(const int32_t* __restrict__ faces, int nf, int nel0,
-                                                          int nel1, int ncp0, int ncp1, int nsc, double alpha,
-                                                          const uint8_t* __restrict__ freem,
-                                                          const T* __restrict__ x, T* __restrict__ y,
-                                                          double* __restrict__ pap, const PcgState* pst) {
-  if (pst != nullptr && pst->done) return;
-  double q = 0.0;
-  const int items = 27 * nf;  // (component, tangential point) major, face minor
-  for (int it = blockIdx.x * FB + threadIdx.x; it < items; it += gridDim.x * FB) {
-    const int cm = it / nf, f = it - cm * nf;
-    const int c = cm / 9, m = cm - 9 * c;
-    const int a = m % 3, b = m / 3;
-    const int code = faces[f];
-    const int e1 = code >> 2, dir = code & 3;
-    const int i = e1 % nel0, j = (e1 / nel0) % nel1, k = e1 / (nel0 * nel1);
-    const int sy = ncp0, sz = ncp0 * ncp1;
-    const int st = dir == 0 ? 1 : (dir == 1 ? sy : sz);
-    const int sa = dir == 0 ? sy : 1;
-    const int sb = dir == 2 ? sy : sz;
-    const int64_t g0 = (int64_t)c * nsc + i + (int64_t)sy * j + (int64_t)sz * k + a * sa + b * sb;
-    const int64_t g3 = g0 + 3 * st;
-    const bool f0 = freem == nullptr || freem[g0] != 0;
-    const bool f3 = freem == nullptr || freem[g3] != 0;
-    const double x0 = f0 ? (double)x[g0] : 0.0;
-    const double x3 = f3 ? (double)x[g3] : 0.0;
-    const double d = alpha * (x0 - x3);
-    if (f0) atomicAdd(y + g0, (T)d);
-    if (f3) atomicAdd(y + g3, (T)(-d));
-    q += d * (x0 - x3);
-  }
-  if (pap == nullptr) return;
-  __shared__ double sh[FB / 64];
-  for (int off = 32; off > 0; off >>= 1) q += __shfl_down(q, off, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = q;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int w = 0; w < FB / 64; ++w) s += sh[w];
-    pap[blockIdx.x] = s;
-  }
-}
-
-// Consistent ghost penalty (rust/src/immersed.rs:234-370) on the same faces:
-//   j(u, v) = gamma E h^3 sum_F int_F [d_n^2 u] . [d_n^2 v]
-// whose face block is coef (j j^T) (x) M_a (x) M_b per component: j the jump
-// of the second normal derivative of the 4 functions across the face's break
-// (normal direction), M_a, M_b the 1D element mass matrices of the face's
-// tangential element indices.  One thread per (face, component): gather the
-// 4 x 3 x 3 masked slab X, contract T = j . X (the jump at each tangential
-// point), W = M_a T M_b^T, scatter y += coef j (x) W to the free DOFs, and
-// with pap != null add coef T : W to the block's p.Ap partial.
-struct GhostTabs {
-  const double* jump;  // per direction d: [nel_d][4], row k = break between elements k-1 and k
-  const double* mass;  // per direction d: [nel_d][9]
+struct FaceParams {
+  const double* jump;  // ghost, per direction d: [nel_d][4], row k = break between elements k-1 and k
+  const double* mass;  // ghost, per direction d: [nel_d][9]
   int joff[3], moff[3];
-  double coef;
+  double coef;  // legacy: alpha; ghost: gamma E mean(h)^3
 };
 
-template <typename T>
-__global__ __launch_bounds__(FB) void ghost_faces_kernel(const int32_t* __restrict__ faces, int nf, int nel0,
-                                                         int nel1, int ncp0, int ncp1, int nsc, GhostTabs tb,
-                                                         const uint8_t* __restrict__ freem,
-                                                         const T* __restrict__ x, T* __restrict__ y,
-                                                         double* __restrict__ pap, const PcgState* pst) {
+template <typename T, int KIND>
+__global__ __launch_bounds__(FB) void faces_blocked_kernel(const int32_t* __restrict__ faces,
+                                                           const int32_t* __restrict__ fblk,
+                                                           const int32_t* __restrict__ fptr, int nblk, int nbx,
+                                                           int nby, int nel0, int nel1, int ncp0, int ncp1, int ncp2,
+                                                           int nsc, FaceParams fp, const uint8_t* __restrict__ freem,
+                                                           const T* __restrict__ x, T* __restrict__ y,
+                                                           double* __restrict__ pap, const PcgState* pst) {
   if (pst != nullptr && pst->done) return;
+  __shared__ double xs[3 * FBOX3];
+  __shared__ double ys[3 * FBOX3];
   double q = 0.0;
-  const int items = 3 * nf;  // component major, face minor
-  for (int it = blockIdx.x * FB + threadIdx.x; it < items; it += gridDim.x * FB) {
-    const int c = it / nf, f = it - c * nf;
-    const int code = faces[f];
-    const int e1 = code >> 2, dir = code & 3;
-    const int e[3] = {e1 % nel0, (e1 / nel0) % nel1, e1 / (nel0 * nel1)};
-    const int sy = ncp0, sz = ncp0 * ncp1;
-    const int ta = dir == 0 ? 1 : 0, tbd = dir == 2 ? 1 : 2;
-    const int st = dir == 0 ? 1 : (dir == 1 ? sy : sz);
-    const int sa = dir == 0 ? sy : 1;
-    const int sb = dir == 2 ? sy : sz;
-    const double* jv = tb.jump + tb.joff[dir] + 4 * (e[dir] + 1);
-    const double* Ma = tb.mass + tb.moff[ta] + 9 * e[ta];
-    const double* Mb = tb.mass + tb.moff[tbd] + 9 * e[tbd];
-    const int64_t base = (int64_t)c * nsc + e[0] + (int64_t)sy * e[1] + (int64_t)sz * e[2];
-    int off[9];
+  for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
+    const int bid = fblk[bi];
+    const int b0 = (bid % nbx) * FACE_BLK, b1 = ((bid / nbx) % nby) * FACE_BLK, b2 = (bid / (nbx * nby)) * FACE_BLK;
+    for (int l = threadIdx.x; l < 3 * FBOX3; l += FB) {
+      const int c = l / FBOX3, r = l - c * FBOX3;
+      const int gx = b0 + r % FBOX, gy = b1 + (r / FBOX) % FBOX, gz = b2 + r / (FBOX * FBOX);
+      double v = 0.0;
+      if (gx < ncp0 && gy < ncp1 && gz < ncp2) {
+        const int64_t g = (int64_t)c * nsc + gx + (int64_t)ncp0 * (gy + (int64_t)ncp1 * gz);
+        if (freem == nullptr || freem[g]) v = (double)x[g];
+      }
+      xs[l] = v;
+      ys[l] = 0.0;
+    }
+    __syncthreads();
+    const int fb = fptr[bi], nfb = fptr[bi + 1] - fb;
+    const int items = 3 * nfb;
+    for (int it = threadIdx.x; it < items; it += FB) {
+      const int c = it / nfb, f = fb + (it - c * nfb);  // component major, face minor
+      const int code = faces[f];
+      const int e1 = code >> 2, dir = code & 3;
+      const int e[3] = {e1 % nel0, (e1 / nel0) % nel1, e1 / (nel0 * nel1)};
+      const int l0 = e[0] - b0, l1 = e[1] - b1, l2 = e[2] - b2;
+      const int sa = dir == 0 ? FBOX : 1;
+      const int sb = dir == 2 ? FBOX : FBOX * FBOX;
+      const int st = dir == 0 ? 1 : (dir == 1 ? FBOX : FBOX * FBOX);
+      const int base = c * FBOX3 + l0 + FBOX * (l1 + FBOX * l2);
+      if (KIND == FACE_LEGACY) {
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
+        for (int b = 0; b < 3; ++b)
 #pragma unroll
-      for (int a = 0; a < 3; ++a) off[3 * b + a] = a * sa + b * sb;
-    double Tm[9];
+          for (int a = 0; a < 3; ++a) {
+            const int p0 = base + a * sa + b * sb, p3 = p0 + 3 * st;
+            const double dx = xs[p0] - xs[p3];
+            const double d = fp.coef * dx;
+            atomicAdd(&ys[p0], d);
+            atomicAdd(&ys[p3], -d);
+            q += d * dx;
+          }
+      } else {
+        const int ta = dir == 0 ? 1 : 0, tb = dir == 2 ? 1 : 2;
+        const double* jv = fp.jump + fp.joff[dir] + 4 * (e[dir] + 1);
+        const double* Ma = fp.mass + fp.moff[ta] + 9 * e[ta];
+        const double* Mb = fp.mass + fp.moff[tb] + 9 * e[tb];
+        double Tm[9];
 #pragma unroll
-    for (int m = 0; m < 9; ++m) Tm[m] = 0.0;
-    // layers are walked one at a time (not unrolled) to keep 9, not 36,
-    // addresses live
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-      const double jt = jv[t];
-      const int64_t lb = base + t * st;
+        for (int m = 0; m < 9; ++m) Tm[m] = 0.0;
 #pragma unroll
-      for (int m = 0; m < 9; ++m) {
-        const int64_t g = lb + off[m];
-        const double xv = (freem == nullptr || freem[g]) ? (double)x[g] : 0.0;
-        Tm[m] += jt * xv;
+        for (int t = 0; t < 4; ++t) {
+          const double jt = jv[t];
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) Tm[3 * b + a] += jt * xs[base + t * st + a * sa + b * sb];
+        }
+        double U[9], W[9];
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            U[3 * b + a] = Ma[3 * a] * Tm[3 * b] + Ma[3 * a + 1] * Tm[3 * b + 1] + Ma[3 * a + 2] * Tm[3 * b + 2];
+        double tw = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const double v = fp.coef * (Mb[3 * b] * U[a] + Mb[3 * b + 1] * U[3 + a] + Mb[3 * b + 2] * U[6 + a]);
+            W[3 * b + a] = v;
+            tw += Tm[3 * b + a] * v;
+          }
+        q += tw;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const double jt = jv[t];
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) atomicAdd(&ys[base + t * st + a * sa + b * sb], jt * W[3 * b + a]);
+        }
       }
     }
-    // U = T M_a^T (rows b), then W = M_b U
-    double U[9], W[9];
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-        U[3 * b + a] = Ma[3 * a] * Tm[3 * b] + Ma[3 * a + 1] * Tm[3 * b + 1] + Ma[3 * a + 2] * Tm[3 * b + 2];
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-        W[3 * b + a] = Mb[3 * b] * U[a] + Mb[3 * b + 1] * U[3 + a] + Mb[3 * b + 2] * U[6 + a];
-    double tw = 0.0;
-#pragma unroll
-    for (int m = 0; m < 9; ++m) {
-      W[m] *= tb.coef;
-      tw += Tm[m] * W[m];
+    __syncthreads();
+    for (int l = threadIdx.x; l < 3 * FBOX3; l += FB) {
+      const double v = ys[l];
+      if (v == 0.0) continue;
+      const int c = l / FBOX3, r = l - c * FBOX3;
+      const int gx = b0 + r % FBOX, gy = b1 + (r / FBOX) % FBOX, gz = b2 + r / (FBOX * FBOX);
+      if (gx >= ncp0 || gy >= ncp1 || gz >= ncp2) continue;
+      const int64_t g = (int64_t)c * nsc + gx + (int64_t)ncp0 * (gy + (int64_t)ncp1 * gz);
+      if (freem == nullptr || freem[g]) atomicAdd(y + g, (T)v);
     }
-    q += tw;
-#pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-      const double jt = jv[t];
-      const int64_t lb = base + t * st;
-#pragma unroll
-      for (int m = 0; m < 9; ++m) {
-        const int64_t g = lb + off[m];
-        if (freem == nullptr || freem[g]) atomicAdd(y + g, (T)(jt * W[m]));
-      }
-    }
+    __syncthreads();
   }
   if (pap == nullptr) return;
   __shared__ double sh[FB / 64];
@@ -177,32 +162,30 @@ __global__ __launch_bounds__(FB) void ghost_faces_kernel(const int32_t* __restri
 
 int face_blocks(const icc_op* op) {
   if (op->f_n <= 0) return 0;
-  const int64_t items = (op->f_kind == FACE_GHOST ? 3 : 27) * op->f_n;
-  const int64_t nb = (items + FB - 1) / FB;
-  return (int)(nb < FACE_MAX_BLOCKS ? nb : FACE_MAX_BLOCKS);
+  return op->f_nblk < FACE_MAX_BLOCKS ? (int)op->f_nblk : FACE_MAX_BLOCKS;
 }
 
 template <typename T>
 hipError_t launch_faces(const icc_op* op, const T* x, T* y, double* pap, const PcgState* pst, hipStream_t st) {
   if (op->f_n <= 0) return hipSuccess;
   const auto& s = op->g->s;
-  if (op->f_kind == FACE_GHOST) {
-    GhostTabs tb;
-    tb.jump = op->d_ftab;
-    tb.mass = op->d_ftab + op->ftab_mass;
-    for (int d = 0; d < 3; ++d) {
-      tb.joff[d] = op->ftab_joff[d];
-      tb.moff[d] = op->ftab_moff[d];
-    }
-    tb.coef = op->f_alpha;
-    hipLaunchKernelGGL(ghost_faces_kernel<T>, dim3(face_blocks(op)), dim3(FB), 0, st, op->d_faces, (int)op->f_n,
-                       (int)s.nel[0], (int)s.nel[1], (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, tb, op->d_free,
-                       x, y, pap, pst);
-  } else {
-    hipLaunchKernelGGL(legacy_faces_kernel<T>, dim3(face_blocks(op)), dim3(FB), 0, st, op->d_faces, (int)op->f_n,
-                       (int)s.nel[0], (int)s.nel[1], (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, op->f_alpha,
-                       op->d_free, x, y, pap, pst);
+  FaceParams fp;
+  fp.jump = op->d_ftab;
+  fp.mass = op->d_ftab ? op->d_ftab + op->ftab_mass : nullptr;
+  for (int d = 0; d < 3; ++d) {
+    fp.joff[d] = op->ftab_joff[d];
+    fp.moff[d] = op->ftab_moff[d];
   }
+  fp.coef = op->f_alpha;
+  const int nbx = (int)((s.nel[0] + FACE_BLK - 1) / FACE_BLK), nby = (int)((s.nel[1] + FACE_BLK - 1) / FACE_BLK);
+  if (op->f_kind == FACE_GHOST)
+    hipLaunchKernelGGL((faces_blocked_kernel<T, FACE_GHOST>), dim3(face_blocks(op)), dim3(FB), 0, st, op->d_faces,
+                       op->d_fblk, op->d_fptr, (int)op->f_nblk, nbx, nby, (int)s.nel[0], (int)s.nel[1],
+                       (int)s.ncp[0], (int)s.ncp[1], (int)s.ncp[2], (int)s.ndof_sc, fp, op->d_free, x, y, pap, pst);
+  else
+    hipLaunchKernelGGL((faces_blocked_kernel<T, FACE_LEGACY>), dim3(face_blocks(op)), dim3(FB), 0, st, op->d_faces,
+                       op->d_fblk, op->d_fptr, (int)op->f_nblk, nbx, nby, (int)s.nel[0], (int)s.nel[1],
+                       (int)s.ncp[0], (int)s.ncp[1], (int)s.ncp[2], (int)s.ndof_sc, fp, op->d_free, x, y, pap, pst);
   return hipGetLastError();
 }
 
