@@ -116,6 +116,22 @@ def cpu_baseline(np, scale, x, budget_s):
                        f"OpenMP threads; CSR built in {t_asm:.1f}s (not timed)"), y
 
 
+def cpu_imm_topopt(np, icc):
+    """The oracle's restatement of ImmersedTopOpt::run (assembled CSR, Jacobi
+    PCG, OpenMP) for the first design iteration of the default 28x20x16
+    tilted-box problem."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    mesh = tilted_box(np, icc)
+    t0 = time.time()
+    comp, _, _, _, its = O.immersed_topopt_run(mesh.vertices, mesh.triangles, (28, 20, 16), 1, 0.35, 3.0, 1.8, 1e-2,
+                                               emin=1e-4, pcg_tol=1e-8, stab_kind=2, full=True)
+    dt = time.time() - t0
+    return dict(s_per_iteration=dt, pcg_iterations=int(its[0]), compliance=float(comp[0]),
+                cores=int(O.lib().orc_num_threads()), kind="port",
+                sample="first design iteration of the 28x20x16 tilted-box immersed SIMP run (set-up included)")
+
+
 def immersed_leg(np, icc, ctx, seed=20261204):
     """Immersed solve on a 128x64x64 background grid over [0,2]x[0,1]^2 minus
     three seeded z-cylinders (radius U[0.08, 0.15]); cells with volume fraction
@@ -250,6 +266,45 @@ def adjoint_leg(np, icc, ctx, oc_iters=20, nfd=32, seed=20261206):
     return out
 
 
+def tilted_box(np, icc):
+    """The reference's tilted-box B-rep (8 vertices, 12 triangles) from its
+    immersed topology-optimisation fixture."""
+    d = json.load(open(os.path.join(ROOT, "tests", "fixtures", "reference", "imm_topopt_tilted_box.json")))
+    return icc.TriangleMesh3D(np.asarray(d["nodes"], dtype=np.float64).reshape(-1, 3),
+                              np.asarray(d["elements"], dtype=np.int64).reshape(-1, 3))
+
+
+def imm_topopt_leg(np, icc, ctx):
+    """Immersed SIMP topology optimisation in the tilted-box B-rep with
+    ImmersedTopOptConfig::default (ghost penalty gamma 1e-2, volfrac 0.35,
+    penal 3, rmin 1.8, emin 1e-4, pcg_tol 1e-8): the default 28x20x16 grid
+    for its 30 iterations, and a 96x72x42 grid (957,264 DOF) for 3
+    iterations with FP64 PCG and with the mixed solver."""
+    mesh = tilted_box(np, icc)
+    out = {}
+    for name, res, iters, solver in (("default_28x20x16", (28, 20, 16), 30, 0),
+                                     ("fp64_96x72x42", (96, 72, 42), 3, 0),
+                                     ("mixed_96x72x42", (96, 72, 42), 3, 1)):
+        cfg = icc.ImmersedTopOptConfig(grid_res=res, max_iter=iters, solver=solver)
+        t0 = time.time()
+        opt = icc.ImmersedTopOpt(ctx, mesh, cfg)
+        t_build = time.time() - t0
+        stamps = []
+        t0 = time.time()
+        h = opt.run(lambda it, c, v, ch, n: stamps.append(time.time()))
+        dt = time.time() - t0
+        out[name] = dict(grid=list(res), ndof=opt.ndof, design_cells=opt.n_active, faces=opt.n_faces,
+                         iterations=iters, build_s=t_build, run_s=dt, s_per_iteration=dt / iters,
+                         first_iteration_s=stamps[0] - t0, pcg_iterations=h.pcg_iterations,
+                         compliance_first=h.compliance[0], compliance_last=h.compliance[-1],
+                         volume_last=h.volume[-1], change_last=h.change[-1])
+        opt.close()
+    a, b = out["fp64_96x72x42"], out["mixed_96x72x42"]
+    b["compliance_rel_diff_vs_fp64"] = abs(b["compliance_last"] - a["compliance_last"]) / abs(a["compliance_last"])
+    out["speedup_mixed_vs_fp64"] = a["run_s"] / b["run_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,6 +318,8 @@ def main():
                     help="skip the immersed, batched and adjoint configuration legs")
     ap.add_argument("--apply-only", action="store_true", help="time only the apply (for profilers)")
     ap.add_argument("--apply-prec", choices=("f64", "f32"), default="f64", help="precision of --apply-only")
+    ap.add_argument("--imm-only", action="store_true",
+                    help="run one design iteration of the 96x72x42 immersed topology optimisation and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of 735,000 doubles 20 times and exit")
     args = ap.parse_args()
@@ -284,6 +341,17 @@ def main():
     import numpy as np
     import torch
     import immersed_catmull_clark_amd as icc
+
+    if args.imm_only:
+        ctx = icc.Context(0)
+        cfg = icc.ImmersedTopOptConfig(grid_res=(96, 72, 42), max_iter=1)
+        opt = icc.ImmersedTopOpt(ctx, tilted_box(np, icc), cfg)
+        t0 = time.time()
+        h = opt.run()
+        print(json.dumps({"workload": "immersed topopt, tilted box, 96x72x42, 1 iteration, ghost penalty 1e-2",
+                          "ndof": opt.ndof, "faces": opt.n_faces, "run_s": time.time() - t0,
+                          "pcg_iterations": h.pcg_iterations, "compliance": h.compliance}), flush=True)
+        return
 
     log("bench: setting up the 96x48x48 grid")
     ctx = icc.Context(0)
@@ -389,12 +457,21 @@ def main():
         out["batch_24_instances"] = batch_leg(np, icc, ctx)
         log("bench: adjoint SIMP 128x64x64")
         out["adjoint_simp_128x64x64"] = adjoint_leg(np, icc, ctx)
+        log("bench: immersed topology optimisation (tilted box)")
+        out["imm_topopt_tilted_box"] = imm_topopt_leg(np, icc, ctx)
 
     if not args.no_cpu:
         log("bench: CPU baseline")
         cb, ycpu = cpu_baseline(np, scale, (free * xh), args.cpu_budget)
         yg = op.apply(xh)
         cb["rel_diff_vs_gpu_f64"] = float(np.linalg.norm(free * ycpu - yg) / np.linalg.norm(yg))
+        if not args.no_configs:
+            log("bench: CPU immersed topology optimisation (1 iteration, 28x20x16)")
+            cb["imm_topopt_28x20x16"] = cpu_imm_topopt(np, icc)
+            leg = out["imm_topopt_tilted_box"]["default_28x20x16"]
+            gpu_it = leg["build_s"] + leg["first_iteration_s"]
+            cb["imm_topopt_28x20x16"]["gpu_setup_and_first_iteration_s"] = gpu_it
+            cb["imm_topopt_28x20x16"]["speedup_gpu_vs_cpu"] = cb["imm_topopt_28x20x16"]["s_per_iteration"] / gpu_it
         out["cpu_baseline"] = cb
     out["wall_s_f64_apply_loop"] = wall
     out["tiling"] = {k: {"variant": v[0], "workgroups_per_cu": v[1], "us_per_apply": v[2]} for k, v in tuned.items()}
