@@ -104,11 +104,19 @@ SIGNATURES = [
     ("icc_imm_topopt_problem", cint, [vp, vp, vp, vp, vp, vp]),
     ("icc_imm_topopt_run", cint, [vp, vp, vp, vp, vp, vp, vp, vp]),
     ("icc_imm_topopt_state", cint, [vp, vp]),
+    ("icc_wq_create", cint, [vp, vpp]),
+    ("icc_wq_destroy", cint, [vp]),
+    ("icc_wq_form", cint, [vp, cint, vp, dbl, dbl]),
+    ("icc_wq_form_dev", cint, [vp, cint, vp, dbl, dbl]),
+    ("icc_wq_apply", cint, [vp, vp, vp]),
+    ("icc_wq_apply_dev", cint, [vp, vp, vp]),
+    ("icc_wq_csr", cint, [vp, vp, vp, vp, vp]),
 ]
 
 REPORT_FN = C.CFUNCTYPE(None, i64, dbl, dbl, dbl, vp)
 IMM_REPORT_FN = C.CFUNCTYPE(None, i64, dbl, dbl, dbl, i64, vp)
 ICC_STAB_NONE, ICC_STAB_LEGACY, ICC_STAB_GHOST = range(3)
+ICC_DENSITY_NONE, ICC_DENSITY_ELEMENT, ICC_DENSITY_SPLINE = range(3)
 
 
 class TopOptConfigC(C.Structure):

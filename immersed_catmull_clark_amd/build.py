@@ -31,12 +31,14 @@ SOURCES = [
     ("k_immersed.hip", ["-ffp-contract=off"]),
     ("k_faces.hip", []),
     ("k_topopt.hip", ["-ffp-contract=off"]),
+    ("k_wq.hip", []),
     ("setup.cpp", None),
     ("wq.cpp", None),
     ("api.cpp", None),
     ("api_ext.cpp", None),
     ("api_imm.cpp", None),
     ("api_imm_topopt.cpp", None),
+    ("api_wq.cpp", None),
 ]
 
 

@@ -309,3 +309,55 @@ def classify_cylinders(ctx, box_lo, box_hi, cyl, grid_bounds, res, sub=(4, 4, 4)
                                        _p(np.asarray(res, dtype=np.int64)), _p(np.asarray(sub, dtype=np.int64)),
                                        _p(status), _p(w)))
     return status, w
+
+
+class WqMatrix:
+    """Weighted-quadrature elasticity stiffness of a grid, formed on the GPU
+    (wq_stiffness / wq_stiffness_layout, rust/src/wq.rs:324-514, interior
+    layout): form(density, x, penal, emin) with density ICC_DENSITY_NONE,
+    ICC_DENSITY_ELEMENT (x[nel]) or ICC_DENSITY_SPLINE (x[ndof_sc])."""
+
+    def __init__(self, grid):
+        self.grid = grid
+        h = C.c_void_p()
+        check(lib().icc_wq_create(grid.h, C.byref(h)))
+        self.h = h
+
+    def form(self, density=0, x=None, penal=3.0, emin=1e-3):
+        if x is not None and _is_dev(x):
+            check(lib().icc_wq_form_dev(self.h, int(density), _p(x), float(penal), float(emin)))
+            return self
+        xx = None if x is None else _f64(x)
+        check(lib().icc_wq_form(self.h, int(density), _p(xx), float(penal), float(emin)))
+        return self
+
+    def apply(self, x, out=None):
+        if _is_dev(x):
+            check(lib().icc_wq_apply_dev(self.h, _p(x), _p(out)))
+            return out
+        y = np.zeros(self.grid.ndof)
+        check(lib().icc_wq_apply(self.h, _p(_f64(x)), _p(y)))
+        return y
+
+    def csr(self):
+        """(indptr, indices, data) of the formed matrix, rows/columns
+        component-blocked, columns ascending."""
+        nnz = C.c_int64(0)
+        check(lib().icc_wq_csr(self.h, C.byref(nnz), None, None, None))
+        ip = np.zeros(self.grid.ndof + 1, dtype=np.int64)
+        ix = np.zeros(nnz.value, dtype=np.int64)
+        dd = np.zeros(nnz.value)
+        check(lib().icc_wq_csr(self.h, C.byref(nnz), _p(ip), _p(ix), _p(dd)))
+        return ip, ix, dd
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().icc_wq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -85,6 +85,16 @@ int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
 
 void ghost_tables_1d(int64_t nel, double length, std::vector<double>& jump, std::vector<double>& mass);
 
+// WQ rules for p = 2 (wq.cpp)
+struct WqRulesP2 {
+  int nel = 0, ndof = 0, npt = 0;
+  std::vector<std::vector<int>> ind;             // points of each test function
+  std::vector<std::vector<double>> w0, w1;       // test-value and test-derivative weights on ind
+  std::vector<double> b, db;                     // [npt][ndof] basis values / parametric derivatives
+};
+void wq_rules_p2(int64_t nel, WqRulesP2& r);
+void wq_form_tables_1d(int64_t nel, std::vector<double>& M);
+
 }  // namespace icc
 
 struct icc_ctx {

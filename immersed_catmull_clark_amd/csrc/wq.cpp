@@ -11,7 +11,8 @@
 // sum over WQ points groups per element: A_{k,d}(i, e) = sum_{q in e} w_{k,d}(i, q),
 // and d(c, ii) = sum_k coef_k sum_{e in supp(ii)} simp_e prod_d A_{k,d}(ii_d, e_d).
 // This file builds A0 = sum W00 B and A1 = sum W11 B' per direction; the
-// GPU kernel does the 27-element gather per control point.
+// GPU kernel does the 27-element gather per control point.  It also builds
+// the per-direction tables of the full sum-factorised formation (k_wq.hip).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -115,9 +116,15 @@ std::vector<double> min_norm(const std::vector<double>& A, int m, int n, const s
 
 }  // namespace
 
-// A0[(i*3 + t)], A1[(i*3 + t)]: sums over the WQ points of element
-// first(i) + t for the 1D function i, first(i) = max(0, i - 2).
-void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1) {
+// WQ rules of rules_interior (wq.rs:141-240) for p = 2 on the open uniform
+// knot vector of `nel` elements (parametric [0, 1]): 3 points per element at
+// (2m - 1)/6 of it; for each test function i the points of the elements of
+// its support (element first(i) + t, t = 0..2, first(i) = max(0, i - 2)),
+// weights w0 (test value) and w1 (test derivative, by the recurrence
+// B_i' = p/da N_{i,p-1} - p/db N_{i+1,p-1}), each the weighted minimum-norm
+// rule exact on the C^0 quadratic trial space; W01 = W00 and W11 = W10 in
+// this layout.  b/db: all basis values / parametric derivatives at the points.
+void wq_rules_p2(int64_t nel, WqRulesP2& r) {
   const int p = 2;
   std::vector<double> knots(3, 0.0);
   for (int64_t i = 1; i < nel; ++i) knots.push_back((double)i / (double)nel);
@@ -127,7 +134,6 @@ void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1)
     if (brk.empty() || brk.back() != k) brk.push_back(k);
   const int ne = (int)brk.size() - 1;
   const int ndof = (int)knots.size() - p - 1;
-  // WQ points: 3 per element at (2m - 1) / 6 of the element (rules_interior, p = 2)
   std::vector<double> pts, hq;
   std::vector<int> el;
   for (int k = 0; k < ne; ++k) {
@@ -156,8 +162,19 @@ void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1)
     basis_all(kt, p, pts[q], Tq[q]);
     basis_all(knots, p - 1, pts[q], Mq[q]);
   }
-  A0.assign((size_t)ndof * 3, 0.0);
-  A1.assign((size_t)ndof * 3, 0.0);
+  r.nel = ne;
+  r.ndof = ndof;
+  r.npt = npt;
+  r.b.assign((size_t)npt * ndof, 0.0);
+  r.db.assign((size_t)npt * ndof, 0.0);
+  for (int q = 0; q < npt; ++q)
+    for (int j = 0; j < ndof; ++j) {
+      r.b[(size_t)q * ndof + j] = Bq[q][j];
+      r.db[(size_t)q * ndof + j] = dBq[q][j];
+    }
+  r.ind.assign(ndof, {});
+  r.w0.assign(ndof, {});
+  r.w1.assign(ndof, {});
   for (int i = 0; i < ndof; ++i) {
     const int e_lo = std::max(0, i - p), e_hi = std::min(i, ne - 1);
     std::vector<int> ind;
@@ -195,11 +212,11 @@ void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1)
       }
       const int m = (int)trial.size(), n = (int)act.size();
       std::vector<double> A((size_t)m * n), rhs(m);
-      for (int r = 0; r < m; ++r) {
-        for (int c = 0; c < n; ++c) A[(size_t)r * n + c] = Tq[ind[act[c]]][trial[r]] * z[act[c]];
+      for (int rr = 0; rr < m; ++rr) {
+        for (int c = 0; c < n; ++c) A[(size_t)rr * n + c] = Tq[ind[act[c]]][trial[rr]] * z[act[c]];
         double s = 0.0;
-        for (size_t g = 0; g < xg.size(); ++g) s += Tg[g][trial[r]] * wg[g] * mgauss(g);
-        rhs[r] = s;
+        for (size_t g = 0; g < xg.size(); ++g) s += Tg[g][trial[rr]] * wg[g] * mgauss(g);
+        rhs[rr] = s;
       }
       std::vector<double> v = min_norm(A, m, n, rhs);
       for (int c = 0; c < n; ++c) out[act[c]] = z[act[c]] * v[c];
@@ -209,19 +226,56 @@ void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1)
     std::vector<double> w1(ind.size(), 0.0);
     const double da = knots[i + p] - knots[i], db = knots[i + p + 1] - knots[i + 1];
     if (da > 0.0) {
-      auto r = rule([&](int q) { return Mq[q][i]; }, [&](size_t g) { return Mg[g][i]; });
-      for (size_t c = 0; c < w1.size(); ++c) w1[c] += (double)p / da * r[c];
+      auto rr = rule([&](int q) { return Mq[q][i]; }, [&](size_t g) { return Mg[g][i]; });
+      for (size_t c = 0; c < w1.size(); ++c) w1[c] += (double)p / da * rr[c];
     }
     if (db > 0.0) {
-      auto r = rule([&](int q) { return Mq[q][i + 1]; }, [&](size_t g) { return Mg[g][i + 1]; });
-      for (size_t c = 0; c < w1.size(); ++c) w1[c] -= (double)p / db * r[c];
+      auto rr = rule([&](int q) { return Mq[q][i + 1]; }, [&](size_t g) { return Mg[g][i + 1]; });
+      for (size_t c = 0; c < w1.size(); ++c) w1[c] -= (double)p / db * rr[c];
     }
-    for (size_t c = 0; c < ind.size(); ++c) {
-      const int q = ind[c], t = el[q] - std::max(0, i - p);
-      A0[(size_t)i * 3 + t] += w0[c] * Bq[q][i];
-      A1[(size_t)i * 3 + t] += w1[c] * dBq[q][i];
-    }
+    r.ind[i] = ind;
+    r.w0[i] = w0;
+    r.w1[i] = w1;
   }
+}
+
+// A0[(i*3 + t)], A1[(i*3 + t)]: sums over the WQ points of element
+// first(i) + t for the 1D function i, first(i) = max(0, i - 2).
+void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1) {
+  WqRulesP2 r;
+  wq_rules_p2(nel, r);
+  A0.assign((size_t)r.ndof * 3, 0.0);
+  A1.assign((size_t)r.ndof * 3, 0.0);
+  for (int i = 0; i < r.ndof; ++i)
+    for (size_t c = 0; c < r.ind[i].size(); ++c) {
+      const int q = r.ind[i][c], t = q / 3 - std::max(0, i - 2);
+      A0[(size_t)i * 3 + t] += r.w0[i][c] * r.b[(size_t)q * r.ndof + i];
+      A1[(size_t)i * 3 + t] += r.w1[i][c] * r.db[(size_t)q * r.ndof + i];
+    }
+}
+
+// Formation tables of one direction for the sum-factorised WQ stiffness:
+// M[((i * 4 + t) * 5 + jr) * 9 + qc] = w_t(i, qc) * val_t(point ind_i[qc], trial i - 2 + jr)
+// with (w, val) = (w0, B), (w1, B), (w0, B'), (w1, B') for t = 0..3 (the
+// W00, W10, W01, W11 contractions of wq.rs:418-438; B' parametric), zero for
+// missing trials or points; the row's points are those of elements
+// first(i) + qc / 3, point qc % 3 of each.
+void wq_form_tables_1d(int64_t nel, std::vector<double>& M) {
+  WqRulesP2 r;
+  wq_rules_p2(nel, r);
+  M.assign((size_t)r.ndof * 4 * 5 * 9, 0.0);
+  for (int i = 0; i < r.ndof; ++i)
+    for (size_t c = 0; c < r.ind[i].size(); ++c) {
+      const int q = r.ind[i][c];
+      const int qc = q - 3 * std::max(0, i - 2);
+      for (int jr = 0; jr < 5; ++jr) {
+        const int j = i - 2 + jr;
+        if (j < 0 || j >= r.ndof) continue;
+        const double bv = r.b[(size_t)q * r.ndof + j], dv = r.db[(size_t)q * r.ndof + j];
+        const double wt[4] = {r.w0[i][c] * bv, r.w1[i][c] * bv, r.w0[i][c] * dv, r.w1[i][c] * dv};
+        for (int t = 0; t < 4; ++t) M[(((size_t)i * 4 + t) * 5 + jr) * 9 + qc] = wt[t];
+      }
+    }
 }
 
 }  // namespace icc

@@ -191,6 +191,28 @@ int icc_filter_bwd(icc_filter* f, const double* g, double* out);
 int icc_wq_diag(icc_grid* g, const double* x, double penal, double emin, double* d);
 int icc_wq_diag_dev(icc_grid* g, const double* x_dev, double penal, double emin, double* d_dev);
 
+/* ---- WQ stiffness formation ------------------------------------------------ */
+/* GPU formation of wq_stiffness / wq_stiffness_layout (rust/src/wq.rs:324-514,
+ * interior layout, p = 2): the WQ elasticity stiffness of the grid's space
+ * (its young / poisson), component-blocked and symmetrised 0.5 (K + K^T),
+ * with the SIMP factor emin + rho^penal (1 - emin) at the WQ points.
+ * density: ICC_DENSITY_NONE (factor 1), ICC_DENSITY_ELEMENT (x[nel],
+ * Density::Element) or ICC_DENSITY_SPLINE (x[ndof_sc] control-point
+ * densities on the displacement space, Density::Spline).  Stored on the
+ * device as a 5x5x5-stencil matrix (9 * 125 * ndof_sc doubles). */
+enum { ICC_DENSITY_NONE = 0, ICC_DENSITY_ELEMENT = 1, ICC_DENSITY_SPLINE = 2 };
+typedef struct icc_wq icc_wq;
+int icc_wq_create(icc_grid* g, icc_wq** out);
+int icc_wq_destroy(icc_wq* m);
+int icc_wq_form(icc_wq* m, int density, const double* x, double penal, double emin);
+int icc_wq_form_dev(icc_wq* m, int density, const double* x_dev, double penal, double emin);
+/* y = K_wq x (unmasked, like CsrMatrix::matvec) */
+int icc_wq_apply(icc_wq* m, const double* x, double* y);
+int icc_wq_apply_dev(icc_wq* m, const double* x_dev, double* y_dev);
+/* CSR copy on the host (the CsrMatrix wq_stiffness returns): rows ndof,
+ * columns ascending, every in-grid stencil entry; NULL arrays: *nnz only. */
+int icc_wq_csr(icc_wq* m, int64_t* nnz, int64_t* indptr, int64_t* indices, double* data);
+
 /* ---- immersed classification --------------------------------------------- */
 /* classify_background_cells + element_weights (rust/src/immersed.rs:88-155)
  * with the analytic box-minus-z-cylinders predicate: inside iff sdf(q) < 0.
