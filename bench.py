@@ -266,6 +266,55 @@ def adjoint_leg(np, icc, ctx, oc_iters=20, nfd=32, seed=20261206):
     return out
 
 
+def wq_leg(np, torch, icc, ctx, grid, rho, reps=5):
+    """Weighted-quadrature stiffness formation (wq_stiffness, interior layout)
+    on the 96x48x48 grid with the benchmark's element densities (penal 3,
+    emin 1e-9), densities already on the device: formation time (SIMP factor
+    at the WQ points, sum-factorised rows, symmetrisation) and the time of
+    one product with the formed 5x5x5-stencil matrix (HIP events)."""
+    m = icc.WqMatrix(grid)
+    xd = torch.from_numpy(rho).cuda()
+    v = torch.randn(grid.ndof, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(v)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    m.form(icc.ICC_DENSITY_ELEMENT, xd, 3.0, 1e-9)
+    m.apply(v, out=y)
+    ctx.sync()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(stream)
+    for _ in range(reps):
+        m.form(icc.ICC_DENSITY_ELEMENT, xd, 3.0, 1e-9)
+    e1.record(stream)
+    for _ in range(reps):
+        m.apply(v, out=y)
+    e2.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    form_ms = e0.elapsed_time(e1) / reps
+    apply_ms = e1.elapsed_time(e2) / reps
+    nsc = grid.ndof_sc
+    out = dict(rows=nsc, stored_doubles=9 * 125 * nsc, form_ms=form_ms, apply_ms=apply_ms,
+               apply_gb_per_s=9 * 125 * nsc * 8 / (apply_ms * 1e-3) / 1e9)
+    m.close()
+    return out
+
+
+def cpu_wq(np):
+    """The oracle's restatement of wq_stiffness (element densities, assembled
+    CSR) on a 32x16x16 sample of the same box."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    nel = [32, 16, 16]
+    sp = O.Space([list(b) for b in BOUNDS], nel, 2)
+    x = np.random.default_rng(SEED).uniform(0.01, 1.0, int(np.prod(nel)))
+    t0 = time.time()
+    sp.assemble_wq(1.0, 0.3, 1, x, 3.0, 1e-9, layout=0)
+    dt = time.time() - t0
+    rows = (nel[0] + 2) * (nel[1] + 2) * (nel[2] + 2)
+    return dict(sample="WQ formation of the 32x16x16 grid (11,016 test functions), element densities",
+                seconds=dt, us_per_row=dt / rows * 1e6, cores=int(O.lib().orc_num_threads()), kind="port")
+
+
 def tilted_box(np, icc):
     """The reference's tilted-box B-rep (8 vertices, 12 triangles) from its
     immersed topology-optimisation fixture."""
@@ -318,6 +367,8 @@ def main():
                     help="skip the immersed, batched and adjoint configuration legs")
     ap.add_argument("--apply-only", action="store_true", help="time only the apply (for profilers)")
     ap.add_argument("--apply-prec", choices=("f64", "f32"), default="f64", help="precision of --apply-only")
+    ap.add_argument("--wq-only", action="store_true",
+                    help="run only the WQ formation leg on the 96x48x48 grid and exit")
     ap.add_argument("--imm-only", action="store_true",
                     help="run one design iteration of the 96x72x42 immersed topology optimisation and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -365,6 +416,10 @@ def main():
     x32 = x.float()
     y32 = torch.zeros(ndof, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
+
+    if args.wq_only:
+        print(json.dumps({"wq_form_96x48x48": wq_leg(np, torch, icc, ctx, grid, rho)}), flush=True)
+        return
 
     flops = 2.0 * 81 * 81 * nel
     # tile shape and workgroups per CU picked by timing (FP64 apply first)
@@ -459,6 +514,8 @@ def main():
         out["adjoint_simp_128x64x64"] = adjoint_leg(np, icc, ctx)
         log("bench: immersed topology optimisation (tilted box)")
         out["imm_topopt_tilted_box"] = imm_topopt_leg(np, icc, ctx)
+        log("bench: WQ formation 96x48x48")
+        out["wq_form_96x48x48"] = wq_leg(np, torch, icc, ctx, grid, rho)
 
     if not args.no_cpu:
         log("bench: CPU baseline")
@@ -472,6 +529,11 @@ def main():
             gpu_it = leg["build_s"] + leg["first_iteration_s"]
             cb["imm_topopt_28x20x16"]["gpu_setup_and_first_iteration_s"] = gpu_it
             cb["imm_topopt_28x20x16"]["speedup_gpu_vs_cpu"] = cb["imm_topopt_28x20x16"]["s_per_iteration"] / gpu_it
+            log("bench: CPU WQ formation (32x16x16 sample)")
+            cb["wq_form"] = cpu_wq(np)
+            wq = out["wq_form_96x48x48"]
+            cb["wq_form"]["gpu_us_per_row_96x48x48"] = wq["form_ms"] * 1e3 / wq["rows"]
+            cb["wq_form"]["speedup_gpu_vs_cpu_per_row"] = cb["wq_form"]["us_per_row"] / (wq["form_ms"] * 1e3 / wq["rows"])
         out["cpu_baseline"] = cb
     out["wall_s_f64_apply_loop"] = wall
     out["tiling"] = {k: {"variant": v[0], "workgroups_per_cu": v[1], "us_per_apply": v[2]} for k, v in tuned.items()}
