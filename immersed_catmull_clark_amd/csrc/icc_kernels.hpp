@@ -83,7 +83,7 @@ hipError_t launch_wq_diag(const icc_grid* g, const double* tab, const double* x,
 // k_faces.hip: matrix-free face stabilisation, at most FACE_MAX_BLOCKS p.Ap partials
 constexpr int FACE_MAX_BLOCKS = 512;
 constexpr int FACE_LEGACY = 1, FACE_GHOST = 2;
-constexpr int FACE_BLK = 8;  // faces are grouped by FACE_BLK^3 element blocks
+constexpr int FACE_BLK = 4;  // faces are grouped by FACE_BLK^3 element blocks
 int face_blocks(const icc_op* op);
 template <typename T>
 hipError_t launch_faces(const icc_op* op, const T* x, T* y, double* pap, const PcgState* pst, hipStream_t st);

@@ -14,8 +14,8 @@
 //     T = j . X (the jump at each tangential point), W = M_a T M_b^T,
 //     y += coef j (x) W, quadratic form coef T : W.
 //
-// Faces are grouped by 8 x 8 x 8 element blocks.  One workgroup takes one
-// block: it stages the masked x of the block's (8+3)^3 control-point box in
+// Faces are grouped by 4 x 4 x 4 element blocks.  One workgroup takes one
+// block: it stages the masked x of the block's (4+3)^3 control-point box in
 // LDS, applies every face of the block with LDS double atomics into a box of
 // y (one thread per face and component), and flushes the nonzero box
 // entries to the free DOFs with one global atomic each (coalesced rows).  Global atomics drop from 18 (legacy) or 36
@@ -58,16 +58,33 @@ __global__ __launch_bounds__(FB) void faces_blocked_kernel(const int32_t* __rest
   for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
     const int bid = fblk[bi];
     const int b0 = (bid % nbx) * FACE_BLK, b1 = ((bid / nbx) % nby) * FACE_BLK, b2 = (bid / (nbx * nby)) * FACE_BLK;
-    for (int l = threadIdx.x; l < 3 * FBOX3; l += FB) {
+    // staging: all loads of a thread are issued before the LDS writes (the
+    // x value and its mask bit are independent loads)
+    constexpr int NST = (3 * FBOX3 + FB - 1) / FB;
+    double sv[NST];
+    bool sf[NST];
+#pragma unroll
+    for (int n = 0; n < NST; ++n) {
+      const int l = threadIdx.x + n * FB;
       const int c = l / FBOX3, r = l - c * FBOX3;
       const int gx = b0 + r % FBOX, gy = b1 + (r / FBOX) % FBOX, gz = b2 + r / (FBOX * FBOX);
       double v = 0.0;
-      if (gx < ncp0 && gy < ncp1 && gz < ncp2) {
+      bool f = false;
+      if (l < 3 * FBOX3 && gx < ncp0 && gy < ncp1 && gz < ncp2) {
         const int64_t g = (int64_t)c * nsc + gx + (int64_t)ncp0 * (gy + (int64_t)ncp1 * gz);
-        if (freem == nullptr || freem[g]) v = (double)x[g];
+        v = (double)x[g];
+        f = freem == nullptr || freem[g];
       }
-      xs[l] = v;
-      ys[l] = 0.0;
+      sv[n] = v;
+      sf[n] = f;
+    }
+#pragma unroll
+    for (int n = 0; n < NST; ++n) {
+      const int l = threadIdx.x + n * FB;
+      if (l < 3 * FBOX3) {
+        xs[l] = sf[n] ? sv[n] : 0.0;
+        ys[l] = 0.0;
+      }
     }
     __syncthreads();
     const int fb = fptr[bi], nfb = fptr[bi + 1] - fb;
