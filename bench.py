@@ -35,6 +35,7 @@ BOUNDS = ((0.0, 2.0), (0.0, 1.0), (0.0, 1.0))
 SEED = 20261203
 PEAK_F64 = 78.6e12   # MI355X dense FP64 MFMA, FLOP/s
 PEAK_F32 = 157.3e12  # MI355X dense FP32, FLOP/s
+PEAK_BF16 = 2.5e15   # MI355X dense BF16 MFMA, FLOP/s
 PEAK_HBM = 8.0e12    # B/s
 
 
@@ -366,7 +367,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the immersed, batched and adjoint configuration legs")
     ap.add_argument("--apply-only", action="store_true", help="time only the apply (for profilers)")
-    ap.add_argument("--apply-prec", choices=("f64", "f32"), default="f64", help="precision of --apply-only")
+    ap.add_argument("--apply-prec", choices=("f64", "f32", "bf16"), default="f64", help="precision of --apply-only")
     ap.add_argument("--wq-only", action="store_true",
                     help="run only the WQ formation leg on the 96x48x48 grid and exit")
     ap.add_argument("--imm-only", action="store_true",
@@ -424,11 +425,12 @@ def main():
     flops = 2.0 * 81 * 81 * nel
     # tile shape and workgroups per CU picked by timing (FP64 apply first)
     tuned = {"f64": op.autotune(icc.ICC_F64, reps=5)}
-    if args.apply_only and args.apply_prec == "f32":
-        op.autotune(icc.ICC_F32, reps=5)
-        ms32 = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F32, args.steps, args.warmup, accumulate=True)
-        print(json.dumps({"kernel": "apply_tiles<float>", "kernel_ms": ms32,
-                          "tflops": flops / (ms32 * 1e-3) / 1e12}), flush=True)
+    if args.apply_only and args.apply_prec in ("f32", "bf16"):
+        pr = icc.ICC_F32 if args.apply_prec == "f32" else icc.ICC_BF16
+        op.autotune(pr, reps=5)
+        ms32 = time_applies(torch, icc, ctx, op, x32, y32, pr, args.steps, args.warmup, accumulate=True)
+        print(json.dumps({"kernel": "apply_tiles<float%s>" % ("" if pr == icc.ICC_F32 else ", true"),
+                          "kernel_ms": ms32, "tflops": flops / (ms32 * 1e-3) / 1e12}), flush=True)
         return
     log("bench: FP64 applies")
     t0 = time.time()
@@ -461,6 +463,14 @@ def main():
     out["apply_f32"] = {"applies_per_s": 1e3 / ms_f32, "ms": ms_f32, "kernel_ms": ms_k32,
                         "tflops": flops / (ms_k32 * 1e-3) / 1e12, "frac_of_fp32_peak": flops / (ms_k32 * 1e-3) / PEAK_F32,
                         "rel_err_vs_f64": float(np.linalg.norm(yh32 - yh64) / np.linalg.norm(yh64))}
+    log("bench: BF16 applies")
+    tuned["bf16"] = op.autotune(icc.ICC_BF16, reps=5)
+    ms_b = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_BF16, args.steps, args.warmup)
+    ms_kb = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_BF16, args.steps, args.warmup, accumulate=True)
+    yhb = op.apply(xh, prec=icc.ICC_BF16)
+    out["apply_bf16"] = {"applies_per_s": 1e3 / ms_b, "ms": ms_b, "kernel_ms": ms_kb,
+                         "tflops": flops / (ms_kb * 1e-3) / 1e12, "frac_of_bf16_peak": flops / (ms_kb * 1e-3) / PEAK_BF16,
+                         "rel_err_vs_f64": float(np.linalg.norm(yhb - yh64) / np.linalg.norm(yh64))}
 
     if not args.no_pcg:
         # traction load on x = 2 (consistent: F = t int N dA), via the driver's builder

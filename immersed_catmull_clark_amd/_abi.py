@@ -11,7 +11,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libicc.so")
 
 ICC_OK, ICC_EINVAL, ICC_ENOMEM, ICC_EHIP, ICC_EBREAKDOWN, ICC_ENOCONV, ICC_ENOTSUP = range(7)
-ICC_F64, ICC_F32 = 0, 1
+ICC_F64, ICC_F32, ICC_BF16 = 0, 1, 2
 
 i64 = C.c_int64
 dbl = C.c_double
@@ -104,6 +104,7 @@ SIGNATURES = [
     ("icc_imm_topopt_problem", cint, [vp, vp, vp, vp, vp, vp]),
     ("icc_imm_topopt_run", cint, [vp, vp, vp, vp, vp, vp, vp, vp]),
     ("icc_imm_topopt_state", cint, [vp, vp]),
+    ("icc_debug_mfma_layout_bf16", cint, [vp, vp]),
     ("icc_wq_create", cint, [vp, vpp]),
     ("icc_wq_destroy", cint, [vp]),
     ("icc_wq_form", cint, [vp, cint, vp, dbl, dbl]),

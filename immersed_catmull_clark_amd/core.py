@@ -60,6 +60,12 @@ class Context:
         check(lib().icc_debug_mfma_layout(self.h, _p(out)))
         return out[:256].reshape(64, 4), out[256:].reshape(64, 4)
 
+    def mfma_layout_bf16(self):
+        """Accumulator contents [lane, reg] of the BF16 16x16x32 layout probe."""
+        out = np.zeros(256)
+        check(lib().icc_debug_mfma_layout_bf16(self.h, _p(out)))
+        return out.reshape(64, 4)
+
 
 class Grid:
     """p = 2 B-spline space on a box with its exact element matrices

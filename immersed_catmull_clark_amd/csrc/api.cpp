@@ -37,6 +37,15 @@ static int dalloc(T** p, size_t n) {
   return ICC_OK;
 }
 
+// BF16 bits of a float, round to nearest even (the conversion the device's
+// v_cvt_pk_bf16_f32 applies to the staged x)
+static uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
 #define ICC_TRY(call)            \
   do {                           \
     int rc__ = (call);           \
@@ -145,6 +154,10 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
   const size_t nfrag = (size_t)g->ntypes * MT * KS * 64;
   std::vector<double> f64(nfrag, 0.0);
   std::vector<float> f32(nfrag, 0.0f);
+  // BF16 fragments (v_mfma_f32_16x16x32_bf16): lane l of (type, mt, ks) holds
+  // A[i = l%16][k = 32 ks + 8 (l/16) + j], j = 0..7, with the FP32 row slots
+  const size_t nfragb = (size_t)g->ntypes * MT * 3 * 64 * 8;
+  std::vector<uint16_t> fb(nfragb, 0);
   std::vector<double> kd((size_t)g->ntypes * NSH);
   for (int t = 0; t < g->ntypes; ++t) {
     const double* K = s.kmat[t].data();
@@ -157,13 +170,22 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
           f64[idx] = (row64 >= 0 && colk < NSH) ? K[row64 * NSH + colk] : 0.0;
           f32[idx] = (float)((row32 >= 0 && colk < NSH) ? K[row32 * NSH + colk] : 0.0);
         }
+    for (int mt = 0; mt < MT; ++mt)
+      for (int ks = 0; ks < 3; ++ks)
+        for (int l = 0; l < 64; ++l)
+          for (int j = 0; j < 8; ++j) {
+            const int i = l % 16, colk = 32 * ks + 8 * (l / 16) + j;
+            const int row32 = perm_row(mt, i / 4, i % 4);
+            const float v = (float)((row32 >= 0 && colk < NSH) ? K[row32 * NSH + colk] : 0.0);
+            fb[((((size_t)t * MT + mt) * 3 + ks) * 64 + l) * 8 + j] = bf16_bits(v);
+          }
     for (int a = 0; a < NSH; ++a) kd[(size_t)t * NSH + a] = K[a * NSH + a];
   }
   std::vector<int32_t> tid((size_t)3 * g->maxnel, 0);
   for (int d = 0; d < 3; ++d)
     for (int64_t e = 0; e < s.nel[d]; ++e) tid[(size_t)d * g->maxnel + e] = s.tid1[d][e];
   hipStream_t st = ctx->stream;
-  if ((rc = dalloc(&g->d_kfrag64, nfrag)) || (rc = dalloc(&g->d_kfrag32, nfrag)) ||
+  if ((rc = dalloc(&g->d_kfrag64, nfrag)) || (rc = dalloc(&g->d_kfrag32, nfrag)) || (rc = dalloc(&g->d_kfragb, nfragb)) ||
       (rc = dalloc(&g->d_kdiag, kd.size())) || (rc = dalloc(&g->d_tid1, tid.size())) ||
       (rc = dalloc(&g->d_typeof, s.type_of.size())) || (rc = dalloc(&g->d_tiles, s.tiles.size()))) {
     icc_grid_destroy(g);
@@ -172,6 +194,7 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
   hipError_t e;
   if ((e = hipMemcpyAsync(g->d_kfrag64, f64.data(), nfrag * 8, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(g->d_kfrag32, f32.data(), nfrag * 4, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(g->d_kfragb, fb.data(), nfragb * 2, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(g->d_kdiag, kd.data(), kd.size() * 8, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(g->d_tid1, tid.data(), tid.size() * 4, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(g->d_typeof, s.type_of.data(), s.type_of.size() * 4, hipMemcpyHostToDevice, st)) ||
@@ -188,6 +211,7 @@ ICC_API int icc_grid_destroy(icc_grid* g) {
   if (!g) return ICC_OK;
   hipFree(g->d_kfrag64);
   hipFree(g->d_kfrag32);
+  hipFree(g->d_kfragb);
   hipFree(g->d_kdiag);
   hipFree(g->d_tid1);
   hipFree(g->d_typeof);
@@ -752,6 +776,22 @@ static hipError_t apply_t(icc_op* op, const T* x, T* y, double* pap) {
   else
     return launch_apply_f32(g, op, x, y, pap, nullptr, st);
 }
+// y = A x with the element contraction in BF16 (FP32 vectors; sparse and
+// face terms in FP32)
+static hipError_t apply_bf16(icc_op* op, const float* x, float* y) {
+  icc_grid* g = op->g;
+  hipStream_t st = g->ctx->stream;
+  hipError_t e;
+  const int64_t n = g->s.ndof;
+  if (op->s_nnz > 0)
+    e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, x, y, op->d_free, nullptr,
+                                      st);
+  else
+    e = launch_fill<float>(y, 0.0f, n, st);
+  if (e == hipSuccess) e = launch_faces<float>(op, x, y, nullptr, nullptr, st);
+  if (e != hipSuccess) return e;
+  return launch_apply_bf16(g, op, x, y, nullptr, nullptr, st);
+}
 }  // namespace icc
 
 ICC_API int icc_apply_dev(icc_op* op, int prec, const void* x, void* y) {
@@ -761,6 +801,8 @@ ICC_API int icc_apply_dev(icc_op* op, int prec, const void* x, void* y) {
     e = apply_t<double>(op, (const double*)x, (double*)y, nullptr);
   else if (prec == ICC_F32)
     e = apply_t<float>(op, (const float*)x, (float*)y, nullptr);
+  else if (prec == ICC_BF16)
+    e = apply_bf16(op, (const float*)x, (float*)y);
   else {
     set_error("icc_apply: unknown precision");
     return ICC_EINVAL;
@@ -780,6 +822,8 @@ ICC_API int icc_apply_accumulate_dev(icc_op* op, int prec, const void* x, void* 
     e = launch_apply_f64(op->g, op, (const double*)x, (double*)y, nullptr, nullptr, op->g->ctx->stream);
   else if (prec == ICC_F32)
     e = launch_apply_f32(op->g, op, (const float*)x, (float*)y, nullptr, nullptr, op->g->ctx->stream);
+  else if (prec == ICC_BF16)
+    e = launch_apply_bf16(op->g, op, (const float*)x, (float*)y, nullptr, nullptr, op->g->ctx->stream);
   else {
     set_error("icc_apply_accumulate_dev: unknown precision");
     return ICC_EINVAL;
@@ -798,9 +842,12 @@ ICC_API int icc_apply(icc_op* op, int prec, const double* x, double* y) {
   if (prec == ICC_F64) {
     ICC_HIP(apply_t<double>(op, w->tmp, w->ap, nullptr));
     ICC_HIP(hipMemcpyAsync(y, w->ap, n * 8, hipMemcpyDeviceToHost, st));
-  } else if (prec == ICC_F32) {
+  } else if (prec == ICC_F32 || prec == ICC_BF16) {
     ICC_HIP(launch_to_f32(w->tmp, w->x32, n, st));
-    ICC_HIP(apply_t<float>(op, w->x32, w->ap32, nullptr));
+    if (prec == ICC_F32)
+      ICC_HIP(apply_t<float>(op, w->x32, w->ap32, nullptr));
+    else
+      ICC_HIP(apply_bf16(op, w->x32, w->ap32));
     std::vector<float> h(n);
     ICC_HIP(hipMemcpyAsync(h.data(), w->ap32, n * 4, hipMemcpyDeviceToHost, st));
     ICC_HIP(hipStreamSynchronize(st));
@@ -1392,6 +1439,18 @@ ICC_API int icc_classify_cylinders(icc_ctx* ctx, const double box_lo[3], const d
   return ICC_OK;
 }
 
+ICC_API int icc_debug_mfma_layout_bf16(icc_ctx* ctx, double* out256) {
+  if (!ctx || !out256) return ICC_EINVAL;
+  double* d = nullptr;
+  ICC_TRY(dalloc(&d, 256));
+  hipError_t e = launch_layout_probe_bf16(d, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out256, d, 256 * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "icc_debug_mfma_layout_bf16");
+  return ICC_OK;
+}
+
 ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out512) {
   if (!ctx || !out512) return ICC_EINVAL;
   double* d = nullptr;
@@ -1410,7 +1469,7 @@ ICC_API int icc_debug_mfma_layout(icc_ctx* ctx, double* out512) {
 // fastest (ties to the lower variant).  The operator's vectors are scratch.
 ICC_API int icc_grid_autotune(icc_op* op, int prec, int reps, int64_t* best_variant, int64_t* best_wg,
                               double* best_us) {
-  if (!op || (prec != ICC_F64 && prec != ICC_F32) || reps < 1) return ICC_EINVAL;
+  if (!op || (prec != ICC_F64 && prec != ICC_F32 && prec != ICC_BF16) || reps < 1) return ICC_EINVAL;
   if (op->s_nnz > 0 || op->f_n > 0) {
     set_error("icc_grid_autotune: operators with a sparse term are not supported");
     return ICC_ENOTSUP;
@@ -1439,8 +1498,10 @@ ICC_API int icc_grid_autotune(icc_op* op, int prec, int reps, int64_t* best_vari
         if (e != hipSuccess) break;
         if (prec == ICC_F64)
           e = launch_apply_f64(g, op, w->tmp, w->ap, nullptr, nullptr, st);
-        else
+        else if (prec == ICC_F32)
           e = launch_apply_f32(g, op, w->x32, w->ap32, nullptr, nullptr, st);
+        else
+          e = launch_apply_bf16(g, op, w->x32, w->ap32, nullptr, nullptr, st);
       }
       if (e == hipSuccess) e = hipEventRecord(e1, st);
       if (e == hipSuccess) e = hipEventSynchronize(e1);

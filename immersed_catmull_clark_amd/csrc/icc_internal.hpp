@@ -110,6 +110,7 @@ struct icc_grid {
   // device data
   double* d_kfrag64 = nullptr;  // [ntypes][MT][KS][64]
   float* d_kfrag32 = nullptr;   // [ntypes][MT][KS][64]
+  uint16_t* d_kfragb = nullptr; // [ntypes][MT][KSB = 3][64][8] BF16 bits
   double* d_kdiag = nullptr;    // [ntypes][81]
   int32_t* d_tid1 = nullptr;    // [3][maxnel] 1D types
   int32_t* d_typeof = nullptr;  // [T0*T1*T2]

@@ -37,11 +37,15 @@ hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x
                             const PcgState* pst, hipStream_t st);
 hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
                             const PcgState* pst, hipStream_t st);
+// K and x rounded to BF16, FP32 accumulation (v_mfma_f32_16x16x32_bf16); float x, y, scale
+hipError_t launch_apply_bf16(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
+                             const PcgState* pst, hipStream_t st);
 hipError_t launch_energies(const icc_grid* g, const double* u, double* ee, const double* x, double penal,
                            double emin, const double* w, double* dc, hipStream_t st);
 hipError_t launch_diag_f64(const icc_grid* g, const double* scale, double* d, hipStream_t st);
 hipError_t launch_diag_f32(const icc_grid* g, const float* scale, float* d, hipStream_t st);
 hipError_t launch_layout_probe(double* out, hipStream_t st);
+hipError_t launch_layout_probe_bf16(double* out, hipStream_t st);
 
 // k_vec.hip
 int vec_blocks(int64_t n);

@@ -31,6 +31,7 @@ namespace icc {
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
 template <typename T>
 struct Mfma;
@@ -139,20 +140,38 @@ struct TileGeom {
 constexpr int APPLY_THREADS = 384;
 constexpr int BCH = 4;                        // K steps per B-operand chunk
 constexpr int NBCH = (KS + BCH - 1) / BCH;    // chunks per element group
-// offset table of a tile: [kq][ks] trial offsets (4 x KS), then [wave][kq][r]
-// output-row offsets (6 x 4 x 4)
-constexpr int OTAB = 4 * KS + MT * 16;
+// BF16 mode (v_mfma_f32_16x16x32_bf16): lane l holds A[l%16][8 (l/16) + j]
+// and B[8 (l/16) + j][l%16], j = 0..7; accumulators as v_mfma_f32_16x16x4.
+constexpr int KSB = 3;                        // 32-deep K steps covering 81 trial columns (96 padded)
+// trial offsets of a tile: [kq][ks] (4 x KS) in the FP64 / FP32 modes,
+// [kq][ks][j] (4 x KSB x 8) in the BF16 mode; then [wave][kq][r] output-row
+// offsets (6 x 4 x 4)
+template <bool BF>
+struct Otab {
+  static constexpr int ntrial = BF ? 4 * KSB * 8 : 4 * KS;
+  static constexpr int size = ntrial + MT * 16;
+};
+constexpr int OTAB_MAX = Otab<true>::size;
 
-// Fills the offset table of a tile (threads 0 .. OTAB-1).  Entries for the
+// Fills the offset table of a tile (threads 0 .. size-1).  Entries for the
 // padding trial columns and rows point at offset 0 (their A entries are zero
 // and padding rows are never stored).
+template <bool BF>
 __device__ __forceinline__ void offset_table(const TileGeom& G, int* tab) {
   const int i = threadIdx.x;
-  if (i < 4 * KS) {
-    const int kq = i / KS, ks = i - KS * kq, b = 4 * ks + kq;
+  constexpr int NT = Otab<BF>::ntrial;
+  if (i < NT) {
+    int b;
+    if (BF) {
+      const int kq = i / (KSB * 8), rem = i - KSB * 8 * kq, ks = rem / 8, j = rem - 8 * ks;
+      b = 32 * ks + 8 * kq + j;
+    } else {
+      const int kq = i / KS, ks = i - KS * kq;
+      b = 4 * ks + kq;
+    }
     tab[i] = b < NSH ? local_offset(b, G.px, G.py, G.npts) : 0;
-  } else if (i < OTAB) {
-    const int j = i - 4 * KS, w = j >> 4, kq = (j >> 2) & 3, r = j & 3;
+  } else if (i < Otab<BF>::size) {
+    const int j = i - NT, w = j >> 4, kq = (j >> 2) & 3, r = j & 3;
     const int row = perm_row(w, kq, r);
     tab[i] = row >= 0 ? local_offset(row, G.px, G.py, G.npts) : 0;
   }
@@ -186,9 +205,13 @@ __device__ __forceinline__ void tile_range(int ntiles, int& first, int& last) {
 // (component, ay, az) triplet (perm_row), so the three contributions each
 // control point of the line receives are summed in registers with two DPP
 // row shifts before a single ds_add per point.
-template <typename T>
+//
+// BF = true (T = float): K and the staged x are rounded to BF16 for
+// v_mfma_f32_16x16x32_bf16 (3 K steps of 32), FP32 accumulation; the rest as
+// the FP32 mode.
+template <typename T, bool BF = false>
 __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void apply_tiles(
-    const Tile* __restrict__ tiles, int ntiles, const T* __restrict__ kfrag, const T* __restrict__ x,
+    const Tile* __restrict__ tiles, int ntiles, const void* __restrict__ kfrag_v, const T* __restrict__ x,
     T* __restrict__ y, const T* __restrict__ scale, const uint8_t* __restrict__ freem,
     const uint8_t* __restrict__ tmask, int ncp0, int ncp1, int ndof_sc, int nel0, int nel1,
     double* __restrict__ pap, const PcgState* st) {
@@ -197,7 +220,8 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
   // full rate, while FP32 LDS atomics back up the LDS data FIFO on gfx950
   __shared__ double ys[3 * TILE_MAX_PTS];
   __shared__ T ss[2][TILE_MAX_EL];
-  __shared__ int otab[2][OTAB];  // per tile: LDS offsets of the B operands and of the output rows
+  __shared__ int otab[2][OTAB_MAX];  // per tile: LDS offsets of the B operands and of the output rows
+  constexpr int NTR = Otab<BF>::ntrial;
   __shared__ double red[6];
   if (st != nullptr && st->done != 0) return;  // solver already finished
   int first, last;
@@ -228,9 +252,10 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
     }
   }
   if (threadIdx.x < G.nel) ss[0][threadIdx.x] = scale[G.elem(t, threadIdx.x, nel0, nel1)];
-  offset_table(G, otab[0]);
+  offset_table<BF>(G, otab[0]);
 
-  T a[KS];
+  T a[BF ? 1 : KS];
+  bf8_t ab[BF ? KSB : 1];
   int cur_type = -1;
   int buf = 0;
   for (int ti = first; ti < last; ++ti) {
@@ -242,9 +267,15 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
     const bool mask_c = tmask != nullptr && tmask[ti] != 0;
     const bool mask_n = tmask != nullptr && has_next && tmask[ti + 1] != 0;
     if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile
-      const T* kf = kfrag + ((size_t)(t.type * MT + wave) * KS) * 64 + lane;
+      if constexpr (BF) {
+        const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + wave) * KSB) * 64 + lane;
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) a[ks] = kf[ks * 64];
+        for (int ks = 0; ks < KSB; ++ks) ab[ks] = kf[ks * 64];
+      } else {
+        const T* kf = (const T*)kfrag_v + ((size_t)(t.type * MT + wave) * KS) * 64 + lane;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) a[ks] = kf[ks * 64];
+      }
       cur_type = t.type;
     }
     // prefetch of the next tile's points (issued unconditionally; unused
@@ -262,12 +293,13 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
     const T* xb = xs[buf];
     const T* sb = ss[buf];
     __syncthreads();  // xs[buf], ss[buf], otab[buf] staged; ys clear
-    int boff[KS];
+    constexpr int NB = NTR / 4;  // trial offsets per row group
+    int boff[NB];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) boff[ks] = otab[buf][kq * KS + ks];
+    for (int m = 0; m < NB; ++m) boff[m] = otab[buf][kq * NB + m];
     int roff[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) roff[r] = otab[buf][4 * KS + (wave * 4 + kq) * 4 + r];
+    for (int r = 0; r < 4; ++r) roff[r] = otab[buf][NTR + (wave * 4 + kq) * 4 + r];
 
     const bool lines = t.lines != 0;
     const int ngroups = lines ? t.ny * t.nz : (nel + 15) / 16;
@@ -306,6 +338,19 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
       // (left to itself the scheduler issues each read right before its MFMA
       // and exposes the full LDS latency)
       typename Mfma<T>::acc_t acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+      if constexpr (BF) {
+#pragma unroll
+        for (int ks = 0; ks < KSB; ++ks) {
+          bf8_t b0, b1;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            b0[j] = (__bf16)xb[eb[0] + boff[ks * 8 + j]];
+            b1[j] = (__bf16)xb[eb[1] + boff[ks * 8 + j]];
+          }
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[ks], b0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[ks], b1, acc1, 0, 0, 0);
+        }
+      } else {
       T bq0[2][BCH], bq1[2][BCH];
 #pragma unroll
       for (int i = 0; i < BCH; ++i) {
@@ -335,6 +380,7 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
           }
         }
         __builtin_amdgcn_sched_barrier(0);
+      }
       }
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
@@ -384,7 +430,7 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
         xs[buf ^ 1][idx] = (!mask_n || freem[Gn.dof(tn, idx, ncp0, ncp1, ndof_sc)] != 0) ? pv[j] : T(0);
     }
     if (has_next && threadIdx.x < Gn.nel) ss[buf ^ 1][threadIdx.x] = psc;
-    if (has_next) offset_table(Gn, otab[buf ^ 1]);
+    if (has_next) offset_table<BF>(Gn, otab[buf ^ 1]);
     if (pap != nullptr) {
       // wave sums, then the six waves in fixed order (red is free: the last
       // reader finished before the barrier above)
@@ -534,6 +580,24 @@ __global__ void mfma_layout_probe(double* out) {
   }
 }
 
+// Lane-layout check of v_mfma_f32_16x16x32_bf16: the lanes are filled under
+// the layout the BF16 apply assumes (lane l holds A[l%16][8 (l/16) + j] and
+// B[8 (l/16) + j][l%16]) with small integers A[i][k] = (7 i + 3 k) % 17 - 8,
+// B[k][n] = (5 k + 11 n) % 13 - 6 (exact in BF16 and FP32); out[4 l + r] =
+// accumulator r, which under the assumed output layout is D[4 (l/16) + r][l%16].
+__global__ void mfma_layout_probe_bf16(double* out) {
+  const int l = threadIdx.x;
+  bf8_t a, b;
+  for (int j = 0; j < 8; ++j) {
+    const int k = 8 * (l / 16) + j, i = l % 16, n = l % 16;
+    a[j] = (__bf16)(float)((7 * i + 3 * k) % 17 - 8);
+    b[j] = (__bf16)(float)((5 * k + 11 * n) % 13 - 6);
+  }
+  f4_t acc = {0, 0, 0, 0};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[l * 4 + r] = (double)acc[r];
+}
+
 // ---- launchers ---------------------------------------------------------------
 // Persistent apply grid: wg_per_cu workgroups per CU (default two; the VGPR
 // budget of the MFMA loop allows twelve waves per CU), a multiple of 8 so the
@@ -549,7 +613,7 @@ hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x
                             const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
   hipLaunchKernelGGL((apply_tiles<double>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->ntiles,
-                     g->d_kfrag64, x, y,
+                     (const void*)g->d_kfrag64, x, y,
                      op->d_scale, op->d_free, op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc,
                      (int)s.nel[0], (int)s.nel[1], pap, pst);
   return hipGetLastError();
@@ -559,9 +623,18 @@ hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x,
                             const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
   hipLaunchKernelGGL((apply_tiles<float>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->ntiles,
-                     g->d_kfrag32, x, y,
+                     (const void*)g->d_kfrag32, x, y,
                      op->d_scale32, op->d_free, op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc,
                      (int)s.nel[0], (int)s.nel[1], pap, pst);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_bf16(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
+                             const PcgState* pst, hipStream_t st) {
+  const auto& s = g->s;
+  hipLaunchKernelGGL((apply_tiles<float, true>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles,
+                     g->ntiles, (const void*)g->d_kfragb, x, y, op->d_scale32, op->d_free, op->d_tmask,
+                     (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0], (int)s.nel[1], pap, pst);
   return hipGetLastError();
 }
 
@@ -592,6 +665,11 @@ hipError_t launch_diag_f32(const icc_grid* g, const float* scale, float* d, hipS
 
 hipError_t launch_layout_probe(double* out, hipStream_t st) {
   hipLaunchKernelGGL(mfma_layout_probe, dim3(1), dim3(64), 0, st, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_layout_probe_bf16(double* out, hipStream_t st) {
+  hipLaunchKernelGGL(mfma_layout_probe_bf16, dim3(1), dim3(64), 0, st, out);
   return hipGetLastError();
 }
 

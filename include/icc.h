@@ -37,7 +37,9 @@ enum icc_status {
 /* Arithmetic of the K.u contraction and of the inner solve. */
 enum icc_prec {
   ICC_F64 = 0, /* FP64 MFMA (v_mfma_f64_16x16x4), FP64 vectors */
-  ICC_F32 = 1  /* FP32 MFMA (v_mfma_f32_16x16x4), FP32 vectors and accumulation */
+  ICC_F32 = 1, /* FP32 MFMA (v_mfma_f32_16x16x4), FP32 vectors and accumulation */
+  ICC_BF16 = 2 /* apply only: K and x rounded to BF16, FP32 accumulation
+                  (v_mfma_f32_16x16x32_bf16), FP32 vectors (x, y float*) */
 };
 
 typedef struct icc_ctx icc_ctx;
@@ -400,6 +402,12 @@ int icc_batch_solve(icc_batch* b, int prec, double tol, double delta, int64_t ma
  * A[i][k] = i + 16k, B[k][j] = (k == j%4) and writes the 64x4 accumulator
  * registers of each (f64 first, then f32): pins the lane layouts in tests */
 int icc_debug_mfma_layout(icc_ctx* ctx, double* out512);
+/* Lane-layout check of v_mfma_f32_16x16x32_bf16 as the BF16 apply uses it:
+ * out[4 l + r] = accumulator r of lane l for A[i][k] = (7 i + 3 k) % 17 - 8,
+ * B[k][n] = (5 k + 11 n) % 13 - 6 loaded as lane l holding A[l%16][8 (l/16) + j]
+ * and B[8 (l/16) + j][l%16]; equals D[4 (l/16) + r][l%16] = (A B) if the
+ * assumed layout holds. */
+int icc_debug_mfma_layout_bf16(icc_ctx* ctx, double* out256);
 
 #ifdef __cplusplus
 }
