@@ -91,6 +91,35 @@ def time_applies(torch, icc, ctx, op, x, y, prec, steps, warmup, accumulate=Fals
     return e0.elapsed_time(e1) / steps  # ms
 
 
+def pcg_c0(np, torch, icc, ctx):
+    """FP64 Jacobi PCG to 1e-8 on the 16x8x8 grid (rho = 0.5, penal 3,
+    Emin 1e-9, x = 0 clamped, unit downward traction on x = 2)."""
+    nel0 = (16, 8, 8)
+    g = icc.Grid(ctx, nel0, BOUNDS)
+    ncp = [n + 2 for n in nel0]
+    nsc = ncp[0] * ncp[1] * ncp[2]
+    fr = np.ones(3 * nsc, dtype=np.uint8)
+    for c in range(3):
+        fr[c * nsc:(c + 1) * nsc][np.arange(nsc) % ncp[0] == 0] = 0
+    op = icc.Operator(g, scale=np.full(g.nel_tot, 1e-9 + (1 - 1e-9) * 0.5 ** 3), free=fr)
+    t = icc.CantileverTopOpt(ctx, icc.CantileverTopOptConfig(nel=nel0, size=(2.0, 1.0, 1.0), load=1, emin=1e-9))
+    f = t.force * fr
+    t.close()
+    fd = torch.from_numpy(f).cuda()
+    u = torch.zeros(g.ndof, dtype=torch.float64, device="cuda")
+    stats = np.zeros(8)
+    icc.lib().icc_solve_ir_dev(op.h, icc.ICC_F64, fd.data_ptr(), 1e-8, 1e-8, 200, 20 * g.ndof, u.data_ptr(),
+                               stats.ctypes.data)  # warm-up (graph capture)
+    ctx.sync()
+    u.zero_()
+    t0 = time.time()
+    rc = icc.lib().icc_solve_ir_dev(op.h, icc.ICC_F64, fd.data_ptr(), 1e-8, 1e-8, 200, 20 * g.ndof, u.data_ptr(),
+                                    stats.ctypes.data)
+    ctx.sync()
+    return dict(time_s=time.time() - t0, status=int(rc), pcg_iterations=int(stats[1]),
+                true_rel_residual=float(stats[2]))
+
+
 def cpu_baseline(np, scale, x, budget_s):
     """The oracle's restatement of the reference CPU K.u (row-parallel CSR
     SpMV, sparse.rs:60-68, OpenMP for rayon) at the full 735k-DOF size."""
@@ -112,9 +141,50 @@ def cpu_baseline(np, scale, x, budget_s):
             break
     dt = (time.time() - t0) / reps
     cores = int(O.lib().orc_num_threads())
-    return dict(value=1.0 / dt, unit="applies/s", cores=cores, kind="port",
-                sample=f"{reps} CSR SpMVs (nnz {nnz}) of the 96x48x48 operator, row-parallel over {cores} "
-                       f"OpenMP threads; CSR built in {t_asm:.1f}s (not timed)"), y
+    out = dict(value=1.0 / dt, unit="applies/s", cores=cores, kind="port",
+               sample=f"{reps} CSR SpMVs (nnz {nnz}) of the 96x48x48 operator, row-parallel over {cores} "
+                      f"OpenMP threads; CSR built in {t_asm:.1f}s (not timed)", csr_build_s=t_asm)
+    # the same SpMV on one thread (a few products)
+    O.lib().orc_set_threads(1)
+    t0 = time.time()
+    n1 = 0
+    while n1 < 2 or time.time() - t0 < 3.0:
+        sp.matvec(x)
+        n1 += 1
+    out["one_thread_applies_per_s"] = n1 / (time.time() - t0)
+    O.lib().orc_set_threads(cores)
+    # Jacobi PCG (solver.rs) on the same operator: a bounded sample of 100
+    # iterations, free DOFs only, traction load
+    return out, y, sp, O
+
+
+def cpu_pcg(np, O, sp, free, force, gpu_iters):
+    """The oracle's restatement of PcgSolver::solve (serial vector loops,
+    the OpenMP CSR SpMV) on the 96x48x48 system: 100 iterations timed, the
+    time to 1e-8 extrapolated with the FP64 GPU iteration count; and the
+    whole solve to 1e-8 on the 16x8x8 grid (rho = 0.5)."""
+    b = force * free
+    d = np.maximum(sp.diag(), 1e-12)
+    t0 = time.time()
+    _, its, _ = sp.pcg(b, d, free, 1e-30, 100)
+    per_it = (time.time() - t0) / max(its, 1)
+    out = dict(c1_ms_per_iteration=per_it * 1e3, c1_sample_iterations=its,
+               c1_time_to_1e8_s_extrapolated=per_it * gpu_iters, c1_gpu_fp64_iterations=gpu_iters)
+    nel0 = [16, 8, 8]
+    sp0 = O.Space([list(bb) for bb in BOUNDS], nel0, 2)
+    sp0.elem_mats(*O.lame(1.0, 0.3))
+    sp0.assemble_structured(np.full(int(np.prod(nel0)), 1e-9 + (1 - 1e-9) * 0.5 ** 3))
+    ncp = [n + 2 for n in nel0]
+    nsc = ncp[0] * ncp[1] * ncp[2]
+    fr0 = np.ones(3 * nsc, dtype=np.uint8)
+    for c in range(3):
+        fr0[c * nsc:(c + 1) * nsc][np.arange(nsc) % ncp[0] == 0] = 0
+    f0 = sp0.traction_xmax([0.0, -1.0, 0.0]) * fr0
+    d0 = np.maximum(sp0.diag(), 1e-12)
+    t0 = time.time()
+    _, its0, res0 = sp0.pcg(f0, d0, fr0, 1e-8, 20 * 3 * nsc)
+    out.update(c0_time_to_1e8_s=time.time() - t0, c0_iterations=its0, c0_residual=res0)
+    return out
 
 
 def cpu_imm_topopt(np, icc):
@@ -478,6 +548,7 @@ def main():
         t = icc.CantileverTopOpt(ctx, cfg)
         f = t.force * free
         t.close()
+        force_c1 = f
         fd = torch.from_numpy(f).cuda()
         res = {}
         for name in ("fp64", "mixed_fp32"):
@@ -513,6 +584,7 @@ def main():
             dc_rel_err_vs_fp64=float(np.linalg.norm(dcm - dcf) / np.linalg.norm(dcf)))
         res["speedup_mixed_vs_fp64"] = res["fp64"]["time_s"] / res["mixed_fp32"]["time_s"]
         res["delta"] = args.delta
+        res["fp64_c0_16x8x8"] = pcg_c0(np, torch, icc, ctx)
         out["pcg_time_to_1e-8"] = res
 
     if not args.no_configs:
@@ -529,7 +601,15 @@ def main():
 
     if not args.no_cpu:
         log("bench: CPU baseline")
-        cb, ycpu = cpu_baseline(np, scale, (free * xh), args.cpu_budget)
+        cb, ycpu, csp, O = cpu_baseline(np, scale, (free * xh), args.cpu_budget)
+        if "pcg_time_to_1e-8" in out:
+            log("bench: CPU PCG (100-iteration sample at 96x48x48, full solve at 16x8x8)")
+            cb["pcg"] = cpu_pcg(np, O, csp, free, force_c1, out["pcg_time_to_1e-8"]["fp64"]["pcg_iterations"])
+            g0 = out["pcg_time_to_1e-8"].get("fp64_c0_16x8x8")
+            if g0:
+                cb["pcg"]["c0_speedup_gpu_vs_cpu"] = cb["pcg"]["c0_time_to_1e8_s"] / g0["time_s"]
+            cb["pcg"]["c1_speedup_gpu_fp64_vs_cpu"] = (cb["pcg"]["c1_time_to_1e8_s_extrapolated"]
+                                                      / out["pcg_time_to_1e-8"]["fp64"]["time_s"])
         yg = op.apply(xh)
         cb["rel_diff_vs_gpu_f64"] = float(np.linalg.norm(free * ycpu - yg) / np.linalg.norm(yg))
         if not args.no_configs:

@@ -562,3 +562,10 @@ ORC_API i64 orc_num_threads(void) {
   return 1;
 #endif
 }
+ORC_API void orc_set_threads(i64 n) {
+#ifdef _OPENMP
+  omp_set_num_threads((int)n);
+#else
+  (void)n;
+#endif
+}

@@ -83,6 +83,7 @@ def lib():
         L.orc_padded_bounds.argtypes = [vp, i64, dbl, vp]
         L.orc_ghost_penalty_matvec.restype = dbl
         L.orc_ghost_penalty_matvec.argtypes = [vp, vp, i64, vp, dbl, dbl, vp, vp]
+        L.orc_set_threads.argtypes = [i64]
         L.orc_wq_rule.restype = i64
         L.orc_wq_rule.argtypes = [i64, i64, i64, i64, i64, vp]
         L.orc_space_assemble_wq.restype = i64
