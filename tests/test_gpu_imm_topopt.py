@@ -246,3 +246,58 @@ def test_imm_topopt_ghost_penalty_matches_oracle(ctx):
     assert np.all(np.abs(np.asarray(h.volume) - vol) < 1e-9)
     assert np.max(np.abs(h.density - dens)) < 1e-6
     assert np.all(np.abs(np.asarray(h.pcg_iterations) - its) <= 2 + 0.06 * its), (h.pcg_iterations, its)
+
+
+def test_mesh_and_topopt_input_errors(ctx):
+    """Empty meshes and out-of-range triangle indices are rejected before any
+    device work; a B-rep that leaves no design cell fails at run time."""
+    d, m = _tilted()
+    bad = icc.TriangleMesh3D(m.vertices, np.array([[0, 1, 99]]))
+    with pytest.raises(icc.IccError) as ei:
+        icc.element_weights(ctx, bad, icc.padded_bounds(m, 0.05), (4, 4, 4))
+    assert ei.value.code == icc.ICC_EINVAL
+    with pytest.raises(icc.IccError):
+        icc.ImmersedTopOpt(ctx, bad, icc.ImmersedTopOptConfig(grid_res=(4, 4, 4), max_iter=1))
+    # two tiny boxes at opposite corners of their bounding box: no cell
+    # reaches w >= 0.05
+    a = icc.TriangleMesh3D.new_box((0.0, 0.0, 0.0), (0.01, 0.01, 0.01))
+    b = icc.TriangleMesh3D.new_box((0.99, 0.99, 0.99), (1.0, 1.0, 1.0))
+    two = icc.TriangleMesh3D(np.vstack([a.vertices, b.vertices]), np.vstack([a.triangles, b.triangles + 8]))
+    opt = icc.ImmersedTopOpt(ctx, two, icc.ImmersedTopOptConfig(grid_res=(4, 4, 3), max_iter=1))
+    assert opt.n_active == 0
+    with pytest.raises(icc.IccError) as ei:
+        opt.run()
+    assert ei.value.code == icc.ICC_EINVAL
+
+
+def test_face_terms_without_cut_cells(ctx):
+    """A status with no CUT cell stabilises no face: the operator is the plain
+    one, for both kinds (to round-off: the apply's atomic scatter order
+    varies between launches)."""
+    g = icc.Grid(ctx, (5, 4, 3), [0.0, 1.0, 0.0, 1.0, 0.0, 1.0])
+    op = icc.Operator(g, scale=np.ones(g.nel_tot))
+    ref = icc.Operator(g, scale=np.ones(g.nel_tot))
+    status = np.ones(g.nel_tot, dtype=np.int8)
+    assert op.set_legacy_stabilization(status, 0.05) == 0
+    assert op.set_ghost_penalty(status, 1e-2) == 0
+    x = np.random.default_rng(0).standard_normal(g.ndof)
+    assert O.rel_err(op.apply(x), ref.apply(x)) < 1e-14
+
+
+@pytest.mark.parametrize("nel", [(1, 1, 1), (2, 1, 3), (3, 2, 2), (4, 5, 1)])
+def test_ghost_penalty_small_grids_match_oracle(ctx, nel):
+    """Grids of one to five cells per direction, where the jump tables see
+    the repeated end knots, against the oracle's assembled matrix."""
+    bounds = np.array([0.0, 1.1, 0.0, 0.7, 0.0, 0.9])
+    rng = np.random.default_rng(sum(nel))
+    status = rng.choice(np.array([-1, 1, 0], dtype=np.int8), size=int(np.prod(nel)), p=[0.6, 0.3, 0.1])
+    g = icc.Grid(ctx, nel, bounds, young=1.7)
+    op = icc.Operator(g, scale=np.zeros(g.nel_tot))
+    op.set_ghost_penalty(status, 0.3)
+    x = rng.standard_normal(g.ndof)
+    want, fro = O.ghost_penalty_matvec(bounds, nel, 2, status, 0.3, 1.7, x)
+    got = op.apply(x)
+    if fro == 0.0:
+        assert np.max(np.abs(got)) == 0.0
+    else:
+        assert O.rel_err(got, want) < 1e-13

@@ -102,3 +102,27 @@ def test_wq_reform_on_device(ctx):
     ref = icc.WqMatrix(g).form(icc.ICC_DENSITY_ELEMENT, x2)
     v = rng.standard_normal(g.ndof)
     assert O.rel_err(m.apply(v), ref.apply(v)) < 1e-15
+
+
+@pytest.mark.parametrize("nel", [(1, 1, 1), (1, 2, 3), (2, 2, 1)])
+def test_wq_form_tiny_grids_match_oracle(ctx, nel):
+    """One- and two-element directions (every WQ rule touches the end knots)."""
+    bounds = [0.0, 0.8, 0.0, 1.1, 0.0, 0.6]
+    rng = np.random.default_rng(7)
+    g = icc.Grid(ctx, nel, bounds)
+    x = rng.uniform(0.1, 1.0, g.nel_tot)
+    m = icc.WqMatrix(g).form(icc.ICC_DENSITY_ELEMENT, x, 3.0, 1e-3)
+    sp = _oracle_wq(bounds, nel, 1, x)
+    v = rng.standard_normal(g.ndof)
+    assert O.rel_err(m.apply(v), sp.matvec(v)) < 1e-13
+
+
+def test_wq_errors(ctx):
+    g = icc.Grid(ctx, (3, 3, 3), [0.0, 1.0, 0.0, 1.0, 0.0, 1.0])
+    m = icc.WqMatrix(g)
+    with pytest.raises(icc.IccError):
+        m.apply(np.zeros(g.ndof))  # not formed yet
+    with pytest.raises(icc.IccError):
+        m.form(icc.ICC_DENSITY_ELEMENT, None)  # density field missing
+    with pytest.raises(icc.IccError):
+        m.form(7, np.ones(g.nel_tot))
