@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <functional>
 #include <type_traits>
 #include <cmath>
@@ -1752,15 +1753,25 @@ ICC_API int icc_batch_solve(icc_batch* b, int prec, double tol, double delta, in
     }
   }
   if (!live.empty()) {
-    IterGraph G;
-    if (prec == ICC_F32)
-      ICC_TRY(capture_batched<float>(b, live, w, G));
-    else
-      ICC_TRY(capture_batched<double>(b, live, w, G));
+    // the graph holds the unfinished instances only: it is captured again
+    // whenever instances finish, so finished ones cost no idle launches
+    std::unique_ptr<IterGraph> G;
+    std::vector<int> in_graph;
     const int32_t done_fin = 4;  // done value that keeps a finished instance's kernels idle
     std::vector<int> pend;
     while (true) {
-      ICC_HIP(hipGraphLaunch(G.exec, m));
+      std::vector<int> running;
+      for (int k : live)
+        if (!finished[k]) running.push_back(k);
+      if (running != in_graph) {
+        G.reset(new IterGraph());
+        if (prec == ICC_F32)
+          ICC_TRY(capture_batched<float>(b, running, w, *G));
+        else
+          ICC_TRY(capture_batched<double>(b, running, w, *G));
+        in_graph = running;
+      }
+      ICC_HIP(hipGraphLaunch(G->exec, m));
       ICC_TRY(read_states(b, live, w));
       pend.clear();
       bool any_running = false;
