@@ -9,7 +9,8 @@ from .core import Context, Filter, Grid, Operator, WqMatrix, classify_cylinders 
 from .topopt import CantileverTopOpt, CantileverTopOptConfig, TopOptHistory  # noqa: F401
 from .immersed import ImmersedConfig, ImmersedSdf  # noqa: F401
 from .batch import Batch  # noqa: F401
-from .imm_topopt import (ImmersedTopOpt, ImmersedTopOptConfig, ImmersedTopOptHistory, Stabilization,  # noqa: F401
-                         TriangleMesh3D, element_weights, padded_bounds)
+from .imm_topopt import (ImmersedOptions, ImmersedProblem, ImmersedSolution, ImmersedTopOpt,  # noqa: F401
+                         ImmersedTopOptConfig, ImmersedTopOptHistory, Stabilization, TriangleMesh3D,
+                         build_immersed_problem, element_weights, padded_bounds, solve_immersed_problem)
 
 __all__ = ["Context", "Grid", "Operator", "Filter", "classify_cylinders", "ICC_F64", "ICC_F32", "IccError"]

@@ -105,6 +105,14 @@ SIGNATURES = [
     ("icc_imm_topopt_run", cint, [vp, vp, vp, vp, vp, vp, vp, vp]),
     ("icc_imm_topopt_state", cint, [vp, vp]),
     ("icc_debug_mfma_layout_bf16", cint, [vp, vp]),
+    ("icc_immersed_mesh_config_default", None, [vp]),
+    ("icc_immersed_mesh_create", cint, [vp, vp, vp, i64, vp, i64, vp, vpp]),
+    ("icc_immersed_mesh_destroy", cint, [vp]),
+    ("icc_immersed_mesh_sizes", cint, [vp, vp, vp]),
+    ("icc_immersed_mesh_problem", cint, [vp, vp, vp, vp, vp]),
+    ("icc_immersed_mesh_set_solver", cint, [vp, dbl, i64, cint]),
+    ("icc_immersed_mesh_apply", cint, [vp, vp, vp]),
+    ("icc_immersed_mesh_solve", cint, [vp, vp, vp]),
     ("icc_wq_create", cint, [vp, vpp]),
     ("icc_wq_destroy", cint, [vp]),
     ("icc_wq_form", cint, [vp, cint, vp, dbl, dbl]),
@@ -118,6 +126,7 @@ REPORT_FN = C.CFUNCTYPE(None, i64, dbl, dbl, dbl, vp)
 IMM_REPORT_FN = C.CFUNCTYPE(None, i64, dbl, dbl, dbl, i64, vp)
 ICC_STAB_NONE, ICC_STAB_LEGACY, ICC_STAB_GHOST = range(3)
 ICC_DENSITY_NONE, ICC_DENSITY_ELEMENT, ICC_DENSITY_SPLINE = range(3)
+ICC_CLAMP_NONE, ICC_CLAMP_BOTTOM, ICC_CLAMP_TOP, ICC_CLAMP_LEFT = -1, 0, 1, 2
 
 
 class TopOptConfigC(C.Structure):
@@ -134,6 +143,14 @@ class ImmersedConfigC(C.Structure):
                 ("poisson", dbl), ("emin", dbl), ("active_min", dbl), ("penalty_factor", dbl),
                 ("traction", dbl * 3), ("pcg_tol", dbl), ("pcg_maxit", i64), ("mode", C.c_int32),
                 ("solver", C.c_int32)]
+
+
+class ImmersedMeshConfigC(C.Structure):
+    """icc_immersed_mesh_config (include/icc.h)."""
+    _fields_ = [("grid_res", i64 * 3), ("padding", dbl), ("young", dbl), ("poisson", dbl),
+                ("stabilization", C.c_int32), ("clamped_face", C.c_int32), ("gamma", dbl), ("emin", dbl),
+                ("subcell_res", i64 * 3), ("pcg_tol", dbl), ("pcg_maxit", i64), ("solver", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class ImmTopOptConfigC(C.Structure):

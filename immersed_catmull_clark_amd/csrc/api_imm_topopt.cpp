@@ -463,3 +463,228 @@ ICC_API int icc_imm_topopt_state(icc_imm_topopt* t, double* u) {
   ICC_HIP(hipStreamSynchronize(t->ctx->stream));
   return ICC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// build_immersed_problem / solve_immersed_problem for a triangle B-rep
+// (rust/src/immersed.rs:392-570): grid on padded_bounds(mesh, padding), ray-
+// parity classification with subcell_res^3 samples, stiffness scale
+// emin + (1 - emin) w on every cell, the selected stabilisation, a clamped
+// background face, the caller's force or the default top-centre load, and
+// Jacobi PCG with the diagonal clamped at 1e-6 (FP64), or the mixed solver.
+struct icc_immersed_mesh {
+  icc_ctx* ctx = nullptr;
+  icc_immersed_mesh_config cfg;
+  icc_grid* g = nullptr;
+  icc_op* op = nullptr;
+  double gb[6];
+  int64_t nel = 0, ndof = 0, n_faces = 0, n_free = 0;
+  std::vector<int8_t> status;
+  std::vector<double> w, force;
+  std::vector<uint8_t> fr;
+  double *d_f = nullptr, *d_u = nullptr, *d_diag = nullptr, *d_tmp = nullptr;
+};
+
+ICC_API void icc_immersed_mesh_config_default(icc_immersed_mesh_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->grid_res[0] = 24;
+  c->grid_res[1] = 16;
+  c->grid_res[2] = 12;
+  c->padding = 0.05;
+  c->young = 1.0;
+  c->poisson = 0.3;
+  c->stabilization = ICC_STAB_GHOST;
+  c->gamma = 1e-2;
+  c->emin = 1e-4;
+  c->subcell_res[0] = c->subcell_res[1] = c->subcell_res[2] = 4;
+  c->clamped_face = ICC_CLAMP_BOTTOM;
+  c->pcg_tol = 1e-8;
+  c->pcg_maxit = 5000;
+  c->solver = 0;
+}
+
+ICC_API int icc_immersed_mesh_destroy(icc_immersed_mesh* h) {
+  if (!h) return ICC_OK;
+  icc_op_destroy(h->op);
+  icc_grid_destroy(h->g);
+  hipFree(h->d_f);
+  hipFree(h->d_u);
+  hipFree(h->d_diag);
+  hipFree(h->d_tmp);
+  delete h;
+  return ICC_OK;
+}
+
+ICC_API int icc_immersed_mesh_create(icc_ctx* ctx, const icc_immersed_mesh_config* cfg, const double* vertices,
+                                     int64_t nv, const int64_t* tris, int64_t ntri, const double* force,
+                                     icc_immersed_mesh** out) {
+  if (!ctx || !cfg || !out) return ICC_EINVAL;
+  if (!mesh_ok(vertices, nv, tris, ntri)) {
+    set_error("icc_immersed_mesh_create: empty mesh or triangle index out of range");
+    return ICC_EINVAL;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (cfg->grid_res[d] < 1 || cfg->subcell_res[d] < 1) {
+      set_error("icc_immersed_mesh_create: grid_res and subcell_res must be >= 1");
+      return ICC_EINVAL;
+    }
+  if ((cfg->stabilization != ICC_STAB_NONE && cfg->stabilization != ICC_STAB_LEGACY &&
+       cfg->stabilization != ICC_STAB_GHOST) ||
+      cfg->clamped_face < ICC_CLAMP_NONE || cfg->clamped_face > ICC_CLAMP_LEFT || (cfg->solver != 0 && cfg->solver != 1)) {
+    set_error("icc_immersed_mesh_create: unknown stabilization, clamped face or solver");
+    return ICC_EINVAL;
+  }
+  auto* h = new icc_immersed_mesh();
+  h->ctx = ctx;
+  h->cfg = *cfg;
+  int rc = icc_mesh_padded_bounds(vertices, nv, cfg->padding, h->gb);
+  if (rc == ICC_OK) rc = icc_grid_create(ctx, cfg->grid_res, h->gb, 2, cfg->young, cfg->poisson, &h->g);
+  if (rc == ICC_OK) rc = icc_op_create(h->g, &h->op);
+  if (rc != ICC_OK) {
+    icc_immersed_mesh_destroy(h);
+    return rc;
+  }
+  const GridSetup& s = h->g->s;
+  const int64_t ne = s.nel_tot, n = s.ndof;
+  h->nel = ne;
+  h->ndof = n;
+  h->status.resize(ne);
+  h->w.resize(ne);
+  rc = icc_classify_mesh(ctx, vertices, nv, tris, ntri, h->gb, cfg->grid_res, cfg->subcell_res, h->status.data(),
+                         h->w.data());
+  // scale emin + (1 - emin) w on every cell (immersed.rs:484)
+  std::vector<double> scale(ne);
+  for (int64_t e = 0; e < ne; ++e) scale[e] = cfg->emin + (1.0 - cfg->emin) * h->w[e];
+  if (rc == ICC_OK) rc = icc_op_set_scale(h->op, scale.data());
+  if (rc == ICC_OK && cfg->stabilization == ICC_STAB_LEGACY)
+    rc = icc_op_set_legacy_stabilization(h->op, h->status.data(), cfg->gamma, &h->n_faces);
+  if (rc == ICC_OK && cfg->stabilization == ICC_STAB_GHOST)
+    rc = icc_op_set_ghost_penalty(h->op, h->status.data(), cfg->gamma, cfg->young, &h->n_faces);
+  if (rc != ICC_OK) {
+    icc_immersed_mesh_destroy(h);
+    return rc;
+  }
+  // clamped face (immersed.rs:512-519) and the load (:520-533, or the caller's)
+  const int64_t nd0 = s.ncp[0], nd1 = s.ncp[1], nd2 = s.ncp[2];
+  h->fr.assign(n, 1);
+  for (int64_t i = 0; i < s.ndof_sc; ++i) {
+    const int64_t i1 = i % nd0, i3 = i / (nd0 * nd1);
+    const bool cl = (cfg->clamped_face == ICC_CLAMP_BOTTOM && i3 == 0) ||
+                    (cfg->clamped_face == ICC_CLAMP_TOP && i3 == nd2 - 1) ||
+                    (cfg->clamped_face == ICC_CLAMP_LEFT && i1 == 0);
+    if (cl)
+      for (int c = 0; c < 3; ++c) h->fr[c * s.ndof_sc + i] = 0;
+  }
+  if (force) {
+    h->force.assign(force, force + n);
+  } else {
+    h->force.assign(n, 0.0);
+    const int64_t mid_x = (int64_t)std::round((double)nd0 / 2.0), mid_y = (int64_t)std::round((double)nd1 / 2.0);
+    std::vector<int64_t> load;
+    for (int64_t i = 0; i < s.ndof_sc; ++i) {
+      const int64_t i1 = i % nd0, i2 = (i / nd0) % nd1, i3 = i / (nd0 * nd1);
+      if (i3 == nd2 - 1 && std::llabs(i1 + 1 - mid_x) <= 2 && std::llabs(i2 + 1 - mid_y) <= 2) load.push_back(i);
+    }
+    for (int64_t i : load) h->force[2 * s.ndof_sc + i] = -1.0 / (double)load.size();
+  }
+  std::vector<double> b(n);
+  for (int64_t i = 0; i < n; ++i) {
+    h->n_free += h->fr[i];
+    b[i] = h->fr[i] ? h->force[i] : 0.0;
+  }
+  if ((rc = icc_op_set_free_mask(h->op, h->fr.data())) || (rc = dalloc(&h->d_f, n)) || (rc = dalloc(&h->d_u, n)) ||
+      (rc = dalloc(&h->d_diag, n)) || (rc = dalloc(&h->d_tmp, n))) {
+    icc_immersed_mesh_destroy(h);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(h->d_f, b.data(), n * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    icc_immersed_mesh_destroy(h);
+    return hip_fail(e, "icc_immersed_mesh_create");
+  }
+  *out = h;
+  return ICC_OK;
+}
+
+ICC_API int icc_immersed_mesh_sizes(const icc_immersed_mesh* h, int64_t out[4], double grid_bounds[6]) {
+  if (!h) return ICC_EINVAL;
+  if (out) {
+    out[0] = h->nel;
+    out[1] = h->ndof;
+    out[2] = h->n_faces;
+    out[3] = h->n_free;
+  }
+  if (grid_bounds) std::memcpy(grid_bounds, h->gb, sizeof(h->gb));
+  return ICC_OK;
+}
+
+ICC_API int icc_immersed_mesh_problem(const icc_immersed_mesh* h, int8_t* status, double* weights, double* force,
+                                      uint8_t* free_mask) {
+  if (!h) return ICC_EINVAL;
+  if (status) std::memcpy(status, h->status.data(), h->nel);
+  if (weights) std::memcpy(weights, h->w.data(), h->nel * 8);
+  if (force) std::memcpy(force, h->force.data(), h->ndof * 8);
+  if (free_mask) std::memcpy(free_mask, h->fr.data(), h->ndof);
+  return ICC_OK;
+}
+
+ICC_API int icc_immersed_mesh_set_solver(icc_immersed_mesh* h, double tol, int64_t maxit, int solver) {
+  if (!h || !(tol > 0.0) || maxit < 1 || (solver != 0 && solver != 1)) return ICC_EINVAL;
+  h->cfg.pcg_tol = tol;
+  h->cfg.pcg_maxit = maxit;
+  h->cfg.solver = solver;
+  return ICC_OK;
+}
+
+ICC_API int icc_immersed_mesh_apply(icc_immersed_mesh* h, const double* x, double* y) {
+  if (!h || !x || !y) return ICC_EINVAL;
+  return icc_apply(h->op, ICC_F64, x, y);
+}
+
+ICC_API int icc_immersed_mesh_solve(icc_immersed_mesh* h, double* u, double* stats) {
+  if (!h || !u) return ICC_EINVAL;
+  const int64_t n = h->ndof;
+  hipStream_t st = h->ctx->stream;
+  double st8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int rc;
+  if (h->cfg.solver == 0) {
+    TRY(icc_diag_dev(h->op, h->d_tmp));
+    ICC_HIP(launch_clamp_min(n, h->d_tmp, 1e-6, h->d_diag, st));
+    int64_t iters = 0;
+    double res = 0.0;
+    rc = icc_pcg_dev(h->op, h->d_f, h->d_diag, h->cfg.pcg_tol, h->cfg.pcg_maxit, h->d_u, &iters, &res);
+    if (rc != ICC_OK && rc != ICC_ENOCONV && rc != ICC_EBREAKDOWN) return rc;
+    st8[0] = (double)iters;
+    st8[1] = res;
+  } else {
+    ICC_HIP(launch_fill<double>(h->d_u, 0.0, n, st));
+    rc = icc_solve_mixed_dev(h->op, h->d_f, h->cfg.pcg_tol, 0.1, h->cfg.pcg_maxit, h->d_u, st8);
+    if (rc != ICC_OK && rc != ICC_ENOCONV && rc != ICC_EBREAKDOWN) return rc;
+    const double upd = st8[0], its = st8[1];
+    st8[0] = its;
+    st8[1] = upd;
+  }
+  // u = free ? x : 0 (immersed.rs:567), true residual and compliance F.u
+  std::vector<double> au(n);
+  ICC_HIP(hipMemcpyAsync(u, h->d_u, n * 8, hipMemcpyDeviceToHost, st));
+  int rc2 = icc_apply_dev(h->op, ICC_F64, h->d_u, h->d_tmp);
+  if (rc2 != ICC_OK) return rc2;
+  ICC_HIP(hipMemcpyAsync(au.data(), h->d_tmp, n * 8, hipMemcpyDeviceToHost, st));
+  ICC_HIP(hipStreamSynchronize(st));
+  double rr = 0.0, ff = 0.0, c = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!h->fr[i]) {
+      u[i] = 0.0;
+      continue;
+    }
+    const double r = h->force[i] - au[i];
+    rr += r * r;
+    ff += h->force[i] * h->force[i];
+  }
+  for (int64_t i = 0; i < n; ++i) c += h->force[i] * u[i];
+  st8[2] = ff > 0.0 ? std::sqrt(rr / ff) : std::sqrt(rr);
+  st8[3] = c;
+  if (stats) std::memcpy(stats, st8, sizeof(st8));
+  return rc;
+}

@@ -10,7 +10,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import (ICC_STAB_GHOST, ICC_STAB_LEGACY, ICC_STAB_NONE, IMM_REPORT_FN, ImmTopOptConfigC, check, lib)
+from ._abi import (ICC_BREAKDOWN_CODES, ICC_CLAMP_BOTTOM, ICC_CLAMP_LEFT, ICC_CLAMP_NONE, ICC_CLAMP_TOP, ICC_OK,
+                   ICC_STAB_GHOST, ICC_STAB_LEGACY, ICC_STAB_NONE, IMM_REPORT_FN, ImmersedMeshConfigC,
+                   ImmTopOptConfigC, check, lib)
 
 
 def _p(a):
@@ -75,6 +77,120 @@ class Stabilization:
     @staticmethod
     def ghost_penalty(gamma):
         return Stabilization(ICC_STAB_GHOST, float(gamma))
+
+
+_CLAMP = {None: ICC_CLAMP_NONE, "bottom": ICC_CLAMP_BOTTOM, "top": ICC_CLAMP_TOP, "left": ICC_CLAMP_LEFT}
+
+
+@dataclass
+class ImmersedOptions:
+    """Fields and defaults of ImmersedOptions (immersed.rs:392-429); degree
+    must be 2.  clamped_face: "bottom", "top", "left" or None; force: None
+    for the default top-centre load.  Extension: solver 0 = FP64 Jacobi
+    PCG, 1 = FP32 PCG with FP64 reliable updates."""
+    grid_res: tuple = (24, 16, 12)
+    padding: float = 0.05
+    degree: int = 2
+    young: float = 1.0
+    poisson: float = 0.3
+    stabilization: Stabilization = field(default_factory=Stabilization)
+    emin: float = 1e-4
+    subcell_res: tuple = (4, 4, 4)
+    clamped_face: object = "bottom"
+    force: np.ndarray = None
+    pcg_tol: float = 1e-8
+    pcg_maxit: int = 5000
+    solver: int = 0
+
+    def to_c(self):
+        if self.degree != 2:
+            raise ValueError("only degree 2 is implemented")
+        c = ImmersedMeshConfigC()
+        lib().icc_immersed_mesh_config_default(C.byref(c))
+        c.grid_res[:] = [int(v) for v in self.grid_res]
+        c.subcell_res[:] = [int(v) for v in self.subcell_res]
+        for k in ("padding", "young", "poisson", "emin", "pcg_tol"):
+            setattr(c, k, float(getattr(self, k)))
+        c.stabilization = int(self.stabilization.kind)
+        c.gamma = float(self.stabilization.gamma)
+        c.clamped_face = _CLAMP[self.clamped_face]
+        c.pcg_maxit = int(self.pcg_maxit)
+        c.solver = int(self.solver)
+        return c
+
+
+@dataclass
+class ImmersedSolution:
+    """ImmersedSolution (immersed.rs:541-547) plus the true relative residual."""
+    u: np.ndarray
+    compliance: float
+    iterations: int
+    residual: float
+    true_residual: float
+    status: int = ICC_OK
+
+
+class ImmersedProblem:
+    """build_immersed_problem(mesh, opts) on the GPU (immersed.rs:466-538)."""
+
+    def __init__(self, ctx, mesh, opts=None):
+        if opts is None:
+            opts = ImmersedOptions()
+        self.ctx, self.mesh, self.opts = ctx, mesh, opts
+        self._c = opts.to_c()
+        f = None if opts.force is None else np.ascontiguousarray(opts.force, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().icc_immersed_mesh_create(ctx.h, C.byref(self._c), _p(mesh.vertices), len(mesh.vertices),
+                                             _p(mesh.triangles), len(mesh.triangles), _p(f), C.byref(h)))
+        self.h = h
+        sz = np.zeros(4, dtype=np.int64)
+        gb = np.zeros(6)
+        check(lib().icc_immersed_mesh_sizes(h, _p(sz), _p(gb)))
+        self.nel, self.ndof, self.n_faces, self.n_free = (int(v) for v in sz)
+        self.grid_bounds = gb.reshape(3, 2)
+        self.status = np.zeros(self.nel, dtype=np.int8)
+        self.weights = np.zeros(self.nel)
+        self.force = np.zeros(self.ndof)
+        fr = np.zeros(self.ndof, dtype=np.uint8)
+        check(lib().icc_immersed_mesh_problem(h, _p(self.status), _p(self.weights), _p(self.force), _p(fr)))
+        self.free = fr.astype(bool)
+
+    def apply(self, x):
+        """M (K + S) M x."""
+        y = np.zeros(self.ndof)
+        check(lib().icc_immersed_mesh_apply(self.h, _p(np.ascontiguousarray(x, dtype=np.float64)), _p(y)))
+        return y
+
+    def solve(self, tol=None, maxit=None, solver=None):
+        """solve_immersed_problem(pb, tol, maxit) (immersed.rs:551-570)."""
+        o = self.opts
+        check(lib().icc_immersed_mesh_set_solver(self.h, float(o.pcg_tol if tol is None else tol),
+                                                 int(o.pcg_maxit if maxit is None else maxit),
+                                                 int(o.solver if solver is None else solver)))
+        u = np.zeros(self.ndof)
+        st = np.zeros(8)
+        rc = lib().icc_immersed_mesh_solve(self.h, _p(u), _p(st))
+        check(rc, allowed=(ICC_OK,) + ICC_BREAKDOWN_CODES)
+        return ImmersedSolution(u, float(st[3]), int(st[0]), float(st[1]), float(st[2]), rc)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().icc_immersed_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_immersed_problem(ctx, mesh, opts=None):
+    return ImmersedProblem(ctx, mesh, opts)
+
+
+def solve_immersed_problem(pb, tol, maxit):
+    return pb.solve(tol, maxit)
 
 
 @dataclass

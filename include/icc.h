@@ -322,6 +322,56 @@ int icc_immersed_apply(icc_immersed* h, const double* x, double* y);
 int icc_immersed_solve(icc_immersed* h, double* u, double* stats);
 int icc_immersed_solve_dev(icc_immersed* h, double* u_dev, double* stats);
 
+/* ---- immersed problem in a triangle B-rep ----------------------------------- */
+/* Mirrors ImmersedOptions / build_immersed_problem / solve_immersed_problem
+ * (rust/src/immersed.rs:392-570): p = 2 grid on padded_bounds(mesh, padding)
+ * with grid_res cells, ray-parity classification with subcell_res^3 samples,
+ * stiffness scale emin + (1 - emin) w on every cell, the stabilisation
+ * (ICC_STAB_*, gamma), the clamped background face, the caller's force (or,
+ * with force = NULL, -1/count on z at the top-face control points with
+ * |i - round(n/2)| <= 2), and Jacobi PCG on the free DOFs with the diagonal
+ * clamped at 1e-6 (solver 0) or the mixed solver to the same true residual
+ * (solver 1). */
+enum { ICC_CLAMP_NONE = -1, ICC_CLAMP_BOTTOM = 0, ICC_CLAMP_TOP = 1, ICC_CLAMP_LEFT = 2 };
+typedef struct icc_immersed_mesh icc_immersed_mesh;
+typedef struct icc_immersed_mesh_config {
+  int64_t grid_res[3];
+  double padding;
+  double young, poisson;
+  int32_t stabilization; /* ICC_STAB_NONE, ICC_STAB_LEGACY, ICC_STAB_GHOST */
+  int32_t clamped_face;  /* ICC_CLAMP_* */
+  double gamma;
+  double emin;
+  int64_t subcell_res[3];
+  double pcg_tol;
+  int64_t pcg_maxit;
+  int32_t solver; /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates */
+  int32_t pad;
+} icc_immersed_mesh_config;
+/* ImmersedOptions::default: 24x16x12, padding 0.05, E = 1, nu = 0.3, ghost
+ * penalty 1e-2, emin 1e-4, 4^3 sub-cells, bottom clamped, pcg_tol 1e-8,
+ * pcg_maxit 5000, solver 0 */
+void icc_immersed_mesh_config_default(icc_immersed_mesh_config* cfg);
+/* force [ndof] or NULL (default load) */
+int icc_immersed_mesh_create(icc_ctx* ctx, const icc_immersed_mesh_config* cfg, const double* vertices, int64_t nv,
+                             const int64_t* tris, int64_t ntri, const double* force, icc_immersed_mesh** out);
+int icc_immersed_mesh_destroy(icc_immersed_mesh* h);
+/* out = nel, ndof, stabilised faces, free DOFs; grid bounds (either may be NULL) */
+int icc_immersed_mesh_sizes(const icc_immersed_mesh* h, int64_t out[4], double grid_bounds[6]);
+/* status, weights [nel]; force, free mask [ndof] (any may be NULL) */
+int icc_immersed_mesh_problem(const icc_immersed_mesh* h, int8_t* status, double* weights, double* force,
+                              uint8_t* free_mask);
+/* tolerance, iteration limit and solver of the following solves
+ * (solve_immersed_problem's tol and maxit) */
+int icc_immersed_mesh_set_solver(icc_immersed_mesh* h, double tol, int64_t maxit, int solver);
+/* y = M (K + S) M x (the solver's operator) */
+int icc_immersed_mesh_apply(icc_immersed_mesh* h, const double* x, double* y);
+/* u [ndof] (zero on fixed DOFs, like ImmersedSolution::u); stats (8, may be
+ * NULL): iterations, recursive relative residual (solver 0) or reliable
+ * updates (solver 1), true relative residual, compliance F.u, 0, 0, 0, 0.
+ * ICC_ENOCONV / ICC_EBREAKDOWN like icc_pcg with u filled. */
+int icc_immersed_mesh_solve(icc_immersed_mesh* h, double* u, double* stats);
+
 /* ---- immersed SIMP topology optimisation in a triangle B-rep ------------------ */
 /* Mirrors ImmersedTopOpt::{new, run} (rust/src/topopt_immersed.rs:21-247):
  * p = 2 background grid on padded_bounds(brep, 0.05) with grid_res cells,

@@ -301,3 +301,76 @@ def test_ghost_penalty_small_grids_match_oracle(ctx, nel):
         assert np.max(np.abs(got)) == 0.0
     else:
         assert O.rel_err(got, want) < 1e-13
+
+
+def test_build_immersed_problem_fixture(ctx):
+    """build_immersed_problem / solve_immersed_problem through the B-rep
+    handle (immersed_vs_matlab.rs:20-73): status exact, weights < 1e-7, load
+    < 1e-15, free mask exact, K v probe < 1e-12 (unmasked: all free), PCG(1e-12)
+    vs the MATLAB direct solution < 1e-6, compliance < 1e-9."""
+    d, m = _tilted()
+    opts = icc.ImmersedOptions(grid_res=tuple(d["grid_res"]), young=d["young"], poisson=d["poisson"],
+                               stabilization=icc.Stabilization.legacy(d["gamma_gp"]), emin=d["emin"])
+    pb = icc.build_immersed_problem(ctx, m, opts)
+    assert np.all(np.abs(pb.grid_bounds.ravel() - np.asarray(d["grid_bounds"]).ravel()) < 1e-14)
+    assert np.array_equal(pb.status, np.asarray(d["status"], dtype=np.int8))
+    assert np.max(np.abs(pb.weights - np.asarray(d["weights"]))) < 1e-7
+    assert O.rel_err(pb.force, d["load"]) < 1e-15
+    assert np.array_equal(pb.free, np.asarray(d["free_mask"]) != 0)
+    sol = icc.solve_immersed_problem(pb, 1e-12, 20 * pb.ndof)
+    assert sol.status == icc.ICC_OK
+    assert O.rel_err(sol.u, d["u_direct"]) < 1e-6
+    assert abs(sol.compliance - d["compliance"]) / d["compliance"] < 1e-9
+    free_all = icc.ImmersedOptions(grid_res=tuple(d["grid_res"]), stabilization=icc.Stabilization.legacy(d["gamma_gp"]),
+                                   emin=d["emin"], clamped_face=None)
+    pb2 = icc.build_immersed_problem(ctx, m, free_all)
+    v = O.probe_vectors(pb2.ndof, 1)[0]
+    assert O.rel_err(pb2.apply(v), d["k_probe"]) < 1e-12
+
+
+@pytest.mark.parametrize("stab,clamp", [("ghost", "bottom"), ("legacy", "left"), ("none", "bottom")])
+def test_build_immersed_problem_matches_oracle(ctx, stab, clamp):
+    """Other stabilisations and clamped faces against the oracle's
+    build_immersed_problem / solve_immersed_problem: solution to 1e-8,
+    compliance to 1e-9, same free mask and load."""
+    m = _hull()
+    res = (12, 9, 7)
+    kind = {"ghost": (icc.Stabilization.ghost_penalty(1e-2), 2), "legacy": (icc.Stabilization.legacy(0.05), 1),
+            "none": (icc.Stabilization.none(), 0)}[stab]
+    ckind = {"bottom": 0, "top": 1, "left": 2}[clamp]
+    opts = icc.ImmersedOptions(grid_res=res, stabilization=kind[0], clamped_face=clamp, pcg_tol=1e-12,
+                               pcg_maxit=200000)
+    pb = icc.build_immersed_problem(ctx, m, opts)
+    ref = O.Immersed(m.vertices, m.triangles, res, stab_kind=kind[1], gamma=kind[0].gamma, emin=1e-4, clamp=ckind)
+    assert np.array_equal(pb.free, ref.free.astype(bool))
+    assert O.rel_err(pb.force, ref.force) < 1e-15
+    sol = pb.solve()
+    u, c, it, res_ = ref.solve(1e-12, 200000)
+    assert O.rel_err(sol.u, u) < 1e-8, (O.rel_err(sol.u, u), sol.iterations, it)
+    assert abs(sol.compliance - c) / abs(c) < 1e-9
+    mixed = pb.solve(tol=1e-10, solver=1)
+    assert mixed.true_residual <= 1e-10 and O.rel_err(mixed.u, u) < 1e-6
+
+
+def test_build_immersed_problem_custom_force_top_clamp(ctx):
+    """A caller's force with the top face clamped (the default load would
+    sit on fixed DOFs): the free mask matches the oracle's top clamp, the
+    solution satisfies M (K + S) M u = M f to the tolerance, u is zero on the
+    clamp, and the compliance is F.u."""
+    m = _hull()
+    res = (10, 8, 6)
+    opts = icc.ImmersedOptions(grid_res=res, clamped_face="top", pcg_tol=1e-10, pcg_maxit=100000)
+    probe = icc.build_immersed_problem(ctx, m, opts)
+    f = np.random.default_rng(12).standard_normal(probe.ndof) * probe.free
+    opts.force = f
+    pb = icc.build_immersed_problem(ctx, m, opts)
+    ref = O.Immersed(m.vertices, m.triangles, res, stab_kind=2, gamma=1e-2, emin=1e-4, clamp=1)
+    assert np.array_equal(pb.free, ref.free.astype(bool))
+    assert np.array_equal(pb.force, f)
+    sol = pb.solve()
+    assert sol.status == icc.ICC_OK
+    r = np.where(pb.free, f - pb.apply(sol.u), 0.0)
+    assert np.linalg.norm(r) / np.linalg.norm(f) < 1e-9
+    assert np.all(sol.u[~pb.free] == 0.0)
+    assert abs(sol.compliance - float(f @ sol.u)) <= 1e-12 * abs(sol.compliance)
+
