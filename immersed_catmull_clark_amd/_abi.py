@@ -95,6 +95,7 @@ SIGNATURES = [
     ("icc_immersed_solve_dev", cint, [vp, vp, vp]),
     ("icc_op_set_legacy_stabilization", cint, [vp, vp, dbl, vp]),
     ("icc_op_set_ghost_penalty", cint, [vp, vp, dbl, dbl, vp]),
+    ("icc_op_face_frobenius", cint, [vp, vp]),
     ("icc_classify_mesh", cint, [vp, vp, i64, vp, i64, vp, vp, vp, vp, vp]),
     ("icc_mesh_padded_bounds", cint, [vp, i64, dbl, vp]),
     ("icc_imm_topopt_config_default", None, [vp]),

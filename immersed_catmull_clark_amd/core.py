@@ -180,6 +180,12 @@ class Operator:
         check(lib().icc_op_set_ghost_penalty(self.h, _p(st), float(gamma), yg, C.byref(nf)))
         return int(nf.value)
 
+    def face_frobenius(self):
+        """Frobenius norm of the assembled face stabilisation term."""
+        f = C.c_double(0.0)
+        check(lib().icc_op_face_frobenius(self.h, C.byref(f)))
+        return f.value
+
     def apply(self, x, prec=ICC_F64, out=None):
         if _is_dev(x):
             check(lib().icc_apply_dev(self.h, prec, _p(x), _p(out)))

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <unordered_map>
 
 #include "icc_internal.hpp"
 #include "icc_kernels.hpp"
@@ -561,6 +562,8 @@ static void clear_faces(icc_op* op) {
   op->d_fptr = nullptr;
   op->f_n = 0;
   op->f_nblk = 0;
+  op->h_faces.clear();
+  op->h_ftab.clear();
   op->f_kind = 0;
   op->f_alpha = 0.0;
 }
@@ -636,6 +639,7 @@ static int upload_faces(icc_op* op, int kind, double coef, const std::vector<int
   ICC_HIP(hipStreamSynchronize(st));
   op->f_n = (int64_t)faces.size();
   op->f_nblk = (int64_t)blk.size();
+  op->h_faces = faces_in;
   op->f_kind = kind;
   op->f_alpha = coef;
   return ICC_OK;
@@ -745,8 +749,60 @@ ICC_API int icc_op_set_ghost_penalty(icc_op* op, const int8_t* status, double ga
     op->ftab_moff[d] = moff[d];
   }
   op->ftab_mass = mass0;
+  op->h_ftab = tab;
   ICC_TRY(upload_faces(op, FACE_GHOST, coef, faces, dg));
   if (nfaces) *nfaces = op->f_n;
+  return ICC_OK;
+}
+
+// Frobenius norm of the assembled face term (the reference reports the
+// norm of its stabilisation matrix, immersed.rs:497): the face blocks are
+// summed entry by entry on the host, then squared.
+ICC_API int icc_op_face_frobenius(const icc_op* op, double* fro) {
+  if (!op || !fro) return ICC_EINVAL;
+  *fro = 0.0;
+  if (op->f_n <= 0) return ICC_OK;
+  const GridSetup& s = op->g->s;
+  const int64_t n0 = s.nel[0], n1 = s.nel[1], n = s.ndof;
+  std::unordered_map<int64_t, double> acc;
+  acc.reserve((size_t)op->f_n * 64);
+  auto add = [&](int64_t r, int64_t c, double v) { acc[r * n + c] += v; };
+  for (int32_t code : op->h_faces) {
+    const int64_t e1 = code >> 2;
+    const int dir = code & 3;
+    const int64_t e[3] = {e1 % n0, (e1 / n0) % n1, e1 / (n0 * n1)};
+    int64_t st, sa, sb;
+    slab_strides(s, dir, st, sa, sb);
+    for (int c = 0; c < 3; ++c) {
+      const int64_t base = c * s.ndof_sc + e[0] + s.ncp[0] * (e[1] + s.ncp[1] * e[2]);
+      if (op->f_kind == FACE_LEGACY) {
+        for (int b = 0; b < 3; ++b)
+          for (int a = 0; a < 3; ++a) {
+            const int64_t p = base + a * sa + b * sb, q = p + 3 * st;
+            add(p, p, op->f_alpha);
+            add(q, q, op->f_alpha);
+            add(p, q, -op->f_alpha);
+            add(q, p, -op->f_alpha);
+          }
+      } else {
+        const int ta = dir == 0 ? 1 : 0, tb = dir == 2 ? 1 : 2;
+        const double* jv = &op->h_ftab[op->ftab_joff[dir] + 4 * (e[dir] + 1)];
+        const double* Ma = &op->h_ftab[op->ftab_mass + op->ftab_moff[ta] + 9 * e[ta]];
+        const double* Mb = &op->h_ftab[op->ftab_mass + op->ftab_moff[tb] + 9 * e[tb]];
+        for (int t1 = 0; t1 < 4; ++t1)
+          for (int b1 = 0; b1 < 3; ++b1)
+            for (int a1 = 0; a1 < 3; ++a1)
+              for (int t2 = 0; t2 < 4; ++t2)
+                for (int b2 = 0; b2 < 3; ++b2)
+                  for (int a2 = 0; a2 < 3; ++a2)
+                    add(base + t1 * st + a1 * sa + b1 * sb, base + t2 * st + a2 * sa + b2 * sb,
+                        op->f_alpha * jv[t1] * jv[t2] * Ma[3 * a1 + a2] * Mb[3 * b1 + b2]);
+      }
+    }
+  }
+  double ss = 0.0;
+  for (auto& kv : acc) ss += kv.second * kv.second;
+  *fro = std::sqrt(ss);
   return ICC_OK;
 }
 

@@ -150,6 +150,8 @@ struct icc_op {
   double* d_f_diag = nullptr;   // [ndof] diagonal of the face term
   double* d_ftab = nullptr;     // ghost: jump tables then mass tables
   int ftab_joff[3] = {0, 0, 0}, ftab_moff[3] = {0, 0, 0}, ftab_mass = 0;
+  std::vector<int32_t> h_faces;  // host copies (Frobenius norm of the face term)
+  std::vector<double> h_ftab;
   // scratch
   double* d_partials = nullptr;  // reduction partials
   int partials_cap = 0;

@@ -117,6 +117,10 @@ int icc_op_set_legacy_stabilization(icc_op* op, const int8_t* status, double gam
  * second normal derivative and M_a, M_b the tangential 1D element mass
  * matrices.  Replaces any face term set before; status NULL removes it. */
 int icc_op_set_ghost_penalty(icc_op* op, const int8_t* status, double gamma, double young, int64_t* nfaces);
+/* Frobenius norm of the assembled face term (ImmersedProblem::
+ * stabilization_frobenius, rust/src/immersed.rs:497), summed on the host;
+ * 0 without a face term. */
+int icc_op_face_frobenius(const icc_op* op, double* fro);
 
 /* y = A x.  Host pointers, FP64 in and out; prec selects the contraction. */
 int icc_apply(icc_op* op, int prec, const double* x, double* y);

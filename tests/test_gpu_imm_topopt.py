@@ -83,6 +83,9 @@ def test_legacy_stabilization_fixture_probe(ctx):
     g, op = _fixture_operator(ctx, d, m, None)
     v = O.probe_vectors(g.ndof, 1)[0]
     assert O.rel_err(op.apply(v), d["k_probe"]) < 1e-12
+    # |K_gp|_F of the assembled stabilisation (immersed_vs_matlab.rs:41-44)
+    kf = d["kgp_frobenius"]
+    assert abs(op.face_frobenius() - kf) <= 1e-12 * kf
 
 
 @pytest.mark.parametrize("case", ["tilted", "hull"])
@@ -217,6 +220,7 @@ def test_ghost_penalty_operator_matches_oracle(ctx, case):
     want, fro = O.ghost_penalty_matvec(bounds, res, 2, status, gamma, young, x)
     assert fro > 0.0
     assert O.rel_err(op.apply(x), want) < 1e-13
+    assert abs(op.face_frobenius() - fro) <= 1e-12 * fro
     fr = (rng.random(g.ndof) > 0.2).astype(np.uint8)
     op.set_free(fr)
     want_m, _ = O.ghost_penalty_matvec(bounds, res, 2, status, gamma, young, fr * x)
