@@ -56,6 +56,7 @@ SIGNATURES = [
     ("icc_sens", cint, [vp, vp, vp, dbl, dbl, vp, vp]),
     ("icc_filter_create", cint, [vp, dbl, vpp]),
     ("icc_filter_destroy", cint, [vp]),
+    ("icc_filter_create_active", cint, [vp, dbl, vp, vpp]),
     ("icc_filter_sens_dev", cint, [vp, vp, vp, vp]),
     ("icc_filter_fwd_dev", cint, [vp, vp, vp]),
     ("icc_filter_bwd_dev", cint, [vp, vp, vp]),

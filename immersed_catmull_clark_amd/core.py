@@ -274,12 +274,25 @@ class Operator:
 
 
 class Filter:
-    """Cartesian filter on element centres (topopt_iga.rs:100-126)."""
+    """Filter on element centres.
 
-    def __init__(self, grid, rmin):
+    Without ``active``: the structured filter of TopOpt3D (topopt_iga.rs:100-126),
+    radius ``rmin`` in elements.  With ``active`` (one flag per element): the
+    design-domain filter of ImmersedTopOpt::new (topopt_immersed.rs:117-151),
+    radius ``rmin * mean(h)``, only active neighbours of active cells; inactive
+    rows are empty.
+    """
+
+    def __init__(self, grid, rmin, active=None):
         self.grid = grid
         h = C.c_void_p()
-        check(lib().icc_filter_create(grid.h, float(rmin), C.byref(h)))
+        if active is None:
+            check(lib().icc_filter_create(grid.h, float(rmin), C.byref(h)))
+        else:
+            act = np.ascontiguousarray(np.asarray(active).reshape(-1) != 0, dtype=np.uint8)
+            if act.size != grid.nel_tot:
+                raise ValueError(f"active has {act.size} entries, the grid has {grid.nel_tot} elements")
+            check(lib().icc_filter_create_active(grid.h, float(rmin), _p(act), C.byref(h)))
         self.h = h
 
     def sens(self, x, dc):

@@ -1392,23 +1392,70 @@ ICC_API int icc_filter_destroy(icc_filter* f) {
   if (!f) return ICC_OK;
   hipFree(f->d_hs);
   hipFree(f->d_tmp);
+  hipFree(f->d_act);
+  hipFree(f->d_off);
+  hipFree(f->d_w);
   delete f;
   return ICC_OK;
 }
 
+// Filter on a design domain (topopt_immersed.rs:117-151): offsets within
+// r = rmin * mean(h) measured from the offset itself, weight r - dist, only
+// active in-range neighbours of active cells; inactive rows are empty.
+ICC_API int icc_filter_create_active(icc_grid* g, double rmin, const uint8_t* active, icc_filter** out) {
+  if (!g || !out || !active || !(rmin > 0)) return ICC_EINVAL;
+  auto* f = new icc_filter();
+  f->g = g;
+  const auto& s = g->s;
+  std::vector<int32_t> offs;
+  std::vector<double> ows;
+  f->r = design_filter_offsets(s, rmin, offs, ows);
+  f->noff = (int)ows.size();
+  const int64_t ne = s.nel_tot;
+  int rc;
+  if ((rc = dalloc(&f->d_hs, ne)) || (rc = dalloc(&f->d_act, ne)) || (rc = dalloc(&f->d_off, offs.size())) ||
+      (rc = dalloc(&f->d_w, ows.size()))) {
+    icc_filter_destroy(f);
+    return rc;
+  }
+  hipStream_t st = g->ctx->stream;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(f->d_act, active, ne, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(f->d_off, offs.data(), offs.size() * 4, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(f->d_w, ows.data(), ows.size() * 8, hipMemcpyHostToDevice, st)) ||
+      (e = launch_imm_filter(1, s.nel, f->d_off, f->d_w, f->noff, f->d_act, nullptr, nullptr, nullptr, f->d_hs, st)) ||
+      (e = hipStreamSynchronize(st))) {
+    icc_filter_destroy(f);
+    return hip_fail(e, "icc_filter_create_active");
+  }
+  *out = f;
+  return ICC_OK;
+}
+
+namespace icc {
+// mode 0 sensitivity, 1 density forward, 2 density backward (structured
+// numbering; the design-domain kernel numbers them 0, 2, 3)
+static hipError_t run_filter(icc_filter* f, int mode, const double* x, const double* v, double* out) {
+  hipStream_t st = f->g->ctx->stream;
+  if (f->d_act == nullptr) return launch_filter(f, mode, x, v, out, st);
+  static const int imm_mode[3] = {0, 2, 3};
+  return launch_imm_filter(imm_mode[mode], f->g->s.nel, f->d_off, f->d_w, f->noff, f->d_act, x, v, f->d_hs, out, st);
+}
+}  // namespace icc
+
 ICC_API int icc_filter_sens_dev(icc_filter* f, const double* x, const double* dc, double* out) {
   if (!f || !x || !dc || !out) return ICC_EINVAL;
-  ICC_HIP(launch_filter(f, 0, x, dc, out, f->g->ctx->stream));
+  ICC_HIP(run_filter(f, 0, x, dc, out));
   return ICC_OK;
 }
 ICC_API int icc_filter_fwd_dev(icc_filter* f, const double* x, double* out) {
   if (!f || !x || !out) return ICC_EINVAL;
-  ICC_HIP(launch_filter(f, 1, nullptr, x, out, f->g->ctx->stream));
+  ICC_HIP(run_filter(f, 1, nullptr, x, out));
   return ICC_OK;
 }
 ICC_API int icc_filter_bwd_dev(icc_filter* f, const double* gr, double* out) {
   if (!f || !gr || !out) return ICC_EINVAL;
-  ICC_HIP(launch_filter(f, 2, nullptr, gr, out, f->g->ctx->stream));
+  ICC_HIP(run_filter(f, 2, nullptr, gr, out));
   return ICC_OK;
 }
 ICC_API int icc_filter_sens(icc_filter* f, const double* x, const double* dc, double* out) {

@@ -266,22 +266,9 @@ ICC_API int icc_imm_topopt_create(icc_ctx* ctx, const icc_imm_topopt_config* cfg
   }
   t->nf = std::sqrt(t->nf);
   // filter offsets on the design domain (topopt_immersed.rs:117-131)
-  const double h[3] = {s.len[0] / (double)s.nel[0], s.len[1] / (double)s.nel[1], s.len[2] / (double)s.nel[2]};
-  const double r = cfg->rmin * (h[0] + h[1] + h[2]) / 3.0;
-  int64_t rmax[3];
-  for (int d = 0; d < 3; ++d) rmax[d] = (int64_t)std::ceil(r / h[d]);
   std::vector<int32_t> offs;
   std::vector<double> ows;
-  for (int64_t dk = -rmax[2]; dk <= rmax[2]; ++dk)
-    for (int64_t dj = -rmax[1]; dj <= rmax[1]; ++dj)
-      for (int64_t di = -rmax[0]; di <= rmax[0]; ++di) {
-        const double a = (double)di * h[0], b = (double)dj * h[1], c = (double)dk * h[2];
-        const double dist = std::sqrt(a * a + b * b + c * c);
-        if (dist <= r) {
-          offs.insert(offs.end(), {(int32_t)di, (int32_t)dj, (int32_t)dk});
-          ows.push_back(r - dist);
-        }
-      }
+  design_filter_offsets(s, cfg->rmin, offs, ows);
   t->noff = (int)ows.size();
   const int G = topopt_blocks(ne);
   if ((rc = dalloc(&t->x, ne)) || (rc = dalloc(&t->xold, ne)) || (rc = dalloc(&t->xnew, ne)) ||

@@ -84,6 +84,10 @@ int n_tile_variants();
 int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
 
 void ghost_tables_1d(int64_t nel, double length, std::vector<double>& jump, std::vector<double>& mass);
+// Design-domain filter offsets (topopt_immersed.rs:117-131): radius
+// r = rmin * mean(h); every (di, dj, dk), dk outermost, with |offset| <= r and
+// weight r - |offset|.  Returns r.
+double design_filter_offsets(const GridSetup& s, double rmin, std::vector<int32_t>& offs, std::vector<double>& ows);
 
 // WQ rules for p = 2 (wq.cpp)
 struct WqRulesP2 {
@@ -161,7 +165,9 @@ struct icc_filter {
   icc_grid* g = nullptr;
   double r = 0;
   int rad[3] = {0, 0, 0};
-  // neighbour offsets (di, dj, dk, w) in increasing linear-index order
+  // design-domain variant (topopt_immersed.rs:117-151): active cells and the
+  // offset table (di, dj, dk) with weights r - dist in (dk, dj, di) order
+  uint8_t* d_act = nullptr;  // [nel], null for the structured filter
   int32_t* d_off = nullptr;  // [noff][3]
   double* d_w = nullptr;     // [noff]
   int noff = 0;

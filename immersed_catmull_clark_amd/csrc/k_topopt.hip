@@ -52,7 +52,9 @@ __device__ __forceinline__ double tb_max(double v, double* sh) {
   return s;  // valid on thread 0
 }
 
-// mode 0: out_e = sum_f w x_f v_f / (Hs_e max(x_e, 1e-3));  mode 1: out_e = Hs_e
+// mode 0: out_e = sum_f w x_f v_f / (Hs_e max(x_e, 1e-3));  mode 1: out_e = Hs_e;
+// mode 2: out_e = sum_f w v_f / Hs_e (density forward);
+// mode 3: out_e = sum_f w v_f / Hs_f (its adjoint: the weights are symmetric)
 __global__ __launch_bounds__(TB) void imm_filter_kernel(int mode, int n0, int n1, int n2,
                                                         const int32_t* __restrict__ off,
                                                         const double* __restrict__ ow, int noff,
@@ -71,14 +73,22 @@ __global__ __launch_bounds__(TB) void imm_filter_kernel(int mode, int n0, int n1
         if (!act[f]) continue;
         if (mode == 0)
           s += ow[o] * x[f] * v[f];
-        else
+        else if (mode == 1)
           s += ow[o];
+        else if (mode == 2)
+          s += ow[o] * v[f];
+        else
+          s += ow[o] * (v[f] / hs[f]);
       }
     }
     if (mode == 0)
       out[e] = s / (hs[e] * fmax(x[e], 1e-3));
-    else
+    else if (mode == 1)
       out[e] = s < 1e-6 ? 1.0 : s;
+    else if (mode == 2)
+      out[e] = s / hs[e];
+    else
+      out[e] = s;
   }
 }
 

@@ -442,4 +442,24 @@ void ghost_tables_1d(int64_t nel, double length, std::vector<double>& jump, std:
     for (int m = 0; m < 9; ++m) mass[e * 9 + m] = em[e][m];
 }
 
+double design_filter_offsets(const GridSetup& s, double rmin, std::vector<int32_t>& offs, std::vector<double>& ows) {
+  const double h[3] = {s.len[0] / (double)s.nel[0], s.len[1] / (double)s.nel[1], s.len[2] / (double)s.nel[2]};
+  const double r = rmin * (h[0] + h[1] + h[2]) / 3.0;
+  int64_t rmax[3];
+  for (int d = 0; d < 3; ++d) rmax[d] = (int64_t)std::ceil(r / h[d]);
+  offs.clear();
+  ows.clear();
+  for (int64_t dk = -rmax[2]; dk <= rmax[2]; ++dk)
+    for (int64_t dj = -rmax[1]; dj <= rmax[1]; ++dj)
+      for (int64_t di = -rmax[0]; di <= rmax[0]; ++di) {
+        const double a = (double)di * h[0], b = (double)dj * h[1], c = (double)dk * h[2];
+        const double dist = std::sqrt(a * a + b * b + c * c);
+        if (dist <= r) {
+          offs.insert(offs.end(), {(int32_t)di, (int32_t)dj, (int32_t)dk});
+          ows.push_back(r - dist);
+        }
+      }
+  return r;
+}
+
 }  // namespace icc

@@ -180,6 +180,11 @@ int icc_sens(icc_op* op, const double* u, const double* x, double penal, double 
 /* Cartesian filter on element centres, r = rmin * mean(h), weight max(r - d, 0)
  * with d^2 <= r^2 (topopt_iga.rs:100-126).  active may be NULL (all active). */
 int icc_filter_create(icc_grid* g, double rmin, icc_filter** out);
+/* The design-domain filter of ImmersedTopOpt::new (rust/src/topopt_immersed.rs:117-151):
+ * offsets within r = rmin * mean(h), weight r - dist, only active in-range
+ * neighbours of active cells (active[nel]); inactive rows are empty (sensitivity
+ * and forward give 0 there).  The same sens / fwd / bwd calls apply. */
+int icc_filter_create_active(icc_grid* g, double rmin, const uint8_t* active, icc_filter** out);
 int icc_filter_destroy(icc_filter* f);
 /* sensitivity filter (topopt_iga.rs:200-205) */
 int icc_filter_sens_dev(icc_filter* f, const double* x_dev, const double* dc_dev, double* out_dev);

@@ -225,6 +225,41 @@ ORC_API void orc_filter_apply(void* p, double rmin, i64 mode, const double* x, c
   }
 }
 
+// Design-domain filter (topopt_immersed.rs:117-151) with the modes of
+// orc_filter_apply; the row sum of an empty row is 1.
+ORC_API void orc_design_filter_apply(void* p, double rmin, i64 mode, const uint8_t* active, const double* x,
+                                     const double* v, double* out) {
+  const SpaceBox& s = ((SpaceH*)p)->sp;
+  std::vector<uint8_t> act(active, active + s.nel);
+  auto rows = build_filter_design(s, rmin, act);
+  usize nel = s.nel;
+  Vec hs(nel);
+  for (usize e = 0; e < nel; ++e) {
+    double a = 0.0;
+    for (double w : rows[e].w) a += w;
+    hs[e] = a < 1e-6 ? 1.0 : a;
+  }
+  for (usize e = 0; e < nel; ++e) {
+    double acc = 0.0;
+    const auto& r = rows[e];
+    for (usize k = 0; k < r.f.size(); ++k) {
+      usize f = r.f[k];
+      if (mode == 0)
+        acc += r.w[k] * x[f] * v[f];
+      else if (mode == 1)
+        acc += r.w[k] * v[f];
+      else
+        acc += r.w[k] * (v[f] / hs[f]);
+    }
+    if (mode == 0)
+      out[e] = acc / (hs[e] * std::fmax(x[e], 1e-3));
+    else if (mode == 1)
+      out[e] = acc / hs[e];
+    else
+      out[e] = acc;
+  }
+}
+
 // Filter row check: returns 1 if the structured rows equal the O(nel^2) scan.
 ORC_API i64 orc_filter_rows_match(void* p, double rmin) {
   const SpaceBox& s = ((SpaceH*)p)->sp;

@@ -553,6 +553,44 @@ ImmersedSolution solve_sdf_immersed(const SdfImmersed& pb, double tol, usize max
 }
 
 // topopt_immersed.rs:78-153 (new) and 155-247 (run)
+// Filter rows on a design domain (topopt_immersed.rs:117-151): offsets within
+// r = rmin * mean(h), taken dk outermost and di innermost, weight r - dist;
+// only active in-range neighbours of active cells, inactive rows empty.
+std::vector<FilterRow> build_filter_design(const SpaceBox& sp, double rmin, const std::vector<uint8_t>& active) {
+  Vec h = sp.element_size();
+  const auto& res = sp.nel_dir;
+  double r = rmin * (h[0] + h[1] + h[2]) / 3.0;
+  long rmax[3];
+  for (int d = 0; d < 3; ++d) rmax[d] = (long)std::ceil(r / h[d]);
+  struct Off {
+    long di, dj, dk;
+    double w;
+  };
+  std::vector<Off> offsets;
+  for (long dk = -rmax[2]; dk <= rmax[2]; ++dk)
+    for (long dj = -rmax[1]; dj <= rmax[1]; ++dj)
+      for (long di = -rmax[0]; di <= rmax[0]; ++di) {
+        double dist = std::sqrt(((double)di * h[0]) * ((double)di * h[0]) + ((double)dj * h[1]) * ((double)dj * h[1]) +
+                                ((double)dk * h[2]) * ((double)dk * h[2]));
+        if (dist <= r) offsets.push_back({di, dj, dk, r - dist});
+      }
+  std::vector<FilterRow> filter(sp.nel);
+  for (usize e = 0; e < sp.nel; ++e) {
+    if (!active[e]) continue;
+    auto s = unravel(e, res);
+    for (auto& o : offsets) {
+      long ni = (long)s[0] + o.di, nj = (long)s[1] + o.dj, nk = (long)s[2] + o.dk;
+      if (ni < 0 || nj < 0 || nk < 0 || ni >= (long)res[0] || nj >= (long)res[1] || nk >= (long)res[2]) continue;
+      usize f = (usize)ni + res[0] * ((usize)nj + res[1] * (usize)nk);
+      if (active[f]) {
+        filter[e].f.push_back(f);
+        filter[e].w.push_back(o.w);
+      }
+    }
+  }
+  return filter;
+}
+
 ImmersedTopOptHistory immersed_topopt_run(const TriangleMesh3D& brep, const ImmersedTopOptCfg& cfg) {
   auto gb = padded_bounds(brep, 0.05);
   std::array<usize, 3> res{cfg.grid_res[0], cfg.grid_res[1], cfg.grid_res[2]};
@@ -591,36 +629,7 @@ ImmersedTopOptHistory immersed_topopt_run(const TriangleMesh3D& brep, const Imme
   Vec force(n, 0.0);
   for (usize i : load) force[2 * sp.ndof_sc + i] = -1.0 / (double)load.size();
   // filter on the design domain (offset stencil, topopt_immersed.rs:117-151)
-  Vec h = sp.element_size();
-  double r = cfg.rmin * (h[0] + h[1] + h[2]) / 3.0;
-  long rmax[3];
-  for (int d = 0; d < 3; ++d) rmax[d] = (long)std::ceil(r / h[d]);
-  struct Off {
-    long di, dj, dk;
-    double w;
-  };
-  std::vector<Off> offsets;
-  for (long dk = -rmax[2]; dk <= rmax[2]; ++dk)
-    for (long dj = -rmax[1]; dj <= rmax[1]; ++dj)
-      for (long di = -rmax[0]; di <= rmax[0]; ++di) {
-        double dist = std::sqrt(((double)di * h[0]) * ((double)di * h[0]) + ((double)dj * h[1]) * ((double)dj * h[1]) +
-                                ((double)dk * h[2]) * ((double)dk * h[2]));
-        if (dist <= r) offsets.push_back({di, dj, dk, r - dist});
-      }
-  std::vector<FilterRow> filter(nel);
-  for (usize e = 0; e < nel; ++e) {
-    if (!active[e]) continue;
-    auto s = unravel(e, sp.nel_dir);
-    for (auto& o : offsets) {
-      long ni = (long)s[0] + o.di, nj = (long)s[1] + o.dj, nk = (long)s[2] + o.dk;
-      if (ni < 0 || nj < 0 || nk < 0 || ni >= (long)res[0] || nj >= (long)res[1] || nk >= (long)res[2]) continue;
-      usize f = (usize)ni + res[0] * ((usize)nj + res[1] * (usize)nk);
-      if (active[f]) {
-        filter[e].f.push_back(f);
-        filter[e].w.push_back(o.w);
-      }
-    }
-  }
+  std::vector<FilterRow> filter = build_filter_design(sp, cfg.rmin, active);
   Vec filter_sum(nel);
   for (usize e = 0; e < nel; ++e) {
     double s = 0.0;

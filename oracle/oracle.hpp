@@ -165,6 +165,9 @@ struct CantileverTopOpt {
 // so the rows equal the O(nel^2) scan entry for entry.
 std::vector<FilterRow> build_filter_structured(const SpaceBox& sp, double rmin);
 std::vector<FilterRow> build_filter_bruteforce(const SpaceBox& sp, double rmin);
+// Filter rows on a design domain (topopt_immersed.rs:117-151): radius
+// rmin * mean(h), only active neighbours of active cells.
+std::vector<FilterRow> build_filter_design(const SpaceBox& sp, double rmin, const std::vector<uint8_t>& active);
 
 // ---- rust/src/topopt2d.rs (fast_sensitivities) ----------------------------
 Vec fast_sensitivities(const SpaceBox& sp, const Vec& u, const Vec& x, bool spline, double penal,

@@ -51,6 +51,8 @@ def lib():
         L.orc_space_energies.argtypes = [vp, vp, vp]
         L.orc_fast_sens.argtypes = [vp, vp, vp, i64, dbl, dbl, dbl, dbl, vp]
         L.orc_filter_apply.argtypes = [vp, dbl, i64, vp, vp, vp]
+        L.orc_design_filter_apply.restype = None
+        L.orc_design_filter_apply.argtypes = [vp, dbl, i64, vp, vp, vp, vp]
         L.orc_filter_rows_match.restype = i64
         L.orc_filter_rows_match.argtypes = [vp, dbl]
         L.orc_topopt_run.restype = i64
@@ -210,6 +212,15 @@ class Space:
         v = np.ascontiguousarray(v, dtype=np.float64)
         out = np.zeros(self.nel)
         lib().orc_filter_apply(self.h, rmin, mode, _p(x), _p(v), _p(out))
+        return out
+
+    def design_filter_apply(self, rmin, mode, active, x, v):
+        """Design-domain filter (topopt_immersed.rs:117-151); modes as filter_apply."""
+        act = np.ascontiguousarray(np.asarray(active).reshape(-1) != 0, dtype=np.uint8)
+        x = np.ascontiguousarray(x, dtype=np.float64) if x is not None else np.zeros(act.size)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(act.size)
+        lib().orc_design_filter_apply(self.h, rmin, mode, _p(act), _p(x), _p(v), _p(out))
         return out
 
     def filter_rows_match(self, rmin):

@@ -136,6 +136,41 @@ def test_filters_match_oracle(ctx, nel, rmin):
     assert O.rel_err(f.bwd(dc), sp.filter_apply(rmin, 2, None, dc)) < 1e-15
 
 
+@pytest.mark.parametrize("nel,rmin,frac", [([8, 4, 2], 1.5, 1.0), ([13, 7, 3], 2.3, 0.6), ([16, 12, 8], 1.5, 0.3)])
+def test_design_filter_matches_oracle(ctx, nel, rmin, frac):
+    """Design-domain filter (topopt_immersed.rs:117-151) on a random active set,
+    every mode against the oracle's rows."""
+    bounds = [[0, 1.2], [0, 0.6], [0, 0.3]]
+    g = icc.Grid(ctx, nel, bounds)
+    sp = O.Space(bounds, nel, 2)
+    rng = np.random.default_rng(11)
+    act = (rng.uniform(size=g.nel_tot) < frac).astype(np.uint8)
+    x = rng.uniform(0.001, 1.0, g.nel_tot)
+    dc = -rng.uniform(0.0, 5.0, g.nel_tot)
+    f = icc.Filter(g, rmin, active=act)
+    assert O.rel_err(f.sens(x, dc), sp.design_filter_apply(rmin, 0, act, x, dc)) < 1e-15
+    assert O.rel_err(f.fwd(x), sp.design_filter_apply(rmin, 1, act, None, x)) < 1e-15
+    assert O.rel_err(f.bwd(dc), sp.design_filter_apply(rmin, 2, act, None, dc)) < 1e-15
+    f.close()
+
+
+def test_design_filter_full_box_equals_structured(ctx):
+    """All cells active on cubic elements: the design-domain filter equals the
+    structured one (weights differ by the factor h only)."""
+    nel = [16, 8, 8]
+    g = icc.Grid(ctx, nel, [[0, 2.0], [0, 1.0], [0, 1.0]])
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0.001, 1.0, g.nel_tot)
+    v = rng.standard_normal(g.nel_tot)
+    a = icc.Filter(g, 1.5, active=np.ones(g.nel_tot, dtype=np.uint8))
+    b = icc.Filter(g, 1.5)
+    assert O.rel_err(a.sens(x, v), b.sens(x, v)) < 1e-14
+    assert O.rel_err(a.fwd(v), b.fwd(v)) < 1e-14
+    assert O.rel_err(a.bwd(v), b.bwd(v)) < 1e-14
+    with pytest.raises(ValueError):
+        icc.Filter(g, 1.5, active=np.ones(5, dtype=np.uint8))
+
+
 def _cantilever_problem(nel, seed, size=(2.0, 1.0, 1.0), rho=(0.01, 1.0), emin=1e-9):
     """Clamp x = 0, consistent downward unit traction on x = L; SIMP scale."""
     bounds = [[0, size[0]], [0, size[1]], [0, size[2]]]

@@ -2,7 +2,8 @@
 them): rigid-body null space and symmetry of K (iga_kernel.rs:50-92), the
 structured filter rows equal the O(nel^2) scan of topopt_iga.rs:109-125, the
 consistent traction integrates to t * area, the density filter is adjoint to
-its backward, the WQ operator equals Gauss at uniform density
+its backward (structured and design-domain variants), the design-domain
+filter reduces to the structured one on a full box of cubic cells, the WQ operator equals Gauss at uniform density
 (wq_vs_matlab.rs:35-51)."""
 import numpy as np
 
@@ -60,6 +61,51 @@ def test_density_filter_adjoint():
     fx = sp.filter_apply(1.5, 1, None, x)
     bg = sp.filter_apply(1.5, 2, None, g)
     assert abs(fx @ g - x @ bg) < 1e-12 * abs(fx @ g)
+
+
+def test_design_filter_all_active_equals_structured():
+    """With cubic elements and every cell active, the design-domain weights
+    r - |d h| are h times the structured weights rmin - |d| (the boundary
+    offsets weigh zero in both), so every normalised mode agrees."""
+    sp = O.Space([[0, 2.0], [0, 1.0], [0, 1.0]], [8, 4, 4], 2)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.001, 1.0, sp.nel)
+    v = rng.standard_normal(sp.nel)
+    act = np.ones(sp.nel, dtype=np.uint8)
+    for rmin in (1.0, 1.5, 2.3):
+        for mode in (0, 1, 2):
+            a = sp.design_filter_apply(rmin, mode, act, x, v)
+            b = sp.filter_apply(rmin, mode, x if mode == 0 else None, v)
+            assert O.rel_err(a, b) < 1e-14, (rmin, mode)
+
+
+def test_design_filter_support_and_adjoint():
+    """Inactive rows are empty, inactive cells never contribute, and the
+    backward mode is the adjoint of the forward mode."""
+    sp = O.Space([[0, 1.2], [0, 0.6], [0, 0.3]], [9, 6, 4], 2)
+    rng = np.random.default_rng(4)
+    act = (rng.uniform(size=sp.nel) < 0.7).astype(np.uint8)
+    x = rng.uniform(0.001, 1.0, sp.nel)
+    v = rng.standard_normal(sp.nel)
+    g = rng.standard_normal(sp.nel)
+    rmin = 1.7
+    outs = [sp.design_filter_apply(rmin, m, act, x, v) for m in (0, 1, 2)]
+    for o in outs:
+        assert np.all(o[act == 0] == 0.0)
+    v2 = v.copy()
+    v2[act == 0] = 1e6
+    x2 = x.copy()
+    x2[act == 0] = 0.5
+    for m, o in zip((0, 1, 2), outs):
+        assert np.array_equal(sp.design_filter_apply(rmin, m, act, x2, v2)[act == 1], o[act == 1])
+    # adjoint on the active subspace (inactive entries of g are ignored)
+    va = v * act
+    fx = sp.design_filter_apply(rmin, 1, act, None, va)
+    bg = sp.design_filter_apply(rmin, 2, act, None, g * act)
+    assert abs(fx @ (g * act) - va @ bg) < 1e-12 * abs(fx @ (g * act))
+    # the forward filter preserves constants on each active cell with a neighbour set
+    one = sp.design_filter_apply(rmin, 1, act, None, act.astype(np.float64))
+    assert np.max(np.abs(one[act == 1] - 1.0)) < 1e-14
 
 
 def test_traction_total_force():
