@@ -126,6 +126,47 @@ ICC_API int icc_ctx_sync(icc_ctx* ctx) {
 ICC_API void* icc_ctx_stream(icc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 // ---------------------------------------------------------------------------
+namespace icc {
+// Partitions the tile list for the persistent apply grid (wg_per_cu
+// workgroups per CU, a multiple of 8 so the XCD grouping applies, at most one
+// per tile): balanced work (balance_tiles) or equal whole-tile counts; then
+// uploads the tiles and the per-workgroup ranges.
+static int upload_tiles(icc_grid* g, bool balance) {
+  int G = g->wg_per_cu * (g->ctx->num_cu > 0 ? g->ctx->num_cu : 256);
+  G -= G % 8;
+  if (G > (int)g->s.tiles.size()) G = (int)g->s.tiles.size();
+  if (G < 1) G = 1;
+  std::vector<int32_t> first;
+  if (balance) {
+    balance_tiles(g->s, G, first);
+  } else {
+    first.resize(G + 1);
+    for (int b = 0; b <= G; ++b) first[b] = (int32_t)(((int64_t)b * (int64_t)g->s.tiles.size()) / G);
+  }
+  Tile* dt = nullptr;
+  int32_t* df = nullptr;
+  int rc;
+  if ((rc = dalloc(&dt, g->s.tiles.size())) || (rc = dalloc(&df, first.size()))) {
+    hipFree(dt);
+    return rc;
+  }
+  if (hipMemcpy(dt, g->s.tiles.data(), g->s.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(df, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(dt);
+    hipFree(df);
+    set_error("tile upload failed");
+    return ICC_EHIP;
+  }
+  hipFree(g->d_tiles);
+  hipFree(g->d_wgfirst);
+  g->d_tiles = dt;
+  g->d_wgfirst = df;
+  g->ntiles = (int)g->s.tiles.size();
+  g->apply_G = G;
+  return ICC_OK;
+}
+}  // namespace icc
+
 ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bounds[6], int64_t degree, double young,
                             double poisson, icc_grid** out) {
   if (!ctx || !nel || !bounds || !out) {
@@ -189,7 +230,7 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
   hipStream_t st = ctx->stream;
   if ((rc = dalloc(&g->d_kfrag64, nfrag)) || (rc = dalloc(&g->d_kfrag32, nfrag)) || (rc = dalloc(&g->d_kfragb, nfragb)) ||
       (rc = dalloc(&g->d_kdiag, kd.size())) || (rc = dalloc(&g->d_tid1, tid.size())) ||
-      (rc = dalloc(&g->d_typeof, s.type_of.size())) || (rc = dalloc(&g->d_tiles, s.tiles.size()))) {
+      (rc = dalloc(&g->d_typeof, s.type_of.size()))) {
     icc_grid_destroy(g);
     return rc;
   }
@@ -200,10 +241,13 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
       (e = hipMemcpyAsync(g->d_kdiag, kd.data(), kd.size() * 8, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(g->d_tid1, tid.data(), tid.size() * 4, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(g->d_typeof, s.type_of.data(), s.type_of.size() * 4, hipMemcpyHostToDevice, st)) ||
-      (e = hipMemcpyAsync(g->d_tiles, s.tiles.data(), s.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st)) ||
       (e = hipStreamSynchronize(st))) {
     icc_grid_destroy(g);
     return hip_fail(e, "icc_grid_create upload");
+  }
+  if ((rc = upload_tiles(g, true)) != ICC_OK) {
+    icc_grid_destroy(g);
+    return rc;
   }
   *out = g;
   return ICC_OK;
@@ -218,6 +262,7 @@ ICC_API int icc_grid_destroy(icc_grid* g) {
   hipFree(g->d_tid1);
   hipFree(g->d_typeof);
   hipFree(g->d_tiles);
+  hipFree(g->d_wgfirst);
   hipFree(g->d_mats1);
   hipFree(g->d_wq);
   delete g;
@@ -472,18 +517,9 @@ ICC_API int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu) {
   if (!g || variant < 0 || variant >= n_tile_variants() || wg_per_cu < 1 || wg_per_cu > 2) return ICC_EINVAL;
   ICC_HIP(hipStreamSynchronize(g->ctx->stream));
   build_tiles(g->s, variant);
-  Tile* d = nullptr;
-  ICC_TRY(dalloc(&d, g->s.tiles.size()));
-  if (hipMemcpy(d, g->s.tiles.data(), g->s.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice) != hipSuccess) {
-    hipFree(d);
-    set_error("icc_grid_set_tiling: upload failed");
-    return ICC_EHIP;
-  }
-  hipFree(g->d_tiles);
-  g->d_tiles = d;
-  g->ntiles = (int)g->s.tiles.size();
-  g->tile_variant = variant;
   g->wg_per_cu = wg_per_cu;
+  ICC_TRY(upload_tiles(g, variant < 6));
+  g->tile_variant = variant;
   for (icc_op* op : g->ops)
     if (op->d_free) ICC_TRY(upload_tile_mask(op));
   return ICC_OK;

@@ -78,8 +78,16 @@ struct GridSetup {
 };
 
 int build_setup(const int64_t nel[3], const double bounds[6], double young, double poisson, GridSetup& s);
-// (Re)builds s.tiles with tiling variant v (0 .. n_tile_variants() - 1).
+// (Re)builds s.tiles with tiling variant v (0 .. n_tile_variants() - 1; the
+// shape order is v % 6, and v < 6 asks for the balanced partition).
 void build_tiles(GridSetup& s, int variant);
+// Re-partitions s.tiles for G persistent workgroups: walks the tiles in
+// order, giving workgroup b a contiguous run that ends as close as possible
+// to the cumulative target (b + 1) W / G, where W is the total work; the tile
+// that crosses a target may be split along z (else y).  Falls back to equal
+// whole-tile counts when that partition is no worse.  first[b] .. first[b + 1]
+// are workgroup b's tiles (G + 1 entries).
+void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first);
 int n_tile_variants();
 int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
 
@@ -126,6 +134,8 @@ struct icc_grid {
   int T[3] = {0, 0, 0};
   int tile_variant = 0;  // build_tiles variant of s.tiles
   int wg_per_cu = 2;     // persistent apply workgroups per CU
+  int apply_G = 1;       // persistent apply workgroups
+  int32_t* d_wgfirst = nullptr;  // [apply_G + 1] tile range of each workgroup (balance_tiles)
   std::vector<struct icc_op*> ops;  // operators on this grid (their tile masks follow the tiling)
 };
 

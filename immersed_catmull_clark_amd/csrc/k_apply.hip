@@ -178,14 +178,15 @@ __device__ __forceinline__ void offset_table(const TileGeom& G, int* tab) {
 }
 constexpr int NPF = (3 * TILE_MAX_PTS + APPLY_THREADS - 1) / APPLY_THREADS;  // staged slots per thread
 
-// Tiles [first, last) of workgroup b among G: contiguous chunks, the chunks of
-// the workgroups that share an XCD (b mod 8) adjacent, so that neighbouring
-// tiles and their shared halo points meet in one L2.
-__device__ __forceinline__ void tile_range(int ntiles, int& first, int& last) {
+// Tiles [first, last) of workgroup b among G: chunk c of the host's balanced
+// partition (wg_first, G + 1 entries), the chunks of the workgroups that share
+// an XCD (b mod 8) adjacent, so that neighbouring tiles and their shared halo
+// points meet in one L2.
+__device__ __forceinline__ void tile_range(const int* __restrict__ wg_first, int& first, int& last) {
   const int G = gridDim.x, b = blockIdx.x;
   const int c = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
-  first = (int)(((long long)c * ntiles) / G);
-  last = (int)(((long long)(c + 1) * ntiles) / G);
+  first = wg_first[c];
+  last = wg_first[c + 1];
 }
 
 // y += M * sum_e s_e K_t x_e (M x); optional per-workgroup partials of
@@ -211,7 +212,8 @@ __device__ __forceinline__ void tile_range(int ntiles, int& first, int& last) {
 // the FP32 mode.
 template <typename T, bool BF = false>
 __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void apply_tiles(
-    const Tile* __restrict__ tiles, int ntiles, const void* __restrict__ kfrag_v, const T* __restrict__ x,
+    const Tile* __restrict__ tiles, const int* __restrict__ wg_first, const void* __restrict__ kfrag_v,
+    const T* __restrict__ x,
     T* __restrict__ y, const T* __restrict__ scale, const uint8_t* __restrict__ freem,
     const uint8_t* __restrict__ tmask, int ncp0, int ncp1, int ndof_sc, int nel0, int nel1,
     double* __restrict__ pap, const PcgState* st) {
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
   __shared__ double red[6];
   if (st != nullptr && st->done != 0) return;  // solver already finished
   int first, last;
-  tile_range(ntiles, first, last);
+  tile_range(wg_first, first, last);
   if (first >= last) {
     if (pap != nullptr && threadIdx.x == 0) pap[blockIdx.x] = 0.0;
     return;
@@ -600,19 +602,14 @@ __global__ void mfma_layout_probe_bf16(double* out) {
 
 // ---- launchers ---------------------------------------------------------------
 // Persistent apply grid: wg_per_cu workgroups per CU (default two; the VGPR
-// budget of the MFMA loop allows twelve waves per CU), a multiple of 8 so the
-// XCD grouping of tile_range applies, never more workgroups than tiles.
-int apply_grid(const icc_grid* g) {
-  int G = g->wg_per_cu * (g->ctx->num_cu > 0 ? g->ctx->num_cu : 256);
-  G -= G % 8;
-  if (G > g->ntiles) G = g->ntiles;
-  return G < 1 ? 1 : G;
-}
+// budget of the MFMA loop allows twelve waves per CU), the size the tile list
+// was balanced for.
+int apply_grid(const icc_grid* g) { return g->apply_G; }
 
 hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x, double* y, double* pap,
                             const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
-  hipLaunchKernelGGL((apply_tiles<double>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->ntiles,
+  hipLaunchKernelGGL((apply_tiles<double>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->d_wgfirst,
                      (const void*)g->d_kfrag64, x, y,
                      op->d_scale, op->d_free, op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc,
                      (int)s.nel[0], (int)s.nel[1], pap, pst);
@@ -622,7 +619,7 @@ hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x
 hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
                             const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
-  hipLaunchKernelGGL((apply_tiles<float>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->ntiles,
+  hipLaunchKernelGGL((apply_tiles<float>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->d_wgfirst,
                      (const void*)g->d_kfrag32, x, y,
                      op->d_scale32, op->d_free, op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc,
                      (int)s.nel[0], (int)s.nel[1], pap, pst);
@@ -633,7 +630,7 @@ hipError_t launch_apply_bf16(const icc_grid* g, const icc_op* op, const float* x
                              const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
   hipLaunchKernelGGL((apply_tiles<float, true>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles,
-                     g->ntiles, (const void*)g->d_kfragb, x, y, op->d_scale32, op->d_free, op->d_tmask,
+                     g->d_wgfirst, (const void*)g->d_kfragb, x, y, op->d_scale32, op->d_free, op->d_tmask,
                      (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0], (int)s.nel[1], pap, pst);
   return hipGetLastError();
 }

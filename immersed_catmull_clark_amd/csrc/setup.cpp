@@ -277,8 +277,9 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
 // are cut into x-line tiles: lines of <= 16 elements spread evenly, and up to
 // a variant-dependent number of lines per tile (y x z candidates tried in the
 // variant's order, the first one that fills the most lines within the
-// staging limits wins).  Shorter sub-boxes use tile_shape.
-int n_tile_variants() { return 6; }
+// staging limits wins).  Shorter sub-boxes use tile_shape.  Variants 6 .. 11
+// repeat the shapes of 0 .. 5 (the caller partitions them differently).
+int n_tile_variants() { return 12; }
 
 void build_tiles(GridSetup& s, int variant) {
   static const int yz_orders[6][10][2] = {
@@ -290,6 +291,7 @@ void build_tiles(GridSetup& s, int variant) {
       {{1, 4}, {4, 1}, {2, 2}, {1, 2}, {2, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}},
   };
   if (variant < 0 || variant >= n_tile_variants()) variant = 0;
+  variant %= 6;
   // Constant-type sub-boxes: runs of equal 1D type along each direction.
   std::vector<std::pair<int64_t, int64_t>> runs[3];
   for (int d = 0; d < 3; ++d) {
@@ -342,6 +344,137 @@ void build_tiles(GridSetup& s, int variant) {
               s.tiles.push_back(t);
             }
       }
+}
+
+// MFMA work of a tile in element-group pairs (each wave runs its element
+// groups two at a time), plus the staging and flush of its points, which
+// cost about a pair per 300 staged points.
+static double tile_cost(const Tile& t) {
+  const int nel = t.nx * t.ny * t.nz;
+  const int groups = t.lines ? t.ny * t.nz : (nel + 15) / 16;
+  const int pts = (t.nx + 2) * (t.ny + 2) * (t.nz + 2);
+  return (double)((groups + 1) / 2) + pts / 300.0;
+}
+
+// One pass of balance_tiles for an assumed total work `total`: workgroup b
+// fills up to the cumulative target (b + 1) total / G, and the tile that
+// crosses it is taken whole, left for the next workgroup, or split at the
+// leading slab, whichever ends closest to the target.
+static double balance_pass(const std::vector<Tile>& in, int G, double total, std::vector<Tile>& out,
+                           std::vector<int32_t>& first) {
+  out.clear();
+  out.reserve(in.size() + (size_t)G);
+  first.assign(1, 0);
+  size_t i = 0;
+  Tile cur{};
+  bool pending = false;  // cur = the remainder of a split tile, next in line
+  double cum = 0.0;
+  auto advance = [&]() {
+    if (pending)
+      pending = false;
+    else
+      ++i;
+  };
+  for (int b = 0; b < G; ++b) {
+    const double target = (double)(b + 1) * total / (double)G;
+    while (pending || i < in.size()) {
+      const Tile t = pending ? cur : in[i];
+      const double c = tile_cost(t);
+      if (b == G - 1 || cum + c <= target + 1e-9) {
+        out.push_back(t);
+        cum += c;
+        advance();
+        continue;
+      }
+      // t crosses the target: 0 = leave it, 1 = take it, 2 = split it
+      int choice = 0;
+      double best = target - cum;
+      if (cum + c - target < best) {
+        choice = 1;
+        best = cum + c - target;
+      }
+      Tile a_best = t, r_best = t;
+      const bool along_z = t.nz > 1;
+      const int n = along_z ? t.nz : t.ny;
+      for (int k = 1; k < n; ++k) {
+        Tile a = t, r = t;
+        if (along_z) {
+          a.nz = k;
+          r.k0 += k;
+          r.nz -= k;
+        } else {
+          a.ny = k;
+          r.j0 += k;
+          r.ny -= k;
+        }
+        const double d = std::fabs(cum + tile_cost(a) - target);
+        if (d < best) {
+          best = d;
+          choice = 2;
+          a_best = a;
+          r_best = r;
+        }
+      }
+      if (choice == 1) {
+        out.push_back(t);
+        cum += c;
+        advance();
+      } else if (choice == 2) {
+        out.push_back(a_best);
+        cum += tile_cost(a_best);
+        if (!pending) ++i;
+        cur = r_best;
+        pending = true;
+      }
+      break;
+    }
+    first.push_back((int32_t)out.size());
+  }
+  return cum;
+}
+
+void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first) {
+  const std::vector<Tile> in = s.tiles;
+  double total = 0.0;
+  for (const Tile& t : in) total += tile_cost(t);
+  // splitting adds staging work, which lands on the last workgroup: re-run
+  // with the total the previous pass found and keep the partition with the
+  // smallest maximum
+  std::vector<Tile> out, best_out;
+  std::vector<int32_t> fst, best_first;
+  double best_max = 0.0;
+  for (int it = 0; it < 8; ++it) {
+    const double found = balance_pass(in, G, total, out, fst);
+    double mx = 0.0;
+    for (int b = 0; b < G; ++b) {
+      double w = 0.0;
+      for (int32_t j = fst[b]; j < fst[b + 1]; ++j) w += tile_cost(out[j]);
+      mx = std::max(mx, w);
+    }
+    if (it == 0 || mx < best_max) {
+      best_max = mx;
+      best_out = out;
+      best_first = fst;
+    }
+    if (found <= total) break;
+    total = found;
+  }
+  // the plain partition (equal tile counts, no splits) when it is no worse
+  double plain_max = 0.0;
+  std::vector<int32_t> plain(G + 1);
+  for (int b = 0; b <= G; ++b) plain[b] = (int32_t)(((int64_t)b * (int64_t)in.size()) / G);
+  for (int b = 0; b < G; ++b) {
+    double w = 0.0;
+    for (int32_t j = plain[b]; j < plain[b + 1]; ++j) w += tile_cost(in[j]);
+    plain_max = std::max(plain_max, w);
+  }
+  if (plain_max <= best_max) {
+    s.tiles = in;
+    first = std::move(plain);
+    return;
+  }
+  s.tiles = std::move(best_out);
+  first = std::move(best_first);
 }
 
 // Consistent traction on the face x = x_max: F(A, c) = t_c * int N_A dA, the

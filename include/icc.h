@@ -75,8 +75,11 @@ int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out);
 
 /* ---- tiling (no counterpart in the reference) ---------------------------------- */
 /* The apply walks the grid in tiles of constant element type.  variant selects
- * the x-line tile shape order (0 .. 5, 0 = default), wg_per_cu the persistent
- * workgroups per CU (1 or 2).  Operators on the grid follow the new tiling. */
+ * the x-line tile shape order (variant % 6) and the partition among the
+ * persistent workgroups: 0 .. 5 balance the work per workgroup (tiles split
+ * where needed), 6 .. 11 give each workgroup an equal number of whole tiles;
+ * 0 is the default.  wg_per_cu: persistent workgroups per CU (1 or 2).
+ * Operators on the grid follow the new tiling. */
 int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu);
 /* out = variant, workgroups per CU, tile count */
 int icc_grid_tiling(const icc_grid* g, int64_t out[3]);

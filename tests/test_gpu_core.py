@@ -334,7 +334,7 @@ def test_tiling_variants_and_autotune(ctx):
     op = icc.Operator(g, scale=s, free=free)
     x = np.random.default_rng(5).standard_normal(g.ndof)
     ref = free * sp.matvec(free * x)
-    for v in range(6):
+    for v in range(12):
         for wg in (1, 2):
             g.set_tiling(v, wg)
             assert g.tiling[:2] == (v, wg)
@@ -343,3 +343,27 @@ def test_tiling_variants_and_autotune(ctx):
     v, wg, us = op.autotune(icc.ICC_F64, reps=3)
     assert g.tiling[:2] == (v, wg) and us > 0
     assert O.rel_err(op.apply(x), ref) < 1e-13
+
+
+def test_balanced_partition_splits_tiles_and_keeps_the_operator(ctx):
+    """On the 96x48x48 grid the work-balanced partition (variants 0..5) splits
+    tiles; every partition gives the same operator (FP64 to round-off of the
+    scatter order; FP32 and BF16 within their rounding) and the same p.Ap."""
+    nel = (96, 48, 48)
+    g = icc.Grid(ctx, nel, [[0, 2.0], [0, 1.0], [0, 1.0]])
+    rng = np.random.default_rng(21)
+    s = 1e-9 + (1 - 1e-9) * rng.uniform(0.01, 1.0, g.nel_tot) ** 3
+    op = icc.Operator(g, scale=s)
+    x = rng.standard_normal(g.ndof)
+    g.set_tiling(6, 2)
+    plain_tiles = g.tiling[2]
+    ref = op.apply(x)
+    ref32 = op.apply(x, prec=icc.ICC_F32)
+    refb = op.apply(x, prec=icc.ICC_BF16)
+    for v in (0, 1, 4):
+        g.set_tiling(v, 2)
+        if v == 0:
+            assert g.tiling[2] > plain_tiles  # tiles were split
+        assert O.rel_err(op.apply(x), ref) < 1e-14, v
+        assert O.rel_err(op.apply(x, prec=icc.ICC_F32), ref32) < 1e-6, v
+        assert O.rel_err(op.apply(x, prec=icc.ICC_BF16), refb) < 1e-5, v
