@@ -104,7 +104,7 @@ class Grid:
     def set_tiling(self, variant, wg_per_cu=2):
         """Tile shape order variant % 6; variants 0..5 balance the work per
         persistent workgroup (splitting tiles), 6..11 give every workgroup an
-        equal number of whole tiles.  wg_per_cu: 1 or 2."""
+        equal number of whole tiles.  wg_per_cu: 1, 2 or 4."""
         check(lib().icc_grid_set_tiling(self.h, int(variant), int(wg_per_cu)))
 
     @property

@@ -127,10 +127,10 @@ ICC_API void* icc_ctx_stream(icc_ctx* ctx) { return ctx ? (void*)ctx->stream : n
 
 // ---------------------------------------------------------------------------
 namespace icc {
-// Partitions the tile list for the persistent apply grid (wg_per_cu
-// workgroups per CU, a multiple of 8 so the XCD grouping applies, at most one
-// per tile): balanced work (balance_tiles) or equal whole-tile counts; then
-// uploads the tiles and the per-workgroup ranges.
+// Partitions the tile list for the apply grid (wg_per_cu workgroups per CU,
+// a multiple of 8 so the XCD grouping applies, at most one per tile):
+// balanced work (balance_tiles) or equal whole-tile counts; then uploads the
+// tiles and the per-workgroup ranges.
 static int upload_tiles(icc_grid* g, bool balance) {
   int G = g->wg_per_cu * (g->ctx->num_cu > 0 ? g->ctx->num_cu : 256);
   G -= G % 8;
@@ -399,7 +399,8 @@ static int ensure_work(icc_op* op, Work*& out) {
     free_work(*w);
     w->n = n;
     int rc;
-    size_t npap = (size_t)2 * (op->g->ctx->num_cu > 0 ? op->g->ctx->num_cu : 256) + VEC_MAX_BLOCKS + FACE_MAX_BLOCKS;
+    size_t npap = (size_t)APPLY_MAX_WG_PER_CU * (op->g->ctx->num_cu > 0 ? op->g->ctx->num_cu : 256) + VEC_MAX_BLOCKS +
+                  FACE_MAX_BLOCKS;
     if ((rc = dalloc(&w->x, n)) || (rc = dalloc(&w->r, n)) || (rc = dalloc(&w->p, n)) || (rc = dalloc(&w->ap, n)) ||
         (rc = dalloc(&w->dinv, n)) || (rc = dalloc(&w->tmp, n)) || (rc = dalloc(&w->f, n)) ||
         (rc = dalloc(&w->x32, n)) || (rc = dalloc(&w->r32, n)) || (rc = dalloc(&w->p32, n)) ||
@@ -514,7 +515,8 @@ ICC_API int icc_op_set_free_mask(icc_op* op, const uint8_t* fm) {
 }
 
 ICC_API int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu) {
-  if (!g || variant < 0 || variant >= n_tile_variants() || wg_per_cu < 1 || wg_per_cu > 2) return ICC_EINVAL;
+  if (!g || variant < 0 || variant >= n_tile_variants() || (wg_per_cu != 1 && wg_per_cu != 2 && wg_per_cu != 4))
+    return ICC_EINVAL;
   ICC_HIP(hipStreamSynchronize(g->ctx->stream));
   build_tiles(g->s, variant);
   g->wg_per_cu = wg_per_cu;
@@ -1630,7 +1632,7 @@ ICC_API int icc_grid_autotune(icc_op* op, int prec, int reps, int64_t* best_vari
   float bt = 1e30f;
   int rc = ICC_OK;
   for (int v = 0; v < n_tile_variants() && rc == ICC_OK; ++v)
-    for (int wg = 1; wg <= 2 && rc == ICC_OK; ++wg) {
+    for (int wg = 1; wg <= APPLY_MAX_WG_PER_CU && rc == ICC_OK; wg *= 2) {
       if ((rc = icc_grid_set_tiling(g, v, wg)) != ICC_OK) break;
       hipError_t e = hipSuccess;
       for (int r = -2; r < reps && e == hipSuccess; ++r) {

@@ -16,6 +16,7 @@ constexpr int MT = 6;        // 16-row M tiles covering 81 test rows (96 padded)
 constexpr int KS = 21;       // 4-deep K steps covering 81 trial columns (84 padded)
 constexpr int TILE_MAX_EL = 128;   // elements per tile (8 waves x 16)
 constexpr int TILE_MAX_PTS = 640;  // control points staged per tile
+constexpr int APPLY_MAX_WG_PER_CU = 4;  // apply workgroups per CU, at most
 
 // Tile of a constant-type element sub-box: elements [i0, i0+nx) x [j0, j0+ny) x [k0, k0+nz).
 // lines = 1: the tile is processed as x-lines of nx <= 16 elements (one MFMA
@@ -133,7 +134,7 @@ struct icc_grid {
   int maxnel = 0;
   int T[3] = {0, 0, 0};
   int tile_variant = 0;  // build_tiles variant of s.tiles
-  int wg_per_cu = 2;     // persistent apply workgroups per CU
+  int wg_per_cu = 2;     // apply workgroups per CU (1 .. APPLY_MAX_WG_PER_CU)
   int apply_G = 1;       // persistent apply workgroups
   int32_t* d_wgfirst = nullptr;  // [apply_G + 1] tile range of each workgroup (balance_tiles)
   std::vector<struct icc_op*> ops;  // operators on this grid (their tile masks follow the tiling)

@@ -3,11 +3,13 @@
 //
 // Work decomposition.  The element grid is cut into tiles of <= 128 elements
 // that never straddle an element-type boundary (host: setup.cpp), so every
-// element of a tile shares one 81x81 matrix K_t.  One workgroup = one tile,
-// six waves, wave w owning the 16-row M tile w of K_t (rows 16w..16w+15; the
-// sixth tile holds row 80 only).  A wave keeps its 21 A fragments (K rows x
-// 4-deep K steps, pre-permuted on the host into lane order) in VGPRs and
-// streams the tile's elements 16 at a time as the MFMA N dimension:
+// element of a tile shares one 81x81 matrix K_t.  A workgroup walks a run of
+// tiles with twelve waves; K_t's rows form six 16-row M tiles (the sixth holds
+// row 80 only), and waves w and w + 6 own M tile w, each for half of the
+// tile's element groups.  A wave keeps its 21
+// A fragments (K rows x 4-deep K steps, pre-permuted on the host into lane
+// order) in VGPRs and streams the tile's elements 16 at a time as the MFMA N
+// dimension:
 //     D[16 x 16] += A[16 x 4] * B[4 x 16],  B[k][j] = x_local(element j, trial k)
 // with the B operand gathered from the LDS copy of the tile's control points
 // (x masked by the free mask).  The 4 accumulator rows per lane are scaled by
@@ -137,7 +139,31 @@ struct TileGeom {
   }
 };
 
-constexpr int APPLY_THREADS = 384;
+// Waves per workgroup W, with wave w on SIMD w mod 4 (the dispatch order):
+//   W = 12 (FP64): two waves per M tile, wave w owns M tile w mod 6 over the
+//     first (w < 6) or second half of the tile's element-group pairs; every
+//     wave carries the same MFMA work, every SIMD three waves of it;
+//   W = 8 (FP32, BF16): waves 0..3 own M tiles 0..3 over all pairs, waves
+//     4..7 own M tiles 4 and 5 (4, 6: tile 4; 5, 7: tile 5) over half of them;
+//     every SIMD carries 1.5 M tiles of work, and at <= 128 VGPRs two
+//     workgroups share a CU.
+// Either way no SIMD holds up the tile's barrier.
+template <int W>
+__device__ __forceinline__ int apply_mtile(int wave) {
+  return W == 12 ? wave % 6 : (wave < 4 ? wave : 4 + (wave & 1));
+}
+// element-group pairs [lo, hi) of a wave among npairs
+template <int W>
+__device__ __forceinline__ void apply_pairs(int wave, int npairs, int& lo, int& hi) {
+  const int half = (npairs + 1) / 2;
+  if (W == 12) {
+    lo = wave < 6 ? 0 : half;
+    hi = wave < 6 ? half : npairs;
+  } else {
+    lo = wave < 6 ? 0 : half;
+    hi = (wave < 4 || wave >= 6) ? npairs : half;
+  }
+}
 constexpr int BCH = 4;                        // K steps per B-operand chunk
 constexpr int NBCH = (KS + BCH - 1) / BCH;    // chunks per element group
 // BF16 mode (v_mfma_f32_16x16x32_bf16): lane l holds A[l%16][8 (l/16) + j]
@@ -176,7 +202,11 @@ __device__ __forceinline__ void offset_table(const TileGeom& G, int* tab) {
     tab[i] = row >= 0 ? local_offset(row, G.px, G.py, G.npts) : 0;
   }
 }
-constexpr int NPF = (3 * TILE_MAX_PTS + APPLY_THREADS - 1) / APPLY_THREADS;  // staged slots per thread
+// staged slots per thread of a W-wave workgroup
+template <int W>
+struct Npf {
+  static constexpr int value = (3 * TILE_MAX_PTS + 64 * W - 1) / (64 * W);
+};
 
 // Tiles [first, last) of workgroup b among G: chunk c of the host's balanced
 // partition (wg_first, G + 1 entries), the chunks of the workgroups that share
@@ -210,8 +240,8 @@ __device__ __forceinline__ void tile_range(const int* __restrict__ wg_first, int
 // BF = true (T = float): K and the staged x are rounded to BF16 for
 // v_mfma_f32_16x16x32_bf16 (3 K steps of 32), FP32 accumulation; the rest as
 // the FP32 mode.
-template <typename T, bool BF = false>
-__global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void apply_tiles(
+template <typename T, bool BF, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 ? 3 : 4))) void apply_tiles(
     const Tile* __restrict__ tiles, const int* __restrict__ wg_first, const void* __restrict__ kfrag_v,
     const T* __restrict__ x,
     T* __restrict__ y, const T* __restrict__ scale, const uint8_t* __restrict__ freem,
@@ -224,7 +254,9 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
   __shared__ T ss[2][TILE_MAX_EL];
   __shared__ int otab[2][OTAB_MAX];  // per tile: LDS offsets of the B operands and of the output rows
   constexpr int NTR = Otab<BF>::ntrial;
-  __shared__ double red[6];
+  __shared__ double red[W];
+  constexpr int NT = 64 * W;
+  constexpr int NPF = Npf<W>::value;
   if (st != nullptr && st->done != 0) return;  // solver already finished
   int first, last;
   tile_range(wg_first, first, last);
@@ -235,6 +267,7 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
   double pap_blk = 0.0;  // thread 0: this workgroup's p.Ap partial, tiles in order
 
   const int wave = threadIdx.x >> 6;
+  const int mt = apply_mtile<W>(wave);
   const int lane = threadIdx.x & 63;
   const int col = lane & 15;
   const int kq = lane >> 4;
@@ -245,7 +278,7 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
   G.init(t);
   {
     const bool masked = tmask != nullptr && tmask[first] != 0;
-    for (int idx = threadIdx.x; idx < 3 * TILE_MAX_PTS; idx += APPLY_THREADS) {
+    for (int idx = threadIdx.x; idx < 3 * TILE_MAX_PTS; idx += NT) {
       ys[idx] = 0.0;
       if (idx < G.n3) {
         const int g = G.dof(t, idx, ncp0, ncp1, ndof_sc);
@@ -268,13 +301,13 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
     // tiles whose staged points include fixed DOFs (few) read the free mask
     const bool mask_c = tmask != nullptr && tmask[ti] != 0;
     const bool mask_n = tmask != nullptr && has_next && tmask[ti + 1] != 0;
-    if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile
+    if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile mt
       if constexpr (BF) {
-        const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + wave) * KSB) * 64 + lane;
+        const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + mt) * KSB) * 64 + lane;
 #pragma unroll
         for (int ks = 0; ks < KSB; ++ks) ab[ks] = kf[ks * 64];
       } else {
-        const T* kf = (const T*)kfrag_v + ((size_t)(t.type * MT + wave) * KS) * 64 + lane;
+        const T* kf = (const T*)kfrag_v + ((size_t)(t.type * MT + mt) * KS) * 64 + lane;
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) a[ks] = kf[ks * 64];
       }
@@ -285,13 +318,13 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
     T pv[NPF];
 #pragma unroll
     for (int j = 0; j < NPF; ++j) {
-      const int idx = threadIdx.x + j * APPLY_THREADS;
+      const int idx = threadIdx.x + j * NT;
       pv[j] = x[Gn.dof(tn, idx < Gn.n3 ? idx : 0, ncp0, ncp1, ndof_sc)];
     }
     const T psc = scale[Gn.elem(tn, threadIdx.x < Gn.nel ? threadIdx.x : 0, nel0, nel1)];
 
     const int px = G.px, py = G.py, nel = G.nel;
-    const bool r3 = perm_row(wave, kq, 3) >= 0;
+    const bool r3 = perm_row(mt, kq, 3) >= 0;
     const T* xb = xs[buf];
     const T* sb = ss[buf];
     __syncthreads();  // xs[buf], ss[buf], otab[buf] staged; ys clear
@@ -301,11 +334,13 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
     for (int m = 0; m < NB; ++m) boff[m] = otab[buf][kq * NB + m];
     int roff[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) roff[r] = otab[buf][NTR + (wave * 4 + kq) * 4 + r];
+    for (int r = 0; r < 4; ++r) roff[r] = otab[buf][NTR + (mt * 4 + kq) * 4 + r];
 
     const bool lines = t.lines != 0;
     const int ngroups = lines ? t.ny * t.nz : (nel + 15) / 16;
-    for (int g0 = 0; g0 < ngroups; g0 += 2) {
+    int p_lo, p_hi;
+    apply_pairs<W>(wave, (ngroups + 1) / 2, p_lo, p_hi);
+    for (int g0 = 2 * p_lo; g0 < 2 * p_hi; g0 += 2) {
       int eb[2];
       bool valid[2];
       T sc[2];
@@ -420,7 +455,7 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
     double pap_acc = 0.0;
 #pragma unroll
     for (int j = 0; j < NPF; ++j) {
-      const int idx = threadIdx.x + j * APPLY_THREADS;
+      const int idx = threadIdx.x + j * NT;
       if (idx < G.n3) {
         const double v = ys[idx];
         ys[idx] = 0.0;
@@ -434,14 +469,14 @@ __global__ __launch_bounds__(APPLY_THREADS) __attribute__((amdgpu_waves_per_eu(3
     if (has_next && threadIdx.x < Gn.nel) ss[buf ^ 1][threadIdx.x] = psc;
     if (has_next) offset_table<BF>(Gn, otab[buf ^ 1]);
     if (pap != nullptr) {
-      // wave sums, then the six waves in fixed order (red is free: the last
+      // wave sums, then the waves in fixed order (red is free: the last
       // reader finished before the barrier above)
       for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
       if (lane == 0) red[wave] = pap_acc;
       __syncthreads();
       if (threadIdx.x == 0) {
         double sum = 0.0;
-        for (int w = 0; w < 6; ++w) sum += red[w];
+        for (int w = 0; w < W; ++w) sum += red[w];
         pap_blk += sum;
         if (!has_next) pap[blockIdx.x] = pap_blk;
       }
@@ -601,15 +636,16 @@ __global__ void mfma_layout_probe_bf16(double* out) {
 }
 
 // ---- launchers ---------------------------------------------------------------
-// Persistent apply grid: wg_per_cu workgroups per CU (default two; the VGPR
-// budget of the MFMA loop allows twelve waves per CU), the size the tile list
-// was balanced for.
+// Apply grid: wg_per_cu workgroups per CU (1, 2 or 4; the size the tile list
+// was partitioned for).  Resident at once: one FP64 workgroup (twelve waves at
+// the MFMA loop's VGPR budget) or two FP32 / BF16 ones per CU; the others
+// start as those finish, which the hardware dispatcher balances.
 int apply_grid(const icc_grid* g) { return g->apply_G; }
 
 hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x, double* y, double* pap,
                             const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
-  hipLaunchKernelGGL((apply_tiles<double>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->d_wgfirst,
+  hipLaunchKernelGGL((apply_tiles<double, false, 12>), dim3(apply_grid(g)), dim3(64 * 12), 0, st, g->d_tiles, g->d_wgfirst,
                      (const void*)g->d_kfrag64, x, y,
                      op->d_scale, op->d_free, op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc,
                      (int)s.nel[0], (int)s.nel[1], pap, pst);
@@ -619,7 +655,7 @@ hipError_t launch_apply_f64(const icc_grid* g, const icc_op* op, const double* x
 hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
                             const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
-  hipLaunchKernelGGL((apply_tiles<float>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles, g->d_wgfirst,
+  hipLaunchKernelGGL((apply_tiles<float, false, 8>), dim3(apply_grid(g)), dim3(64 * 8), 0, st, g->d_tiles, g->d_wgfirst,
                      (const void*)g->d_kfrag32, x, y,
                      op->d_scale32, op->d_free, op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc,
                      (int)s.nel[0], (int)s.nel[1], pap, pst);
@@ -629,7 +665,7 @@ hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x,
 hipError_t launch_apply_bf16(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
                              const PcgState* pst, hipStream_t st) {
   const auto& s = g->s;
-  hipLaunchKernelGGL((apply_tiles<float, true>), dim3(apply_grid(g)), dim3(APPLY_THREADS), 0, st, g->d_tiles,
+  hipLaunchKernelGGL((apply_tiles<float, true, 8>), dim3(apply_grid(g)), dim3(64 * 8), 0, st, g->d_tiles,
                      g->d_wgfirst, (const void*)g->d_kfragb, x, y, op->d_scale32, op->d_free, op->d_tmask,
                      (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0], (int)s.nel[1], pap, pst);
   return hipGetLastError();

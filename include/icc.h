@@ -78,7 +78,7 @@ int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out);
  * the x-line tile shape order (variant % 6) and the partition among the
  * persistent workgroups: 0 .. 5 balance the work per workgroup (tiles split
  * where needed), 6 .. 11 give each workgroup an equal number of whole tiles;
- * 0 is the default.  wg_per_cu: persistent workgroups per CU (1 or 2).
+ * 0 is the default.  wg_per_cu: apply workgroups per CU (1, 2 or 4).
  * Operators on the grid follow the new tiling. */
 int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu);
 /* out = variant, workgroups per CU, tile count */

@@ -335,7 +335,7 @@ def test_tiling_variants_and_autotune(ctx):
     x = np.random.default_rng(5).standard_normal(g.ndof)
     ref = free * sp.matvec(free * x)
     for v in range(12):
-        for wg in (1, 2):
+        for wg in (1, 2, 4):
             g.set_tiling(v, wg)
             assert g.tiling[:2] == (v, wg)
             assert O.rel_err(op.apply(x), ref) < 1e-13, (v, wg)
