@@ -438,6 +438,9 @@ def main():
                     help="skip the immersed, batched and adjoint configuration legs")
     ap.add_argument("--apply-only", action="store_true", help="time only the apply (for profilers)")
     ap.add_argument("--apply-prec", choices=("f64", "f32", "bf16"), default="f64", help="precision of --apply-only")
+    ap.add_argument("--tiling", default=None, metavar="V,W",
+                    help="fixed tiling variant and workgroups per CU for the apply legs (skips the autotuner; "
+                         "for profiling the configuration a default run picked)")
     ap.add_argument("--wq-only", action="store_true",
                     help="run only the WQ formation leg on the 96x48x48 grid and exit")
     ap.add_argument("--imm-only", action="store_true",
@@ -494,10 +497,16 @@ def main():
 
     flops = 2.0 * 81 * 81 * nel
     # tile shape and workgroups per CU picked by timing (FP64 apply first)
-    tuned = {"f64": op.autotune(icc.ICC_F64, reps=5)}
+    fixed = tuple(int(a) for a in args.tiling.split(",")) if args.tiling else None
+    if fixed:
+        grid.set_tiling(*fixed)
+        tuned = {"f64": (fixed[0], fixed[1], None)}
+    else:
+        tuned = {"f64": op.autotune(icc.ICC_F64, reps=5)}
     if args.apply_only and args.apply_prec in ("f32", "bf16"):
         pr = icc.ICC_F32 if args.apply_prec == "f32" else icc.ICC_BF16
-        op.autotune(pr, reps=5)
+        if not fixed:
+            op.autotune(pr, reps=5)
         ms32 = time_applies(torch, icc, ctx, op, x32, y32, pr, args.steps, args.warmup, accumulate=True)
         print(json.dumps({"kernel": "apply_tiles<float%s>" % ("" if pr == icc.ICC_F32 else ", true"),
                           "kernel_ms": ms32, "tflops": flops / (ms32 * 1e-3) / 1e12}), flush=True)
