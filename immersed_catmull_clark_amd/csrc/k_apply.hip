@@ -293,14 +293,18 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   bf8_t ab[BF ? KSB : 1];
   int cur_type = -1;
   int buf = 0;
+  // descriptor and mask flag of the next tile, loaded one tile ahead so that
+  // the prefetch below never waits for them (tiles whose staged points
+  // include fixed DOFs, few, read the free mask)
+  Tile tn = tiles[first + 1 < last ? first + 1 : first];
+  bool mask_c = tmask != nullptr && tmask[first] != 0;
+  bool mask_n = tmask != nullptr && first + 1 < last && tmask[first + 1] != 0;
   for (int ti = first; ti < last; ++ti) {
     const bool has_next = ti + 1 < last;
-    const Tile tn = tiles[has_next ? ti + 1 : ti];
+    const Tile tnn = tiles[ti + 2 < last ? ti + 2 : ti];
+    const bool mask_nn = tmask != nullptr && ti + 2 < last && tmask[ti + 2] != 0;
     TileGeom Gn;
     Gn.init(tn);
-    // tiles whose staged points include fixed DOFs (few) read the free mask
-    const bool mask_c = tmask != nullptr && tmask[ti] != 0;
-    const bool mask_n = tmask != nullptr && has_next && tmask[ti + 1] != 0;
     if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile mt
       if constexpr (BF) {
         const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + mt) * KSB) * 64 + lane;
@@ -483,6 +487,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     }
     t = tn;
     G = Gn;
+    tn = tnn;
+    mask_c = mask_n;
+    mask_n = mask_nn;
     buf ^= 1;
   }
 }
