@@ -345,6 +345,18 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     int p_lo, p_hi;
     apply_pairs<W>(wave, (ngroups + 1) / 2, p_lo, p_hi);
     for (int g0 = 2 * p_lo; g0 < 2 * p_hi; g0 += 2) {
+      if constexpr (W == 8) {
+        // waves with more pairs left issue first, so that the full-M-tile
+        // waves (twice the work of the half-tile ones on their SIMD) are not
+        // left to finish alone
+        const int rem = p_hi - g0 / 2;
+        if (rem >= 3)
+          __builtin_amdgcn_s_setprio(3);
+        else if (rem == 2)
+          __builtin_amdgcn_s_setprio(2);
+        else
+          __builtin_amdgcn_s_setprio(1);
+      }
       int eb[2];
       bool valid[2];
       T sc[2];
@@ -452,6 +464,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
         }
       }
     }
+    if constexpr (W == 8) __builtin_amdgcn_s_setprio(0);
     __syncthreads();  // ys complete; xs[buf ^ 1] free
     // flush this tile (masked) and clear ys; p.Ap partial = sum over the
     // staged points of (M x)_i times the tile's contribution to (A x)_i, which
