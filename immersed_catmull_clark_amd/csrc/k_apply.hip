@@ -222,12 +222,12 @@ __device__ __forceinline__ void tile_range(const int* __restrict__ wg_first, int
 // y += M * sum_e s_e K_t x_e (M x); optional per-workgroup partials of
 // x^T M K M x into pap[blockIdx.x].
 //
-// Persistent workgroups, each walking a contiguous chunk of tiles with a
-// two-stage software pipeline: while the MFMA loop runs on tile i (LDS
-// buffer b), the masked control points, free flags and scales of tile i+1
-// are already in flight into registers; after the loop they are written to
-// buffer b^1, and tile i's output points are flushed from LDS to y with
-// global atomic adds (y must hold the sum to accumulate into).  The A
+// Each workgroup walks a contiguous run of tiles through a three-deep
+// pipeline with double-buffered LDS (points, scales, offset tables and
+// output points): at the top of tile i a wave flushes tile i - 1's output
+// points to y with global atomic adds (y must hold the sum to accumulate
+// into), stages tile i + 1 from registers and issues the loads of tile
+// i + 2, then runs its MFMA loop on tile i; one barrier per tile.  The A
 // fragments are reloaded only when the element type changes.
 //
 // Each wave runs two element groups at a time as independent MFMA chains.
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   __shared__ T xs[2][3 * TILE_MAX_PTS];
   // output points accumulate in FP64 for both precisions: ds_add_f64 runs at
   // full rate, while FP32 LDS atomics back up the LDS data FIFO on gfx950
-  __shared__ double ys[3 * TILE_MAX_PTS];
+  __shared__ double ys[2][3 * TILE_MAX_PTS];
   __shared__ T ss[2][TILE_MAX_EL];
   __shared__ int otab[2][OTAB_MAX];  // per tile: LDS offsets of the B operands and of the output rows
   constexpr int NTR = Otab<BF>::ntrial;
@@ -264,7 +264,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     if (pap != nullptr && threadIdx.x == 0) pap[blockIdx.x] = 0.0;
     return;
   }
-  double pap_blk = 0.0;  // thread 0: this workgroup's p.Ap partial, tiles in order
 
   const int wave = threadIdx.x >> 6;
   const int mt = apply_mtile<W>(wave);
@@ -272,14 +271,51 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   const int col = lane & 15;
   const int kq = lane >> 4;
 
-  // prologue: stage the first tile, clear the output buffer
+  // Flush of a finished tile tf from output buffer b (masked): every staged
+  // point's sum goes to y with a global atomic add and the buffer is cleared.
+  // The p.Ap partial is the sum over the staged points of (M x)_i times the
+  // tile's contribution to (A x)_i, which summed over tiles is x^T M K M x.
+  double pap_acc = 0.0;  // this thread's share, tiles in order
+  auto flush = [&](const Tile& tf, bool maskf, int b) {
+    TileGeom Gf;
+    Gf.init(tf);
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int idx = threadIdx.x + j * NT;
+      if (idx < Gf.n3) {
+        const double v = ys[b][idx];
+        ys[b][idx] = 0.0;
+        if (pap != nullptr) pap_acc += (double)xs[b][idx] * v;
+        const int g = Gf.dof(tf, idx, ncp0, ncp1, ndof_sc);
+        if (!maskf || freem[g] != 0) unsafeAtomicAdd(&y[g], (T)v);
+      }
+    }
+  };
+  // loads of tile tl's points and scales into registers (unused slots are not
+  // loaded)
+  T pv[NPF];
+  T psc = T(0);
+  auto load_regs = [&](const Tile& tl) {
+    TileGeom Gl;
+    Gl.init(tl);
+#pragma unroll
+    for (int j = 0; j < NPF; ++j) {
+      const int idx = threadIdx.x + j * NT;
+      if (idx < Gl.n3) pv[j] = x[Gl.dof(tl, idx, ncp0, ncp1, ndof_sc)];
+    }
+    if (threadIdx.x < Gl.nel) psc = scale[Gl.elem(tl, threadIdx.x, nel0, nel1)];
+  };
+
+  // prologue: stage the first tile into buffer 0, clear both output buffers,
+  // load the second tile into registers
   Tile t = tiles[first];
   TileGeom G;
   G.init(t);
   {
     const bool masked = tmask != nullptr && tmask[first] != 0;
     for (int idx = threadIdx.x; idx < 3 * TILE_MAX_PTS; idx += NT) {
-      ys[idx] = 0.0;
+      ys[0][idx] = 0.0;
+      ys[1][idx] = 0.0;
       if (idx < G.n3) {
         const int g = G.dof(t, idx, ncp0, ncp1, ndof_sc);
         xs[0][idx] = (!masked || freem[g] != 0) ? x[g] : T(0);
@@ -289,22 +325,45 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   if (threadIdx.x < G.nel) ss[0][threadIdx.x] = scale[G.elem(t, threadIdx.x, nel0, nel1)];
   offset_table<BF>(G, otab[0]);
 
+  // The tile pipeline, three deep.  At the top of tile i each wave flushes
+  // tile i - 1 (its output buffer is complete once every wave has passed the
+  // barrier that ends tile i - 1), stages tile i + 1 from the registers its
+  // loads went to at the top of tile i - 1, and issues the loads of tile
+  // i + 2; then it runs tile i.  Waves that are through with this start their
+  // MFMA loops while the others still flush, and one barrier per tile
+  // suffices.  Descriptors and mask flags (tiles whose staged points include
+  // fixed DOFs, few, read the free mask) are loaded a tile before they are
+  // needed.
+  const auto tile_at = [&](int i) { return tiles[i < last ? i : first]; };
+  const auto mask_at = [&](int i) { return tmask != nullptr && i < last && tmask[i] != 0; };
+  Tile tp = t;                // tile i - 1
+  Tile tn = tile_at(first + 1);   // tile i + 1
+  Tile tnn = tile_at(first + 2);  // tile i + 2
+  bool mask_p = false, mask_c = mask_at(first), mask_n = mask_at(first + 1), mask_nn = mask_at(first + 2);
+  if (first + 1 < last) load_regs(tn);
+  __syncthreads();  // buffer 0 staged, output buffers clear
+
   T a[BF ? 1 : KS];
   bf8_t ab[BF ? KSB : 1];
   int cur_type = -1;
   int buf = 0;
-  // descriptor and mask flag of the next tile, loaded one tile ahead so that
-  // the prefetch below never waits for them (tiles whose staged points
-  // include fixed DOFs, few, read the free mask)
-  Tile tn = tiles[first + 1 < last ? first + 1 : first];
-  bool mask_c = tmask != nullptr && tmask[first] != 0;
-  bool mask_n = tmask != nullptr && first + 1 < last && tmask[first + 1] != 0;
   for (int ti = first; ti < last; ++ti) {
-    const bool has_next = ti + 1 < last;
-    const Tile tnn = tiles[ti + 2 < last ? ti + 2 : ti];
-    const bool mask_nn = tmask != nullptr && ti + 2 < last && tmask[ti + 2] != 0;
-    TileGeom Gn;
-    Gn.init(tn);
+    const Tile t3 = tile_at(ti + 3);
+    const bool mask_3 = mask_at(ti + 3);
+    if (ti > first) flush(tp, mask_p, buf ^ 1);
+    if (ti + 1 < last) {  // stage tile i + 1 into buffer buf ^ 1
+      TileGeom Gn;
+      Gn.init(tn);
+#pragma unroll
+      for (int j = 0; j < NPF; ++j) {
+        const int idx = threadIdx.x + j * NT;
+        if (idx < Gn.n3)
+          xs[buf ^ 1][idx] = (!mask_n || freem[Gn.dof(tn, idx, ncp0, ncp1, ndof_sc)] != 0) ? pv[j] : T(0);
+      }
+      if (threadIdx.x < Gn.nel) ss[buf ^ 1][threadIdx.x] = psc;
+      offset_table<BF>(Gn, otab[buf ^ 1]);
+    }
+    if (ti + 2 < last) load_regs(tnn);
     if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile mt
       if constexpr (BF) {
         const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + mt) * KSB) * 64 + lane;
@@ -317,21 +376,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
       }
       cur_type = t.type;
     }
-    // prefetch of the next tile's points (issued unconditionally; unused
-    // slots read slot 0 of the tile, which always exists)
-    T pv[NPF];
-#pragma unroll
-    for (int j = 0; j < NPF; ++j) {
-      const int idx = threadIdx.x + j * NT;
-      pv[j] = x[Gn.dof(tn, idx < Gn.n3 ? idx : 0, ncp0, ncp1, ndof_sc)];
-    }
-    const T psc = scale[Gn.elem(tn, threadIdx.x < Gn.nel ? threadIdx.x : 0, nel0, nel1)];
 
     const int px = G.px, py = G.py, nel = G.nel;
     const bool r3 = perm_row(mt, kq, 3) >= 0;
     const T* xb = xs[buf];
     const T* sb = ss[buf];
-    __syncthreads();  // xs[buf], ss[buf], otab[buf] staged; ys clear
+    double* yb = ys[buf];
     constexpr int NB = NTR / 4;  // trial offsets per row group
     int boff[NB];
 #pragma unroll
@@ -449,61 +499,45 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
           if (g < ngroups) {
             const int ey = g % t.ny, ez = g / t.ny;
             const int lb = px * (ey + py * ez) + roff[0];  // point x = 0 of this line, ax = 0 row
-            if (col <= t.nx + 1) unsafeAtomicAdd(&ys[lb + col], (double)comb);
+            if (col <= t.nx + 1) unsafeAtomicAdd(&yb[lb + col], (double)comb);
             if (col == 15) {  // points 16 and 17 exist only for lines of 15 and 16 elements
-              if (t.nx >= 15) unsafeAtomicAdd(&ys[lb + 16], (double)e16);
-              if (t.nx >= 16) unsafeAtomicAdd(&ys[lb + 17], (double)v2);
+              if (t.nx >= 15) unsafeAtomicAdd(&yb[lb + 16], (double)e16);
+              if (t.nx >= 16) unsafeAtomicAdd(&yb[lb + 17], (double)v2);
             }
-            if (r3 && valid[q]) unsafeAtomicAdd(&ys[eb[q] + roff[3]], (double)v3);
+            if (r3 && valid[q]) unsafeAtomicAdd(&yb[eb[q] + roff[3]], (double)v3);
           }
         } else if (valid[q]) {
-          unsafeAtomicAdd(&ys[eb[q] + roff[0]], (double)v0);
-          unsafeAtomicAdd(&ys[eb[q] + roff[1]], (double)v1);
-          unsafeAtomicAdd(&ys[eb[q] + roff[2]], (double)v2);
-          if (r3) unsafeAtomicAdd(&ys[eb[q] + roff[3]], (double)v3);
+          unsafeAtomicAdd(&yb[eb[q] + roff[0]], (double)v0);
+          unsafeAtomicAdd(&yb[eb[q] + roff[1]], (double)v1);
+          unsafeAtomicAdd(&yb[eb[q] + roff[2]], (double)v2);
+          if (r3) unsafeAtomicAdd(&yb[eb[q] + roff[3]], (double)v3);
         }
       }
     }
     if constexpr (W == 8) __builtin_amdgcn_s_setprio(0);
-    __syncthreads();  // ys complete; xs[buf ^ 1] free
-    // flush this tile (masked) and clear ys; p.Ap partial = sum over the
-    // staged points of (M x)_i times the tile's contribution to (A x)_i, which
-    // summed over tiles is x^T M K M x; then stage the next tile
-    double pap_acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < NPF; ++j) {
-      const int idx = threadIdx.x + j * NT;
-      if (idx < G.n3) {
-        const double v = ys[idx];
-        ys[idx] = 0.0;
-        if (pap != nullptr) pap_acc += (double)xb[idx] * v;
-        const int g = G.dof(t, idx, ncp0, ncp1, ndof_sc);
-        if (!mask_c || freem[g] != 0) unsafeAtomicAdd(&y[g], (T)v);
-      }
-      if (has_next && idx < Gn.n3)
-        xs[buf ^ 1][idx] = (!mask_n || freem[Gn.dof(tn, idx, ncp0, ncp1, ndof_sc)] != 0) ? pv[j] : T(0);
-    }
-    if (has_next && threadIdx.x < Gn.nel) ss[buf ^ 1][threadIdx.x] = psc;
-    if (has_next) offset_table<BF>(Gn, otab[buf ^ 1]);
-    if (pap != nullptr) {
-      // wave sums, then the waves in fixed order (red is free: the last
-      // reader finished before the barrier above)
-      for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
-      if (lane == 0) red[wave] = pap_acc;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double sum = 0.0;
-        for (int w = 0; w < W; ++w) sum += red[w];
-        pap_blk += sum;
-        if (!has_next) pap[blockIdx.x] = pap_blk;
-      }
-    }
+    __syncthreads();  // tile i's output complete; buffer buf ^ 1 staged
+    tp = t;
+    mask_p = mask_c;
     t = tn;
-    G = Gn;
-    tn = tnn;
     mask_c = mask_n;
+    tn = tnn;
     mask_n = mask_nn;
+    tnn = t3;
+    mask_nn = mask_3;
+    G.init(t);
     buf ^= 1;
+  }
+  flush(tp, mask_p, buf ^ 1);  // the last tile
+  if (pap != nullptr) {
+    // wave sums, then the waves in fixed order
+    for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
+    if (lane == 0) red[wave] = pap_acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < W; ++w) sum += red[w];
+      pap[blockIdx.x] = sum;
+    }
   }
 }
 
