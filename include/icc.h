@@ -31,7 +31,7 @@ enum icc_status {
   ICC_EHIP = 3,        /* HIP runtime error */
   ICC_EBREAKDOWN = 4,  /* PCG breakdown: p.Ap <= 0 (solver.rs:47-49 stops silently; we report it) */
   ICC_ENOCONV = 5,     /* iteration limit reached before the tolerance */
-  ICC_ENOTSUP = 6      /* valid request this build does not implement */
+  ICC_ENOTSUP = 6      /* valid request the library does not implement */
 };
 
 /* Arithmetic of the K.u contraction and of the inner solve. */

@@ -5,7 +5,7 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Nothing in the product (libicc.so, the Python
 // package) links or imports this code; only tests/, the smoke check and the
-// bench's cpu_baseline leg load it, as the checker.
+// bench's cpu_baseline leg load it, to check the product against.
 //
 // Loops keep the reference's order and the file is compiled with
 // -ffp-contract=off (Rust never fuses FMAs), so FP64 results are comparable

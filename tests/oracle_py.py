@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 Test infrastructure: used by tests/, the smoke check and the bench's
-cpu_baseline leg only, as the checker.  The product never imports it.
+cpu_baseline leg only, to check the product against.  The product never
+imports it.
 """
 import ctypes as C
 import os

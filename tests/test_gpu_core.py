@@ -1,7 +1,7 @@
 """GPU parity of the HIP hot path against the CPU oracle, through the C ABI.
 
-Tolerances: FP64 paths 1e-13 relative (operator) / 1e-8 (solves, as the
-north-star bar); FP32 contraction 2e-6 relative on the operator.
+Tolerances: FP64 paths 1e-13 relative (operator) / 1e-8 (solves); FP32
+contraction 2e-6 relative on the operator.
 """
 import numpy as np
 import pytest
