@@ -97,27 +97,36 @@ __device__ __forceinline__ double row_shr(double v) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Exact quotient n / d for n < 2^12 and 3 <= d < 2^12 with m = magic(d):
-// floor(n * m / 2^32), m = ceil(2^32 / d) (the error n (m - 2^32/d) / 2^32
-// stays below 1/d).
-__host__ __device__ __forceinline__ uint32_t magic_div(uint32_t d) { return 0xffffffffu / d + 1u; }
+// Exact quotient n / d for n < 2^12 and 2 <= d < 2^10 with m = ceil(2^32 / d):
+// floor(n * m / 2^32) (the error n (m - 2^32/d) / 2^32 stays below 1/d).  The
+// multipliers sit in a constant-memory table, so a tile's geometry costs a few
+// scalar loads instead of integer divisions.
+constexpr int MAGIC_MAX = 1024;
+struct MagicTab {
+  uint32_t v[MAGIC_MAX];
+  constexpr MagicTab() : v() {
+    for (uint32_t d = 2; d < (uint32_t)MAGIC_MAX; ++d) v[d] = 0xffffffffu / d + 1u;
+  }
+};
+__constant__ MagicTab kMagic = MagicTab();
+static_assert(TILE_MAX_PTS < MAGIC_MAX, "tile point counts index the multiplier table");
 __device__ __forceinline__ int qdiv(int n, uint32_t m) { return (int)__umulhi((uint32_t)n, m); }
+// n / d for 0 <= n < 2^12, 1 <= d < MAGIC_MAX
+__device__ __forceinline__ int sdiv(int n, int d) { return d == 1 ? n : qdiv(n, kMagic.v[d]); }
 
 // Geometry of a staged tile: points [c][lz][ly][lx] of extents px, py, pz.
 struct TileGeom {
   int px, py, npts, n3, nel;
-  uint32_t m_npts, m_px, m_py, m_nx, m_ny;
+  uint32_t m_npts, m_px, m_py;
   __device__ __forceinline__ void init(const Tile& t) {
     px = t.nx + 2;
     py = t.ny + 2;
     npts = px * py * (t.nz + 2);
     n3 = 3 * npts;
     nel = t.nx * t.ny * t.nz;
-    m_npts = magic_div((uint32_t)npts);
-    m_px = magic_div((uint32_t)px);
-    m_py = magic_div((uint32_t)py);
-    m_nx = t.nx >= 3 ? magic_div((uint32_t)t.nx) : 0u;
-    m_ny = t.ny >= 3 ? magic_div((uint32_t)t.ny) : 0u;
+    m_npts = kMagic.v[npts];
+    m_px = kMagic.v[px];
+    m_py = kMagic.v[py];
   }
   // global DOF of staged slot idx < n3
   __device__ __forceinline__ int dof(const Tile& t, int idx, int ncp0, int ncp1, int ndof_sc) const {
@@ -131,9 +140,9 @@ struct TileGeom {
   }
   // global element of tile element el < nel
   __device__ __forceinline__ int elem(const Tile& t, int el, int nel0, int nel1) const {
-    const int r = t.nx >= 3 ? qdiv(el, m_nx) : el / t.nx;
+    const int r = sdiv(el, t.nx);
     const int ex = el - r * t.nx;
-    const int ez = t.ny >= 3 ? qdiv(r, m_ny) : r / t.ny;
+    const int ez = sdiv(r, t.ny);
     const int ey = r - ez * t.ny;
     return (t.i0 + ex) + nel0 * ((t.j0 + ey) + nel1 * (t.k0 + ez));
   }
@@ -415,7 +424,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
         const int g = g0 + q;
         int e;
         if (lines) {
-          const int ey = g % t.ny, ez = g / t.ny;
+          const int ez = sdiv(g, t.ny), ey = g - ez * t.ny;
           valid[q] = g < ngroups && col < t.nx;
           e = col + t.nx * g;
           eb[q] = valid[q] ? col + px * (ey + py * ez) : 0;
@@ -424,10 +433,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
           valid[q] = g < ngroups && e < nel;
           int ex = 0, ey = 0, ez = 0;
           if (valid[q]) {
-            ex = e % t.nx;
-            const int rr = e / t.nx;
-            ey = rr % t.ny;
-            ez = rr / t.ny;
+            const int rr = sdiv(e, t.nx);
+            ex = e - rr * t.nx;
+            ez = sdiv(rr, t.ny);
+            ey = rr - ez * t.ny;
           }
           eb[q] = ex + px * (ey + py * ez);
         }
@@ -497,7 +506,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
           const T e16 = v1 + row_shr<1>(v2);  // at lane 15: point 16 = v1[15] + v2[14]
           const int g = g0 + q;
           if (g < ngroups) {
-            const int ey = g % t.ny, ez = g / t.ny;
+            const int ez = sdiv(g, t.ny), ey = g - ez * t.ny;
             const int lb = px * (ey + py * ez) + roff[0];  // point x = 0 of this line, ax = 0 row
             if (col <= t.nx + 1) unsafeAtomicAdd(&yb[lb + col], (double)comb);
             if (col == 15) {  // points 16 and 17 exist only for lines of 15 and 16 elements
