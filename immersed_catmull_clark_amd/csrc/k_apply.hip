@@ -341,15 +341,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   // i + 2; then it runs tile i.  Waves that are through with this start their
   // MFMA loops while the others still flush, and one barrier per tile
   // suffices.  Descriptors and mask flags (tiles whose staged points include
-  // fixed DOFs, few, read the free mask) are loaded a tile before they are
-  // needed.
-  const auto tile_at = [&](int i) { return tiles[i < last ? i : first]; };
-  const auto mask_at = [&](int i) { return tmask != nullptr && i < last && tmask[i] != 0; };
-  Tile tp = t;                // tile i - 1
-  Tile tn = tile_at(first + 1);   // tile i + 1
-  Tile tnn = tile_at(first + 2);  // tile i + 2
-  bool mask_p = false, mask_c = mask_at(first), mask_n = mask_at(first + 1), mask_nn = mask_at(first + 2);
-  if (first + 1 < last) load_regs(tn);
+  // fixed DOFs, few, read the free mask) are read where they are used: after
+  // their first use they sit in the scalar cache, and no descriptor has to
+  // be carried (and spilled) across tiles.
+  const auto mask_at = [&](int i) { return tmask != nullptr && tmask[i] != 0; };
+  if (first + 1 < last) load_regs(tiles[first + 1]);
   __syncthreads();  // buffer 0 staged, output buffers clear
 
   T a[BF ? 1 : KS];
@@ -357,10 +353,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   int cur_type = -1;
   int buf = 0;
   for (int ti = first; ti < last; ++ti) {
-    const Tile t3 = tile_at(ti + 3);
-    const bool mask_3 = mask_at(ti + 3);
-    if (ti > first) flush(tp, mask_p, buf ^ 1);
+    if (ti > first) {
+      t = tiles[ti];
+      G.init(t);
+      flush(tiles[ti - 1], mask_at(ti - 1), buf ^ 1);
+    }
     if (ti + 1 < last) {  // stage tile i + 1 into buffer buf ^ 1
+      const Tile tn = tiles[ti + 1];
+      const bool mask_n = mask_at(ti + 1);
       TileGeom Gn;
       Gn.init(tn);
 #pragma unroll
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
       if (threadIdx.x < Gn.nel) ss[buf ^ 1][threadIdx.x] = psc;
       offset_table<BF>(Gn, otab[buf ^ 1]);
     }
-    if (ti + 2 < last) load_regs(tnn);
+    if (ti + 2 < last) load_regs(tiles[ti + 2]);
     if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile mt
       if constexpr (BF) {
         const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + mt) * KSB) * 64 + lane;
@@ -525,18 +525,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     }
     if constexpr (W == 8) __builtin_amdgcn_s_setprio(0);
     __syncthreads();  // tile i's output complete; buffer buf ^ 1 staged
-    tp = t;
-    mask_p = mask_c;
-    t = tn;
-    mask_c = mask_n;
-    tn = tnn;
-    mask_n = mask_nn;
-    tnn = t3;
-    mask_nn = mask_3;
-    G.init(t);
     buf ^= 1;
   }
-  flush(tp, mask_p, buf ^ 1);  // the last tile
+  flush(tiles[last - 1], mask_at(last - 1), buf ^ 1);  // the last tile
   if (pap != nullptr) {
     // wave sums, then the waves in fixed order
     for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
