@@ -474,7 +474,7 @@ namespace icc {
 static int upload_tile_mask(icc_op* op) {
   const auto& s = op->g->s;
   const std::vector<uint8_t>& fm = op->h_free;
-  std::vector<uint8_t> tm(s.tiles.size(), 0);
+  std::vector<int32_t> tm(s.tiles.size(), 0);  // 32-bit flags: scalar loads in the kernel
   for (size_t ti = 0; ti < s.tiles.size(); ++ti) {
     const Tile& t = s.tiles[ti];
     bool fixed = false;
@@ -491,7 +491,8 @@ static int upload_tile_mask(icc_op* op) {
   hipFree(op->d_tmask);
   op->d_tmask = nullptr;
   ICC_TRY(dalloc(&op->d_tmask, (int64_t)tm.size()));
-  ICC_HIP(hipMemcpyAsync(op->d_tmask, tm.data(), tm.size(), hipMemcpyHostToDevice, op->g->ctx->stream));
+  ICC_HIP(hipMemcpyAsync(op->d_tmask, tm.data(), tm.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                         op->g->ctx->stream));
   ICC_HIP(hipStreamSynchronize(op->g->ctx->stream));
   return ICC_OK;
 }

@@ -145,7 +145,7 @@ struct icc_op {
   double* d_scale = nullptr;    // [nel]
   float* d_scale32 = nullptr;   // [nel]
   uint8_t* d_free = nullptr;    // [ndof], null = all free
-  uint8_t* d_tmask = nullptr;   // [ntiles]: 1 if the tile stages a fixed DOF
+  int32_t* d_tmask = nullptr;   // [ntiles]: 1 if the tile stages a fixed DOF
   std::vector<uint8_t> h_free;  // host copy of the free mask
   // optional sparse add-on (CSR, int32 indices)
   int64_t s_nnz = 0;

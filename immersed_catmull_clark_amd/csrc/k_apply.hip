@@ -254,7 +254,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     const Tile* __restrict__ tiles, const int* __restrict__ wg_first, const void* __restrict__ kfrag_v,
     const T* __restrict__ x,
     T* __restrict__ y, const T* __restrict__ scale, const uint8_t* __restrict__ freem,
-    const uint8_t* __restrict__ tmask, int ncp0, int ncp1, int ndof_sc, int nel0, int nel1,
+    const int32_t* __restrict__ tmask, int ncp0, int ncp1, int ndof_sc, int nel0, int nel1,
     double* __restrict__ pap, const PcgState* st) {
   __shared__ T xs[2][3 * TILE_MAX_PTS];
   // output points accumulate in FP64 for both precisions: ds_add_f64 runs at
@@ -356,8 +356,31 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     if (ti > first) {
       t = tiles[ti];
       G.init(t);
-      flush(tiles[ti - 1], mask_at(ti - 1), buf ^ 1);
     }
+    // the A fragments are (re)loaded before any other memory operation of
+    // the tile, so that waiting for them never waits for the flush's atomics
+    // or the prefetch below (the vector memory counter retires in order)
+    if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile mt
+      // each fragment is consumed by an empty asm right here, so the loads are
+      // waited for on this (rare) path and are never pending in the MFMA
+      // loop, whose waits would otherwise also wait for the prefetch of tile
+      // i + 2 issued below
+      if constexpr (BF) {
+        const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + mt) * KSB) * 64 + lane;
+#pragma unroll
+        for (int ks = 0; ks < KSB; ++ks) ab[ks] = kf[ks * 64];
+#pragma unroll
+        for (int ks = 0; ks < KSB; ++ks) asm volatile("" ::"v"(ab[ks]));
+      } else {
+        const T* kf = (const T*)kfrag_v + ((size_t)(t.type * MT + mt) * KS) * 64 + lane;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) a[ks] = kf[ks * 64];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) asm volatile("" ::"v"(a[ks]));
+      }
+      cur_type = t.type;
+    }
+    if (ti > first) flush(tiles[ti - 1], mask_at(ti - 1), buf ^ 1);
     if (ti + 1 < last) {  // stage tile i + 1 into buffer buf ^ 1
       const Tile tn = tiles[ti + 1];
       const bool mask_n = mask_at(ti + 1);
@@ -373,18 +396,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
       offset_table<BF>(Gn, otab[buf ^ 1]);
     }
     if (ti + 2 < last) load_regs(tiles[ti + 2]);
-    if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile mt
-      if constexpr (BF) {
-        const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + mt) * KSB) * 64 + lane;
-#pragma unroll
-        for (int ks = 0; ks < KSB; ++ks) ab[ks] = kf[ks * 64];
-      } else {
-        const T* kf = (const T*)kfrag_v + ((size_t)(t.type * MT + mt) * KS) * 64 + lane;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) a[ks] = kf[ks * 64];
-      }
-      cur_type = t.type;
-    }
 
     const int px = G.px, py = G.py, nel = G.nel;
     const bool r3 = perm_row(mt, kq, 3) >= 0;
