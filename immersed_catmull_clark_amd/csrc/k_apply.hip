@@ -3,20 +3,20 @@
 //
 // Work decomposition.  The element grid is cut into tiles of <= 128 elements
 // that never straddle an element-type boundary (host: setup.cpp), so every
-// element of a tile shares one 81x81 matrix K_t.  A workgroup walks a run of
-// tiles with twelve waves; K_t's rows form six 16-row M tiles (the sixth holds
-// row 80 only), and waves w and w + 6 own M tile w, each for half of the
-// tile's element groups.  A wave keeps its 21
-// A fragments (K rows x 4-deep K steps, pre-permuted on the host into lane
-// order) in VGPRs and streams the tile's elements 16 at a time as the MFMA N
-// dimension:
+// element of a tile shares one 81x81 matrix K_t.  K_t's rows form six 16-row
+// M tiles (the sixth holds row 80 only).  A workgroup walks a run of tiles
+// (host: balance_tiles) with W waves, each owning one M tile for all or half
+// of the tile's element groups (apply_mtile, apply_pairs: W = 12 in FP64,
+// 8 in FP32 / BF16).  A wave keeps its 21 A fragments (K rows x 4-deep K
+// steps, pre-permuted on the host into lane order) in VGPRs and streams the
+// tile's elements 16 at a time as the MFMA N dimension:
 //     D[16 x 16] += A[16 x 4] * B[4 x 16],  B[k][j] = x_local(element j, trial k)
 // with the B operand gathered from the LDS copy of the tile's control points
 // (x masked by the free mask).  The 4 accumulator rows per lane are scaled by
 // s_e and scatter-added into an LDS copy of the tile's output points
-// (ds_add), and the tile is flushed to HBM with global atomic adds (y must be
-// zero on entry).  The element energy p_e^T K p_e, needed for p.Ap, is
-// accumulated on the fly from the same accumulators.
+// (ds_add_f64), and the tile is flushed to HBM with global atomic adds (y
+// holds the sum to accumulate into).  The p.Ap partial of PCG is accumulated
+// in that flush from the staged points and the tile's contributions.
 //
 // MFMA lane layouts (v_mfma_{f64,f32}_16x16x4): lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; accumulator register r of lane l holds D[i][l%16] with
