@@ -5,9 +5,9 @@
 // that never straddle an element-type boundary (host: setup.cpp), so every
 // element of a tile shares one 81x81 matrix K_t.  K_t's rows form six 16-row
 // M tiles (the sixth holds row 80 only).  A workgroup walks a run of tiles
-// (host: balance_tiles) with W waves, each owning one M tile for all or half
-// of the tile's element groups (apply_mtile, apply_pairs: W = 12 in FP64,
-// 8 in FP32 / BF16).  A wave keeps its 21 A fragments (K rows x 4-deep K
+// (host: balance_tiles) with W waves, each owning one M tile (apply_mtile;
+// W = 12 in FP64, two waves per M tile claiming its element groups
+// dynamically; W = 8 in FP32 / BF16, apply_pairs).  A wave keeps its 21 A fragments (K rows x 4-deep K
 // steps, pre-permuted on the host into lane order) in VGPRs and streams the
 // tile's elements 16 at a time as the MFMA N dimension:
 //     D[16 x 16] += A[16 x 4] * B[4 x 16],  B[k][j] = x_local(element j, trial k)
@@ -149,9 +149,9 @@ struct TileGeom {
 };
 
 // Waves per workgroup W, with wave w on SIMD w mod 4 (the dispatch order):
-//   W = 12 (FP64): two waves per M tile, wave w owns M tile w mod 6 over the
-//     first (w < 6) or second half of the tile's element-group pairs; every
-//     wave carries the same MFMA work, every SIMD three waves of it;
+//   W = 12 (FP64): two waves per M tile, wave w owns M tile w mod 6 (the
+//     static split below is the fallback; the kernel claims pairs from an
+//     LDS counter); every SIMD carries three waves;
 //   W = 8 (FP32, BF16): waves 0..3 own M tiles 0..3 over all pairs, waves
 //     4..7 own M tiles 4 and 5 (4, 6: tile 4; 5, 7: tile 5) over half of them;
 //     every SIMD carries 1.5 M tiles of work, and at <= 128 VGPRs two
@@ -192,8 +192,7 @@ constexpr int OTAB_MAX = Otab<true>::size;
 // padding trial columns and rows point at offset 0 (their A entries are zero
 // and padding rows are never stored).
 template <bool BF>
-__device__ __forceinline__ void offset_table(const TileGeom& G, int* tab) {
-  const int i = threadIdx.x;
+__device__ __forceinline__ void offset_table(const TileGeom& G, int* tab, int i) {
   constexpr int NT = Otab<BF>::ntrial;
   if (i < NT) {
     int b;
@@ -264,8 +263,16 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   __shared__ int otab[2][OTAB_MAX];  // per tile: LDS offsets of the B operands and of the output rows
   constexpr int NTR = Otab<BF>::ntrial;
   __shared__ double red[W];
+  __shared__ int claim[2][8];  // FP64: next element-group pair of each M tile, per buffer
   constexpr int NT = 64 * W;
-  constexpr int NPF = Npf<W>::value;
+  // FP64 (twelve waves): waves 6..11 do the per-tile flush, staging and
+  // prefetch while waves 0..5 start their MFMA loops at once, and the two
+  // waves of an M tile claim its element-group pairs from an LDS counter, so
+  // that whichever is free takes the next one.  (With eight waves the split
+  // gained nothing.)
+  constexpr bool SPEC = W == 12;
+  constexpr int NBG = SPEC ? 6 * 64 : NT;
+  constexpr int NPF = (3 * TILE_MAX_PTS + NBG - 1) / NBG;
   if (st != nullptr && st->done != 0) return;  // solver already finished
   int first, last;
   tile_range(wg_first, first, last);
@@ -279,6 +286,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   const int lane = threadIdx.x & 63;
   const int col = lane & 15;
   const int kq = lane >> 4;
+  const int tb = SPEC ? (int)threadIdx.x - 6 * 64 : (int)threadIdx.x;  // background thread index
+  const bool bgw = !SPEC || wave >= 6;
 
   // Flush of a finished tile tf from output buffer b (masked): every staged
   // point's sum goes to y with a global atomic add and the buffer is cleared.
@@ -286,11 +295,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   // tile's contribution to (A x)_i, which summed over tiles is x^T M K M x.
   double pap_acc = 0.0;  // this thread's share, tiles in order
   auto flush = [&](const Tile& tf, bool maskf, int b) {
+    if (!bgw) return;
     TileGeom Gf;
     Gf.init(tf);
 #pragma unroll
     for (int j = 0; j < NPF; ++j) {
-      const int idx = threadIdx.x + j * NT;
+      const int idx = tb + j * NBG;
       if (idx < Gf.n3) {
         const double v = ys[b][idx];
         ys[b][idx] = 0.0;
@@ -305,14 +315,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   T pv[NPF];
   T psc = T(0);
   auto load_regs = [&](const Tile& tl) {
+    if (!bgw) return;
     TileGeom Gl;
     Gl.init(tl);
 #pragma unroll
     for (int j = 0; j < NPF; ++j) {
-      const int idx = threadIdx.x + j * NT;
+      const int idx = tb + j * NBG;
       if (idx < Gl.n3) pv[j] = x[Gl.dof(tl, idx, ncp0, ncp1, ndof_sc)];
     }
-    if (threadIdx.x < Gl.nel) psc = scale[Gl.elem(tl, threadIdx.x, nel0, nel1)];
+    if (tb < Gl.nel) psc = scale[Gl.elem(tl, tb, nel0, nel1)];
   };
 
   // prologue: stage the first tile into buffer 0, clear both output buffers,
@@ -325,6 +336,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     for (int idx = threadIdx.x; idx < 3 * TILE_MAX_PTS; idx += NT) {
       ys[0][idx] = 0.0;
       ys[1][idx] = 0.0;
+      if (idx < 16) claim[idx >> 3][idx & 7] = 0;
       if (idx < G.n3) {
         const int g = G.dof(t, idx, ncp0, ncp1, ndof_sc);
         xs[0][idx] = (!masked || freem[g] != 0) ? x[g] : T(0);
@@ -332,14 +344,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     }
   }
   if (threadIdx.x < G.nel) ss[0][threadIdx.x] = scale[G.elem(t, threadIdx.x, nel0, nel1)];
-  offset_table<BF>(G, otab[0]);
+  offset_table<BF>(G, otab[0], threadIdx.x);
 
-  // The tile pipeline, three deep.  At the top of tile i each wave flushes
-  // tile i - 1 (its output buffer is complete once every wave has passed the
-  // barrier that ends tile i - 1), stages tile i + 1 from the registers its
-  // loads went to at the top of tile i - 1, and issues the loads of tile
-  // i + 2; then it runs tile i.  Waves that are through with this start their
-  // MFMA loops while the others still flush, and one barrier per tile
+  // The tile pipeline, three deep.  At the top of tile i the background
+  // waves flush tile i - 1 (its output buffer is complete once every wave has
+  // passed the barrier that ends tile i - 1), stage tile i + 1 from the
+  // registers their loads went to at the top of tile i - 1, and issue the
+  // loads of tile i + 2; then they join the MFMA loop on tile i, which the
+  // other waves started right after the barrier.  One barrier per tile
   // suffices.  Descriptors and mask flags (tiles whose staged points include
   // fixed DOFs, few, read the free mask) are read where they are used: after
   // their first use they sit in the scalar cache, and no descriptor has to
@@ -381,19 +393,20 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
       cur_type = t.type;
     }
     if (ti > first) flush(tiles[ti - 1], mask_at(ti - 1), buf ^ 1);
-    if (ti + 1 < last) {  // stage tile i + 1 into buffer buf ^ 1
+    if (bgw && tb < 8) claim[buf ^ 1][tb] = 0;
+    if (bgw && ti + 1 < last) {  // stage tile i + 1 into buffer buf ^ 1
       const Tile tn = tiles[ti + 1];
       const bool mask_n = mask_at(ti + 1);
       TileGeom Gn;
       Gn.init(tn);
 #pragma unroll
       for (int j = 0; j < NPF; ++j) {
-        const int idx = threadIdx.x + j * NT;
+        const int idx = tb + j * NBG;
         if (idx < Gn.n3)
           xs[buf ^ 1][idx] = (!mask_n || freem[Gn.dof(tn, idx, ncp0, ncp1, ndof_sc)] != 0) ? pv[j] : T(0);
       }
-      if (threadIdx.x < Gn.nel) ss[buf ^ 1][threadIdx.x] = psc;
-      offset_table<BF>(Gn, otab[buf ^ 1]);
+      if (tb < Gn.nel) ss[buf ^ 1][tb] = psc;
+      offset_table<BF>(Gn, otab[buf ^ 1], tb);
     }
     if (ti + 2 < last) load_regs(tiles[ti + 2]);
 
@@ -414,7 +427,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     const int ngroups = lines ? t.ny * t.nz : (nel + 15) / 16;
     int p_lo, p_hi;
     apply_pairs<W>(wave, (ngroups + 1) / 2, p_lo, p_hi);
-    for (int g0 = 2 * p_lo; g0 < 2 * p_hi; g0 += 2) {
+    const int npairs = (ngroups + 1) / 2;
+    auto next_pair = [&]() {
+      int p = 0;
+      if (lane == 0) p = atomicAdd(&claim[buf][mt], 1);
+      p = __shfl(p, 0, 64);
+      return __builtin_amdgcn_readfirstlane(p);
+    };
+    for (int g0 = SPEC ? 2 * next_pair() : 2 * p_lo; g0 < (SPEC ? 2 * npairs : 2 * p_hi);
+         g0 = SPEC ? 2 * next_pair() : g0 + 2) {
       if constexpr (W == 8) {
         // waves with more pairs left issue first, so that the full-M-tile
         // waves (twice the work of the half-tile ones on their SIMD) are not
