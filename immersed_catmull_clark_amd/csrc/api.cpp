@@ -657,11 +657,24 @@ static int upload_faces(icc_op* op, int kind, double coef, const std::vector<int
     if ((a & 3) != (b & 3)) return (a & 3) < (b & 3);
     return a < b;
   });
-  std::vector<int32_t> blk, ptr;
+  // the device gets the faces as packed element coordinates and direction
+  // (ex | ey << 10 | ez << 20 | dir << 30) and the blocks as packed block
+  // coordinates, so that the kernel decodes them without divisions
+  if (s.nel[0] > 1024 || s.nel[1] > 1024 || s.nel[2] > 1024) {
+    set_error("face stabilisation: at most 1024 elements per direction");
+    return ICC_ENOTSUP;
+  }
+  std::vector<int32_t> blk, ptr, dev(faces.size());
+  int64_t last_b = -1;
   for (size_t f = 0; f < faces.size(); ++f) {
     const int64_t b = block_of(faces[f]);
-    if (blk.empty() || blk.back() != (int32_t)b) {
-      blk.push_back((int32_t)b);
+    const int64_t e1 = faces[f] >> 2;
+    const int64_t i = e1 % s.nel[0], j = (e1 / s.nel[0]) % s.nel[1], k = e1 / (s.nel[0] * s.nel[1]);
+    dev[f] = (int32_t)((uint32_t)i | ((uint32_t)j << 10) | ((uint32_t)k << 20) | ((uint32_t)(faces[f] & 3) << 30));
+    if (b != last_b) {
+      last_b = b;
+      const int64_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+      blk.push_back((int32_t)(bx | (by << 10) | (bz << 20)));
       ptr.push_back((int32_t)f);
     }
   }
@@ -671,7 +684,7 @@ static int upload_faces(icc_op* op, int kind, double coef, const std::vector<int
   ICC_TRY(dalloc(&op->d_fptr, (int64_t)ptr.size()));
   ICC_TRY(dalloc(&op->d_f_diag, s.ndof));
   hipStream_t st = op->g->ctx->stream;
-  ICC_HIP(hipMemcpyAsync(op->d_faces, faces.data(), faces.size() * 4, hipMemcpyHostToDevice, st));
+  ICC_HIP(hipMemcpyAsync(op->d_faces, dev.data(), dev.size() * 4, hipMemcpyHostToDevice, st));
   ICC_HIP(hipMemcpyAsync(op->d_fblk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, st));
   ICC_HIP(hipMemcpyAsync(op->d_fptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice, st));
   ICC_HIP(hipMemcpyAsync(op->d_f_diag, dg.data(), s.ndof * 8, hipMemcpyHostToDevice, st));

@@ -56,8 +56,9 @@ __global__ __launch_bounds__(FB) void faces_blocked_kernel(const int32_t* __rest
   __shared__ double ys[3 * FBOX3];
   double q = 0.0;
   for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
-    const int bid = fblk[bi];
-    const int b0 = (bid % nbx) * FACE_BLK, b1 = ((bid / nbx) % nby) * FACE_BLK, b2 = (bid / (nbx * nby)) * FACE_BLK;
+    const uint32_t bc = (uint32_t)fblk[bi];  // packed block coordinates
+    const int b0 = (int)(bc & 1023u) * FACE_BLK, b1 = (int)((bc >> 10) & 1023u) * FACE_BLK,
+              b2 = (int)(bc >> 20) * FACE_BLK;
     // staging: all loads of a thread are issued before the LDS writes (the
     // x value and its mask bit are independent loads)
     constexpr int NST = (3 * FBOX3 + FB - 1) / FB;
@@ -90,10 +91,10 @@ __global__ __launch_bounds__(FB) void faces_blocked_kernel(const int32_t* __rest
     const int fb = fptr[bi], nfb = fptr[bi + 1] - fb;
     const int items = 3 * nfb;
     for (int it = threadIdx.x; it < items; it += FB) {
-      const int c = it / nfb, f = fb + (it - c * nfb);  // component major, face minor
-      const int code = faces[f];
-      const int e1 = code >> 2, dir = code & 3;
-      const int e[3] = {e1 % nel0, (e1 / nel0) % nel1, e1 / (nel0 * nel1)};
+      const int c = (it >= nfb) + (it >= 2 * nfb), f = fb + (it - c * nfb);  // component major, face minor
+      const uint32_t code = (uint32_t)faces[f];  // ex | ey << 10 | ez << 20 | dir << 30
+      const int dir = (int)(code >> 30);
+      const int e[3] = {(int)(code & 1023u), (int)((code >> 10) & 1023u), (int)((code >> 20) & 1023u)};
       const int l0 = e[0] - b0, l1 = e[1] - b1, l2 = e[2] - b2;
       const int sa = dir == 0 ? FBOX : 1;
       const int sb = dir == 2 ? FBOX : FBOX * FBOX;
