@@ -347,23 +347,35 @@ def test_tiling_variants_and_autotune(ctx):
 
 def test_balanced_partition_splits_tiles_and_keeps_the_operator(ctx):
     """On the 96x48x48 grid the work-balanced partition (variants 0..5) splits
-    tiles; every partition gives the same operator (FP64 to round-off of the
-    scatter order; FP32 and BF16 within their rounding) and the same p.Ap."""
+    tiles; every partition and workgroup count gives the same masked operator
+    (FP64 to round-off of the scatter order; FP32 and BF16 within their
+    rounding; the clamped face puts fixed DOFs into split tiles too) and the
+    same PCG iterates (the per-workgroup p.Ap partials)."""
     nel = (96, 48, 48)
     g = icc.Grid(ctx, nel, [[0, 2.0], [0, 1.0], [0, 1.0]])
     rng = np.random.default_rng(21)
     s = 1e-9 + (1 - 1e-9) * rng.uniform(0.01, 1.0, g.nel_tot) ** 3
-    op = icc.Operator(g, scale=s)
+    free = np.ones(g.ndof, dtype=np.uint8)
+    ix = np.arange(g.ndof_sc) % g.ncp[0]
+    for c in range(3):
+        free[c * g.ndof_sc:(c + 1) * g.ndof_sc][ix == 0] = 0
+    op = icc.Operator(g, scale=s, free=free)
     x = rng.standard_normal(g.ndof)
+    b = free * rng.standard_normal(g.ndof)
     g.set_tiling(6, 2)
     plain_tiles = g.tiling[2]
     ref = op.apply(x)
     ref32 = op.apply(x, prec=icc.ICC_F32)
     refb = op.apply(x, prec=icc.ICC_BF16)
-    for v in (0, 1, 4):
-        g.set_tiling(v, 2)
+    diag = np.maximum(op.diag(), 1e-12)
+    xp, _, _, _ = op.pcg(b, diag, 0.0, 30)
+    assert np.array_equal(ref * (1 - free), np.zeros(g.ndof))
+    for v, wg in ((0, 2), (1, 2), (4, 2), (0, 1), (8, 4)):
+        g.set_tiling(v, wg)
         if v == 0:
             assert g.tiling[2] > plain_tiles  # tiles were split
-        assert O.rel_err(op.apply(x), ref) < 1e-14, v
-        assert O.rel_err(op.apply(x, prec=icc.ICC_F32), ref32) < 1e-6, v
-        assert O.rel_err(op.apply(x, prec=icc.ICC_BF16), refb) < 1e-5, v
+        assert O.rel_err(op.apply(x), ref) < 1e-14, (v, wg)
+        assert O.rel_err(op.apply(x, prec=icc.ICC_F32), ref32) < 1e-6, (v, wg)
+        assert O.rel_err(op.apply(x, prec=icc.ICC_BF16), refb) < 1e-5, (v, wg)
+        xv, _, _, _ = op.pcg(b, diag, 0.0, 30)
+        assert O.rel_err(xv, xp) < 1e-10, (v, wg)
