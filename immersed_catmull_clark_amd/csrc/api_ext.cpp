@@ -1,7 +1,7 @@
 // Structured SIMP cantilever driver above the C ABI: the host side of
 // CantileverTopOpt (rust/src/topopt_iga.rs) with the state solve, energies,
-// sensitivities and filter on the GPU and the optimality-criteria update on
-// the host, in the reference's loop order.
+// sensitivities, filter and optimality-criteria bisection on the GPU, in the
+// reference's loop order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +37,10 @@ struct icc_topopt {
   PcgState* st = nullptr;
   PcgState* st_h = nullptr;
   double* parts = nullptr;
+  // optimality-criteria update: previous and candidate designs, bisection partials and state
+  double *xold = nullptr, *xnew = nullptr, *pmax = nullptr, *ocparts = nullptr, *sparts = nullptr;
+  OcState* ocs = nullptr;
+  OcState* ocs_h = nullptr;
 };
 
 namespace {
@@ -104,15 +108,48 @@ int solve_state_dev(icc_topopt* t, const double* x_dev, const double* u0_dev, do
   return ICC_OK;
 }
 
-// x (host, design) -> t->xt (device, physical densities)
-int physical_densities(icc_topopt* t, const double* x_host) {
+// t->x (device, design) -> t->xt (device, physical densities)
+int physical_densities_dev(icc_topopt* t) {
   hipStream_t st = t->ctx->stream;
-  ICC_HIP(hipMemcpyAsync(t->x, x_host, t->nel * 8, hipMemcpyHostToDevice, st));
   if (t->cfg.filter == 1)
     TRY(icc_filter_fwd_dev(t->filt, t->x, t->xt));
   else
     ICC_HIP(hipMemcpyAsync(t->xt, t->x, t->nel * 8, hipMemcpyDeviceToDevice, st));
   return ICC_OK;
+}
+
+// x (host, design) -> t->xt (device, physical densities)
+int physical_densities(icc_topopt* t, const double* x_host) {
+  ICC_HIP(hipMemcpyAsync(t->x, x_host, t->nel * 8, hipMemcpyHostToDevice, t->ctx->stream));
+  return physical_densities_dev(t);
+}
+
+// OC bisection (topopt_iga.rs:206-222) on the device: t->xnew = the candidate
+// of the last bisection step from t->xold and the filtered sensitivities dc.
+// The sensitivity filter bisects on sum x = volfrac nel. The density filter
+// bisects on sum H x / Hs = sum dv x with dv = H^T (1 / Hs), and divides dc by
+// dv under the square root.
+int oc_update_dev(icc_topopt* t, const double* dc) {
+  const icc_topopt_config& cf = t->cfg;
+  hipStream_t st = t->ctx->stream;
+  const int64_t ne = t->nel;
+  const double* dv = cf.filter == 1 ? t->dv : nullptr;
+  const double target = cf.volfrac * (double)ne;
+  ICC_HIP(launch_oc_init(ne, nullptr, dc, t->xold, t->xnew, t->pmax, st));
+  constexpr int CHUNK = 8, MAX_STEPS = 4096;
+  int k = 0;
+  for (;;) {
+    for (int q = 0; q < CHUNK; ++q, ++k)
+      ICC_HIP(launch_oc_step(k, ne, nullptr, dv, dv, dc, t->xold, t->xnew, target, cf.move_limit, t->pmax,
+                             t->ocparts, t->ocs, st));
+    ICC_HIP(hipMemcpyAsync(t->ocs_h, t->ocs + (k & 1), sizeof(OcState), hipMemcpyDeviceToHost, st));
+    ICC_HIP(hipStreamSynchronize(st));
+    if (t->ocs_h->done) return ICC_OK;
+    if (k >= MAX_STEPS) {
+      set_error("icc_topopt_run: OC bisection did not terminate");
+      return ICC_ENOCONV;
+    }
+  }
 }
 
 double dot_host(const std::vector<double>& a, const std::vector<double>& b) {
@@ -148,11 +185,15 @@ ICC_API int icc_topopt_destroy(icc_topopt* t) {
   icc_filter_destroy(t->filt);
   icc_op_destroy(t->op);
   icc_grid_destroy(t->g);
-  double* bufs[] = {t->x, t->xt, t->s, t->u, t->u0, t->b, t->du, t->dg, t->f, t->dc, t->tmp, t->ones, t->dv, t->parts};
+  double* bufs[] = {t->x,  t->xt,   t->s,    t->u,    t->u0,   t->b,       t->du,      t->dg,
+                    t->f,  t->dc,   t->tmp,  t->ones, t->dv,   t->parts,   t->xold,    t->xnew,
+                    t->pmax, t->ocparts, t->sparts};
   for (double* p : bufs) hipFree(p);
   hipFree(t->fr_d);
   hipFree(t->st);
+  hipFree(t->ocs);
   if (t->st_h) hipHostFree(t->st_h);
+  if (t->ocs_h) hipHostFree(t->ocs_h);
   delete t;
   return ICC_OK;
 }
@@ -196,11 +237,15 @@ ICC_API int icc_topopt_create(icc_ctx* ctx, const icc_topopt_config* cfg, icc_to
       (rc = dalloc(&t->u0, n)) || (rc = dalloc(&t->b, n)) || (rc = dalloc(&t->du, n)) || (rc = dalloc(&t->dg, n)) ||
       (rc = dalloc(&t->f, n)) || (rc = dalloc(&t->dc, ne)) || (rc = dalloc(&t->tmp, n)) ||
       (rc = dalloc(&t->ones, ne)) || (rc = dalloc(&t->dv, ne)) || (rc = dalloc(&t->parts, 2 * VEC_MAX_BLOCKS)) ||
-      (rc = dalloc(&t->fr_d, n)) || (rc = dalloc(&t->st, 1))) {
+      (rc = dalloc(&t->fr_d, n)) || (rc = dalloc(&t->st, 1)) || (rc = dalloc(&t->xold, ne)) ||
+      (rc = dalloc(&t->xnew, ne)) || (rc = dalloc(&t->pmax, VEC_MAX_BLOCKS)) ||
+      (rc = dalloc(&t->ocparts, 2 * VEC_MAX_BLOCKS)) || (rc = dalloc(&t->sparts, 3 * VEC_MAX_BLOCKS)) ||
+      (rc = dalloc(&t->ocs, 2))) {
     icc_topopt_destroy(t);
     return rc;
   }
-  if (hipHostMalloc((void**)&t->st_h, sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&t->st_h, sizeof(PcgState), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&t->ocs_h, sizeof(OcState), hipHostMallocDefault) != hipSuccess) {
     icc_topopt_destroy(t);
     return ICC_ENOMEM;
   }
@@ -289,60 +334,36 @@ ICC_API int icc_topopt_run(icc_topopt* t, icc_report_fn report, void* user, doub
   const icc_topopt_config& cf = t->cfg;
   hipStream_t st = t->ctx->stream;
   const int64_t nel = t->nel, n = t->ndof;
-  std::vector<double> x(nel, cf.volfrac), uh(n), dc(nel), dv(nel, 1.0), xt_h(nel);
+  const int G = topopt_blocks(nel);
+  std::vector<double> sp(3 * (size_t)G);
+  ICC_HIP(launch_fill<double>(t->x, cf.volfrac, nel, st));
   ICC_HIP(launch_fill<double>(t->u, 0.0, n, st));
-  if (cf.filter == 1) ICC_HIP(hipMemcpyAsync(dv.data(), t->dv, nel * 8, hipMemcpyDeviceToHost, st));
   int64_t it_done = 0;
   for (int64_t iter = 1; iter <= cf.max_iter; ++iter) {
-    std::vector<double> x_old = x;
-    TRY(physical_densities(t, x.data()));
+    ICC_HIP(hipMemcpyAsync(t->xold, t->x, nel * 8, hipMemcpyDeviceToDevice, st));
+    TRY(physical_densities_dev(t));
     ICC_HIP(hipMemcpyAsync(t->u0, t->u, n * 8, hipMemcpyDeviceToDevice, st));
     int64_t its;
     double res;
     TRY(solve_state_dev(t, t->xt, t->u0, t->u, &its, &res));
-    ICC_HIP(hipMemcpyAsync(uh.data(), t->u, n * 8, hipMemcpyDeviceToHost, st));
     // dc_raw (topopt_iga.rs:199), fused with the element energies
     TRY(icc_sens_dev(t->op, t->u, t->xt, cf.penal, cf.emin, nullptr, t->dc));
     if (cf.filter == 1)
       TRY(icc_filter_bwd_dev(t->filt, t->dc, t->tmp));
     else
       TRY(icc_filter_sens_dev(t->filt, t->x, t->dc, t->tmp));  // topopt_iga.rs:200-205
-    ICC_HIP(hipMemcpyAsync(dc.data(), t->tmp, nel * 8, hipMemcpyDeviceToHost, st));
+    TRY(oc_update_dev(t, t->tmp));
+    std::swap(t->x, t->xnew);
+    // c = F.u (topopt_iga.rs:197), change = max |x - x_old|, volume = mean x
+    ICC_HIP(launch_topopt_stats(n, t->f, t->u, nel, nullptr, nullptr, t->x, t->xold, t->sparts, st));
+    ICC_HIP(hipMemcpyAsync(sp.data(), t->sparts, sp.size() * 8, hipMemcpyDeviceToHost, st));
     ICC_HIP(hipStreamSynchronize(st));
-    const double c = dot_host(t->force, uh);  // topopt_iga.rs:197
-    // OC bisection (topopt_iga.rs:206-222); the density filter bisects on mean(H x / Hs)
-    double l1 = 0.0, mx = 0.0;
-    for (double v : dc) mx = std::fmax(mx, std::fabs(v));
-    double l2 = 2.0 * mx;
-    std::vector<double> cand = x_old;
-    while ((l2 - l1) / (l1 + l2 + 1e-10) > 1e-4) {
-      const double lmid = 0.5 * (l1 + l2);
-      for (int64_t e = 0; e < nel; ++e) {
-        const double be = std::sqrt(-dc[e] / (dv[e] * lmid));
-        cand[e] = std::fmax(std::fmax(std::fmin(std::fmin(x_old[e] * be, x_old[e] + cf.move_limit), 1.0),
-                                      x_old[e] - cf.move_limit),
-                            0.001);
-      }
-      double sum = 0.0;
-      if (cf.filter == 1) {
-        ICC_HIP(hipMemcpyAsync(t->x, cand.data(), nel * 8, hipMemcpyHostToDevice, st));
-        TRY(icc_filter_fwd_dev(t->filt, t->x, t->xt));
-        ICC_HIP(hipMemcpyAsync(xt_h.data(), t->xt, nel * 8, hipMemcpyDeviceToHost, st));
-        ICC_HIP(hipStreamSynchronize(st));
-        for (double v : xt_h) sum += v;
-      } else {
-        for (double v : cand) sum += v;
-      }
-      if (sum / (double)nel > cf.volfrac)
-        l1 = lmid;
-      else
-        l2 = lmid;
+    double c = 0.0, ch = 0.0, vol = 0.0;
+    for (int i = 0; i < G; ++i) {
+      c += sp[i];
+      ch = std::fmax(ch, sp[G + i]);
+      vol += sp[2 * G + i];
     }
-    x = cand;
-    double ch = 0.0;
-    for (int64_t e = 0; e < nel; ++e) ch = std::fmax(ch, std::fabs(x[e] - x_old[e]));
-    double vol = 0.0;
-    for (double v : x) vol += v;
     vol /= (double)nel;
     if (compliance) compliance[iter - 1] = std::fabs(c);
     if (change) change[iter - 1] = ch;
@@ -351,7 +372,8 @@ ICC_API int icc_topopt_run(icc_topopt* t, icc_report_fn report, void* user, doub
     if (report) report(iter, std::fabs(c), vol, ch, user);
     if (ch < 0.01 && iter >= 15) break;
   }
-  std::memcpy(density, x.data(), nel * 8);
+  ICC_HIP(hipMemcpyAsync(density, t->x, nel * 8, hipMemcpyDeviceToHost, st));
+  ICC_HIP(hipStreamSynchronize(st));
   if (n_iter) *n_iter = it_done;
   return ICC_OK;
 }

@@ -384,8 +384,8 @@ int imm_iteration(icc_imm_topopt* t, double* c_out, double* vol_out, double* ch_
   int k = 0;
   for (;;) {
     for (int q = 0; q < CHUNK; ++q, ++k)
-      ICC_HIP(launch_oc_step(k, ne, t->act_d, t->wd, t->dc, t->xold, t->xnew, target, 0.2, t->pmax, t->parts, t->ocs,
-                             st));
+      ICC_HIP(launch_oc_step(k, ne, t->act_d, t->wd, nullptr, t->dc, t->xold, t->xnew, target, 0.2, t->pmax,
+                             t->parts, t->ocs, st));
     ICC_HIP(hipMemcpyAsync(t->ocs_h, t->ocs + (k & 1), sizeof(OcState), hipMemcpyDeviceToHost, st));
     ICC_HIP(hipStreamSynchronize(st));
     if (t->ocs_h->done) break;

@@ -104,9 +104,9 @@ hipError_t launch_imm_filter(int mode, const int64_t nel[3], const int32_t* off,
                              hipStream_t st);
 hipError_t launch_oc_init(int64_t nel, const uint8_t* act, const double* dc, const double* xold, double* xnew,
                           double* parts_max, hipStream_t st);
-hipError_t launch_oc_step(int k, int64_t nel, const uint8_t* act, const double* w, const double* dc,
-                          const double* xold, double* xnew, double target, double move, const double* parts_max,
-                          double* parts, OcState* st, hipStream_t s);
+hipError_t launch_oc_step(int k, int64_t nel, const uint8_t* act, const double* w, const double* dvs,
+                          const double* dc, const double* xold, double* xnew, double target, double move,
+                          const double* parts_max, double* parts, OcState* st, hipStream_t s);
 hipError_t launch_topopt_stats(int64_t n, const double* f, const double* u, int64_t nel, const uint8_t* act,
                                const double* w, const double* x, const double* xold, double* parts, hipStream_t st);
 hipError_t launch_fill_active(int64_t nel, const uint8_t* act, double v, double* x, hipStream_t st);

@@ -93,14 +93,14 @@ __global__ __launch_bounds__(TB) void imm_filter_kernel(int mode, int n0, int n1
 }
 
 // x_new = active ? x_old : 0 (the candidate before the first bisection step)
-// and per-block max |dc| over the active cells.
+// and per-block max |dc| over the active cells. act == nullptr: every cell is active.
 __global__ __launch_bounds__(TB) void oc_init_kernel(int nel, const uint8_t* __restrict__ act,
                                                      const double* __restrict__ dc, const double* __restrict__ xold,
                                                      double* __restrict__ xnew, double* __restrict__ parts_max) {
   __shared__ double sh[TB / 64];
   double m = 0.0;
   for (int e = blockIdx.x * TB + threadIdx.x; e < nel; e += gridDim.x * TB) {
-    const bool a = act[e] != 0;
+    const bool a = !act || act[e] != 0;
     xnew[e] = a ? xold[e] : 0.0;
     if (a) m = fmax(m, fabs(dc[e]));
   }
@@ -108,8 +108,13 @@ __global__ __launch_bounds__(TB) void oc_init_kernel(int nel, const uint8_t* __r
   if (threadIdx.x == 0) parts_max[blockIdx.x] = m;
 }
 
+// One bisection step k: update [l1, l2] from step k-1's volume, stop on the
+// interval test, else x_new = clamp(x_old sqrt(-dc / (dvs lmid))) and the
+// per-block volume sum of w x_new. act, w and dvs may be null (all active,
+// unit weights, no divisor).
 __global__ __launch_bounds__(TB) void oc_step_kernel(int k, int nel, const uint8_t* __restrict__ act,
-                                                     const double* __restrict__ w, const double* __restrict__ dc,
+                                                     const double* __restrict__ w, const double* __restrict__ dvs,
+                                                     const double* __restrict__ dc,
                                                      const double* __restrict__ xold, double* __restrict__ xnew,
                                                      double target, double move,
                                                      const double* __restrict__ parts_max,
@@ -165,18 +170,19 @@ __global__ __launch_bounds__(TB) void oc_step_kernel(int k, int nel, const uint8
   const double lmid = s_lmid;
   double acc = 0.0;
   for (int e = blockIdx.x * TB + threadIdx.x; e < nel; e += G * TB) {
-    if (!act[e]) continue;
+    if (act && !act[e]) continue;
     const double xo = xold[e];
-    const double be = sqrt(-dc[e] / lmid);
+    const double be = sqrt(-dc[e] / (dvs ? dvs[e] * lmid : lmid));
     const double c = fmax(fmax(fmin(fmin(xo * be, xo + move), 1.0), xo - move), 0.001);
     xnew[e] = c;
-    acc += w[e] * c;
+    acc += w ? w[e] * c : c;
   }
   acc = tb_sum(acc, sh);
   if (threadIdx.x == 0) pout[blockIdx.x] = acc;
 }
 
-// parts[0][b] = sum f.u, parts[1][b] = max |x - x_old| (active), parts[2][b] = sum w x (active)
+// parts[0][b] = sum f.u, parts[1][b] = max |x - x_old| (active), parts[2][b] = sum w x (active);
+// act and w may be null (all active, unit weights)
 __global__ __launch_bounds__(TB) void stats_kernel(int64_t n, const double* __restrict__ f,
                                                    const double* __restrict__ u, int nel,
                                                    const uint8_t* __restrict__ act, const double* __restrict__ w,
@@ -188,9 +194,9 @@ __global__ __launch_bounds__(TB) void stats_kernel(int64_t n, const double* __re
   for (int64_t i = blockIdx.x * (int64_t)TB + threadIdx.x; i < n; i += (int64_t)G * TB) c += f[i] * u[i];
   double ch = 0.0, vol = 0.0;
   for (int e = blockIdx.x * TB + threadIdx.x; e < nel; e += G * TB)
-    if (act[e]) {
+    if (!act || act[e]) {
       ch = fmax(ch, fabs(x[e] - xold[e]));
-      vol += w[e] * x[e];
+      vol += w ? w[e] * x[e] : x[e];
     }
   c = tb_sum(c, sh);
   ch = tb_max(ch, sh);
@@ -231,11 +237,11 @@ hipError_t launch_oc_init(int64_t nel, const uint8_t* act, const double* dc, con
   return hipGetLastError();
 }
 
-hipError_t launch_oc_step(int k, int64_t nel, const uint8_t* act, const double* w, const double* dc,
-                          const double* xold, double* xnew, double target, double move, const double* parts_max,
-                          double* parts, OcState* st, hipStream_t s) {
-  hipLaunchKernelGGL(oc_step_kernel, dim3(topopt_blocks(nel)), dim3(TB), 0, s, k, (int)nel, act, w, dc, xold, xnew,
-                     target, move, parts_max, parts, st);
+hipError_t launch_oc_step(int k, int64_t nel, const uint8_t* act, const double* w, const double* dvs,
+                          const double* dc, const double* xold, double* xnew, double target, double move,
+                          const double* parts_max, double* parts, OcState* st, hipStream_t s) {
+  hipLaunchKernelGGL(oc_step_kernel, dim3(topopt_blocks(nel)), dim3(TB), 0, s, k, (int)nel, act, w, dvs, dc, xold,
+                     xnew, target, move, parts_max, parts, st);
   return hipGetLastError();
 }
 

@@ -278,7 +278,9 @@ int icc_topopt_element_energies(icc_topopt* t, const double* u, double* ee);
  * in, state out.  Tolerance: cfg.pcg_tol. */
 int icc_topopt_compliance(icc_topopt* t, const double* x, double* u_inout, double* c, double* dcdx);
 typedef void (*icc_report_fn)(int64_t iter, double compliance, double volume, double change, void* user);
-/* run (topopt_iga.rs:187-234); arrays sized max_iter (history) and nel (density) */
+/* run (topopt_iga.rs:187-234); arrays sized max_iter (history) and nel (density).
+ * The OC bisection runs on the device; ICC_ENOCONV if it has not ended after
+ * 4096 steps. */
 int icc_topopt_run(icc_topopt* t, icc_report_fn report, void* user, double* compliance, double* change,
                    int64_t* pcg_iterations, double* density, int64_t* n_iter);
 

@@ -47,6 +47,45 @@ def test_topopt_matches_oracle_run(ctx):
     assert np.max(np.abs(h.density - dens)) < 1e-7
 
 
+def test_density_filter_run_matches_host_restatement(ctx):
+    """The run with the density filter (no reference counterpart; the crate
+    filters sensitivities) against the same loop restated here: state solve
+    and dc/dx through the driver's compliance(), warm-started like the run,
+    and the OC bisection of topopt_iga.rs:206-222 in numpy on the physical
+    volume mean(H x / Hs), with the oracle's filter rows and dc divided by
+    H^T (1 / Hs) under the square root."""
+    nel, rmin, vf, move = (16, 8, 4), 1.5, 0.3, 0.2
+    cfg = icc.CantileverTopOptConfig(nel=nel, max_iter=6, pcg_tol=1e-12, filter=1, rmin=rmin, volfrac=vf)
+    t = icc.CantileverTopOpt(ctx, cfg)
+    vols = []
+    h = t.run(report=lambda it, c, v, ch: vols.append(v))
+    sp = O.Space([[0, 1.2], [0, 0.6], [0, 0.3]], list(nel), 2)
+    dv = sp.filter_apply(rmin, 2, None, np.ones(t.nel))
+    x, u = np.full(t.nel, vf), np.zeros(t.ndof)
+    comp, change = [], []
+    for _ in range(len(h.compliance)):
+        c, g, u = t.compliance(x, u0=u)
+        l1, l2 = 0.0, 2.0 * np.max(np.abs(g))
+        cand = x.copy()
+        while (l2 - l1) / (l1 + l2 + 1e-10) > 1e-4:
+            lmid = 0.5 * (l1 + l2)
+            be = np.sqrt(-g / (dv * lmid))
+            cand = np.maximum(np.maximum(np.minimum(np.minimum(x * be, x + move), 1.0), x - move), 0.001)
+            if np.mean(sp.filter_apply(rmin, 1, None, cand)) > vf:
+                l1 = lmid
+            else:
+                l2 = lmid
+        comp.append(abs(c))
+        change.append(np.max(np.abs(cand - x)))
+        x = cand
+    assert np.all(np.abs(np.asarray(h.compliance) - comp) / np.asarray(comp) < 1e-8), (h.compliance, comp)
+    assert np.max(np.abs(np.asarray(h.change) - change)) < 1e-7
+    assert np.max(np.abs(h.density - x)) < 1e-7
+    assert abs(vols[-1] - np.mean(h.density)) < 1e-12
+    # the bisection meets the physical volume target to its interval tolerance
+    assert abs(np.mean(sp.filter_apply(rmin, 1, None, h.density)) - vf) < 1e-3
+
+
 def test_solve_state_matches_oracle(ctx):
     cfg = icc.CantileverTopOptConfig(nel=(12, 6, 4), pcg_tol=1e-11, emin=1e-3)
     t = icc.CantileverTopOpt(ctx, cfg)
