@@ -116,6 +116,8 @@ SIGNATURES = [
     ("icc_immersed_mesh_apply", cint, [vp, vp, vp]),
     ("icc_immersed_mesh_solve", cint, [vp, vp, vp]),
     ("icc_wq_create", cint, [vp, vpp]),
+    ("icc_wq_create_layout", cint, [vp, cint, vpp]),
+    ("icc_debug_wq_rule", i64, [i64, cint, cint, i64, vp]),
     ("icc_wq_destroy", cint, [vp]),
     ("icc_wq_form", cint, [vp, cint, vp, dbl, dbl]),
     ("icc_wq_form_dev", cint, [vp, cint, vp, dbl, dbl]),

@@ -341,14 +341,21 @@ def classify_cylinders(ctx, box_lo, box_hi, cyl, grid_bounds, res, sub=(4, 4, 4)
 
 class WqMatrix:
     """Weighted-quadrature elasticity stiffness of a grid, formed on the GPU
-    (wq_stiffness / wq_stiffness_layout, rust/src/wq.rs:324-514, interior
-    layout): form(density, x, penal, emin) with density ICC_DENSITY_NONE,
-    ICC_DENSITY_ELEMENT (x[nel]) or ICC_DENSITY_SPLINE (x[ndof_sc])."""
+    (wq_stiffness / wq_stiffness_layout, rust/src/wq.rs:324-514):
+    form(density, x, penal, emin) with density ICC_DENSITY_NONE,
+    ICC_DENSITY_ELEMENT (x[nel]) or ICC_DENSITY_SPLINE (x[ndof_sc]).
+    layout: "interior" (WqLayout::Interior, the default of wq_stiffness) or
+    "calabro" (WqLayout::Calabro)."""
 
-    def __init__(self, grid):
+    LAYOUTS = {"interior": 0, "calabro": 1}
+
+    def __init__(self, grid, layout="interior"):
+        if layout not in self.LAYOUTS:
+            raise ValueError(f"unknown WQ layout {layout!r}; expected one of {sorted(self.LAYOUTS)}")
         self.grid = grid
+        self.layout = layout
         h = C.c_void_p()
-        check(lib().icc_wq_create(grid.h, C.byref(h)))
+        check(lib().icc_wq_create_layout(grid.h, self.LAYOUTS[layout], C.byref(h)))
         self.h = h
 
     def form(self, density=0, x=None, penal=3.0, emin=1e-3):

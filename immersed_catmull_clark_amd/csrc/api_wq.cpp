@@ -1,6 +1,6 @@
 // Weighted-quadrature stiffness formation above the C ABI: the GPU
 // counterpart of wq_stiffness / wq_stiffness_layout (rust/src/wq.rs:324-514,
-// interior layout, p = 2).  The per-direction 1D rules and formation tables
+// interior or Calabro point layout, p = 2).  The per-direction 1D rules and formation tables
 // are built on the host once per handle (wq.cpp); forming the matrix for a
 // density field runs three kernels (k_wq.hip): the SIMP factor at the WQ
 // points, the sum-factorised rows, and the in-place symmetrisation.
@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <exception>
 #include <vector>
 
 #include "icc_internal.hpp"
@@ -20,8 +21,11 @@ using namespace icc;
 struct icc_wq {
   icc_grid* g = nullptr;
   double* d_S = nullptr;      // [9 * 125][ndof_sc] stencil matrix
-  double* d_simp = nullptr;   // [27 nel] SIMP factor at the WQ points
+  int np[3] = {0, 0, 0};      // WQ points per direction
+  double* d_simp = nullptr;   // [np0 np1 np2] SIMP factor at the WQ points
   double* d_tabs = nullptr;   // formation tables, directions 0, 1, 2
+  int* d_pstart = nullptr;    // first WQ point of each row, per direction
+  int* d_pbin = nullptr;      // element of each WQ point, per direction
   int* d_pfirst = nullptr;    // first nonzero basis index at each WQ point, per direction
   double* d_pval = nullptr;   // the 3 nonzero basis values there
   double* d_x = nullptr;      // density upload buffer
@@ -55,6 +59,8 @@ ICC_API int icc_wq_destroy(icc_wq* m) {
   hipFree(m->d_S);
   hipFree(m->d_simp);
   hipFree(m->d_tabs);
+  hipFree(m->d_pstart);
+  hipFree(m->d_pbin);
   hipFree(m->d_pfirst);
   hipFree(m->d_pval);
   hipFree(m->d_x);
@@ -64,30 +70,50 @@ ICC_API int icc_wq_destroy(icc_wq* m) {
   return ICC_OK;
 }
 
-ICC_API int icc_wq_create(icc_grid* g, icc_wq** out) {
-  if (!g || !out) return ICC_EINVAL;
+ICC_API int icc_wq_create_layout(icc_grid* g, int layout, icc_wq** out) {
+  if (!g || !out || (layout != ICC_WQ_INTERIOR && layout != ICC_WQ_CALABRO)) return ICC_EINVAL;
   const GridSetup& s = g->s;
+  if (layout == ICC_WQ_CALABRO && (s.nel[0] < 2 || s.nel[1] < 2 || s.nel[2] < 2)) {
+    set_error("icc_wq_create_layout: the Calabro layout needs at least 2 elements per direction (one element "
+              "gives a singular system for the middle function's rule)");
+    return ICC_ENOTSUP;
+  }
   auto* m = new icc_wq();
   m->g = g;
   std::vector<double> tabs, pval;
-  std::vector<int> pfirst;
+  std::vector<int> pstart, pbin, pfirst;
   for (int d = 0; d < 3; ++d) {
-    std::vector<double> t;
-    wq_form_tables_1d(s.nel[d], t);
-    tabs.insert(tabs.end(), t.begin(), t.end());
-    // interior WQ points: 3 per element, the element's 3 functions are nonzero
     WqRulesP2 r;
-    wq_rules_p2(s.nel[d], r);
+    if (layout == ICC_WQ_CALABRO)
+      wq_rules_calabro_p2(s.nel[d], r);
+    else
+      wq_rules_p2(s.nel[d], r);
+    std::vector<double> t;
+    std::vector<int> ps;
+    try {
+      wq_form_tables(r, t, ps);
+    } catch (const std::exception& ex) {
+      delete m;
+      set_error(ex.what());
+      return ICC_ENOTSUP;
+    }
+    tabs.insert(tabs.end(), t.begin(), t.end());
+    pstart.insert(pstart.end(), ps.begin(), ps.end());
+    m->np[d] = r.npt;
+    // the point's element holds every nonzero function there (a knot belongs
+    // to its right element, where the left neighbour's last function is zero)
     for (int q = 0; q < r.npt; ++q) {
-      const int e = q / 3;
+      const int e = r.bin[q];
+      pbin.push_back(e);
       pfirst.push_back(e);
       for (int a = 0; a < 3; ++a) pval.push_back(r.b[(size_t)q * r.ndof + e + a]);
     }
   }
-  const int64_t nsc = s.ndof_sc;
+  const int64_t nsc = s.ndof_sc, npt = (int64_t)m->np[0] * m->np[1] * m->np[2];
   int rc;
-  if ((rc = dalloc(&m->d_S, (size_t)9 * 125 * nsc)) || (rc = dalloc(&m->d_simp, (size_t)27 * s.nel_tot)) ||
-      (rc = dalloc(&m->d_tabs, tabs.size())) || (rc = dalloc(&m->d_pfirst, pfirst.size())) ||
+  if ((rc = dalloc(&m->d_S, (size_t)9 * 125 * nsc)) || (rc = dalloc(&m->d_simp, (size_t)npt)) ||
+      (rc = dalloc(&m->d_tabs, tabs.size())) || (rc = dalloc(&m->d_pstart, pstart.size())) ||
+      (rc = dalloc(&m->d_pbin, pbin.size())) || (rc = dalloc(&m->d_pfirst, pfirst.size())) ||
       (rc = dalloc(&m->d_pval, pval.size())) || (rc = dalloc(&m->d_x, std::max<int64_t>(s.nel_tot, nsc))) ||
       (rc = dalloc(&m->d_v, s.ndof)) || (rc = dalloc(&m->d_w, s.ndof))) {
     icc_wq_destroy(m);
@@ -96,6 +122,8 @@ ICC_API int icc_wq_create(icc_grid* g, icc_wq** out) {
   hipStream_t st = g->ctx->stream;
   hipError_t e;
   if ((e = hipMemcpyAsync(m->d_tabs, tabs.data(), tabs.size() * 8, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(m->d_pstart, pstart.data(), pstart.size() * 4, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(m->d_pbin, pbin.data(), pbin.size() * 4, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(m->d_pfirst, pfirst.data(), pfirst.size() * 4, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(m->d_pval, pval.data(), pval.size() * 8, hipMemcpyHostToDevice, st)) ||
       (e = hipStreamSynchronize(st))) {
@@ -106,12 +134,31 @@ ICC_API int icc_wq_create(icc_grid* g, icc_wq** out) {
   return ICC_OK;
 }
 
+ICC_API int icc_wq_create(icc_grid* g, icc_wq** out) { return icc_wq_create_layout(g, ICC_WQ_INTERIOR, out); }
+
+ICC_API int64_t icc_debug_wq_rule(int64_t nel, int layout, int which, int64_t i, double* out) {
+  if (nel < 1 || (layout != ICC_WQ_INTERIOR && layout != ICC_WQ_CALABRO) || which < 0 || which > 4)
+    return -(int64_t)ICC_EINVAL;
+  if (layout == ICC_WQ_CALABRO && nel < 2) return -(int64_t)ICC_ENOTSUP;
+  WqRulesP2 r;
+  if (layout == ICC_WQ_CALABRO)
+    wq_rules_calabro_p2(nel, r);
+  else
+    wq_rules_p2(nel, r);
+  if (which > 0 && (i < 0 || i >= r.ndof)) return -(int64_t)ICC_EINVAL;
+  const std::vector<double>* w[4] = {&r.w0[0], &r.w1[0], &r.w2[0], &r.w3[0]};
+  const std::vector<double>& v = which == 0 ? r.pts : w[which - 1][i];
+  if (out) std::memcpy(out, v.data(), v.size() * sizeof(double));
+  return (int64_t)v.size();
+}
+
 ICC_API int icc_wq_form_dev(icc_wq* m, int density, const double* x_dev, double penal, double emin) {
   if (!m || density < ICC_DENSITY_NONE || density > ICC_DENSITY_SPLINE || (density != ICC_DENSITY_NONE && !x_dev))
     return ICC_EINVAL;
   hipStream_t st = m->g->ctx->stream;
-  ICC_HIP(launch_wq_simp(m->g, density, x_dev, penal, emin, m->d_pfirst, m->d_pval, m->d_simp, st));
-  ICC_HIP(launch_wq_form(m->g, m->d_tabs, m->d_simp, m->d_S, st));
+  ICC_HIP(launch_wq_simp(m->g, density, m->np, x_dev, penal, emin, m->d_pbin, m->d_pfirst, m->d_pval, m->d_simp,
+                         st));
+  ICC_HIP(launch_wq_form(m->g, m->np, m->d_pstart, m->d_tabs, m->d_simp, m->d_S, st));
   m->formed = 1;
   return ICC_OK;
 }

@@ -101,11 +101,20 @@ double design_filter_offsets(const GridSetup& s, double rmin, std::vector<int32_
 // WQ rules for p = 2 (wq.cpp)
 struct WqRulesP2 {
   int nel = 0, ndof = 0, npt = 0;
-  std::vector<std::vector<int>> ind;             // points of each test function
-  std::vector<std::vector<double>> w0, w1;       // test-value and test-derivative weights on ind
+  std::vector<double> pts;                       // WQ points, ascending
+  std::vector<int> bin;                          // element of each point (a knot goes to its right element)
+  std::vector<std::vector<int>> ind;             // points of each test function (contiguous)
+  // weights on ind of the four contractions: W00 (trial B, test B), W10 (trial B,
+  // test B'), W01 (trial B', test B), W11 (trial B', test B')
+  std::vector<std::vector<double>> w0, w1, w2, w3;
   std::vector<double> b, db;                     // [npt][ndof] basis values / parametric derivatives
 };
+enum { WQ_LAYOUT_INTERIOR = 0, WQ_LAYOUT_CALABRO = 1 };
 void wq_rules_p2(int64_t nel, WqRulesP2& r);
+void wq_rules_calabro_p2(int64_t nel, WqRulesP2& r);
+// Formation tables M[((i * 4 + t) * 5 + jr) * 9 + qc] of one direction and the
+// first point pstart[i] of each row (rows have at most 9 points).
+void wq_form_tables(const WqRulesP2& r, std::vector<double>& M, std::vector<int>& pstart);
 void wq_form_tables_1d(int64_t nel, std::vector<double>& M);
 
 }  // namespace icc

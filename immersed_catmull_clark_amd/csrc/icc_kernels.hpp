@@ -111,10 +111,12 @@ hipError_t launch_topopt_stats(int64_t n, const double* f, const double* u, int6
                                const double* w, const double* x, const double* xold, double* parts, hipStream_t st);
 hipError_t launch_fill_active(int64_t nel, const uint8_t* act, double v, double* x, hipStream_t st);
 
-// k_wq.hip: WQ stiffness formation (stencil storage [9 * 125][ndof_sc]) and its product
-hipError_t launch_wq_simp(const icc_grid* g, int kind, const double* x, double penal, double emin,
-                          const int* pfirst, const double* pval, double* simp, hipStream_t st);
-hipError_t launch_wq_form(const icc_grid* g, const double* tabs, const double* simp, double* S, hipStream_t st);
+// k_wq.hip: WQ stiffness formation (stencil storage [9 * 125][ndof_sc]) and its product;
+// np[d] WQ points per direction, pstart / pbin per-direction tables (concatenated)
+hipError_t launch_wq_simp(const icc_grid* g, int kind, const int np[3], const double* x, double penal, double emin,
+                          const int* pbin, const int* pfirst, const double* pval, double* simp, hipStream_t st);
+hipError_t launch_wq_form(const icc_grid* g, const int np[3], const int* pstart, const double* tabs,
+                          const double* simp, double* S, hipStream_t st);
 hipError_t launch_wq_spmv(const icc_grid* g, const double* S, const double* x, double* y, hipStream_t st);
 
 // k_immersed.hip

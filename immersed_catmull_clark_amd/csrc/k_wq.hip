@@ -1,6 +1,6 @@
 // Weighted-quadrature (WQ) formation of the elasticity stiffness on the GPU:
 // the sum-factorised row assembly of wq_stiffness_layout
-// (rust/src/wq.rs:329-514, interior layout, p = 2), written into a stencil
+// (rust/src/wq.rs:329-514, interior or Calabro point layout, p = 2), written into a stencil
 // matrix (every scalar control point couples to the 5 x 5 x 5 box of control
 // points around it), then symmetrised in place, 0.5 (K + K^T), as the
 // reference does with its mirrored triplets.
@@ -8,7 +8,8 @@
 // Row of test function P = (i0, i1, i2), components (i, j):
 //   K[(i,P),(j,Q)] = sum_{k1,k2} coef(i,j,k1,k2) R_{k1k2}[Q - P],
 //   R_{k1k2} = (B_0^{t0} x B_1^{t1} x B_2^{t2}) s_loc,
-// with s_loc the SIMP factor on the row's 9 x 9 x 9 WQ points and B_d^t the
+// with s_loc the SIMP factor on the row's 9 x 9 x 9 WQ points (the row's
+// points in each direction are contiguous from pstart, at most 9) and B_d^t the
 // 5 x 9 per-direction tables of wq.cpp (t = 0: W00 B, 1: W10 B, 2: W01 B',
 // 3: W11 B'), t_d = 3 on k1 = k2 = d, 1 on d = k1 != k2, 2 on d = k2 != k1,
 // else 0; coef(i,j,k1,k2) = det / (L_k1 L_k2) (lambda d_{i k1} d_{j k2} +
@@ -37,20 +38,21 @@ struct WqCoef {
   double c[81];  // coef(i, j, k1, k2) at ((i * 3 + j) * 3 + k1) * 3 + k2
 };
 
-// simp factor at every WQ point (points x fastest; 3 per element and direction)
-// kind 0: 1; kind 1: element densities x[nel]; kind 2: spline densities x[ndof_sc]
-// evaluated with the per-direction point bases bt[d] (first index, 3 values).
-__global__ __launch_bounds__(256) void wq_simp_kernel(int kind, int nel0, int nel1, int nel2, int ncp0, int ncp1,
-                                                      const double* __restrict__ x, double penal, double emin,
+// simp factor at every WQ point (points x fastest; np_d points in direction d)
+// kind 0: 1; kind 1: element densities x[nel], the element of a point from
+// pbin (per direction, concatenated); kind 2: spline densities x[ndof_sc]
+// evaluated with the per-direction point bases (first index pfirst, 3 values pval).
+__global__ __launch_bounds__(256) void wq_simp_kernel(int kind, int np0, int np1, int np2, int nel0, int nel1,
+                                                      int ncp0, int ncp1, const double* __restrict__ x, double penal,
+                                                      double emin, const int* __restrict__ pbin,
                                                       const int* __restrict__ pfirst, const double* __restrict__ pval,
                                                       double* __restrict__ simp) {
-  const int np0 = 3 * nel0, np1 = 3 * nel1, np2 = 3 * nel2;
   const int64_t npt = (int64_t)np0 * np1 * np2;
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < npt; g += (int64_t)gridDim.x * blockDim.x) {
     const int p0 = (int)(g % np0), p1 = (int)((g / np0) % np1), p2 = (int)(g / ((int64_t)np0 * np1));
     double v = 1.0;
     if (kind == 1) {
-      const double xe = x[p0 / 3 + nel0 * (p1 / 3 + nel1 * (p2 / 3))];
+      const double xe = x[pbin[p0] + nel0 * (pbin[np0 + p1] + nel1 * pbin[np0 + np1 + p2])];
       v = emin + pow(xe, penal) * (1.0 - emin);
     } else if (kind == 2) {
       const int q[3] = {p0, p1 + np0, p2 + np0 + np1};
@@ -71,8 +73,9 @@ __global__ __launch_bounds__(256) void wq_simp_kernel(int kind, int nel0, int ne
   }
 }
 
-__global__ __launch_bounds__(64 * WQ_WAVES) void wq_form_kernel(int ncp0, int ncp1, int ncp2, int nel0, int nel1,
-                                                                int nel2, const double* __restrict__ tab0,
+__global__ __launch_bounds__(64 * WQ_WAVES) void wq_form_kernel(int ncp0, int ncp1, int ncp2, int np0, int np1,
+                                                                int np2, const int* __restrict__ pstart,
+                                                                const double* __restrict__ tab0,
                                                                 const double* __restrict__ tab1,
                                                                 const double* __restrict__ tab2,
                                                                 const double* __restrict__ simp, WqCoef cf,
@@ -87,9 +90,8 @@ __global__ __launch_bounds__(64 * WQ_WAVES) void wq_form_kernel(int ncp0, int nc
   const bool live = P < nsc;
   const int i0 = live ? (int)(P % ncp0) : 0, i1 = live ? (int)((P / ncp0) % ncp1) : 0,
             i2 = live ? (int)(P / ((int64_t)ncp0 * ncp1)) : 0;
-  const int np0 = 3 * nel0, np1 = 3 * nel1, np2 = 3 * nel2;
-  // first WQ point of the row in each direction: 3 * max(0, i - 2)
-  const int f0 = 3 * (i0 > 2 ? i0 - 2 : 0), f1 = 3 * (i1 > 2 ? i1 - 2 : 0), f2 = 3 * (i2 > 2 ? i2 - 2 : 0);
+  // first WQ point of the row in each direction
+  const int f0 = pstart[i0], f1 = pstart[ncp0 + i1], f2 = pstart[ncp0 + ncp1 + i2];
   for (int k = l; k < 729; k += 64) {
     const int q0 = k % 9, q1 = (k / 9) % 9, q2 = k / 81;
     const int p0 = f0 + q0, p1 = f1 + q1, p2 = f2 + q2;
@@ -214,16 +216,18 @@ static int grid_blocks(int64_t n) {
   return (int)(nb > 65536 ? 65536 : (nb < 1 ? 1 : nb));
 }
 
-hipError_t launch_wq_simp(const icc_grid* g, int kind, const double* x, double penal, double emin,
-                          const int* pfirst, const double* pval, double* simp, hipStream_t st) {
+hipError_t launch_wq_simp(const icc_grid* g, int kind, const int np[3], const double* x, double penal, double emin,
+                          const int* pbin, const int* pfirst, const double* pval, double* simp, hipStream_t st) {
   const auto& s = g->s;
-  const int64_t npt = 27 * s.nel_tot;
-  hipLaunchKernelGGL(wq_simp_kernel, dim3(grid_blocks(npt)), dim3(256), 0, st, kind, (int)s.nel[0], (int)s.nel[1],
-                     (int)s.nel[2], (int)s.ncp[0], (int)s.ncp[1], x, penal, emin, pfirst, pval, simp);
+  const int64_t npt = (int64_t)np[0] * np[1] * np[2];
+  hipLaunchKernelGGL(wq_simp_kernel, dim3(grid_blocks(npt)), dim3(256), 0, st, kind, np[0], np[1], np[2],
+                     (int)s.nel[0], (int)s.nel[1], (int)s.ncp[0], (int)s.ncp[1], x, penal, emin, pbin, pfirst, pval,
+                     simp);
   return hipGetLastError();
 }
 
-hipError_t launch_wq_form(const icc_grid* g, const double* tabs, const double* simp, double* S, hipStream_t st) {
+hipError_t launch_wq_form(const icc_grid* g, const int np[3], const int* pstart, const double* tabs,
+                          const double* simp, double* S, hipStream_t st) {
   const auto& s = g->s;
   WqCoef cf;
   const double det = s.len[0] * s.len[1] * s.len[2];
@@ -241,7 +245,7 @@ hipError_t launch_wq_form(const icc_grid* g, const double* tabs, const double* s
   const int64_t nsc = s.ndof_sc;
   const int nb = (int)((nsc + WQ_WAVES - 1) / WQ_WAVES);
   hipLaunchKernelGGL(wq_form_kernel, dim3(nb), dim3(64 * WQ_WAVES), 0, st, (int)s.ncp[0], (int)s.ncp[1],
-                     (int)s.ncp[2], (int)s.nel[0], (int)s.nel[1], (int)s.nel[2], t0, t1, t2, simp, cf, S);
+                     (int)s.ncp[2], np[0], np[1], np[2], pstart, t0, t1, t2, simp, cf, S);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(wq_sym_kernel, dim3(grid_blocks(9 * 125 * nsc)), dim3(256), 0, st, (int)s.ncp[0],

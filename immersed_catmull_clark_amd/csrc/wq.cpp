@@ -15,6 +15,7 @@
 // the per-direction tables of the full sum-factorised formation (k_wq.hip).
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
 #include <vector>
 
 #include "icc_internal.hpp"
@@ -24,12 +25,13 @@ namespace icc {
 namespace {
 
 // Cox-de Boor values of all degree-p functions of `knots` at x (general knot
-// vector, span clamped to [p, ndof-1] like the reference's find_span).
-void basis_all(const std::vector<double>& knots, int p, double x, std::vector<double>& out) {
+// vector, span clamped to [p, smax], smax = ndof-1 by default, like the
+// reference's find_span).
+void basis_all(const std::vector<double>& knots, int p, double x, std::vector<double>& out, int smax = -1) {
   const int ndof = (int)knots.size() - p - 1;
   out.assign(ndof, 0.0);
   int cnt = (int)(std::upper_bound(knots.begin(), knots.end(), x) - knots.begin());
-  int s = std::min(std::max(cnt - 1, p), ndof - 1);
+  int s = std::min(std::max(cnt - 1, p), smax >= 0 ? smax : ndof - 1);
   std::vector<double> b(1, 1.0);
   for (int k = 1; k <= p; ++k) {
     std::vector<double> bn(k + 1, 0.0);
@@ -55,7 +57,10 @@ void basis_all(const std::vector<double>& knots, int p, double x, std::vector<do
 void dbasis_all(const std::vector<double>& knots, int p, double x, std::vector<double>& out) {
   const int ndof = (int)knots.size() - p - 1;
   std::vector<double> lower;
-  basis_all(knots, p - 1, x, lower);  // degree p-1 on the same knots: ndof + 1 functions
+  // degree p-1 on the same knots (ndof + 1 functions), on the span of the
+  // degree-p evaluation: at the last knot, the degree-(p-1) clamp would pick
+  // the empty span between repeated end knots, where every value is 0
+  basis_all(knots, p - 1, x, lower, ndof - 1);
   out.assign(ndof, 0.0);
   for (int i = 0; i < ndof; ++i) {
     double v = 0.0;
@@ -114,6 +119,47 @@ std::vector<double> min_norm(const std::vector<double>& A, int m, int n, const s
   return y;
 }
 
+// Minimum-norm solution through the normal equations: (A A^T) y = rhs by
+// Gaussian elimination with partial pivoting (the last row of largest
+// magnitude on ties), w = A^T y.  The Calabro rules are defined with this
+// solver (wq.rs:33-62), which also fixes their value where A A^T is singular
+// (a single-element direction).
+std::vector<double> min_norm_normal(const std::vector<double>& A, int m, int n, const std::vector<double>& rhs) {
+  std::vector<double> G((size_t)m * m), y = rhs;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < n; ++k) s += A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      G[(size_t)i * m + j] = s;
+    }
+  for (int c = 0; c < m; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < m; ++i)
+      if (std::fabs(G[(size_t)i * m + c]) >= std::fabs(G[(size_t)piv * m + c])) piv = i;
+    if (piv != c) {
+      for (int j = 0; j < m; ++j) std::swap(G[(size_t)c * m + j], G[(size_t)piv * m + j]);
+      std::swap(y[c], y[piv]);
+    }
+    for (int i = c + 1; i < m; ++i) {
+      const double f = G[(size_t)i * m + c] / G[(size_t)c * m + c];
+      for (int j = c; j < m; ++j) G[(size_t)i * m + j] -= f * G[(size_t)c * m + j];
+      y[i] -= f * y[c];
+    }
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double s = 0.0;
+    for (int j = i + 1; j < m; ++j) s += G[(size_t)i * m + j] * y[j];
+    y[i] = (y[i] - s) / G[(size_t)i * m + i];
+  }
+  std::vector<double> w(n);
+  for (int k = 0; k < n; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < m; ++i) s += A[(size_t)i * n + k] * y[i];
+    w[k] = s;
+  }
+  return w;
+}
+
 }  // namespace
 
 // WQ rules of rules_interior (wq.rs:141-240) for p = 2 on the open uniform
@@ -165,6 +211,8 @@ void wq_rules_p2(int64_t nel, WqRulesP2& r) {
   r.nel = ne;
   r.ndof = ndof;
   r.npt = npt;
+  r.pts = pts;
+  r.bin = el;
   r.b.assign((size_t)npt * ndof, 0.0);
   r.db.assign((size_t)npt * ndof, 0.0);
   for (int q = 0; q < npt; ++q)
@@ -237,10 +285,117 @@ void wq_rules_p2(int64_t nel, WqRulesP2& r) {
     r.w0[i] = w0;
     r.w1[i] = w1;
   }
+  r.w2 = r.w0;
+  r.w3 = r.w1;
+}
+
+namespace {
+// open uniform knot vector of degree 2 on [0, 1] with nel elements
+std::vector<double> open_knots_p2(int64_t nel) {
+  std::vector<double> knots(3, 0.0);
+  for (int64_t i = 1; i < nel; ++i) knots.push_back((double)i / (double)nel);
+  for (int i = 0; i < 3; ++i) knots.push_back(1.0);
+  return knots;
+}
+
+// n equispaced values from a to b with both ends exact (wq.rs:26-31)
+void push_linspace(double a, double b, int n, std::vector<double>& out) {
+  for (int i = 0; i < n; ++i) out.push_back(i == 0 ? a : (i == n - 1 ? b : a + (double)i * (b - a) / (double)(n - 1)));
+}
+}  // namespace
+
+// WQ rules of rules_calabro (wq.rs:242-310) for p = 2: p + 2 equispaced points
+// in each end element, the interior knots and the midpoints of the interior
+// elements; for each test function i its points are those where B_i is
+// nonzero, and the four rules are unweighted minimum-norm solutions exact on
+// the spline space (W00, W10: trials B_j, j in the support's elements) and on
+// its derivative space (W01, W11: degree-1 functions of the knot vector
+// without its end knots), against 3-point Gauss integrals over supp(B_i).
+void wq_rules_calabro_p2(int64_t nel, WqRulesP2& r) {
+  const int p = 2;
+  const std::vector<double> knots = open_knots_p2(nel);
+  std::vector<double> brk;
+  for (double k : knots)
+    if (brk.empty() || brk.back() != k) brk.push_back(k);
+  const int nb = (int)brk.size(), ne = nb - 1;
+  std::vector<double> pts;
+  push_linspace(brk[0], brk[1], p + 2, pts);
+  for (int k = 2; k < nb - 2; ++k) pts.push_back(brk[k]);
+  for (int k = 1; k < nb - 2; ++k) pts.push_back(0.5 * (brk[k] + brk[k + 1]));
+  push_linspace(brk[nb - 2], brk[nb - 1], p + 2, pts);
+  std::sort(pts.begin(), pts.end());
+  pts.erase(std::unique(pts.begin(), pts.end()), pts.end());
+  const int npt = (int)pts.size(), ndof = (int)knots.size() - p - 1;
+  // derivative space: degree 1 on the knot vector without its first and last knot
+  const std::vector<double> kd(knots.begin() + 1, knots.end() - 1);
+  r.nel = ne;
+  r.ndof = ndof;
+  r.npt = npt;
+  r.pts = pts;
+  r.bin.resize(npt);
+  r.b.assign((size_t)npt * ndof, 0.0);
+  r.db.assign((size_t)npt * ndof, 0.0);
+  std::vector<std::vector<double>> Bd(npt);
+  std::vector<double> tmp;
+  for (int q = 0; q < npt; ++q) {
+    const int e = (int)(std::upper_bound(brk.begin(), brk.end(), pts[q]) - brk.begin()) - 1;
+    r.bin[q] = std::min(std::max(e, 0), ne - 1);
+    basis_all(knots, p, pts[q], tmp);
+    for (int j = 0; j < ndof; ++j) r.b[(size_t)q * ndof + j] = tmp[j];
+    dbasis_all(knots, p, pts[q], tmp);
+    for (int j = 0; j < ndof; ++j) r.db[(size_t)q * ndof + j] = tmp[j];
+    basis_all(kd, p - 1, pts[q], Bd[q]);
+  }
+  const double gx[3] = {-std::sqrt(0.6), 0.0, std::sqrt(0.6)}, gw[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  r.ind.assign(ndof, {});
+  r.w0.assign(ndof, {});
+  r.w1.assign(ndof, {});
+  r.w2.assign(ndof, {});
+  r.w3.assign(ndof, {});
+  for (int i = 0; i < ndof; ++i) {
+    std::vector<int> ind;
+    for (int q = 0; q < npt; ++q)
+      if (r.b[(size_t)q * ndof + i] != 0.0) ind.push_back(q);
+    // support elements of B_i; trials: functions e..e+2 (space) and e..e+1 (derivative space) of each
+    const int e_lo = std::max(0, i - p), e_hi = std::min(i, ne - 1);
+    const int jp_lo = e_lo, jp_hi = e_hi + p, jd_lo = e_lo, jd_hi = e_hi + p - 1;
+    std::vector<double> xg, wg;
+    for (int e = e_lo; e <= e_hi; ++e) {
+      const double a = brk[e], h = brk[e + 1] - brk[e];
+      for (int g = 0; g < 3; ++g) {
+        xg.push_back(a + 0.5 * (gx[g] + 1.0) * h);
+        wg.push_back(0.5 * gw[g] * h);
+      }
+    }
+    const int nq = (int)ind.size(), ng = (int)xg.size();
+    std::vector<std::vector<double>> Bg(ng), dBg(ng), Bdg(ng);
+    for (int g = 0; g < ng; ++g) {
+      basis_all(knots, p, xg[g], Bg[g]);
+      dbasis_all(knots, p, xg[g], dBg[g]);
+      basis_all(kd, p - 1, xg[g], Bdg[g]);
+    }
+    auto rule = [&](bool deriv_trial, bool deriv_test) {
+      const int lo = deriv_trial ? jd_lo : jp_lo, m = (deriv_trial ? jd_hi : jp_hi) - lo + 1;
+      std::vector<double> A((size_t)m * nq), rhs(m);
+      for (int rr = 0; rr < m; ++rr) {
+        const int j = lo + rr;
+        for (int c = 0; c < nq; ++c) A[(size_t)rr * nq + c] = deriv_trial ? Bd[ind[c]][j] : r.b[(size_t)ind[c] * ndof + j];
+        double s = 0.0;
+        for (int g = 0; g < ng; ++g) s += (deriv_trial ? Bdg[g][j] : Bg[g][j]) * wg[g] * (deriv_test ? dBg[g][i] : Bg[g][i]);
+        rhs[rr] = s;
+      }
+      return min_norm_normal(A, m, nq, rhs);
+    };
+    r.ind[i] = ind;
+    r.w0[i] = rule(false, false);
+    r.w1[i] = rule(false, true);
+    r.w2[i] = rule(true, false);
+    r.w3[i] = rule(true, true);
+  }
 }
 
 // A0[(i*3 + t)], A1[(i*3 + t)]: sums over the WQ points of element
-// first(i) + t for the 1D function i, first(i) = max(0, i - 2).
+// first(i) + t for the 1D function i, first(i) = max(0, i - 2) (interior layout).
 void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1) {
   WqRulesP2 r;
   wq_rules_p2(nel, r);
@@ -255,27 +410,36 @@ void wq_tables_1d(int64_t nel, std::vector<double>& A0, std::vector<double>& A1)
 }
 
 // Formation tables of one direction for the sum-factorised WQ stiffness:
-// M[((i * 4 + t) * 5 + jr) * 9 + qc] = w_t(i, qc) * val_t(point ind_i[qc], trial i - 2 + jr)
-// with (w, val) = (w0, B), (w1, B), (w0, B'), (w1, B') for t = 0..3 (the
-// W00, W10, W01, W11 contractions of wq.rs:418-438; B' parametric), zero for
-// missing trials or points; the row's points are those of elements
-// first(i) + qc / 3, point qc % 3 of each.
-void wq_form_tables_1d(int64_t nel, std::vector<double>& M) {
-  WqRulesP2 r;
-  wq_rules_p2(nel, r);
+// M[((i * 4 + t) * 5 + jr) * 9 + qc] = W_t(i, qc) * val_t(point pstart[i] + qc, trial i - 2 + jr)
+// with (W, val) = (W00, B), (W10, B), (W01, B'), (W11, B') for t = 0..3 (the
+// contractions of wq.rs:418-438; B' parametric), zero for missing trials or
+// points; pstart[i] is the first of the row's (contiguous) points.
+void wq_form_tables(const WqRulesP2& r, std::vector<double>& M, std::vector<int>& pstart) {
   M.assign((size_t)r.ndof * 4 * 5 * 9, 0.0);
-  for (int i = 0; i < r.ndof; ++i)
+  pstart.assign(r.ndof, 0);
+  for (int i = 0; i < r.ndof; ++i) {
+    if (r.ind[i].empty()) continue;
+    pstart[i] = r.ind[i][0];
     for (size_t c = 0; c < r.ind[i].size(); ++c) {
       const int q = r.ind[i][c];
-      const int qc = q - 3 * std::max(0, i - 2);
+      const int qc = q - pstart[i];
+      if (qc >= 9) throw std::logic_error("wq_form_tables: more than 9 points in a row");
       for (int jr = 0; jr < 5; ++jr) {
         const int j = i - 2 + jr;
         if (j < 0 || j >= r.ndof) continue;
         const double bv = r.b[(size_t)q * r.ndof + j], dv = r.db[(size_t)q * r.ndof + j];
-        const double wt[4] = {r.w0[i][c] * bv, r.w1[i][c] * bv, r.w0[i][c] * dv, r.w1[i][c] * dv};
+        const double wt[4] = {r.w0[i][c] * bv, r.w1[i][c] * bv, r.w2[i][c] * dv, r.w3[i][c] * dv};
         for (int t = 0; t < 4; ++t) M[(((size_t)i * 4 + t) * 5 + jr) * 9 + qc] = wt[t];
       }
     }
+  }
+}
+
+void wq_form_tables_1d(int64_t nel, std::vector<double>& M) {
+  WqRulesP2 r;
+  wq_rules_p2(nel, r);
+  std::vector<int> pstart;
+  wq_form_tables(r, M, pstart);
 }
 
 }  // namespace icc

@@ -207,7 +207,7 @@ int icc_wq_diag_dev(icc_grid* g, const double* x_dev, double penal, double emin,
 
 /* ---- WQ stiffness formation ------------------------------------------------ */
 /* GPU formation of wq_stiffness / wq_stiffness_layout (rust/src/wq.rs:324-514,
- * interior layout, p = 2): the WQ elasticity stiffness of the grid's space
+ * p = 2): the WQ elasticity stiffness of the grid's space
  * (its young / poisson), component-blocked and symmetrised 0.5 (K + K^T),
  * with the SIMP factor emin + rho^penal (1 - emin) at the WQ points.
  * density: ICC_DENSITY_NONE (factor 1), ICC_DENSITY_ELEMENT (x[nel],
@@ -215,8 +215,15 @@ int icc_wq_diag_dev(icc_grid* g, const double* x_dev, double penal, double emin,
  * densities on the displacement space, Density::Spline).  Stored on the
  * device as a 5x5x5-stencil matrix (9 * 125 * ndof_sc doubles). */
 enum { ICC_DENSITY_NONE = 0, ICC_DENSITY_ELEMENT = 1, ICC_DENSITY_SPLINE = 2 };
+/* WqLayout (wq.rs:116-126): Interior (the default of wq_stiffness) or Calabro */
+enum { ICC_WQ_INTERIOR = 0, ICC_WQ_CALABRO = 1 };
 typedef struct icc_wq icc_wq;
+/* wq_stiffness: interior layout */
 int icc_wq_create(icc_grid* g, icc_wq** out);
+/* wq_stiffness_layout (wq.rs:329): layout ICC_WQ_INTERIOR or ICC_WQ_CALABRO.
+ * ICC_ENOTSUP for the Calabro layout on a grid with one element in a
+ * direction: the middle function's rule there solves a singular system. */
+int icc_wq_create_layout(icc_grid* g, int layout, icc_wq** out);
 int icc_wq_destroy(icc_wq* m);
 int icc_wq_form(icc_wq* m, int density, const double* x, double penal, double emin);
 int icc_wq_form_dev(icc_wq* m, int density, const double* x_dev, double penal, double emin);
@@ -472,6 +479,12 @@ int icc_debug_mfma_layout(icc_ctx* ctx, double* out512);
  * and B[8 (l/16) + j][l%16]; equals D[4 (l/16) + r][l%16] = (A B) if the
  * assumed layout holds. */
 int icc_debug_mfma_layout_bf16(icc_ctx* ctx, double* out256);
+/* Host only (no device work): the 1D p = 2 WQ rule the formation tables are
+ * built from, on the open uniform knot vector of nel elements over [0, 1]
+ * (wq_rules_1d_layout, wq.rs:134-139).  which 0: the points; 1..4: the
+ * weights W00, W10, W01, W11 of test function i on its points.  Returns the
+ * length (out may be NULL), -ICC_EINVAL, or -ICC_ENOTSUP (Calabro, nel = 1). */
+int64_t icc_debug_wq_rule(int64_t nel, int layout, int which, int64_t i, double* out);
 
 #ifdef __cplusplus
 }

@@ -43,3 +43,47 @@ def test_library_reports_errors_without_gpu():
     import immersed_catmull_clark_amd as m
     with pytest.raises(m.IccError):
         m.Context(0)
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_host_wq_rules_match_oracle(layout):
+    """The 1D p = 2 WQ rules the GPU formation tables are built from (host
+    code, no device work) against the oracle's wq_rules_1d_layout
+    (wq.rs:134-310), which the wq_fastformation.json points and weights pin:
+    points, and the four weights of every test function, for 1 to 9 elements
+    (the end-element cases, and interior rows far from both ends). Each row's
+    points are contiguous and at most 9, as the formation kernel assumes.
+    The Calabro layout starts at 2 elements: with one, the middle function
+    has 3 trials on 2 points, its Gram matrix is singular, and the rule is
+    whatever the rounding of a zero pivot makes of it; the library reports
+    ENOTSUP there."""
+    import numpy as np
+    import oracle_py as O
+    from immersed_catmull_clark_amd._abi import lib
+
+    def rule(nel, which, i=0):
+        n = int(lib().icc_debug_wq_rule(nel, layout, which, i, None))
+        assert n > 0
+        out = np.zeros(n)
+        lib().icc_debug_wq_rule(nel, layout, which, i, out.ctypes.data)
+        return out
+
+    for nel in range(1 + layout, 10):
+        pts = rule(nel, 0)
+        assert np.max(np.abs(pts - O.wq_rule(nel, 2, layout, 0))) < 1e-15
+        for i in range(nel + 2):
+            for which in range(1, 5):
+                w, wr = rule(nel, which, i), O.wq_rule(nel, 2, layout, which, i)
+                assert w.shape == wr.shape and len(w) <= 9, (nel, i, which)
+                assert np.max(np.abs(w - wr)) <= 1e-12 * max(1.0, np.max(np.abs(wr))), (nel, i, which, w, wr)
+    assert lib().icc_debug_wq_rule(4, 2, 0, 0, None) < 0  # unknown layout
+    assert lib().icc_debug_wq_rule(4, layout, 1, 6, None) < 0  # test function out of range
+    assert (lib().icc_debug_wq_rule(1, layout, 0, 0, None) < 0) == (layout == 1)
+    # closed form on 5 elements: interior, 3 points per element at (2m-1)/6 of it;
+    # Calabro, thirds of the end elements, the 4 interior knots and 3 interior midpoints
+    p5 = rule(5, 0)
+    if layout == 0:
+        want = np.array([(e + (2 * m - 1) / 6.0) / 5.0 for e in range(5) for m in (1, 2, 3)])
+    else:
+        want = np.array(sorted({0.0, 1 / 15, 2 / 15, 0.2, 0.4, 0.6, 0.8, 13 / 15, 14 / 15, 1.0, 0.3, 0.5, 0.7}))
+    assert p5.shape == want.shape and np.max(np.abs(p5 - want)) < 1e-15
