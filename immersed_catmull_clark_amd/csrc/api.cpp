@@ -1013,7 +1013,8 @@ static int write_tol(icc_op* op, Work* w, double tol) {
 // One PCG iteration in precision T enqueued on the op stream (sparse term
 // first, then the tiles, then the two vector kernels).
 template <typename T>
-static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, hipStream_t st) {
+static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, hipStream_t st,
+                                    bool bf16 = false) {
   icc_grid* g = op->g;
   const int64_t n = g->s.ndof;
   const int npap = npap_of(op);
@@ -1029,7 +1030,8 @@ static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap
       e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, p, ap, op->d_free,
                                         w->pap + apply_grid(g), st);
     if (e == hipSuccess) e = launch_faces<float>(op, p, ap, w->pap + apply_grid(g) + csr_blocks(op), w->st, st);
-    if (e == hipSuccess) e = launch_apply_f32(g, op, p, ap, w->pap, w->st, st);
+    if (e == hipSuccess)
+      e = bf16 ? launch_apply_bf16(g, op, p, ap, w->pap, w->st, st) : launch_apply_f32(g, op, p, ap, w->pap, w->st, st);
   }
   if (e == hipSuccess) e = launch_pcg_update<T>(n, w->st, w->pap, npap, p, ap, x, r, dinv, w->parts, st);
   return e;
@@ -1046,11 +1048,12 @@ struct IterGraph {
 };
 
 template <typename T>
-static int capture_batch(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, int batch, IterGraph& G) {
+static int capture_batch(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, int batch, IterGraph& G,
+                         bool bf16 = false) {
   hipStream_t st = op->g->ctx->stream;
   ICC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
-  for (int i = 0; i < batch && e == hipSuccess; ++i) e = enqueue_iteration<T>(op, w, x, r, p, ap, dinv, st);
+  for (int i = 0; i < batch && e == hipSuccess; ++i) e = enqueue_iteration<T>(op, w, x, r, p, ap, dinv, st, bf16);
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(st, &graph);
   if (e != hipSuccess) return hip_fail(e, "PCG batch capture");
@@ -1098,16 +1101,17 @@ constexpr int PCG_BATCH = 16;
 
 // Jacobi PCG in precision T on device vectors: solves A x = bscale * b from
 // x = 0 with dinv already formed.  status: 1 converged, 2 breakdown, 3 limit.
+// bf16 (T = float only): the element contraction of A in BF16.
 template <typename T, typename TB>
 static int pcg_core(icc_op* op, Work* w, const TB* b, double bscale, T* x, T* r, T* p, T* ap, const T* dinv,
-                    double tol, int64_t maxit, int64_t* iters, double* res, int* status) {
+                    double tol, int64_t maxit, int64_t* iters, double* res, int* status, bool bf16 = false) {
   icc_grid* g = op->g;
   hipStream_t st = g->ctx->stream;
   const int64_t n = g->s.ndof;
   ICC_TRY(set_limits(op, w, tol, maxit));
   ICC_HIP((launch_pcg_init<T, TB>(n, b, bscale, dinv, x, r, p, ap, w->st, w->parts, st)));
   IterGraph G;
-  ICC_TRY(capture_batch<T>(op, w, x, r, p, ap, dinv, PCG_BATCH, G));
+  ICC_TRY(capture_batch<T>(op, w, x, r, p, ap, dinv, PCG_BATCH, G, bf16));
   PcgState hs;
   if (maxit > 0) {
     ICC_TRY(run_until_done(op, w, G, hs));
@@ -1162,7 +1166,7 @@ ICC_API int icc_pcg(icc_op* op, const double* b, const double* diag, double tol,
 ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol, double inner_tol, int64_t max_outer,
                              int64_t max_inner, double* u, double* stats) {
   if (!op || !f || !u) return ICC_EINVAL;
-  if (inner != ICC_F64 && inner != ICC_F32) {
+  if (inner != ICC_F64 && inner != ICC_F32 && inner != ICC_BF16) {
     set_error("icc_solve_ir: unknown inner precision");
     return ICC_EINVAL;
   }
@@ -1195,9 +1199,9 @@ ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol,
     int64_t its = 0;
     double res = 0;
     int status = 0;
-    if (inner == ICC_F32) {
+    if (inner == ICC_F32 || inner == ICC_BF16) {
       ICC_TRY((pcg_core<float, double>(op, w, w->r, 1.0 / nr, w->x32, w->r32, w->p32, w->ap32, w->dinv32, inner_tol,
-                                       max_inner, &its, &res, &status)));
+                                       max_inner, &its, &res, &status, inner == ICC_BF16)));
       ICC_HIP(launch_ir_update<float>(n, u, w->x32, nr, st));
     } else {
       ICC_TRY((pcg_core<double, double>(op, w, w->r, 1.0 / nr, w->x, w->tmp, w->p, w->ap, w->dinv, inner_tol,

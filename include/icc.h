@@ -38,8 +38,9 @@ enum icc_status {
 enum icc_prec {
   ICC_F64 = 0, /* FP64 MFMA (v_mfma_f64_16x16x4), FP64 vectors */
   ICC_F32 = 1, /* FP32 MFMA (v_mfma_f32_16x16x4), FP32 vectors and accumulation */
-  ICC_BF16 = 2 /* apply only: K and x rounded to BF16, FP32 accumulation
-                  (v_mfma_f32_16x16x32_bf16), FP32 vectors (x, y float*) */
+  ICC_BF16 = 2 /* K and x rounded to BF16, FP32 accumulation
+                  (v_mfma_f32_16x16x32_bf16), FP32 vectors (x, y float*);
+                  the apply, and the inner solve of icc_solve_ir */
 };
 
 typedef struct icc_ctx icc_ctx;
@@ -149,7 +150,11 @@ int icc_pcg_dev(icc_op* op, const double* b_dev, const double* diag_dev, double 
 
 /* Mixed-precision iterative refinement: FP64 outer residual r = f - A u with
  * compensated dot products, FP64 update u += d, inner Jacobi PCG in `inner`
- * precision solving A d = r to relative `inner_tol`.  Stops when the TRUE
+ * precision (ICC_F64, ICC_F32, or ICC_BF16: FP32 vectors with the BF16
+ * element contraction) solving A d = r to relative `inner_tol`.  The BF16
+ * inner operator is perturbed by about 2^-9 relative, so refinement converges
+ * only while that perturbation times the condition number stays below 1;
+ * otherwise ICC_ENOCONV after max_outer steps.  Stops when the TRUE
  * relative residual |f - A u| / |f| <= tol.  stats (may be NULL), 8 doubles:
  * outer steps, total inner iterations, final true residual, FP64 applies,
  * inner applies, 0, 0, 0.  u_dev in/out (initial guess). */

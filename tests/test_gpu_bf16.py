@@ -84,3 +84,55 @@ def test_bf16_apply_on_device_and_autotune(ctx):
     ctx.sync()
     got = yd.cpu().numpy().astype(np.float64)
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-6
+
+
+def _bf16_ir_problem(ctx, nel, lo):
+    import oracle_py as O
+    bounds = [[0, 2.0], [0, 1.0], [0, 1.0]]
+    rng = np.random.default_rng(sum(nel))
+    sp = O.Space(bounds, list(nel), 2)
+    lam, mu = O.lame(1.0, 0.3)
+    sp.elem_mats(lam, mu)
+    nelt = int(np.prod(nel))
+    s = rng.uniform(lo, 1.0, nelt) ** 3 if lo < 1 else np.ones(nelt)
+    sp.assemble(s)
+    g = icc.Grid(ctx, nel, [0.0, 2.0, 0.0, 1.0, 0.0, 1.0])
+    gx = sp.greville()[:, 0]
+    free = np.ones(g.ndof, dtype=np.uint8)
+    for c in range(3):
+        free[c * g.ndof_sc + np.nonzero(gx < 1e-12)[0]] = 0
+    f = free * rng.standard_normal(g.ndof)
+    return sp, g, free, f, icc.Operator(g, scale=s, free=free)
+
+
+@pytest.mark.parametrize("nel,lo", [((6, 3, 3), 1.0), ((8, 4, 4), 0.3)])
+def test_ir_with_bf16_inner_solve(ctx, nel, lo):
+    """Iterative refinement whose inner Jacobi PCG runs on the BF16 element
+    contraction (FP32 vectors): on clamped cantilevers with densities in
+    [lo, 1] (condition numbers 2.5e3 and 1.4e4) it reaches a true relative
+    residual of 1e-10, recomputed here with the oracle's assembled FP64
+    operator, and u agrees with an FP64 PCG solve to that residual times the
+    conditioning."""
+    sp, g, free, f, op = _bf16_ir_problem(ctx, nel, lo)
+    u, st, rc = op.solve_ir(f, tol=1e-10, inner=icc.ICC_BF16, inner_tol=1e-2, max_outer=60, max_inner=20000)
+    assert rc == icc.ICC_OK, st
+    r = free * (f - sp.matvec(free * u))
+    assert np.linalg.norm(r) / np.linalg.norm(f) <= 1.01e-10
+    assert st["true_res"] <= 1e-10 and st["outer"] <= 40 and st["inner_applies"] > 0
+    u64, it, res, rc64 = op.pcg(f, np.maximum(op.diag(), 1e-12), 1e-12, 50 * g.ndof)
+    assert np.linalg.norm(u - u64) / np.linalg.norm(u64) < 1e-5
+
+
+def test_ir_with_bf16_inner_solve_reports_no_convergence(ctx):
+    """Refinement needs the BF16 perturbation (about 2^-9) times the
+    condition number below 1. The 16x8x8 cantilever with densities in
+    [0.3, 1] is past that: the refinement does not converge, stops at
+    max_outer with ENOCONV, and reports the true residual of the u it
+    returns."""
+    sp, g, free, f, op = _bf16_ir_problem(ctx, (16, 8, 8), 0.3)
+    u, st, rc = op.solve_ir(f, tol=1e-10, inner=icc.ICC_BF16, inner_tol=1e-2, max_outer=8, max_inner=20000)
+    assert rc == icc.ICC_ENOCONV and st["outer"] == 8
+    r = free * (f - sp.matvec(free * u))
+    assert np.isfinite(u).all()
+    assert abs(np.linalg.norm(r) / np.linalg.norm(f) - st["true_res"]) <= 1e-8 * st["true_res"]
+    assert st["true_res"] > 1e-10
