@@ -338,8 +338,8 @@ def adjoint_leg(np, icc, ctx, oc_iters=20, nfd=32, seed=20261206):
 
 
 def wq_leg(np, torch, icc, ctx, grid, rho, reps=5):
-    """Weighted-quadrature stiffness formation (wq_stiffness, interior layout)
-    on the 96x48x48 grid with the benchmark's element densities (penal 3,
+    """Weighted-quadrature stiffness formation (wq_stiffness, interior layout;
+    also timed with the Calabro layout) on the 96x48x48 grid with the benchmark's element densities (penal 3,
     emin 1e-9), densities already on the device: formation time (SIMP factor
     at the WQ points, sum-factorised rows, symmetrisation) and the time of
     one product with the formed 5x5x5-stencil matrix (HIP events)."""
@@ -367,6 +367,18 @@ def wq_leg(np, torch, icc, ctx, grid, rho, reps=5):
     out = dict(rows=nsc, stored_doubles=9 * 125 * nsc, form_ms=form_ms, apply_ms=apply_ms,
                apply_gb_per_s=9 * 125 * nsc * 8 / (apply_ms * 1e-3) / 1e9)
     m.close()
+    # the same formation with the Calabro point layout (wq_stiffness_layout)
+    mc = icc.WqMatrix(grid, "calabro")
+    mc.form(icc.ICC_DENSITY_ELEMENT, xd, 3.0, 1e-9)
+    ctx.sync()
+    e0.record(stream)
+    for _ in range(reps):
+        mc.form(icc.ICC_DENSITY_ELEMENT, xd, 3.0, 1e-9)
+    e1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    out["form_ms_calabro"] = e0.elapsed_time(e1) / reps
+    mc.close()
     return out
 
 
