@@ -51,8 +51,13 @@ def _run(cmd):
     return out
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compiles every source (in parallel, at most `jobs` at once) and links
+    libicc.so and the topopt_3d driver."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(BUILD, exist_ok=True)
+    cmds = []
     objs = []
     for name, extra in SOURCES:
         src = os.path.join(CSRC, name)
@@ -63,10 +68,13 @@ def build(verbose=False):
             cmd = [HIPCC] + HOST_FLAGS + ["-c", src, "-o", obj]
         else:
             cmd = [HIPCC] + HIP_FLAGS + extra + ["-c", src, "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            print(out)
+        cmds.append(cmd)
         objs.append(obj)
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)

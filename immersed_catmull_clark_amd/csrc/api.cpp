@@ -58,6 +58,7 @@ static uint16_t bf16_bits(float f) {
 struct Work {
   int64_t n = 0;
   double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *dinv = nullptr, *tmp = nullptr, *f = nullptr;
+  double* ubest = nullptr;  // best refinement iterate (icc_solve_ir)
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *dinv32 = nullptr;
   double* parts = nullptr;  // 2 * VEC_MAX_BLOCKS vector partials
   double* pap = nullptr;    // apply workgroups + csr blocks
@@ -353,6 +354,7 @@ static void free_work(Work& w) {
   hipFree(w.dinv);
   hipFree(w.tmp);
   hipFree(w.f);
+  hipFree(w.ubest);
   hipFree(w.x32);
   hipFree(w.r32);
   hipFree(w.p32);
@@ -364,27 +366,18 @@ static void free_work(Work& w) {
   if (w.st_host) hipHostFree(w.st_host);
   w = Work();
 }
-// The workspace hangs off the op through a side table keyed by pointer.
-struct WorkEntry {
-  icc_op* op;
-  Work* w;
-};
-static std::vector<WorkEntry> g_work;
+// The workspace is owned by the op (created on first use, freed with it), so
+// contexts driven from different host threads share no table.
 static Work* work_of(icc_op* op) {
-  for (auto& e : g_work)
-    if (e.op == op) return e.w;
-  Work* w = new Work();
-  g_work.push_back({op, w});
-  return w;
+  if (op->work == nullptr) op->work = new Work();
+  return static_cast<Work*>(op->work);
 }
 static void drop_work(icc_op* op) {
-  for (size_t i = 0; i < g_work.size(); ++i)
-    if (g_work[i].op == op) {
-      free_work(*g_work[i].w);
-      delete g_work[i].w;
-      g_work.erase(g_work.begin() + (long)i);
-      return;
-    }
+  if (op->work == nullptr) return;
+  Work* w = static_cast<Work*>(op->work);
+  free_work(*w);
+  delete w;
+  op->work = nullptr;
 }
 
 static int csr_blocks(const icc_op* op) { return op->s_nnz > 0 ? vec_blocks(op->g->s.ndof) : 0; }
@@ -402,7 +395,7 @@ static int ensure_work(icc_op* op, Work*& out) {
     size_t npap = (size_t)APPLY_MAX_WG_PER_CU * (op->g->ctx->num_cu > 0 ? op->g->ctx->num_cu : 256) + VEC_MAX_BLOCKS +
                   FACE_MAX_BLOCKS;
     if ((rc = dalloc(&w->x, n)) || (rc = dalloc(&w->r, n)) || (rc = dalloc(&w->p, n)) || (rc = dalloc(&w->ap, n)) ||
-        (rc = dalloc(&w->dinv, n)) || (rc = dalloc(&w->tmp, n)) || (rc = dalloc(&w->f, n)) ||
+        (rc = dalloc(&w->dinv, n)) || (rc = dalloc(&w->tmp, n)) || (rc = dalloc(&w->f, n)) || (rc = dalloc(&w->ubest, n)) ||
         (rc = dalloc(&w->x32, n)) || (rc = dalloc(&w->r32, n)) || (rc = dalloc(&w->p32, n)) ||
         (rc = dalloc(&w->ap32, n)) || (rc = dalloc(&w->dinv32, n)) || (rc = dalloc(&w->parts, 2 * VEC_MAX_BLOCKS)) ||
         (rc = dalloc(&w->pap, npap)) || (rc = dalloc(&w->st, 1))) {
@@ -1182,6 +1175,13 @@ ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol,
   double st_outer = 0, st_inner = 0, st_res = 1, st_app64 = 0, st_appin = 0;
   int rc_final = ICC_ENOCONV;
   PcgState hs;
+  // The true residual of every outer step is kept: the best iterate is what
+  // a failed solve returns, and a step that does not cut the residual by
+  // IR_STALL ends the refinement (an inner operator too coarse for the
+  // system's conditioning makes no progress, and running every outer step
+  // would only repeat full inner solves).
+  constexpr double IR_STALL = 0.5;
+  double res_prev = 0.0, res_best = 0.0;
   for (int64_t outer = 0; outer <= max_outer; ++outer) {
     // r = M (f - A u) in FP64, compensated norms
     ICC_HIP(apply_t<double>(op, u, w->ap, nullptr));
@@ -1194,7 +1194,20 @@ ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol,
       rc_final = ICC_OK;
       break;
     }
-    if (outer == max_outer) break;
+    if (outer == 0 || st_res < res_best) {
+      res_best = st_res;
+      ICC_HIP(hipMemcpyAsync(w->ubest, u, n * 8, hipMemcpyDeviceToDevice, st));
+    }
+    if (outer == max_outer || (outer > 0 && !(st_res <= IR_STALL * res_prev))) {
+      if (outer < max_outer) set_error("icc_solve_ir: the refinement stagnated (the inner operator is too coarse)");
+      if (st_res > res_best) {  // return the best iterate seen
+        ICC_HIP(hipMemcpyAsync(u, w->ubest, n * 8, hipMemcpyDeviceToDevice, st));
+        st_res = res_best;
+      }
+      rc_final = ICC_ENOCONV;
+      break;
+    }
+    res_prev = st_res;
     st_outer += 1;
     int64_t its = 0;
     double res = 0;
@@ -1224,7 +1237,8 @@ ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol,
     stats[4] = st_appin;
     stats[5] = stats[6] = stats[7] = 0.0;
   }
-  if (rc_final == ICC_ENOCONV) set_error("icc_solve_ir: outer iteration limit reached");
+  if (rc_final == ICC_ENOCONV && st_outer >= (double)max_outer)
+    set_error("icc_solve_ir: outer iteration limit reached");
   return rc_final;
 }
 

@@ -179,6 +179,7 @@ struct icc_op {
   // scratch
   double* d_partials = nullptr;  // reduction partials
   int partials_cap = 0;
+  void* work = nullptr;  // solver workspace (api.cpp), created on first use
 };
 
 struct icc_filter {

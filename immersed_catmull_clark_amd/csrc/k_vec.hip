@@ -150,6 +150,7 @@ __global__ __launch_bounds__(VB) void pcg_update1(int64_t n, PcgState* st, const
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->pap = pap;
       st->it_cur = it;
+      st->iters = it + 1;  // the reference counts the breaking iteration (final_iter = iter + 1, solver.rs:41)
       st->done = 2;
     }
     return;

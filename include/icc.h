@@ -69,7 +69,16 @@ int icc_grid_destroy(icc_grid* g);
 int icc_grid_sizes(const icc_grid* g, int64_t out[11]);
 /* connectivity[e*81 + c*27 + a] (space.rs:133-155), bit-exact */
 int icc_grid_connectivity(const icc_grid* g, int64_t* out);
-/* type_id[e] in first-appearance order (elasticity.rs:85-98) */
+/* type_id[e] in first-appearance order (elasticity.rs:85-98).
+ * Deviation: the reference groups 1D element matrices by the rounded key
+ * round(v / scale * 1e12) (elasticity.rs:73); here a 1D type is every
+ * element whose matrices agree with the type's first element to
+ * 1e-9 * scale.  Where the knot spacing's rounding noise straddles the
+ * reference's key (a length-2 side with 32 or 160 elements) the reference
+ * splits one mathematical type in two and this map merges them, so its
+ * numbering (and ntypes) then differs from the reference's; every element's
+ * matrix (icc_grid_element_matrix of its type) equals the reference's matrix
+ * for that element to round-off. */
 int icc_grid_type_id(const icc_grid* g, int64_t* out);
 /* row-major 81x81 element matrix of a type (test x trial) */
 int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out);

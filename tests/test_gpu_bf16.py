@@ -126,13 +126,15 @@ def test_ir_with_bf16_inner_solve(ctx, nel, lo):
 def test_ir_with_bf16_inner_solve_reports_no_convergence(ctx):
     """Refinement needs the BF16 perturbation (about 2^-9) times the
     condition number below 1. The 16x8x8 cantilever with densities in
-    [0.3, 1] is past that: the refinement does not converge, stops at
-    max_outer with ENOCONV, and reports the true residual of the u it
-    returns."""
+    [0.3, 1] is past that: an outer step fails to halve the true residual,
+    so the refinement stops early (before max_outer) with ENOCONV, returns
+    the best iterate it saw, and reports that iterate's true residual, which
+    is no worse than the initial one (u = 0)."""
     sp, g, free, f, op = _bf16_ir_problem(ctx, (16, 8, 8), 0.3)
     u, st, rc = op.solve_ir(f, tol=1e-10, inner=icc.ICC_BF16, inner_tol=1e-2, max_outer=8, max_inner=20000)
-    assert rc == icc.ICC_ENOCONV and st["outer"] == 8
+    assert rc == icc.ICC_ENOCONV and 1 <= st["outer"] < 8
     r = free * (f - sp.matvec(free * u))
     assert np.isfinite(u).all()
-    assert abs(np.linalg.norm(r) / np.linalg.norm(f) - st["true_res"]) <= 1e-8 * st["true_res"]
-    assert st["true_res"] > 1e-10
+    true_res = np.linalg.norm(r) / np.linalg.norm(f)
+    assert abs(true_res - st["true_res"]) <= 1e-8 * st["true_res"]
+    assert 1e-10 < st["true_res"] <= 1.0
