@@ -449,7 +449,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the immersed, batched and adjoint configuration legs")
     ap.add_argument("--apply-only", action="store_true", help="time only the apply (for profilers)")
-    ap.add_argument("--apply-prec", choices=("f64", "f32", "bf16"), default="f64", help="precision of --apply-only")
+    ap.add_argument("--apply-prec", choices=("f64", "f32", "bf16", "f16x"), default="f64",
+                    help="precision of --apply-only")
     ap.add_argument("--tiling", default=None, metavar="V,W",
                     help="fixed tiling variant and workgroups per CU for the apply legs (skips the autotuner; "
                          "for profiling the configuration a default run picked)")
@@ -515,13 +516,14 @@ def main():
         tuned = {"f64": (fixed[0], fixed[1], None)}
     else:
         tuned = {"f64": op.autotune(icc.ICC_F64, reps=5)}
-    if args.apply_only and args.apply_prec in ("f32", "bf16"):
-        pr = icc.ICC_F32 if args.apply_prec == "f32" else icc.ICC_BF16
-        if not fixed:
+    if args.apply_only and args.apply_prec in ("f32", "bf16", "f16x"):
+        pr = {"f32": icc.ICC_F32, "bf16": icc.ICC_BF16, "f16x": icc.ICC_F16X}[args.apply_prec]
+        if not fixed and pr != icc.ICC_F16X:
             op.autotune(pr, reps=5)
         ms32 = time_applies(torch, icc, ctx, op, x32, y32, pr, args.steps, args.warmup, accumulate=True)
-        print(json.dumps({"kernel": "apply_tiles<float%s>" % ("" if pr == icc.ICC_F32 else ", true"),
-                          "kernel_ms": ms32, "tflops": flops / (ms32 * 1e-3) / 1e12}), flush=True)
+        name = {"f32": "apply_tiles<float, false, 8>", "bf16": "apply_tiles<float, true, 8>",
+                "f16x": "apply_split_kernel"}[args.apply_prec]
+        print(json.dumps({"kernel": name, "kernel_ms": ms32, "tflops": flops / (ms32 * 1e-3) / 1e12}), flush=True)
         return
     log("bench: FP64 applies")
     t0 = time.time()
@@ -563,6 +565,16 @@ def main():
                          "tflops": flops / (ms_kb * 1e-3) / 1e12, "frac_of_bf16_peak": flops / (ms_kb * 1e-3) / PEAK_BF16,
                          "rel_err_vs_f64": float(np.linalg.norm(yhb - yh64) / np.linalg.norm(yh64))}
 
+    log("bench: split-FP16 applies")
+    ms_s = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F16X, args.steps, args.warmup)
+    ms_ks = time_applies(torch, icc, ctx, op, x32, y32, icc.ICC_F16X, args.steps, args.warmup, accumulate=True)
+    yhs = op.apply(xh, prec=icc.ICC_F16X)
+    out["apply_f16x"] = {"applies_per_s": 1e3 / ms_s, "ms": ms_s, "kernel_ms": ms_ks,
+                         "tflops": flops / (ms_ks * 1e-3) / 1e12,
+                         "frac_of_fp16_peak": flops / (ms_ks * 1e-3) / PEAK_BF16,
+                         "executed_mfma_flops_per_element": 2 * 96 * 112 * 3,
+                         "rel_err_vs_f64": float(np.linalg.norm(yhs - yh64) / np.linalg.norm(yh64))}
+
     if not args.no_pcg:
         # traction load on x = 2 (consistent: F = t int N dA), via the driver's builder
         cfg = icc.CantileverTopOptConfig(nel=NEL, size=(2.0, 1.0, 1.0), load=1, emin=1e-9)
@@ -572,9 +584,11 @@ def main():
         force_c1 = f
         fd = torch.from_numpy(f).cuda()
         res = {}
-        for name in ("fp64", "mixed_fp32"):
+        legs = (("fp64", None), ("mixed_fp32", icc.ICC_F32), ("mixed_f16x", icc.ICC_F16X))
+        for name, inner in legs:
             log(f"bench: PCG {name}")
-            tuned[name] = op.autotune(icc.ICC_F64 if name == "fp64" else icc.ICC_F32, reps=5)
+            if inner != icc.ICC_F16X:
+                tuned[name] = op.autotune(icc.ICC_F64 if name == "fp64" else icc.ICC_F32, reps=5)
             u = torch.zeros(ndof, dtype=torch.float64, device="cuda")
             stats = np.zeros(8)
             torch.cuda.synchronize()
@@ -586,24 +600,28 @@ def main():
                                                 u.data_ptr(), stats.ctypes.data)
             else:
                 # FP32 recurrence with FP64 reliable updates every `delta` drop
-                rc = icc.lib().icc_solve_mixed_dev(op.h, fd.data_ptr(), 1e-8, args.delta, 20 * ndof, u.data_ptr(),
-                                                   stats.ctypes.data)
+                rc = icc.lib().icc_solve_mixed_prec_dev(op.h, inner, fd.data_ptr(), 1e-8, args.delta, 20 * ndof,
+                                                        u.data_ptr(), stats.ctypes.data)
             ctx.sync()
             dt = time.time() - t0
             res[name] = dict(time_s=dt, status=int(rc), updates=int(stats[0]), pcg_iterations=int(stats[1]),
                              true_rel_residual=float(stats[2]), fp64_applies=int(stats[3]),
                              ms_per_iteration=1e3 * dt / max(1, int(stats[1])), u=u.cpu().numpy())
-        uf, um = res["fp64"]["u"], res["mixed_fp32"]["u"]
-        cf, cm = float(f @ uf), float(f @ um)
+        uf = res["fp64"]["u"]
+        cf = float(f @ uf)
         dcf = op.sens(uf, rho, 3.0, 1e-9)
-        dcm = op.sens(um, rho, 3.0, 1e-9)
-        for k in res:
+        for name, _ in legs[1:]:
+            um = res[name]["u"]
+            cm = float(f @ um)
+            dcm = op.sens(um, rho, 3.0, 1e-9)
+            res[name].update(
+                u_rel_err_vs_fp64=float(np.linalg.norm(um - uf) / np.linalg.norm(uf)),
+                compliance_rel_err_vs_fp64=abs(cm - cf) / abs(cf),
+                dc_rel_err_vs_fp64=float(np.linalg.norm(dcm - dcf) / np.linalg.norm(dcf)))
+            res["speedup_" + name + "_vs_fp64"] = res["fp64"]["time_s"] / res[name]["time_s"]
+        for k in ("fp64",) + tuple(n for n, _ in legs[1:]):
             res[k].pop("u")
-        res["mixed_fp32"].update(
-            u_rel_err_vs_fp64=float(np.linalg.norm(um - uf) / np.linalg.norm(uf)),
-            compliance_rel_err_vs_fp64=abs(cm - cf) / abs(cf),
-            dc_rel_err_vs_fp64=float(np.linalg.norm(dcm - dcf) / np.linalg.norm(dcf)))
-        res["speedup_mixed_vs_fp64"] = res["fp64"]["time_s"] / res["mixed_fp32"]["time_s"]
+        res["speedup_mixed_vs_fp64"] = res["speedup_mixed_f16x_vs_fp64"]
         res["delta"] = args.delta
         res["fp64_c0_16x8x8"] = pcg_c0(np, torch, icc, ctx)
         out["pcg_time_to_1e-8"] = res

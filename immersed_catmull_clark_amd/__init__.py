@@ -2,7 +2,7 @@
 
 Host API over libicc.so (HIP kernels for gfx950 behind a C ABI, include/icc.h).
 """
-from ._abi import (ICC_BF16, ICC_F32, ICC_F64, ICC_OK, ICC_EINVAL, ICC_ENOMEM, ICC_EHIP, ICC_EBREAKDOWN,  # noqa: F401
+from ._abi import (ICC_BF16, ICC_F16X, ICC_F32, ICC_F64, ICC_OK, ICC_EINVAL, ICC_ENOMEM, ICC_EHIP, ICC_EBREAKDOWN,  # noqa: F401
                    ICC_ENOCONV, ICC_ENOTSUP, ICC_DENSITY_NONE, ICC_DENSITY_ELEMENT, ICC_DENSITY_SPLINE, IccError,
                    LIB_PATH, lib)
 from .core import Context, Filter, Grid, Operator, WqMatrix, classify_cylinders  # noqa: F401
@@ -13,4 +13,4 @@ from .imm_topopt import (ImmersedOptions, ImmersedProblem, ImmersedSolution, Imm
                          ImmersedTopOptConfig, ImmersedTopOptHistory, Stabilization, TriangleMesh3D,
                          build_immersed_problem, element_weights, padded_bounds, solve_immersed_problem)
 
-__all__ = ["Context", "Grid", "Operator", "Filter", "classify_cylinders", "ICC_F64", "ICC_F32", "IccError"]
+__all__ = ["Context", "Grid", "Operator", "Filter", "classify_cylinders", "ICC_F64", "ICC_F32", "ICC_BF16", "ICC_F16X", "IccError"]

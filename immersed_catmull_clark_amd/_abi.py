@@ -11,7 +11,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libicc.so")
 
 ICC_OK, ICC_EINVAL, ICC_ENOMEM, ICC_EHIP, ICC_EBREAKDOWN, ICC_ENOCONV, ICC_ENOTSUP = range(7)
-ICC_F64, ICC_F32, ICC_BF16 = 0, 1, 2
+ICC_F64, ICC_F32, ICC_BF16, ICC_F16X = 0, 1, 2, 3
 
 i64 = C.c_int64
 dbl = C.c_double
@@ -50,6 +50,8 @@ SIGNATURES = [
     ("icc_solve_ir", cint, [vp, cint, vp, dbl, dbl, i64, i64, vp, vp]),
     ("icc_solve_mixed_dev", cint, [vp, vp, dbl, dbl, i64, vp, vp]),
     ("icc_solve_mixed", cint, [vp, vp, dbl, dbl, i64, vp, vp]),
+    ("icc_solve_mixed_prec_dev", cint, [vp, cint, vp, dbl, dbl, i64, vp, vp]),
+    ("icc_solve_mixed_prec", cint, [vp, cint, vp, dbl, dbl, i64, vp, vp]),
     ("icc_energies", cint, [vp, vp, vp]),
     ("icc_energies_dev", cint, [vp, vp, vp]),
     ("icc_sens_dev", cint, [vp, vp, vp, dbl, dbl, vp, vp]),
@@ -107,6 +109,7 @@ SIGNATURES = [
     ("icc_imm_topopt_run", cint, [vp, vp, vp, vp, vp, vp, vp, vp]),
     ("icc_imm_topopt_state", cint, [vp, vp]),
     ("icc_debug_mfma_layout_bf16", cint, [vp, vp]),
+    ("icc_debug_mfma_layout_f16", cint, [vp, vp]),
     ("icc_immersed_mesh_config_default", None, [vp]),
     ("icc_immersed_mesh_create", cint, [vp, vp, vp, i64, vp, i64, vp, vpp]),
     ("icc_immersed_mesh_destroy", cint, [vp]),

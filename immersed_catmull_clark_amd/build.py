@@ -32,6 +32,7 @@ SOURCES = [
     ("k_faces.hip", []),
     ("k_topopt.hip", ["-ffp-contract=off"]),
     ("k_wq.hip", []),
+    ("k_split.hip", []),
     ("setup.cpp", None),
     ("wq.cpp", None),
     ("api.cpp", None),

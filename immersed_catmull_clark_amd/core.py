@@ -66,6 +66,13 @@ class Context:
         check(lib().icc_debug_mfma_layout_bf16(self.h, _p(out)))
         return out.reshape(64, 4)
 
+    def mfma_layout_f16(self):
+        """(16x16x32, 16x16x16) accumulator contents [lane, reg] of the FP16
+        layout probe."""
+        out = np.zeros(512)
+        check(lib().icc_debug_mfma_layout_f16(self.h, _p(out)))
+        return out[:256].reshape(64, 4), out[256:].reshape(64, 4)
+
 
 class Grid:
     """p = 2 B-spline space on a box with its exact element matrices
@@ -241,13 +248,15 @@ class Operator:
         check(lib().icc_grid_autotune(self.h, int(prec), int(reps), _p(v), _p(wg), _p(us)))
         return int(v[0]), int(wg[0]), float(us[0])
 
-    def solve_mixed(self, f, tol=1e-8, delta=0.1, maxit=1000000, u0=None):
-        """FP32 PCG with reliable updates against the FP64 operator: the FP32
-        residual is replaced by the FP64 true residual each time it drops by
-        `delta`.  Returns (u, stats dict, status)."""
+    def solve_mixed(self, f, tol=1e-8, delta=0.1, maxit=1000000, u0=None, inner=ICC_F32):
+        """PCG with FP32 vectors and the inner contraction in `inner` (ICC_F32,
+        ICC_F16X or ICC_BF16), with reliable updates against the FP64
+        operator: the inner residual is replaced by the FP64 true residual
+        each time it drops by `delta`.  Returns (u, stats dict, status)."""
         u = np.zeros(self.ndof) if u0 is None else _f64(u0).copy()
         stats = np.zeros(8)
-        rc = lib().icc_solve_mixed(self.h, _p(_f64(f)), float(tol), float(delta), int(maxit), _p(u), _p(stats))
+        rc = lib().icc_solve_mixed_prec(self.h, int(inner), _p(_f64(f)), float(tol), float(delta), int(maxit), _p(u),
+                                        _p(stats))
         check(rc, allowed=(ICC_OK,) + ICC_BREAKDOWN_CODES)
         st = dict(updates=int(stats[0]), inner_iters=int(stats[1]), true_res=float(stats[2]),
                   fp64_applies=int(stats[3]), inner_applies=int(stats[4]))

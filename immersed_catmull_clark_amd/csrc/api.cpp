@@ -166,6 +166,88 @@ static int upload_tiles(icc_grid* g, bool balance) {
   g->apply_G = G;
   return ICC_OK;
 }
+// Partitions the split-apply tiles over one workgroup per CU (balanced
+// work, tiles split where needed) and uploads them.
+static int upload_split_tiles(icc_grid* g) {
+  int G = g->ctx->num_cu > 0 ? g->ctx->num_cu : 256;
+  G -= G % 8;
+  if (G > (int)g->s.stiles.size()) G = (int)g->s.stiles.size();
+  if (G < 1) G = 1;
+  std::vector<int32_t> first;
+  balance_tile_list(g->s.stiles, G, split_tile_cost, first);
+  Tile* dt = nullptr;
+  int32_t* df = nullptr;
+  int rc;
+  if ((rc = dalloc(&dt, g->s.stiles.size())) || (rc = dalloc(&df, first.size()))) {
+    hipFree(dt);
+    return rc;
+  }
+  if (hipMemcpy(dt, g->s.stiles.data(), g->s.stiles.size() * sizeof(Tile), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(df, first.data(), first.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(dt);
+    hipFree(df);
+    set_error("split tile upload failed");
+    return ICC_EHIP;
+  }
+  hipFree(g->d_stiles);
+  hipFree(g->d_swgfirst);
+  g->d_stiles = dt;
+  g->d_swgfirst = df;
+  g->nstiles = (int)g->s.stiles.size();
+  g->split_G = G;
+  return ICC_OK;
+}
+
+// A fragments of the split-FP16 apply.  K_t is scaled by 2^-kexp (so that
+// max |K_t| 2^-kexp lies in [2^14, 2^15), keeping the lo parts normal) and split
+// into hi = fp16(v), lo = fp16(v - hi).  Lane l of (type, half h, local M
+// tile mm, K step ks) holds row i = l % 16 of M tile m = 3 h + mm (the K
+// row perm_row assigns to FP32 accumulator slot i = 4 kq + r) against
+// trial columns k = 8 (l / 16) + j (ks < 3) or 4 (l / 16) + j (ks = 3,
+// j < 4): triplet T = 8 ks + 2 (l / 16) + j / 4, slot j % 4 (ks < 3), or
+// triplet 24 + l / 16, slot j (ks = 3); slot 3 and triplets past 26 are
+// zero columns.
+static void split_fragments(const GridSetup& s, std::vector<uint16_t>& frag, std::vector<int32_t>& kexp) {
+  const int nt = (int)s.kmat.size();
+  frag.assign((size_t)nt * 2 * 3 * 4 * 64 * 2 * 8, 0);
+  kexp.assign(nt, 0);
+  auto half_bits = [](double v) {
+    const _Float16 hv = (_Float16)v;
+    uint16_t b;
+    std::memcpy(&b, &hv, 2);
+    return b;
+  };
+  for (int t = 0; t < nt; ++t) {
+    const double* K = s.kmat[t].data();
+    double mx = 0.0;
+    for (int q = 0; q < NSH * NSH; ++q) mx = std::max(mx, std::fabs(K[q]));
+    int e = 0;
+    if (mx > 0.0) std::frexp(mx, &e);
+    e -= SP_SCALE_BITS;
+    kexp[t] = e;
+    for (int h = 0; h < 2; ++h)
+      for (int mm = 0; mm < 3; ++mm)
+        for (int ks = 0; ks < 4; ++ks)
+          for (int l = 0; l < 64; ++l)
+            for (int j = 0; j < (ks < 3 ? 8 : 4); ++j) {
+              const int i = l % 16;
+              const int row = perm_row(3 * h + mm, i / 4, i % 4);
+              const int T = ks < 3 ? 8 * ks + 2 * (l / 16) + j / 4 : 24 + l / 16;
+              const int slot = ks < 3 ? j % 4 : j;
+              int col = -1;
+              if (T < 27 && slot < 3) {
+                const int c = T / 9, rem = T % 9, ay = rem % 3, az = rem / 3;
+                col = c * NSC + slot + 3 * ay + 9 * az;
+              }
+              const double v = (row >= 0 && col >= 0) ? std::ldexp(K[row * NSH + col], -e) : 0.0;
+              const _Float16 hi = (_Float16)v;
+              const double lo = v - (double)hi;
+              const size_t base = ((((((size_t)t * 2 + h) * 3 + mm) * 4 + ks) * 64 + l) * 2) * 8;
+              frag[base + j] = half_bits(v);
+              frag[base + 8 + j] = half_bits(lo);
+            }
+  }
+}
 }  // namespace icc
 
 ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bounds[6], int64_t degree, double young,
@@ -250,6 +332,24 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
     icc_grid_destroy(g);
     return rc;
   }
+  {
+    std::vector<uint16_t> f16;
+    std::vector<int32_t> kx;
+    split_fragments(s, f16, kx);
+    if ((rc = dalloc(&g->d_afrag16, f16.size() / 8)) || (rc = dalloc(&g->d_kexp16, kx.size()))) {
+      icc_grid_destroy(g);
+      return rc;
+    }
+    if ((e = hipMemcpy(g->d_afrag16, f16.data(), f16.size() * 2, hipMemcpyHostToDevice)) ||
+        (e = hipMemcpy(g->d_kexp16, kx.data(), kx.size() * 4, hipMemcpyHostToDevice))) {
+      icc_grid_destroy(g);
+      return hip_fail(e, "icc_grid_create upload");
+    }
+    if ((rc = upload_split_tiles(g)) != ICC_OK) {
+      icc_grid_destroy(g);
+      return rc;
+    }
+  }
   *out = g;
   return ICC_OK;
 }
@@ -266,6 +366,10 @@ ICC_API int icc_grid_destroy(icc_grid* g) {
   hipFree(g->d_wgfirst);
   hipFree(g->d_mats1);
   hipFree(g->d_wq);
+  hipFree(g->d_afrag16);
+  hipFree(g->d_kexp16);
+  hipFree(g->d_stiles);
+  hipFree(g->d_swgfirst);
   delete g;
   return ICC_OK;
 }
@@ -428,6 +532,7 @@ ICC_API int icc_op_destroy(icc_op* op) {
   hipFree(op->d_scale32);
   hipFree(op->d_free);
   hipFree(op->d_tmask);
+  hipFree(op->d_stmask);
   hipFree(op->d_s_indptr);
   hipFree(op->d_s_indices);
   hipFree(op->d_s_data);
@@ -464,12 +569,11 @@ ICC_API int icc_op_set_scale(icc_op* op, const double* s) {
 namespace icc {
 // Per tile: does any staged point carry a fixed DOF?  Only such tiles read
 // the mask inside the apply kernel.  Rebuilt whenever the grid is re-tiled.
-static int upload_tile_mask(icc_op* op) {
-  const auto& s = op->g->s;
-  const std::vector<uint8_t>& fm = op->h_free;
-  std::vector<int32_t> tm(s.tiles.size(), 0);  // 32-bit flags: scalar loads in the kernel
-  for (size_t ti = 0; ti < s.tiles.size(); ++ti) {
-    const Tile& t = s.tiles[ti];
+// 32-bit flags (scalar loads in the kernels): 1 if the tile stages a fixed DOF
+static std::vector<int32_t> tile_mask_of(const GridSetup& s, const std::vector<Tile>& tiles, const uint8_t* fm) {
+  std::vector<int32_t> tm(tiles.size(), 0);
+  for (size_t ti = 0; ti < tiles.size(); ++ti) {
+    const Tile& t = tiles[ti];
     bool fixed = false;
     for (int c = 0; c < 3 && !fixed; ++c)
       for (int64_t k = t.k0; k < t.k0 + t.nz + 2 && !fixed; ++k)
@@ -481,10 +585,22 @@ static int upload_tile_mask(icc_op* op) {
             }
     tm[ti] = fixed ? 1 : 0;
   }
+  return tm;
+}
+
+static int upload_tile_mask(icc_op* op) {
+  const GridSetup& s = op->g->s;
+  const uint8_t* fm = op->h_free.data();
+  const std::vector<int32_t> tm = tile_mask_of(s, s.tiles, fm), stm = tile_mask_of(s, s.stiles, fm);
   hipFree(op->d_tmask);
+  hipFree(op->d_stmask);
   op->d_tmask = nullptr;
+  op->d_stmask = nullptr;
   ICC_TRY(dalloc(&op->d_tmask, (int64_t)tm.size()));
+  ICC_TRY(dalloc(&op->d_stmask, (int64_t)stm.size()));
   ICC_HIP(hipMemcpyAsync(op->d_tmask, tm.data(), tm.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                         op->g->ctx->stream));
+  ICC_HIP(hipMemcpyAsync(op->d_stmask, stm.data(), stm.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                          op->g->ctx->stream));
   ICC_HIP(hipStreamSynchronize(op->g->ctx->stream));
   return ICC_OK;
@@ -497,8 +613,10 @@ ICC_API int icc_op_set_free_mask(icc_op* op, const uint8_t* fm) {
   if (!fm) {
     hipFree(op->d_free);
     hipFree(op->d_tmask);
+    hipFree(op->d_stmask);
     op->d_free = nullptr;
     op->d_tmask = nullptr;
+    op->d_stmask = nullptr;
     op->h_free.clear();
     return ICC_OK;
   }
@@ -878,9 +996,15 @@ static hipError_t apply_t(icc_op* op, const T* x, T* y, double* pap) {
   else
     return launch_apply_f32(g, op, x, y, pap, nullptr, st);
 }
-// y = A x with the element contraction in BF16 (FP32 vectors; sparse and
-// face terms in FP32)
-static hipError_t apply_bf16(icc_op* op, const float* x, float* y) {
+// Element contraction of a reduced-precision mode (BF16 or split FP16) on
+// FP32 vectors, accumulating into y.
+static hipError_t launch_apply_lowp(int prec, const icc_grid* g, const icc_op* op, const float* x, float* y,
+                                    double* pap, const PcgState* pst, hipStream_t st) {
+  return prec == ICC_F16X ? launch_apply_split(g, op, x, y, pap, pst, st) : launch_apply_bf16(g, op, x, y, pap, pst, st);
+}
+// y = A x with the element contraction in BF16 or split FP16 (FP32 vectors;
+// sparse and face terms in FP32)
+static hipError_t apply_lowp(icc_op* op, int prec, const float* x, float* y) {
   icc_grid* g = op->g;
   hipStream_t st = g->ctx->stream;
   hipError_t e;
@@ -892,7 +1016,7 @@ static hipError_t apply_bf16(icc_op* op, const float* x, float* y) {
     e = launch_fill<float>(y, 0.0f, n, st);
   if (e == hipSuccess) e = launch_faces<float>(op, x, y, nullptr, nullptr, st);
   if (e != hipSuccess) return e;
-  return launch_apply_bf16(g, op, x, y, nullptr, nullptr, st);
+  return launch_apply_lowp(prec, g, op, x, y, nullptr, nullptr, st);
 }
 }  // namespace icc
 
@@ -903,8 +1027,8 @@ ICC_API int icc_apply_dev(icc_op* op, int prec, const void* x, void* y) {
     e = apply_t<double>(op, (const double*)x, (double*)y, nullptr);
   else if (prec == ICC_F32)
     e = apply_t<float>(op, (const float*)x, (float*)y, nullptr);
-  else if (prec == ICC_BF16)
-    e = apply_bf16(op, (const float*)x, (float*)y);
+  else if (prec == ICC_BF16 || prec == ICC_F16X)
+    e = apply_lowp(op, prec, (const float*)x, (float*)y);
   else {
     set_error("icc_apply: unknown precision");
     return ICC_EINVAL;
@@ -924,8 +1048,8 @@ ICC_API int icc_apply_accumulate_dev(icc_op* op, int prec, const void* x, void* 
     e = launch_apply_f64(op->g, op, (const double*)x, (double*)y, nullptr, nullptr, op->g->ctx->stream);
   else if (prec == ICC_F32)
     e = launch_apply_f32(op->g, op, (const float*)x, (float*)y, nullptr, nullptr, op->g->ctx->stream);
-  else if (prec == ICC_BF16)
-    e = launch_apply_bf16(op->g, op, (const float*)x, (float*)y, nullptr, nullptr, op->g->ctx->stream);
+  else if (prec == ICC_BF16 || prec == ICC_F16X)
+    e = launch_apply_lowp(prec, op->g, op, (const float*)x, (float*)y, nullptr, nullptr, op->g->ctx->stream);
   else {
     set_error("icc_apply_accumulate_dev: unknown precision");
     return ICC_EINVAL;
@@ -944,12 +1068,12 @@ ICC_API int icc_apply(icc_op* op, int prec, const double* x, double* y) {
   if (prec == ICC_F64) {
     ICC_HIP(apply_t<double>(op, w->tmp, w->ap, nullptr));
     ICC_HIP(hipMemcpyAsync(y, w->ap, n * 8, hipMemcpyDeviceToHost, st));
-  } else if (prec == ICC_F32 || prec == ICC_BF16) {
+  } else if (prec == ICC_F32 || prec == ICC_BF16 || prec == ICC_F16X) {
     ICC_HIP(launch_to_f32(w->tmp, w->x32, n, st));
     if (prec == ICC_F32)
       ICC_HIP(apply_t<float>(op, w->x32, w->ap32, nullptr));
     else
-      ICC_HIP(apply_bf16(op, w->x32, w->ap32));
+      ICC_HIP(apply_lowp(op, prec, w->x32, w->ap32));
     std::vector<float> h(n);
     ICC_HIP(hipMemcpyAsync(h.data(), w->ap32, n * 4, hipMemcpyDeviceToHost, st));
     ICC_HIP(hipStreamSynchronize(st));
@@ -1007,7 +1131,7 @@ static int write_tol(icc_op* op, Work* w, double tol) {
 // first, then the tiles, then the two vector kernels).
 template <typename T>
 static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, hipStream_t st,
-                                    bool bf16 = false) {
+                                    int prec = ICC_F32) {
   icc_grid* g = op->g;
   const int64_t n = g->s.ndof;
   const int npap = npap_of(op);
@@ -1024,7 +1148,8 @@ static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap
                                         w->pap + apply_grid(g), st);
     if (e == hipSuccess) e = launch_faces<float>(op, p, ap, w->pap + apply_grid(g) + csr_blocks(op), w->st, st);
     if (e == hipSuccess)
-      e = bf16 ? launch_apply_bf16(g, op, p, ap, w->pap, w->st, st) : launch_apply_f32(g, op, p, ap, w->pap, w->st, st);
+      e = prec == ICC_F32 ? launch_apply_f32(g, op, p, ap, w->pap, w->st, st)
+                          : launch_apply_lowp(prec, g, op, p, ap, w->pap, w->st, st);
   }
   if (e == hipSuccess) e = launch_pcg_update<T>(n, w->st, w->pap, npap, p, ap, x, r, dinv, w->parts, st);
   return e;
@@ -1042,11 +1167,11 @@ struct IterGraph {
 
 template <typename T>
 static int capture_batch(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, int batch, IterGraph& G,
-                         bool bf16 = false) {
+                         int prec = ICC_F32) {
   hipStream_t st = op->g->ctx->stream;
   ICC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
-  for (int i = 0; i < batch && e == hipSuccess; ++i) e = enqueue_iteration<T>(op, w, x, r, p, ap, dinv, st, bf16);
+  for (int i = 0; i < batch && e == hipSuccess; ++i) e = enqueue_iteration<T>(op, w, x, r, p, ap, dinv, st, prec);
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(st, &graph);
   if (e != hipSuccess) return hip_fail(e, "PCG batch capture");
@@ -1094,17 +1219,17 @@ constexpr int PCG_BATCH = 16;
 
 // Jacobi PCG in precision T on device vectors: solves A x = bscale * b from
 // x = 0 with dinv already formed.  status: 1 converged, 2 breakdown, 3 limit.
-// bf16 (T = float only): the element contraction of A in BF16.
+// prec (T = float only): the element contraction of A in FP32, BF16 or split FP16.
 template <typename T, typename TB>
 static int pcg_core(icc_op* op, Work* w, const TB* b, double bscale, T* x, T* r, T* p, T* ap, const T* dinv,
-                    double tol, int64_t maxit, int64_t* iters, double* res, int* status, bool bf16 = false) {
+                    double tol, int64_t maxit, int64_t* iters, double* res, int* status, int prec = ICC_F32) {
   icc_grid* g = op->g;
   hipStream_t st = g->ctx->stream;
   const int64_t n = g->s.ndof;
   ICC_TRY(set_limits(op, w, tol, maxit));
   ICC_HIP((launch_pcg_init<T, TB>(n, b, bscale, dinv, x, r, p, ap, w->st, w->parts, st)));
   IterGraph G;
-  ICC_TRY(capture_batch<T>(op, w, x, r, p, ap, dinv, PCG_BATCH, G, bf16));
+  ICC_TRY(capture_batch<T>(op, w, x, r, p, ap, dinv, PCG_BATCH, G, prec));
   PcgState hs;
   if (maxit > 0) {
     ICC_TRY(run_until_done(op, w, G, hs));
@@ -1159,7 +1284,7 @@ ICC_API int icc_pcg(icc_op* op, const double* b, const double* diag, double tol,
 ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol, double inner_tol, int64_t max_outer,
                              int64_t max_inner, double* u, double* stats) {
   if (!op || !f || !u) return ICC_EINVAL;
-  if (inner != ICC_F64 && inner != ICC_F32 && inner != ICC_BF16) {
+  if (inner != ICC_F64 && inner != ICC_F32 && inner != ICC_BF16 && inner != ICC_F16X) {
     set_error("icc_solve_ir: unknown inner precision");
     return ICC_EINVAL;
   }
@@ -1212,9 +1337,9 @@ ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol,
     int64_t its = 0;
     double res = 0;
     int status = 0;
-    if (inner == ICC_F32 || inner == ICC_BF16) {
+    if (inner != ICC_F64) {
       ICC_TRY((pcg_core<float, double>(op, w, w->r, 1.0 / nr, w->x32, w->r32, w->p32, w->ap32, w->dinv32, inner_tol,
-                                       max_inner, &its, &res, &status, inner == ICC_BF16)));
+                                       max_inner, &its, &res, &status, inner)));
       ICC_HIP(launch_ir_update<float>(n, u, w->x32, nr, st));
     } else {
       ICC_TRY((pcg_core<double, double>(op, w, w->r, 1.0 / nr, w->x, w->tmp, w->p, w->ap, w->dinv, inner_tol,
@@ -1250,9 +1375,13 @@ ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol,
 // while the search direction is kept (no restart).  Stops when the true
 // relative residual <= tol.  stats: updates, FP32 iterations, true residual,
 // FP64 applies, FP32 applies.
-ICC_API int icc_solve_mixed_dev(icc_op* op, const double* f, double tol, double delta, int64_t maxit, double* u,
-                                double* stats) {
+ICC_API int icc_solve_mixed_prec_dev(icc_op* op, int inner, const double* f, double tol, double delta, int64_t maxit,
+                                     double* u, double* stats) {
   if (!op || !f || !u || !(delta > 0.0 && delta < 1.0)) return ICC_EINVAL;
+  if (inner != ICC_F32 && inner != ICC_BF16 && inner != ICC_F16X) {
+    set_error("icc_solve_mixed: the inner precision must be ICC_F32, ICC_BF16 or ICC_F16X");
+    return ICC_EINVAL;
+  }
   Work* w;
   ICC_TRY(ensure_work(op, w));
   icc_grid* g = op->g;
@@ -1279,7 +1408,7 @@ ICC_API int icc_solve_mixed_dev(icc_op* op, const double* f, double tol, double 
     ICC_HIP((launch_pcg_init<float, double>(n, w->r, 1.0, w->dinv32, w->x32, w->r32, w->p32, w->ap32, w->st, w->parts,
                                              st)));
     IterGraph G;
-    ICC_TRY(capture_batch<float>(op, w, w->x32, w->r32, w->p32, w->ap32, w->dinv32, PCG_BATCH, G));
+    ICC_TRY(capture_batch<float>(op, w, w->x32, w->r32, w->p32, w->ap32, w->dinv32, PCG_BATCH, G, inner));
     for (;;) {
       ICC_TRY(run_until_done(op, w, G, hs));
       iters = hs.iters;
@@ -1318,6 +1447,11 @@ ICC_API int icc_solve_mixed_dev(icc_op* op, const double* f, double tol, double 
   }
   if (rc_final == ICC_ENOCONV) set_error("icc_solve_mixed: iteration limit reached");
   return rc_final;
+}
+
+ICC_API int icc_solve_mixed_dev(icc_op* op, const double* f, double tol, double delta, int64_t maxit, double* u,
+                                double* stats) {
+  return icc_solve_mixed_prec_dev(op, ICC_F32, f, tol, delta, maxit, u, stats);
 }
 
 namespace icc {
@@ -1361,6 +1495,14 @@ ICC_API int icc_solve_ir(icc_op* op, int inner, const double* f, double tol, dou
   if (!op || !f || !u) return ICC_EINVAL;
   return host_solve(op, f, u, [&](const double* df, double* du) {
     return icc_solve_ir_dev(op, inner, df, tol, inner_tol, max_outer, max_inner, du, stats);
+  });
+}
+
+ICC_API int icc_solve_mixed_prec(icc_op* op, int inner, const double* f, double tol, double delta, int64_t maxit,
+                                 double* u, double* stats) {
+  if (!op || !f || !u) return ICC_EINVAL;
+  return host_solve(op, f, u, [&](const double* df, double* du) {
+    return icc_solve_mixed_prec_dev(op, inner, df, tol, delta, maxit, du, stats);
   });
 }
 
@@ -1622,6 +1764,18 @@ ICC_API int icc_debug_mfma_layout_bf16(icc_ctx* ctx, double* out256) {
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   hipFree(d);
   if (e != hipSuccess) return hip_fail(e, "icc_debug_mfma_layout_bf16");
+  return ICC_OK;
+}
+
+ICC_API int icc_debug_mfma_layout_f16(icc_ctx* ctx, double* out512) {
+  if (!ctx || !out512) return ICC_EINVAL;
+  double* d = nullptr;
+  ICC_TRY(dalloc(&d, 512));
+  hipError_t e = launch_layout_probe_f16(d, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out512, d, 512 * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "icc_debug_mfma_layout_f16");
   return ICC_OK;
 }
 

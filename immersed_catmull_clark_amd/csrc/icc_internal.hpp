@@ -17,6 +17,15 @@ constexpr int KS = 21;       // 4-deep K steps covering 81 trial columns (84 pad
 constexpr int TILE_MAX_EL = 128;   // elements per tile (8 waves x 16)
 constexpr int TILE_MAX_PTS = 640;  // control points staged per tile
 constexpr int APPLY_MAX_WG_PER_CU = 4;  // apply workgroups per CU, at most
+// Split-FP16 apply (k_split.hip): one workgroup of 8 MFMA + 4 memory waves
+// per CU; tiles of <= 16 element groups (x-lines or 16-element groups).
+constexpr int SP_THREADS = 768;
+constexpr int SP_MAX_GROUPS = 16;   // element groups per tile
+constexpr int SP_MAX_EL = 256;      // elements per tile
+constexpr int SP_MAX_PTS = 720;     // staged control points per tile
+constexpr int SP_MAX_WIN = 1920;    // staged x-windows per tile (3 nx (ny + 2) (nz + 2))
+constexpr int SP_TASK_ROUNDS = 3;   // staging tasks per memory thread
+constexpr int SP_SCALE_BITS = 15;   // K_t and x scaled to a maximum in [2^14, 2^15)
 
 // Tile of a constant-type element sub-box: elements [i0, i0+nx) x [j0, j0+ny) x [k0, k0+nz).
 // lines = 1: the tile is processed as x-lines of nx <= 16 elements (one MFMA
@@ -76,12 +85,17 @@ struct GridSetup {
   std::vector<int32_t> type_of;  // index t0 + T0*(t1 + T1*t2) -> element type
   std::vector<std::vector<double>> kmat;  // [ntypes][81*81]
   std::vector<Tile> tiles;
+  std::vector<Tile> stiles;  // split-FP16 apply tiles
 };
 
 int build_setup(const int64_t nel[3], const double bounds[6], double young, double poisson, GridSetup& s);
 // (Re)builds s.tiles with tiling variant v (0 .. n_tile_variants() - 1; the
 // shape order is v % 6, and v < 6 asks for the balanced partition).
 void build_tiles(GridSetup& s, int variant);
+// Tiles of the split-FP16 apply (s.stiles), within the SP_* limits.
+void build_split_tiles(GridSetup& s);
+// Work of a split-apply tile (its element groups plus a per-tile overhead).
+double split_tile_cost(const Tile& t);
 // Re-partitions s.tiles for G persistent workgroups: walks the tiles in
 // order, giving workgroup b a contiguous run that ends as close as possible
 // to the cumulative target (b + 1) W / G, where W is the total work; the tile
@@ -89,6 +103,8 @@ void build_tiles(GridSetup& s, int variant);
 // whole-tile counts when that partition is no worse.  first[b] .. first[b + 1]
 // are workgroup b's tiles (G + 1 entries).
 void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first);
+// The same partition of an arbitrary tile list under a cost model.
+void balance_tile_list(std::vector<Tile>& tiles, int G, double (*cost)(const Tile&), std::vector<int32_t>& first);
 int n_tile_variants();
 int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
 
@@ -147,6 +163,13 @@ struct icc_grid {
   int apply_G = 1;       // persistent apply workgroups
   int32_t* d_wgfirst = nullptr;  // [apply_G + 1] tile range of each workgroup (balance_tiles)
   std::vector<struct icc_op*> ops;  // operators on this grid (their tile masks follow the tiling)
+  // split-FP16 apply
+  uint4* d_afrag16 = nullptr;     // [ntypes][2 halves][3 M tiles][4 K steps][64 lanes][hi, lo]
+  int32_t* d_kexp16 = nullptr;    // [ntypes] K_t was scaled by 2^-kexp
+  icc::Tile* d_stiles = nullptr;
+  int nstiles = 0;
+  int split_G = 1;                // workgroups (one per CU)
+  int32_t* d_swgfirst = nullptr;  // [split_G + 1]
 };
 
 struct icc_op {
@@ -155,6 +178,7 @@ struct icc_op {
   float* d_scale32 = nullptr;   // [nel]
   uint8_t* d_free = nullptr;    // [ndof], null = all free
   int32_t* d_tmask = nullptr;   // [ntiles]: 1 if the tile stages a fixed DOF
+  int32_t* d_stmask = nullptr;  // [nstiles]: the same for the split-apply tiles
   std::vector<uint8_t> h_free;  // host copy of the free mask
   // optional sparse add-on (CSR, int32 indices)
   int64_t s_nnz = 0;

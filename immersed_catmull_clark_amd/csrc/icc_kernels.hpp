@@ -40,6 +40,11 @@ hipError_t launch_apply_f32(const icc_grid* g, const icc_op* op, const float* x,
 // K and x rounded to BF16, FP32 accumulation (v_mfma_f32_16x16x32_bf16); float x, y, scale
 hipError_t launch_apply_bf16(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
                              const PcgState* pst, hipStream_t st);
+// k_split.hip: K and x split into FP16 hi + lo parts, three MFMA passes,
+// FP32 accumulation; float x, y, scale
+hipError_t launch_apply_split(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
+                              const PcgState* pst, hipStream_t st);
+hipError_t launch_layout_probe_f16(double* out, hipStream_t st);
 hipError_t launch_energies(const icc_grid* g, const double* u, double* ee, const double* x, double penal,
                            double emin, const double* w, double* dc, hipStream_t st);
 hipError_t launch_diag_f64(const icc_grid* g, const double* scale, double* d, hipStream_t st);

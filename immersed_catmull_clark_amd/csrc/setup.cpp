@@ -270,6 +270,7 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
   }
 
   build_tiles(s, 0);
+  build_split_tiles(s);
   return ICC_OK;
 }
 
@@ -360,8 +361,8 @@ static double tile_cost(const Tile& t) {
 // fills up to the cumulative target (b + 1) total / G, and the tile that
 // crosses it is taken whole, left for the next workgroup, or split at the
 // leading slab, whichever ends closest to the target.
-static double balance_pass(const std::vector<Tile>& in, int G, double total, std::vector<Tile>& out,
-                           std::vector<int32_t>& first) {
+static double balance_pass(const std::vector<Tile>& in, int G, double total, double (*tile_cost)(const Tile&),
+                           std::vector<Tile>& out, std::vector<int32_t>& first) {
   out.clear();
   out.reserve(in.size() + (size_t)G);
   first.assign(1, 0);
@@ -433,8 +434,8 @@ static double balance_pass(const std::vector<Tile>& in, int G, double total, std
   return cum;
 }
 
-void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first) {
-  const std::vector<Tile> in = s.tiles;
+void balance_tile_list(std::vector<Tile>& tiles, int G, double (*tile_cost)(const Tile&), std::vector<int32_t>& first) {
+  const std::vector<Tile> in = tiles;
   double total = 0.0;
   for (const Tile& t : in) total += tile_cost(t);
   // splitting adds staging work, which lands on the last workgroup: re-run
@@ -444,7 +445,7 @@ void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first) {
   std::vector<int32_t> fst, best_first;
   double best_max = 0.0;
   for (int it = 0; it < 8; ++it) {
-    const double found = balance_pass(in, G, total, out, fst);
+    const double found = balance_pass(in, G, total, tile_cost, out, fst);
     double mx = 0.0;
     for (int b = 0; b < G; ++b) {
       double w = 0.0;
@@ -469,12 +470,103 @@ void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first) {
     plain_max = std::max(plain_max, w);
   }
   if (plain_max <= best_max) {
-    s.tiles = in;
+    tiles = in;
     first = std::move(plain);
     return;
   }
-  s.tiles = std::move(best_out);
+  tiles = std::move(best_out);
   first = std::move(best_first);
+}
+
+void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first) {
+  balance_tile_list(s.tiles, G, tile_cost, first);
+}
+
+// Tiles of the split-FP16 apply.  Constant-type sub-boxes at least 8
+// elements long in x are cut into x-line tiles (lines of <= 16 elements
+// spread evenly) of up to 16 lines, the (ny, nz) shape with the most lines
+// and then the fewest staged points winning; shorter sub-boxes (the
+// x-boundary layers) into boxes processed as 16-element groups, the shape
+// with the most elements (<= 16 groups) and then the fewest points per
+// element winning.  Every shape keeps within the SP_* staging limits.
+static bool split_fits(int nx, int ny, int nz) {
+  const int py = ny + 2, pz = nz + 2;
+  return (nx + 2) * py * pz <= SP_MAX_PTS && 3 * nx * py * pz <= SP_MAX_WIN &&
+         ((nx + 3) / 4) * 3 * py * pz <= 256 * SP_TASK_ROUNDS && nx * ny * nz <= SP_MAX_EL;
+}
+
+double split_tile_cost(const Tile& t) {
+  const int nel = t.nx * t.ny * t.nz;
+  const int groups = t.lines ? t.ny * t.nz : (nel + 15) / 16;
+  return (double)groups + 0.5;
+}
+
+void build_split_tiles(GridSetup& s) {
+  std::vector<std::pair<int64_t, int64_t>> runs[3];
+  for (int d = 0; d < 3; ++d) {
+    int64_t st = 0;
+    for (int64_t e = 1; e <= s.nel[d]; ++e)
+      if (e == s.nel[d] || s.tid1[d][e] != s.tid1[d][st]) {
+        runs[d].push_back({st, e});
+        st = e;
+      }
+  }
+  s.stiles.clear();
+  for (auto& rz : runs[2])
+    for (auto& ry : runs[1])
+      for (auto& rx : runs[0]) {
+        const int64_t ex = rx.second - rx.first, ey = ry.second - ry.first, ez = rz.second - rz.first;
+        int tx, ty = 1, tz = 1, lines;
+        if (ex >= 8) {
+          const int64_t ntx = (ex + 15) / 16;
+          tx = (int)((ex + ntx - 1) / ntx);
+          lines = 1;
+          int best_lines = 0, best_pts = 0;
+          for (int by = 1; by <= std::min<int64_t>(ey, SP_MAX_GROUPS); ++by)
+            for (int bz = 1; bz <= std::min<int64_t>(ez, SP_MAX_GROUPS); ++bz) {
+              if (by * bz > SP_MAX_GROUPS || !split_fits(tx, by, bz)) continue;
+              const int pts = (tx + 2) * (by + 2) * (bz + 2);
+              if (by * bz > best_lines || (by * bz == best_lines && pts < best_pts)) {
+                best_lines = by * bz;
+                best_pts = pts;
+                ty = by;
+                tz = bz;
+              }
+            }
+        } else {
+          tx = (int)ex;
+          lines = 0;
+          int best_el = 0;
+          double best_ppe = 0.0;
+          for (int by = 1; by <= std::min<int64_t>(ey, 64); ++by)
+            for (int bz = 1; bz <= std::min<int64_t>(ez, 64); ++bz) {
+              const int nel = tx * by * bz;
+              if ((nel + 15) / 16 > SP_MAX_GROUPS || !split_fits(tx, by, bz)) continue;
+              const double ppe = (double)((tx + 2) * (by + 2) * (bz + 2)) / nel;
+              if (nel > best_el || (nel == best_el && ppe < best_ppe)) {
+                best_el = nel;
+                best_ppe = ppe;
+                ty = by;
+                tz = bz;
+              }
+            }
+        }
+        const int32_t type = element_type(s, rx.first, ry.first, rz.first);
+        for (int64_t k0 = rz.first; k0 < rz.second; k0 += tz)
+          for (int64_t j0 = ry.first; j0 < ry.second; j0 += ty)
+            for (int64_t i0 = rx.first; i0 < rx.second; i0 += tx) {
+              Tile t;
+              t.i0 = (int32_t)i0;
+              t.j0 = (int32_t)j0;
+              t.k0 = (int32_t)k0;
+              t.nx = (int32_t)std::min<int64_t>(tx, rx.second - i0);
+              t.ny = (int32_t)std::min<int64_t>(ty, ry.second - j0);
+              t.nz = (int32_t)std::min<int64_t>(tz, rz.second - k0);
+              t.type = type;
+              t.lines = lines;
+              s.stiles.push_back(t);
+            }
+      }
 }
 
 // Consistent traction on the face x = x_max: F(A, c) = t_c * int N_A dA, the

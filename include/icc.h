@@ -38,9 +38,15 @@ enum icc_status {
 enum icc_prec {
   ICC_F64 = 0, /* FP64 MFMA (v_mfma_f64_16x16x4), FP64 vectors */
   ICC_F32 = 1, /* FP32 MFMA (v_mfma_f32_16x16x4), FP32 vectors and accumulation */
-  ICC_BF16 = 2 /* K and x rounded to BF16, FP32 accumulation
+  ICC_BF16 = 2, /* K and x rounded to BF16, FP32 accumulation
                   (v_mfma_f32_16x16x32_bf16), FP32 vectors (x, y float*);
                   the apply, and the inner solve of icc_solve_ir */
+  ICC_F16X = 3  /* split FP16: K (scaled per element type) and x (scaled per
+                  tile) each as an FP16 hi + lo pair, three MFMA passes
+                  K_hi x_hi + K_hi x_lo + K_lo x_hi (v_mfma_f32_16x16x32_f16),
+                  FP32 accumulation, about 2^-21 relative error per element
+                  product; FP32 vectors (x, y float*).  The default inner
+                  operator of the mixed-precision solvers. */
 };
 
 typedef struct icc_ctx icc_ctx;
@@ -182,6 +188,16 @@ int icc_solve_ir(icc_op* op, int inner, const double* f, double tol, double inne
 int icc_solve_mixed_dev(icc_op* op, const double* f_dev, double tol, double delta, int64_t maxit, double* u_dev,
                         double* stats);
 int icc_solve_mixed(icc_op* op, const double* f, double tol, double delta, int64_t maxit, double* u, double* stats);
+/* The same solver with the inner contraction in precision `inner`: ICC_F32,
+ * ICC_F16X (split FP16, the fastest inner operator with FP32-level accuracy)
+ * or ICC_BF16 (converges only where 2^-9 times the condition number is below
+ * 1).  The inner vectors are FP32 in every case.  Replaces the solve closure
+ * of solve_state (topopt_iga.rs:137-162, solver.rs:14-76) by an FP64-accurate
+ * solve to a true residual <= tol. */
+int icc_solve_mixed_prec_dev(icc_op* op, int inner, const double* f_dev, double tol, double delta, int64_t maxit,
+                             double* u_dev, double* stats);
+int icc_solve_mixed_prec(icc_op* op, int inner, const double* f, double tol, double delta, int64_t maxit, double* u,
+                         double* stats);
 
 /* ---- sensitivities --------------------------------------------------------- */
 /* ee_e = u_e^T K_t(e) u_e, unscaled (topopt_iga.rs:165-184) */
@@ -493,6 +509,13 @@ int icc_debug_mfma_layout(icc_ctx* ctx, double* out512);
  * and B[8 (l/16) + j][l%16]; equals D[4 (l/16) + r][l%16] = (A B) if the
  * assumed layout holds. */
 int icc_debug_mfma_layout_bf16(icc_ctx* ctx, double* out256);
+/* Lane-layout check of the two FP16 MFMAs of the split-FP16 apply, same
+ * operands: out512[4 l + r] = register r of lane l after
+ * v_mfma_f32_16x16x32_f16 (A[l%16][8 (l/16) + j], B[8 (l/16) + j][l%16]) and
+ * out512[256 + 4 l + r] after v_mfma_f32_16x16x16_f16 (A[l%16][4 (l/16) + j],
+ * B[4 (l/16) + j][l%16]); both equal D[4 (l/16) + r][l%16] if the assumed
+ * layouts hold. */
+int icc_debug_mfma_layout_f16(icc_ctx* ctx, double* out512);
 /* Host only (no device work): the 1D p = 2 WQ rule the formation tables are
  * built from, on the open uniform knot vector of nel elements over [0, 1]
  * (wq_rules_1d_layout, wq.rs:134-139).  which 0: the points; 1..4: the

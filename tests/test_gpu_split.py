@@ -1,0 +1,187 @@
+"""Split-FP16 element contraction (ICC_F16X): K and x as FP16 hi + lo pairs,
+three v_mfma_f32_16x16x32_f16 passes with FP32 accumulation.
+
+Tolerances: the split parts carry 22 significant bits together, so one
+element product is accurate to about 2^-21 relative (5e-7); the operator is
+checked against the oracle's assembled FP64 operator at 2e-6 relative (the
+FP32 contraction's bar), and at full size against the FP64 GPU apply, which
+the oracle pins at small sizes.  The mixed solver with this inner operator is
+checked by the true residual recomputed with the oracle's FP64 operator.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+icc = pytest.importorskip("immersed_catmull_clark_amd")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = icc.Context(0)
+    yield c
+    c.close()
+
+
+GRIDS = [
+    ([[0.5, 2.5], [-1, 0.2], [0.1, 0.4]], [4, 3, 3]),
+    ([[0, 1.2], [0, 0.6], [0, 0.3]], [8, 4, 2]),
+    ([[0, 2.0], [0, 1.0], [0, 1.0]], [16, 8, 8]),
+    ([[0, 1.0], [0, 1.0], [0, 1.0]], [1, 1, 1]),
+    ([[0, 1.0], [0, 0.5], [0, 2.0]], [2, 1, 5]),
+    ([[0, 3.0], [0, 1.0], [0, 1.0]], [13, 7, 3]),
+    ([[0, 2.0], [0, 1.0], [0, 1.0]], [35, 9, 11]),
+]
+
+
+def test_f16_mfma_layouts(ctx):
+    """16x16x32: lane l holds A[l%16][8 (l/16) + j], B[8 (l/16) + j][l%16];
+    16x16x16: A[l%16][4 (l/16) + j], B[4 (l/16) + j][l%16]; register r holds
+    D[4 (l/16) + r][l%16] for both, checked against the exact integer
+    products."""
+    o32, o16 = ctx.mfma_layout_f16()
+    lane = np.arange(64)[:, None]
+    r = np.arange(4)[None, :]
+    for K, out in ((32, o32), (16, o16)):
+        i = np.arange(16)[:, None]
+        k = np.arange(K)[None, :]
+        A = ((7 * i + 3 * k) % 17 - 8).astype(np.float64)
+        B = ((5 * np.arange(K)[:, None] + 11 * np.arange(16)[None, :]) % 13 - 6).astype(np.float64)
+        want = (A @ B)[4 * (lane // 16) + r, lane % 16]
+        assert np.array_equal(out, want), (K, out[:4], want[:4])
+
+
+@pytest.mark.parametrize("bounds,nel", GRIDS)
+def test_split_apply_matches_oracle(ctx, bounds, nel):
+    """Masked, SIMP-scaled operator (s in [1e-6, 1]) against the oracle's
+    assembled FP64 operator; the split error is within a small factor of the
+    FP32 contraction's and far below BF16's."""
+    rng = np.random.default_rng(7 + sum(nel))
+    g = icc.Grid(ctx, nel, bounds)
+    sp = O.Space(bounds, nel, 2)
+    sp.elem_mats(*O.lame(1.0, 0.3))
+    s = 1e-9 + (1 - 1e-9) * rng.uniform(0.01, 1.0, g.nel_tot) ** 3
+    free = (rng.uniform(size=g.ndof) > 0.2).astype(np.uint8)
+    op = icc.Operator(g, scale=s, free=free)
+    sp.assemble(s)
+    x = rng.standard_normal(g.ndof)
+    ref = free * sp.matvec(free * x)
+    y = op.apply(x, prec=icc.ICC_F16X)
+    err = O.rel_err(y, ref)
+    assert err < 2e-6, err
+    assert np.all(y[free == 0] == 0.0)
+    err32 = O.rel_err(op.apply(x, prec=icc.ICC_F32), ref)
+    print(f"split {err:.3e} fp32 {err32:.3e}")
+    assert err < max(30 * err32, 1e-6), (err, err32)
+
+
+def test_split_apply_scales_per_tile(ctx):
+    """x spanning six decades across the grid: each tile's power-of-two scale
+    keeps the small-magnitude region as accurate, relative to its own size,
+    as the large one (a single global scale would push its low parts into
+    FP16 subnormals)."""
+    bounds, nel = [[0, 2.0], [0, 1.0], [0, 1.0]], [48, 8, 8]
+    rng = np.random.default_rng(11)
+    g = icc.Grid(ctx, nel, bounds)
+    sp = O.Space(bounds, nel, 2)
+    sp.elem_mats(*O.lame(1.0, 0.3))
+    s = rng.uniform(0.01, 1.0, g.nel_tot) ** 3
+    op = icc.Operator(g, scale=s)
+    sp.assemble(s)
+    ix = np.arange(g.ndof) % g.ndof_sc % (nel[0] + 2)
+    x = rng.standard_normal(g.ndof) * 10.0 ** (-6.0 * ix / (nel[0] + 1))
+    ref = sp.matvec(x)
+    y = op.apply(x, prec=icc.ICC_F16X)
+    small = ix < 12
+    assert O.rel_err(y[small], ref[small]) < 2e-6
+    assert O.rel_err(y, ref) < 2e-6
+
+
+def test_split_apply_accumulate_and_zero(ctx):
+    """apply_accumulate adds into y; x = 0 gives exactly 0 (scale exponent 0)."""
+    import torch
+    bounds, nel = [[0, 2.0], [0, 1.0], [0, 1.0]], [16, 8, 8]
+    rng = np.random.default_rng(3)
+    g = icc.Grid(ctx, nel, bounds)
+    op = icc.Operator(g, scale=rng.uniform(0.1, 1.0, g.nel_tot))
+    x = torch.tensor(rng.standard_normal(g.ndof), dtype=torch.float32, device="cuda")
+    y0 = torch.tensor(rng.standard_normal(g.ndof), dtype=torch.float32, device="cuda")
+    y = y0.clone()
+    torch.cuda.synchronize()  # the library runs on its own stream
+    op.apply_accumulate(x, y, prec=icc.ICC_F16X)
+    ax = torch.zeros_like(y)
+    op.apply(x, prec=icc.ICC_F16X, out=ax)
+    torch.cuda.synchronize()
+    assert torch.allclose(y, y0 + ax, rtol=1e-5, atol=1e-5 * ax.abs().max().item())
+    z = torch.zeros_like(x)
+    op.apply(z, prec=icc.ICC_F16X, out=ax)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(ax).item() == 0
+
+
+def test_split_apply_full_size(ctx):
+    """96x48x48 (735,000 DOFs), the benchmark's cantilever: against the FP64
+    GPU apply (pinned by the oracle at small sizes), and symmetry
+    z.(A x) = x.(A z), a size-independent property."""
+    import torch
+    nel = [96, 48, 48]
+    rng = np.random.default_rng(20261204)
+    g = icc.Grid(ctx, nel, [0.0, 2.0, 0.0, 1.0, 0.0, 1.0])
+    s = 1e-9 + (1 - 1e-9) * rng.uniform(0.01, 1.0, g.nel_tot) ** 3
+    free = np.ones(g.ndof, dtype=np.uint8)
+    ix = np.arange(g.ndof) % g.ndof_sc % (nel[0] + 2)
+    free[ix == 0] = 0
+    op = icc.Operator(g, scale=s, free=free)
+    x = rng.standard_normal(g.ndof)
+    z = rng.standard_normal(g.ndof)
+    y64 = op.apply(x)
+    y = op.apply(x, prec=icc.ICC_F16X)
+    assert O.rel_err(y, y64) < 2e-6
+    yz = op.apply(z, prec=icc.ICC_F16X)
+    a, b = float(z @ y), float(x @ yz)
+    assert abs(a - b) <= 2e-6 * np.linalg.norm(z) * np.linalg.norm(y)
+    del torch
+
+
+def _cantilever(ctx, nel, lo, seed):
+    bounds = [[0, 2.0], [0, 1.0], [0, 1.0]]
+    rng = np.random.default_rng(seed)
+    g = icc.Grid(ctx, nel, bounds)
+    sp = O.Space(bounds, nel, 2)
+    sp.elem_mats(*O.lame(1.0, 0.3))
+    s = 1e-9 + (1 - 1e-9) * rng.uniform(lo, 1.0, g.nel_tot) ** 3
+    sp.assemble(s)
+    gx = sp.greville()[:, 0]
+    free = np.ones(g.ndof, dtype=np.uint8)
+    for c in range(3):
+        free[c * g.ndof_sc + np.nonzero(gx < 1e-12)[0]] = 0
+    f = free * rng.standard_normal(g.ndof)
+    return sp, g, free, f, icc.Operator(g, scale=s, free=free)
+
+
+@pytest.mark.parametrize("nel", [(16, 8, 8), (24, 12, 12)])
+def test_mixed_solve_with_split_inner_reaches_1e8(ctx, nel):
+    """Reliable-update PCG with the split-FP16 inner contraction on the
+    cantilever with densities in [0.01, 1] (s down to 1e-6, the case where a
+    BF16 inner operator does not converge): the true relative residual,
+    recomputed with the oracle's FP64 operator, reaches 1e-8, and u agrees
+    with the FP64 PCG solution to the residual times the conditioning."""
+    sp, g, free, f, op = _cantilever(ctx, nel, 0.01, 5 + nel[0])
+    u, st, rc = op.solve_mixed(f, tol=1e-8, delta=0.1, inner=icc.ICC_F16X)
+    assert rc == icc.ICC_OK, st
+    r = free * (f - sp.matvec(free * u))
+    assert np.linalg.norm(r) / np.linalg.norm(f) <= 1.05e-8
+    assert st["true_res"] <= 1e-8 and st["updates"] >= 1
+    u64, it64, res, rc64 = op.pcg(f, np.maximum(op.diag(), 1e-12), 1e-10, 50 * g.ndof)
+    assert rc64 == icc.ICC_OK
+    assert np.linalg.norm(u - u64) / np.linalg.norm(u64) < 1e-4
+    # the split inner solve needs no more iterations than the FP32 one, within 15 %
+    u32, st32, rc32 = op.solve_mixed(f, tol=1e-8, delta=0.1, inner=icc.ICC_F32)
+    assert rc32 == icc.ICC_OK
+    print(st, st32)
+    assert st["inner_iters"] <= 1.15 * st32["inner_iters"] + 10, (st, st32)
