@@ -174,7 +174,7 @@ static int upload_split_tiles(icc_grid* g) {
   if (G > (int)g->s.stiles.size()) G = (int)g->s.stiles.size();
   if (G < 1) G = 1;
   std::vector<int32_t> first;
-  balance_tile_list(g->s.stiles, G, split_tile_cost, first);
+  partition_split_tiles(g->s.stiles, G, first);
   Tile* dt = nullptr;
   int32_t* df = nullptr;
   int rc;

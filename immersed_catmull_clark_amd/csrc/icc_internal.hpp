@@ -23,7 +23,8 @@ constexpr int SP_THREADS = 768;
 constexpr int SP_MAX_GROUPS = 16;   // element groups per tile
 constexpr int SP_MAX_EL = 256;      // elements per tile
 constexpr int SP_MAX_PTS = 720;     // staged control points per tile
-constexpr int SP_MAX_WIN = 1920;    // staged x-windows per tile (3 nx (ny + 2) (nz + 2))
+constexpr int SP_MAX_HPTS = 4096;   // staged FP16 points per part, rows padded to 8 points
+constexpr int SP_MAX_ROWS = 720;    // staged (component, y, z) rows per tile (3 (ny + 2) (nz + 2))
 constexpr int SP_TASK_ROUNDS = 3;   // staging tasks per memory thread
 constexpr int SP_SCALE_BITS = 15;   // K_t and x scaled to a maximum in [2^14, 2^15)
 
@@ -39,6 +40,10 @@ struct Tile {
 #else
 #define ICC_HD
 #endif
+
+// Padded row length (points) of a split-apply tile's FP16 rows: at least
+// nx + 3, the last element's 4-point run (x .. x + 3), rounded up to 8.
+ICC_HD inline int sp_pxp(int nx) { return (nx + 3 + 7) & ~7; }
 
 // Row of K held by accumulator register r (0..3) of row group kq (0..3) in
 // M tile w (0..5), or -1 for padding.  Rows are permuted so that registers
@@ -103,6 +108,9 @@ double split_tile_cost(const Tile& t);
 // whole-tile counts when that partition is no worse.  first[b] .. first[b + 1]
 // are workgroup b's tiles (G + 1 entries).
 void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first);
+// Split-apply partition: contiguous runs of minimal maximum cost
+// (split_tile_cost), tiles split where that lowers it.
+void partition_split_tiles(std::vector<Tile>& tiles, int G, std::vector<int32_t>& first);
 // The same partition of an arbitrary tile list under a cost model.
 void balance_tile_list(std::vector<Tile>& tiles, int G, double (*cost)(const Tile&), std::vector<int32_t>& first);
 int n_tile_variants();

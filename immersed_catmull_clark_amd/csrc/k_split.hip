@@ -15,14 +15,17 @@
 //
 // B operand without a gather.  The trial columns of an element are ordered
 // by triplet (component c, ay, az) with four slots ax = 0..3 per triplet (slot
-// 3 has zero K entries).  The staging writes, for every element x-position of
-// every staged (c, y, z) row, the window of the four control points
-// x .. x + 3 as 16 bytes (hi parts, then lo parts).  A lane's eight K values
-// of one 32-deep step are two triplets of its element, so the B operand of a
-// step is two 8-byte LDS reads per part, shared by three M tiles and three
-// passes.  24 triplets fill three 32-deep steps; the last three take one
-// 16-deep step (four slots per lane group).  Executed MFMA work per element:
-// 96 rows x 112 columns x 3 passes.
+// 3 has zero K entries).  The staged points are kept as FP16 hi and lo rows,
+// one row of x-points per (c, y, z), padded to sp_pxp points (at least one
+// zero past the row, for the last element's run), each row twice: as is and
+// shifted by one point.  A lane's eight K values of one 32-deep step are two
+// triplets of its element, i.e. two runs of 4 consecutive x-points; a run
+// starting at an even point is a dword-aligned 8-byte read from the first
+// copy, one starting at an odd point from the shifted copy.  So the B operand
+// of a step is two aligned LDS reads per part, shared by three M tiles and
+// three passes.  24 triplets fill three 32-deep steps; the last three take
+// one 16-deep step (four slots per lane group).  Executed MFMA work per
+// element: 96 rows x 112 columns x 3 passes.
 //
 // Work decomposition.  One workgroup of twelve waves per CU walks a
 // contiguous run of tiles (sub-boxes of constant element type, up to 16
@@ -32,11 +35,13 @@
 //     of A fragments (both parts) resident; scale, DPP combine along the
 //     x-line (rows are permuted so that accumulator registers 0..2 hold the
 //     ax = 0, 1, 2 rows of one triplet, as in the FP64 apply) and ds_add_f64
-//     into the tile's LDS output box;
+//     into the tile's LDS output box; after their groups of tile i they
+//     flush tile i - 1's output box to y with global atomic adds (masked),
+//     accumulating the p.Ap partial;
 //   * waves 8..11 (memory waves): one tile ahead they load the points of
-//     tile i + 2 and reduce their maximum, convert and write the windows of
-//     tile i + 1, and flush tile i - 1's output box to y with global atomic
-//     adds (masked), accumulating the p.Ap partial.
+//     tile i + 2 and reduce their maximum, and convert and write the windows
+//     of tile i + 1.  They issue no global stores or atomics, so their waits
+//     for the loads wait for nothing else.
 // One barrier per tile.  Windows and output boxes are double-buffered, the
 // FP32 copy of the staged points (the p.Ap partial of the flush) is
 // triple-buffered.
@@ -88,48 +93,62 @@ __device__ __forceinline__ void sp_triplet(int t, int& c, int& ay, int& az) {
   az = rem / 3;
 }
 
-// Staging of one tile by the 256 memory threads: task = (row, quad), row =
-// (c, lz, ly) of the staged box, quad q = windows 4q .. 4q + 3 of the row,
-// which need points 4q .. 4q + 6.
+// Staging of one tile by the 256 memory threads: task = (row, chunk), row =
+// (c, lz, ly) of the staged box, chunk = 8 consecutive x-points of the row
+// (rows padded to sp_pxp points, a multiple of 8 with at least one zero past
+// the row) plus the next point, which the shifted copy needs.
 struct SpLoad {
-  float v[SP_TASK_ROUNDS][7];
+  float v[SP_TASK_ROUNDS][9];
   float sc;
 };
 
-__device__ __forceinline__ void sp_geom(const Tile& t, int& px, int& py, int& pz, int& nq, int& ntask) {
+__device__ __forceinline__ void sp_geom(const Tile& t, int& px, int& py, int& pz, int& pxp, int& ntask) {
   px = t.nx + 2;
   py = t.ny + 2;
   pz = t.nz + 2;
-  nq = (t.nx + 3) >> 2;
-  ntask = nq * 3 * py * pz;
+  pxp = sp_pxp(t.nx);
+  ntask = (pxp >> 3) * 3 * py * pz;
 }
 
-// Loads tile t's points (masked when `masked`) and element scales into L;
-// returns the largest |value| this thread loaded.
-__device__ __forceinline__ float sp_load(const Tile& t, bool masked, int tm, const float* __restrict__ x,
-                                         const float* __restrict__ scale, const uint8_t* __restrict__ freem,
-                                         int ncp0, int ncp1, int ndof_sc, int nel0, int nel1, SpLoad& L) {
-  int px, py, pz, nq, ntask;
-  sp_geom(t, px, py, pz, nq, ntask);
-  float mx = 0.0f;
+// Issues the loads of tile t's points (masked when `masked`) and element
+// scales into L; the values are consumed later (sp_absmax, sp_stage), so
+// the loads stay in flight meanwhile.  Full chunks load as two 16-byte
+// vectors (4-byte aligned: gfx950 supports unaligned global access).
+__device__ __forceinline__ void sp_load(const Tile& t, bool masked, int tm, const float* __restrict__ x,
+                                        const float* __restrict__ scale, const uint8_t* __restrict__ freem,
+                                        int ncp0, int ncp1, int ndof_sc, int nel0, int nel1, SpLoad& L) {
+  int px, py, pz, pxp, ntask;
+  sp_geom(t, px, py, pz, pxp, ntask);
+  const int nch = pxp >> 3;
 #pragma unroll
   for (int k = 0; k < SP_TASK_ROUNDS; ++k) {
     const int task = tm + 256 * k;
     if (task < ntask) {
-      const int row = sp_div(task, nq), q = task - row * nq;
+      const int row = sp_div(task, nch), ch = task - row * nch;
       const int r2 = sp_div(row, py), ly = row - r2 * py;
       const int c = sp_div(r2, pz), lz = r2 - c * pz;
-      const int gb = (t.i0 + 4 * q) + ncp0 * ((t.j0 + ly) + ncp1 * (t.k0 + lz)) + c * ndof_sc;
+      const int gb = (t.i0 + 8 * ch) + ncp0 * ((t.j0 + ly) + ncp1 * (t.k0 + lz)) + c * ndof_sc;
+      if (8 * ch + 8 <= px) {
+        float4 a, b;
+        __builtin_memcpy(&a, x + gb, 16);
+        __builtin_memcpy(&b, x + gb + 4, 16);
+        L.v[k][0] = a.x;
+        L.v[k][1] = a.y;
+        L.v[k][2] = a.z;
+        L.v[k][3] = a.w;
+        L.v[k][4] = b.x;
+        L.v[k][5] = b.y;
+        L.v[k][6] = b.z;
+        L.v[k][7] = b.w;
+        L.v[k][8] = (8 * ch + 8 < px) ? x[gb + 8] : 0.0f;
+      } else {
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
-        float v = 0.0f;
-        if (4 * q + j < px) {
-          v = x[gb + j];
-          if (masked && freem[gb + j] == 0) v = 0.0f;
-        }
-        L.v[k][j] = v;
-        mx = fmaxf(mx, fabsf(v));
+        for (int j = 0; j < 9; ++j) L.v[k][j] = (8 * ch + j < px) ? x[gb + j] : 0.0f;
       }
+      if (masked)
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+          if (8 * ch + j < px && freem[gb + j] == 0) L.v[k][j] = 0.0f;
     }
   }
   const int nel = t.nx * t.ny * t.nz;
@@ -139,6 +158,18 @@ __device__ __forceinline__ float sp_load(const Tile& t, bool masked, int tm, con
     const int ez = sp_div(r, t.ny), ey = r - ez * t.ny;
     L.sc = scale[(t.i0 + ex) + nel0 * ((t.j0 + ey) + nel1 * (t.k0 + ez))];
   }
+}
+
+// Largest |value| of the tasks this thread loaded for tile t.
+__device__ __forceinline__ float sp_absmax(const Tile& t, int tm, const SpLoad& L) {
+  int px, py, pz, pxp, ntask;
+  sp_geom(t, px, py, pz, pxp, ntask);
+  float mx = 0.0f;
+#pragma unroll
+  for (int k = 0; k < SP_TASK_ROUNDS; ++k)
+    if (tm + 256 * k < ntask)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) mx = fmaxf(mx, fabsf(L.v[k][j]));  // point 8 belongs to the next chunk
   return mx;
 }
 
@@ -152,41 +183,75 @@ __device__ __forceinline__ int sp_exponent(uint32_t maxbits) {
   return maxbits == 0u ? 0 : E - 126 - SP_SCALE_BITS;
 }
 
-// Writes tile t's windows and FP32 point copy from L, scaled by 2^-ex.
-__device__ __forceinline__ void sp_stage(const Tile& t, int tm, int ex, const SpLoad& L, uint4* __restrict__ win,
-                                         float* __restrict__ xs, float* __restrict__ ss) {
-  int px, py, pz, nq, ntask;
-  sp_geom(t, px, py, pz, nq, ntask);
+// Writes tile t's FP16 hi and lo point rows (both copies, padded layout),
+// FP32 point copy (unpadded) and global row bases from L, scaled by 2^-ex.
+// Row r's points p, p + 1 (p even) are dword p / 2 of copy 0; points p + 1,
+// p + 2 are dword p / 2 of copy 1.
+__device__ __forceinline__ void sp_stage(const Tile& t, int tm, int ex, const SpLoad& L, uint4* __restrict__ h0,
+                                         uint4* __restrict__ h1, uint4* __restrict__ l0, uint4* __restrict__ l1,
+                                         float* __restrict__ xs, float* __restrict__ ss, int* __restrict__ rowg,
+                                         int ncp0, int ncp1, int ndof_sc) {
+  int px, py, pz, pxp, ntask;
+  sp_geom(t, px, py, pz, pxp, ntask);
+  const int nch = pxp >> 3;
 #pragma unroll
   for (int k = 0; k < SP_TASK_ROUNDS; ++k) {
     const int task = tm + 256 * k;
     if (task < ntask) {
-      const int row = sp_div(task, nq), q = task - row * nq;
-      _Float16 hi[7], lo[7];
+      const int row = sp_div(task, nch), ch = task - row * nch;
+      _Float16 hi[9], lo[9];
 #pragma unroll
-      for (int j = 0; j < 7; ++j) {
+      for (int j = 0; j < 9; ++j) {
         const float v = __builtin_amdgcn_ldexpf(L.v[k][j], -ex);
         hi[j] = (_Float16)v;
         lo[j] = (_Float16)(v - (float)hi[j]);
       }
-      uint32_t ph[6], pl[6];
+      const int w = row * nch + ch;
+      h0[w] = make_uint4(sp_pack(hi[0], hi[1]), sp_pack(hi[2], hi[3]), sp_pack(hi[4], hi[5]), sp_pack(hi[6], hi[7]));
+      h1[w] = make_uint4(sp_pack(hi[1], hi[2]), sp_pack(hi[3], hi[4]), sp_pack(hi[5], hi[6]), sp_pack(hi[7], hi[8]));
+      l0[w] = make_uint4(sp_pack(lo[0], lo[1]), sp_pack(lo[2], lo[3]), sp_pack(lo[4], lo[5]), sp_pack(lo[6], lo[7]));
+      l1[w] = make_uint4(sp_pack(lo[1], lo[2]), sp_pack(lo[3], lo[4]), sp_pack(lo[5], lo[6]), sp_pack(lo[7], lo[8]));
+      float* xrow = xs + row * px + 8 * ch;
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        ph[j] = sp_pack(hi[j], hi[j + 1]);
-        pl[j] = sp_pack(lo[j], lo[j + 1]);
-      }
-      uint4* wrow = win + row * t.nx + 4 * q;
-#pragma unroll
-      for (int w = 0; w < 4; ++w)
-        if (4 * q + w < t.nx) wrow[w] = make_uint4(ph[w], ph[w + 2], pl[w], pl[w + 2]);
-      float* xrow = xs + row * px + 4 * q;
-      const int jmax = (q == nq - 1) ? 7 : 4;
-#pragma unroll
-      for (int j = 0; j < 7; ++j)
-        if (j < jmax && 4 * q + j < px) xrow[j] = L.v[k][j];
+      for (int j = 0; j < 8; ++j)
+        if (8 * ch + j < px) xrow[j] = L.v[k][j];
     }
   }
+  const int nrows = 3 * py * pz;
+  for (int row = tm; row < nrows; row += 256) {
+    const int r2 = sp_div(row, py), ly = row - r2 * py;
+    const int c = sp_div(r2, pz), lz = r2 - c * pz;
+    rowg[row] = t.i0 + ncp0 * ((t.j0 + ly) + ncp1 * (t.k0 + lz)) + c * ndof_sc;
+  }
   if (tm < t.nx * t.ny * t.nz) ss[tm] = L.sc;
+}
+
+// Row rotation within 16-lane rows: lane c gets lane (c - K) mod 16.
+template <int K>
+__device__ __forceinline__ float sp_ror(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x120 + K, 0xf, 0xf, true));
+}
+
+// Scatter of one accumulator set (registers 0..2 = ax 0, 1, 2 of one triplet
+// per row group) of an x-line into the LDS output box: point x = col
+// receives s[col] v0[col] + s[col-1] v1[col-1] + s[col-2] v2[col-2] (s1, s2:
+// the element scales shifted like v1, v2); points 16 and 17 (lines of 15 and
+// 16 elements) are s[15] v1[15] + s[14] v2[14] and s[15] v2[15], moved to
+// lanes 0 and 1 by row rotations (r1 = rotated scale on lane 0 only, r2).
+// yl: this lane's point; cmask: col <= nx + 1; emask: lanes 0, 1 of lines
+// that reach points 16, 17.
+__device__ __forceinline__ void sp_line_scatter(double* yl, bool cmask, bool emask, float s, float s1, float s2,
+                                                float r1, float r2, float v0, float v1, float v2) {
+  float comb = v0 * s;
+  comb = fmaf(sp_shr<1>(v1), s1, comb);
+  comb = fmaf(sp_shr<2>(v2), s2, comb);
+  float ext = sp_ror<2>(v2) * r2;
+  ext = fmaf(sp_ror<1>(v1), r1, ext);
+  // the DPP moves read other lanes: they must run with every lane active,
+  // so their results are pinned here, before the lane-dependent branches
+  asm("" : "+v"(comb), "+v"(ext));
+  if (cmask) unsafeAtomicAdd(yl, (double)comb);
+  if (emask) unsafeAtomicAdd(yl + 16, (double)ext);
 }
 
 // y += M sum_e s_e K_t(e) x_e (M x) in split FP16; optional per-workgroup
@@ -199,13 +264,20 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
     const int* __restrict__ kexp, const float* __restrict__ x, float* __restrict__ y,
     const float* __restrict__ scale, const uint8_t* __restrict__ freem, const int32_t* __restrict__ tmask,
     int ncp0, int ncp1, int ndof_sc, int nel0, int nel1, double* __restrict__ pap, const PcgState* st) {
-  __shared__ uint4 win[2][SP_MAX_WIN];
+  // FP16 hi and lo parts of the staged points: [buffer][copy] (copy 1
+  // shifted by one point)
+  __shared__ uint4 hh[2][2][SP_MAX_HPTS / 8];
+  __shared__ uint4 hl[2][2][SP_MAX_HPTS / 8];
   __shared__ float xs[3][3 * SP_MAX_PTS];
+  __shared__ int rowg[3][SP_MAX_ROWS];
+  // output boxes accumulate in FP64: ds_add_f64 runs at full rate, while
+  // FP32 LDS atomics back up the LDS data FIFO on gfx950 and stall every
+  // other LDS access of the workgroup
   __shared__ double ys[2][3 * SP_MAX_PTS];
   __shared__ float ss[2][SP_MAX_EL];
   __shared__ uint32_t pmax[2][4];
   __shared__ int sexp[2];
-  __shared__ double red[4];
+  __shared__ double red[8];
 
   if (st != nullptr && st->done != 0) return;  // solver already finished
   int first, last;
@@ -228,73 +300,98 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
   // The two roles run separate loops with the same barrier sequence: two in
   // the prologue, one per tile, and one before the p.Ap reduction.
   if (memw) {
-    SpLoad L;
-    double pap_acc = 0.0;
+    // Two register sets alternate: at the top of tile i one holds tile i + 1
+    // (loaded during tile i - 1, its maximum published) and the other is
+    // free; the loads of tile i + 2 go into the free set first, so their
+    // latency is covered by the staging.  The memory waves issue no stores
+    // or atomics to global memory, so waiting for these loads never waits
+    // for anything else.
+    SpLoad La, Lb;
     auto publish_max = [&](float mx, int slot) {
       for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
       if (lane == 0) pmax[slot][wave - 8] = __builtin_bit_cast(uint32_t, mx);
     };
-    auto stage = [&](int i) {  // tile i from L into its buffers
+    auto load = [&](int i, SpLoad& L) {
+      sp_load(tiles[i], masked_at(i), tm, x, scale, freem, ncp0, ncp1, ndof_sc, nel0, nel1, L);
+    };
+    auto stage = [&](int i, const SpLoad& L) {  // tile i from L into its buffers
       const Tile t = tiles[i];
       uint32_t m = pmax[i & 1][0];
       m = max(m, pmax[i & 1][1]);
       m = max(m, pmax[i & 1][2]);
       m = max(m, pmax[i & 1][3]);
       const int ex = sp_exponent(m);
-      sp_stage(t, tm, ex, L, win[i & 1], xs[i % 3], ss[i & 1]);
-      if (tm == 0) sexp[i & 1] = ex;
+      sp_stage(t, tm, ex, L, hh[i & 1][0], hh[i & 1][1], hl[i & 1][0], hl[i & 1][1], xs[i % 3], ss[i & 1],
+               rowg[i % 3], ncp0, ncp1, ndof_sc);
+      if (tm == 0) sexp[i & 1] = ex + kexp[t.type];
     };
-    auto flush = [&](int i) {  // tile i's output box to y, cleared
-      const Tile t = tiles[i];
-      const bool masked = masked_at(i);
-      const int px = t.nx + 2, py = t.ny + 2, pz = t.nz + 2;
-      const int n3 = 3 * px * py * pz;
-      double* yb = ys[i & 1];
-      const float* xb = xs[i % 3];
-      for (int idx = tm; idx < n3; idx += 256) {
-        const double v = yb[idx];
-        yb[idx] = 0.0;
-        if (pap != nullptr) pap_acc += (double)xb[idx] * v;
-        const int r = sp_div(idx, px), lx = idx - r * px;
-        const int r2 = sp_div(r, py), ly = r - r2 * py;
-        const int c = sp_div(r2, pz), lz = r2 - c * pz;
-        const int g = (t.i0 + lx) + ncp0 * ((t.j0 + ly) + ncp1 * (t.k0 + lz)) + c * ndof_sc;
-        if (!masked || freem[g] != 0) unsafeAtomicAdd(&y[g], (float)v);
-      }
+    // tile i: stage i + 1 from Lh, load i + 2 into Lf
+    auto body = [&](int i, SpLoad& Lh, SpLoad& Lf) {
+      if (i + 2 < last) load(i + 2, Lf);
+      if (i + 1 < last) stage(i + 1, Lh);
+      if (i + 2 < last) publish_max(sp_absmax(tiles[i + 2], tm, Lf), (i + 2) & 1);
+      __syncthreads();
     };
     for (int idx = tm; idx < 3 * SP_MAX_PTS; idx += 256) {
       ys[0][idx] = 0.0;
       ys[1][idx] = 0.0;
     }
-    publish_max(sp_load(tiles[first], masked_at(first), tm, x, scale, freem, ncp0, ncp1, ndof_sc, nel0, nel1, L),
-                first & 1);
+    load(first, La);
+    if (first + 1 < last) load(first + 1, Lb);
+    publish_max(sp_absmax(tiles[first], tm, La), first & 1);
     __syncthreads();
-    stage(first);
-    if (first + 1 < last)
-      publish_max(sp_load(tiles[first + 1], masked_at(first + 1), tm, x, scale, freem, ncp0, ncp1, ndof_sc, nel0,
-                          nel1, L),
-                  (first + 1) & 1);
+    stage(first, La);
+    if (first + 1 < last) publish_max(sp_absmax(tiles[first + 1], tm, Lb), (first + 1) & 1);
     __syncthreads();
-    for (int ti = first; ti < last; ++ti) {
-      // flush tile ti - 1, stage ti + 1, load and reduce ti + 2
-      if (ti > first) flush(ti - 1);
-      if (ti + 1 < last) stage(ti + 1);
-      if (ti + 2 < last)
-        publish_max(sp_load(tiles[ti + 2], masked_at(ti + 2), tm, x, scale, freem, ncp0, ncp1, ndof_sc, nel0, nel1,
-                            L),
-                    (ti + 2) & 1);
-      __syncthreads();
+    for (int ti = first; ti < last; ti += 2) {
+      body(ti, Lb, La);
+      if (ti + 1 < last) body(ti + 1, La, Lb);
     }
-    flush(last - 1);
-    if (pap != nullptr) {
-      for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
-      if (lane == 0) red[wave - 8] = pap_acc;
-    }
+    __syncthreads();  // the MFMA waves' flush of the last tile
   } else {
     const int h = wave >> 2;  // M tiles 3h .. 3h + 2
     const int gs = wave & 3;  // element groups gs, gs + 4, ...
     const int col = lane & 15;
-    const int kq = lane >> 4;
+    const int kq0 = lane >> 4;
+    const int tf = threadIdx.x;  // flush thread index (0..511)
+    double pap_acc = 0.0;
+    // Flush of tile i's output box to y (masked) with global atomic adds,
+    // clearing it; the p.Ap partial is the sum over the staged points of
+    // (M x)_i times the tile's contribution to (A x)_i, which summed over
+    // tiles is x^T M K M x.
+    auto flush = [&](int i) {
+      const Tile t = tiles[i];
+      const bool masked = masked_at(i);
+      const int px = t.nx + 2;
+      const int n3 = 3 * px * (t.ny + 2) * (t.nz + 2);
+      double* yb = ys[i & 1];
+      const float* xb = xs[i % 3];
+      const int* rb = rowg[i % 3];
+      for (int base = tf; base < n3; base += 2 * 512) {
+        double v[2];
+        float xv[2];
+        int gg[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const int idx = base + 512 * k;
+          if (idx < n3) {
+            v[k] = yb[idx];
+            xv[k] = xb[idx];
+            const int r = sp_div(idx, px);
+            gg[k] = rb[r] + (idx - r * px);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const int idx = base + 512 * k;
+          if (idx < n3) {
+            yb[idx] = 0.0;
+            if (pap != nullptr) pap_acc += (double)xv[k] * v[k];
+            if (!masked || freem[gg[k]] != 0) unsafeAtomicAdd(&y[gg[k]], (float)v[k]);
+          }
+        }
+      }
+    };
     sp_h8 ahi[3][3], alo[3][3];
     sp_h4 ahi3[3], alo3[3];
     int cur_type = -1;
@@ -318,16 +415,31 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
           ahi3[mm] = __builtin_bit_cast(sp_h4, make_uint2(a0.x, a0.y));
           alo3[mm] = __builtin_bit_cast(sp_h4, make_uint2(a1.x, a1.y));
         }
+        // consumed here, so the loads are waited for on this (rare) path and
+        // the group loop carries no vector-memory waits, which would also
+        // wait for the flush's atomics
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) {
+#pragma unroll
+          for (int ks = 0; ks < 3; ++ks) asm volatile("" ::"v"(ahi[mm][ks]), "v"(alo[mm][ks]));
+          asm volatile("" ::"v"(ahi3[mm]), "v"(alo3[mm]));
+        }
         cur_type = t.type;
       }
       const int nx = t.nx, px = nx + 2, py = t.ny + 2, pz = t.nz + 2;
       const int nel = nx * t.ny * t.nz;
       const bool lines = t.lines != 0;
       const int ngroups = lines ? t.ny * t.nz : (nel + 15) >> 4;
-      const float mult = __builtin_amdgcn_ldexpf(1.0f, sexp[b] + kexp[t.type]);
-      const int wplane = nx * py * pz;  // windows per component
+      const float mult = __builtin_amdgcn_ldexpf(1.0f, sexp[b]);
+      // the row-group index, laundered so that the triplet decomposition
+      // below is recomputed per tile instead of being held (and spilled)
+      // across the tile loop
+      int kq = kq0;
+      asm volatile("" : "+v"(kq));
+      const int pxp = sp_pxp(nx);      // padded row length of the FP16 parts
+      const int wplane = pxp * py * pz;  // padded points per component
       const int pplane = px * py * pz;  // points per component
-      // window offsets of this lane's B triplets (K steps 0..2: 8 ks + 2 kq
+      // point offsets of this lane's B triplets (K steps 0..2: 8 ks + 2 kq
       // and + 1; step 3: 24 + kq, kq = 3 reads triplet 26 against zero A)
       int woff[7];
 #pragma unroll
@@ -336,12 +448,12 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
         for (int u = 0; u < 2; ++u) {
           int c, ay, az;
           sp_triplet(8 * ks + 2 * kq + u, c, ay, az);
-          woff[2 * ks + u] = c * wplane + nx * (ay + py * az);
+          woff[2 * ks + u] = c * wplane + pxp * (ay + py * az);
         }
       {
         int c, ay, az;
         sp_triplet(kq < 3 ? 24 + kq : 26, c, ay, az);
-        woff[6] = c * wplane + nx * (ay + py * az);
+        woff[6] = c * wplane + pxp * (ay + py * az);
       }
       // output offsets of this lane's rows: triplet 4 (3h + mm) + kq, and
       // for h = 0 the fourth register's triplet 24 + kq (kq < 3)
@@ -357,17 +469,21 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
         sp_triplet(24 + kq, c, ay, az);
         roff3 = c * pplane + px * (ay + py * az);
       }
-      const uint4* wb = win[b];
+      // a 4-point run starting at point p: dword p / 2 of copy (p & 1)
+      const uint32_t* whb0 = reinterpret_cast<const uint32_t*>(hh[b][0]);
+      const uint32_t* whb1 = reinterpret_cast<const uint32_t*>(hh[b][1]);
+      const uint32_t* wlb0 = reinterpret_cast<const uint32_t*>(hl[b][0]);
+      const uint32_t* wlb1 = reinterpret_cast<const uint32_t*>(hl[b][1]);
       double* yb = ys[b];
       const float* sb = ss[b];
-      for (int g = gs; g < ngroups; g += 4) {
-        int wbase, pbase, eidx;
-        bool valid;
+      const bool cmask = col <= nx + 1, emask = col < 2 && nx >= 15 + col;
+      // element of group g seen by this lane: window base, output base, scale index
+      auto group = [&](int g, int& wbase, int& pbase, int& eidx, bool& valid) {
         if (lines) {
           const int lz = sp_div(g, t.ny), ly = g - lz * t.ny;
           valid = col < nx;
           eidx = col + nx * g;
-          wbase = valid ? col + nx * (ly + py * lz) : 0;
+          wbase = valid ? col + pxp * (ly + py * lz) : 0;
           pbase = px * (ly + py * lz);
         } else {
           const int e = 16 * g + col;
@@ -380,89 +496,142 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
             ey = r - ez * t.ny;
           }
           eidx = valid ? e : 0;
-          wbase = ex + nx * (ey + py * ez);
+          wbase = ex + pxp * (ey + py * ez);
           pbase = ex + px * (ey + py * ez);
         }
+      };
+      // The triplet offsets are even (rows are padded to a multiple of 8), so
+      // a group's runs all start with the parity of its element base: the
+      // copy is picked once per group, and a run at point wbase + o is
+      // dwords (wbase >> 1) + o / 2 .. + 1 of that copy.
+      auto rd4 = [&](const uint32_t* w, int o) { return make_uint2(w[o], w[o + 1]); };
+      auto rd8 = [&](const uint32_t* w, int o1, int o2) {  // two runs as one 8-half operand
+        const uint2 a = rd4(w, o1), c = rd4(w, o2);
+        return __builtin_bit_cast(sp_h8, make_uint4(a.x, a.y, c.x, c.y));
+      };
+      auto copies = [&](int wb, const uint32_t*& wh, const uint32_t*& wl) {
+        wh = ((wb & 1) ? whb1 : whb0) + (wb >> 1);
+        wl = ((wb & 1) ? wlb1 : wlb0) + (wb >> 1);
+      };
+#pragma unroll
+      for (int i = 0; i < 7; ++i) woff[i] >>= 1;
+      // B operands in a two-deep rolling pipeline: the reads of K step
+      // ks + 1 (and of the next group's first step) are issued before the
+      // MFMAs of step ks; scheduling barriers keep the compiler from
+      // hoisting all of them (register pressure).
+      int g = gs, wbase = 0, pbase = 0, eidx = 0;
+      bool valid = false;
+      sp_h8 bh0, bl0;
+      const uint32_t *wh = whb0, *wl = wlb0;
+      if (g < ngroups) {
+        group(g, wbase, pbase, eidx, valid);
+        copies(wbase, wh, wl);
+        bh0 = rd8(wh, woff[0], woff[1]);
+        bl0 = rd8(wl, woff[0], woff[1]);
+      }
+      for (; g < ngroups; g += 4) {
         const float s = valid ? sb[eidx] * mult : 0.0f;
+        const int pb = pbase;
+        int wn = 0, pn = 0, en = 0;
+        bool vn = false;
+        const bool more = g + 4 < ngroups;
+        if (more) group(g + 4, wn, pn, en, vn);
         sp_f4 acc[3];
 #pragma unroll
         for (int mm = 0; mm < 3; ++mm) acc[mm] = sp_f4{0.0f, 0.0f, 0.0f, 0.0f};
+        sp_h8 bh1 = rd8(wh, woff[2], woff[3]);
+        sp_h8 bl1 = rd8(wl, woff[2], woff[3]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int ks = 0; ks < 3; ++ks) {
-          const uint4 w1 = wb[wbase + woff[2 * ks]];
-          const uint4 w2 = wb[wbase + woff[2 * ks + 1]];
-          const sp_h8 bh = __builtin_bit_cast(sp_h8, make_uint4(w1.x, w1.y, w2.x, w2.y));
-          const sp_h8 bl = __builtin_bit_cast(sp_h8, make_uint4(w1.z, w1.w, w2.z, w2.w));
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][0], bh0, acc[mm], 0, 0, 0);
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm)
-            acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][ks], bh, acc[mm], 0, 0, 0);
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[mm][0], bh0, acc[mm], 0, 0, 0);
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm)
-            acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][ks], bl, acc[mm], 0, 0, 0);
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][0], bl0, acc[mm], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        bh0 = rd8(wh, woff[4], woff[5]);
+        bl0 = rd8(wl, woff[4], woff[5]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm)
-            acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[mm][ks], bh, acc[mm], 0, 0, 0);
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][1], bh1, acc[mm], 0, 0, 0);
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[mm][1], bh1, acc[mm], 0, 0, 0);
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][1], bl1, acc[mm], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        const sp_h4 bh3 = __builtin_bit_cast(sp_h4, rd4(wh, woff[6]));
+        const sp_h4 bl3 = __builtin_bit_cast(sp_h4, rd4(wl, woff[6]));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][2], bh0, acc[mm], 0, 0, 0);
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[mm][2], bh0, acc[mm], 0, 0, 0);
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][2], bl0, acc[mm], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) {
+          copies(wn, wh, wl);
+          bh0 = rd8(wh, woff[0], woff[1]);
+          bl0 = rd8(wl, woff[0], woff[1]);
         }
-        {
-          const uint4 w3 = wb[wbase + woff[6]];
-          const sp_h4 bh = __builtin_bit_cast(sp_h4, make_uint2(w3.x, w3.y));
-          const sp_h4 bl = __builtin_bit_cast(sp_h4, make_uint2(w3.z, w3.w));
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi3[mm], bh, acc[mm], 0, 0, 0);
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi3[mm], bh3, acc[mm], 0, 0, 0);
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi3[mm], bl, acc[mm], 0, 0, 0);
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo3[mm], bh3, acc[mm], 0, 0, 0);
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo3[mm], bh, acc[mm], 0, 0, 0);
-        }
+        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi3[mm], bl3, acc[mm], 0, 0, 0);
         if (lines) {
-          // point x = col of the line receives v0[col] + v1[col - 1] + v2[col - 2];
-          // points 16 and 17 (lines of 15 and 16 elements) come from lane 15
+          float s1 = sp_shr<1>(s), s2 = sp_shr<2>(s), r2 = sp_ror<2>(s), q1 = sp_ror<1>(s);
+          asm("" : "+v"(s1), "+v"(s2), "+v"(r2), "+v"(q1));  // full-wave DPP, as above
+          const float r1 = col == 0 ? q1 : 0.0f;
+          double* yl = yb + pb + col;
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm) {
-            const float v0 = acc[mm][0] * s, v1 = acc[mm][1] * s, v2 = acc[mm][2] * s;
-            const float comb = v0 + sp_shr<1>(v1) + sp_shr<2>(v2);
-            const float e16 = v1 + sp_shr<1>(v2);
-            double* yl = yb + pbase + roff[mm];
-            if (col <= nx + 1) unsafeAtomicAdd(yl + col, (double)comb);
-            if (col == 15) {
-              if (nx >= 15) unsafeAtomicAdd(yl + 16, (double)e16);
-              if (nx >= 16) unsafeAtomicAdd(yl + 17, (double)v2);
-            }
-          }
-          if (h == 0) {
-            const float r0 = acc[0][3] * s, r1 = acc[1][3] * s, r2 = acc[2][3] * s;
-            const float comb = r0 + sp_shr<1>(r1) + sp_shr<2>(r2);
-            const float e16 = r1 + sp_shr<1>(r2);
-            if (kq < 3) {
-              double* yl = yb + pbase + roff3;
-              if (col <= nx + 1) unsafeAtomicAdd(yl + col, (double)comb);
-              if (col == 15) {
-                if (nx >= 15) unsafeAtomicAdd(yl + 16, (double)e16);
-                if (nx >= 16) unsafeAtomicAdd(yl + 17, (double)r2);
-              }
-            }
-          }
+          for (int mm = 0; mm < 3; ++mm)
+            sp_line_scatter(yl + roff[mm], cmask, emask, s, s1, s2, r1, r2, acc[mm][0], acc[mm][1], acc[mm][2]);
+          // the fourth registers hold ax = mm of triplet 24 + kq (h = 0,
+          // kq < 3): the same combine across the three M tiles
+          if (h == 0)
+            sp_line_scatter(yl + roff3, cmask && kq < 3, emask && kq < 3, s, s1, s2, r1, r2, acc[0][3], acc[1][3],
+                            acc[2][3]);
         } else if (valid) {
 #pragma unroll
           for (int mm = 0; mm < 3; ++mm) {
-            double* ye = yb + pbase + roff[mm];
+            double* ye = yb + pb + roff[mm];
             unsafeAtomicAdd(ye + 0, (double)(acc[mm][0] * s));
             unsafeAtomicAdd(ye + 1, (double)(acc[mm][1] * s));
             unsafeAtomicAdd(ye + 2, (double)(acc[mm][2] * s));
           }
           if (h == 0 && kq < 3) {
-            double* ye = yb + pbase + roff3;
+            double* ye = yb + pb + roff3;
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm) unsafeAtomicAdd(ye + mm, (double)(acc[mm][3] * s));
           }
         }
+        wbase = wn;
+        pbase = pn;
+        eidx = en;
+        valid = vn;
       }
+      // tile ti - 1's output box (complete since the last barrier) goes out
+      // while other waves still run their MFMAs
+      if (ti > first) flush(ti - 1);
       __syncthreads();
     }
+    flush(last - 1);
+    if (pap != nullptr) {
+      for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
+      if (lane == 0) red[wave] = pap_acc;
+    }
+    __syncthreads();
   }
   if (pap != nullptr) {
     __syncthreads();
-    if (threadIdx.x == 0) pap[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < 8; ++w) sum += red[w];
+      pap[blockIdx.x] = sum;
+    }
   }
 }
 

@@ -482,6 +482,99 @@ void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first) {
   balance_tile_list(s.tiles, G, tile_cost, first);
 }
 
+// Min-max contiguous partition: the smallest bound L such that walking the
+// tiles in order, each workgroup takes tiles while its cost stays <= L and
+// splits the tile that would cross L (along z, else y; the leading slabs
+// that fit stay, the rest carries over), needs at most G workgroups.  Found
+// by bisection on L; every workgroup but possibly the last few gets work.
+static bool minmax_walk(const std::vector<Tile>& in, int G, double L, double (*cost)(const Tile&),
+                        std::vector<Tile>* out, std::vector<int32_t>* first) {
+  if (out) {
+    out->clear();
+    first->assign(1, 0);
+  }
+  size_t i = 0;
+  Tile cur{};
+  bool pending = false;
+  int used = 0;
+  while (pending || i < in.size()) {
+    if (used == G) return false;
+    double cum = 0.0;
+    bool took = false;
+    while (pending || i < in.size()) {
+      const Tile t = pending ? cur : in[i];
+      const double c = cost(t);
+      if (cum + c <= L) {
+        if (out) out->push_back(t);
+        cum += c;
+        took = true;
+        if (pending)
+          pending = false;
+        else
+          ++i;
+        continue;
+      }
+      // split t: the largest leading part that fits
+      const bool along_z = t.nz > 1;
+      const int n = along_z ? t.nz : t.ny;
+      int kbest = 0;
+      for (int k = 1; k < n; ++k) {
+        Tile a = t;
+        if (along_z)
+          a.nz = k;
+        else
+          a.ny = k;
+        if (cum + cost(a) <= L) kbest = k;
+      }
+      if (kbest > 0) {
+        Tile a = t, r = t;
+        if (along_z) {
+          a.nz = kbest;
+          r.k0 += kbest;
+          r.nz -= kbest;
+        } else {
+          a.ny = kbest;
+          r.j0 += kbest;
+          r.ny -= kbest;
+        }
+        if (out) out->push_back(a);
+        took = true;
+        if (!pending) ++i;
+        cur = r;
+        pending = true;
+      } else if (!took) {
+        return false;  // a single slab does not fit
+      }
+      break;
+    }
+    ++used;
+    if (out) first->push_back((int32_t)out->size());
+  }
+  if (out)
+    while ((int)first->size() < G + 1) first->push_back((int32_t)out->size());
+  return true;
+}
+
+static void partition_minmax(std::vector<Tile>& tiles, int G, double (*cost)(const Tile&), std::vector<int32_t>& first) {
+  double lo = 0.0, hi = 0.0;
+  for (const Tile& t : tiles) hi += cost(t);
+  hi += 1.0;
+  for (int it = 0; it < 60 && hi - lo > 1e-3 * hi; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (minmax_walk(tiles, G, mid, cost, nullptr, nullptr))
+      hi = mid;
+    else
+      lo = mid;
+  }
+  std::vector<Tile> out;
+  minmax_walk(tiles, G, hi, cost, &out, &first);
+  tiles = std::move(out);
+}
+
+void partition_split_tiles(std::vector<Tile>& tiles, int G, std::vector<int32_t>& first) {
+  partition_minmax(tiles, G, split_tile_cost, first);
+}
+
 // Tiles of the split-FP16 apply.  Constant-type sub-boxes at least 8
 // elements long in x are cut into x-line tiles (lines of <= 16 elements
 // spread evenly) of up to 16 lines, the (ny, nz) shape with the most lines
@@ -490,15 +583,17 @@ void balance_tiles(GridSetup& s, int G, std::vector<int32_t>& first) {
 // with the most elements (<= 16 groups) and then the fewest points per
 // element winning.  Every shape keeps within the SP_* staging limits.
 static bool split_fits(int nx, int ny, int nz) {
-  const int py = ny + 2, pz = nz + 2;
-  return (nx + 2) * py * pz <= SP_MAX_PTS && 3 * nx * py * pz <= SP_MAX_WIN &&
-         ((nx + 3) / 4) * 3 * py * pz <= 256 * SP_TASK_ROUNDS && nx * ny * nz <= SP_MAX_EL;
+  const int px = nx + 2, py = ny + 2, pz = nz + 2, pxp = sp_pxp(nx);
+  return px * py * pz <= SP_MAX_PTS && 3 * py * pz <= SP_MAX_ROWS && 3 * py * pz * pxp <= SP_MAX_HPTS &&
+         (pxp / 8) * 3 * py * pz <= 256 * SP_TASK_ROUNDS && nx * ny * nz <= SP_MAX_EL;
 }
 
+// A tile costs its element groups (MFMA work) plus a fixed part for the
+// staging, flush and barrier that every tile pays whatever its size.
 double split_tile_cost(const Tile& t) {
   const int nel = t.nx * t.ny * t.nz;
   const int groups = t.lines ? t.ny * t.nz : (nel + 15) / 16;
-  return (double)groups + 0.5;
+  return (double)groups + 4.0;
 }
 
 void build_split_tiles(GridSetup& s) {

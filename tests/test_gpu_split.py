@@ -36,6 +36,8 @@ GRIDS = [
     ([[0, 1.0], [0, 0.5], [0, 2.0]], [2, 1, 5]),
     ([[0, 3.0], [0, 1.0], [0, 1.0]], [13, 7, 3]),
     ([[0, 2.0], [0, 1.0], [0, 1.0]], [35, 9, 11]),
+    ([[0, 2.0], [0, 1.0], [0, 1.0]], [18, 5, 4]),      # x-lines of 16 elements (points 16, 17 of a line)
+    ([[0, 2.0], [0, 0.5], [0, 0.5]], [17, 3, 6]),      # x-lines of 15 elements
 ]
 
 
