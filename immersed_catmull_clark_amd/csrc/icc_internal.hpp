@@ -17,9 +17,9 @@ constexpr int KS = 21;       // 4-deep K steps covering 81 trial columns (84 pad
 constexpr int TILE_MAX_EL = 128;   // elements per tile (8 waves x 16)
 constexpr int TILE_MAX_PTS = 640;  // control points staged per tile
 constexpr int APPLY_MAX_WG_PER_CU = 4;  // apply workgroups per CU, at most
-// Split-FP16 apply (k_split.hip): one workgroup of 8 MFMA + 4 memory waves
+// Split-FP16 apply (k_split.hip): one workgroup of 4 MFMA + 4 memory waves
 // per CU; tiles of <= 16 element groups (x-lines or 16-element groups).
-constexpr int SP_THREADS = 768;
+constexpr int SP_THREADS = 512;
 constexpr int SP_MAX_GROUPS = 16;   // element groups per tile
 constexpr int SP_MAX_EL = 256;      // elements per tile
 constexpr int SP_MAX_PTS = 720;     // staged control points per tile

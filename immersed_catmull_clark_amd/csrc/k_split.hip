@@ -277,7 +277,8 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
   __shared__ float ss[2][SP_MAX_EL];
   __shared__ uint32_t pmax[2][4];
   __shared__ int sexp[2];
-  __shared__ double red[8];
+  __shared__ double red[4];
+  __shared__ double sdummy[4][64];
 
   if (st != nullptr && st->done != 0) return;  // solver already finished
   int first, last;
@@ -293,8 +294,8 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
   }
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const bool memw = wave >= 8;
-  const int tm = threadIdx.x - 512;  // memory thread index (memory waves)
+  const bool memw = wave >= 4;
+  const int tm = threadIdx.x - 256;  // memory thread index (memory waves)
   const auto masked_at = [&](int i) { return tmask != nullptr && tmask[i] != 0; };
 
   // The two roles run separate loops with the same barrier sequence: two in
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
     SpLoad La, Lb;
     auto publish_max = [&](float mx, int slot) {
       for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-      if (lane == 0) pmax[slot][wave - 8] = __builtin_bit_cast(uint32_t, mx);
+      if (lane == 0) pmax[slot][wave - 4] = __builtin_bit_cast(uint32_t, mx);
     };
     auto load = [&](int i, SpLoad& L) {
       sp_load(tiles[i], masked_at(i), tm, x, scale, freem, ncp0, ncp1, ndof_sc, nel0, nel1, L);
@@ -349,17 +350,19 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
     }
     __syncthreads();  // the MFMA waves' flush of the last tile
   } else {
-    const int h = wave >> 2;  // M tiles 3h .. 3h + 2
-    const int gs = wave & 3;  // element groups gs, gs + 4, ...
+    // MFMA waves 0..3, one per SIMD: all six M tiles of element groups
+    // w, w + 4, ... of each tile.
+    const int gs = wave;
     const int col = lane & 15;
     const int kq0 = lane >> 4;
-    const int tf = threadIdx.x;  // flush thread index (0..511)
+    const int tf = threadIdx.x;  // flush thread index (0..255)
+    double* dummy = &sdummy[wave][lane];  // target of the lanes whose output point does not exist
     double pap_acc = 0.0;
-    // Flush of tile i's output box to y (masked) with global atomic adds,
-    // clearing it; the p.Ap partial is the sum over the staged points of
-    // (M x)_i times the tile's contribution to (A x)_i, which summed over
-    // tiles is x^T M K M x.
-    auto flush = [&](int i) {
+    // Part [k0, k1) of the flush of tile i's output box to y (masked) with
+    // global atomic adds, clearing it; the p.Ap partial is the sum over the
+    // staged points of (M x)_i times the tile's contribution to (A x)_i,
+    // which summed over tiles is x^T M K M x.
+    auto flush = [&](int i, int k0, int k1) {
       const Tile t = tiles[i];
       const bool masked = masked_at(i);
       const int px = t.nx + 2;
@@ -367,62 +370,52 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
       double* yb = ys[i & 1];
       const float* xb = xs[i % 3];
       const int* rb = rowg[i % 3];
-      for (int base = tf; base < n3; base += 2 * 512) {
-        double v[2];
-        float xv[2];
-        int gg[2];
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          const int idx = base + 512 * k;
-          if (idx < n3) {
-            v[k] = yb[idx];
-            xv[k] = xb[idx];
-            const int r = sp_div(idx, px);
-            gg[k] = rb[r] + (idx - r * px);
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          const int idx = base + 512 * k;
-          if (idx < n3) {
-            yb[idx] = 0.0;
-            if (pap != nullptr) pap_acc += (double)xv[k] * v[k];
-            if (!masked || freem[gg[k]] != 0) unsafeAtomicAdd(&y[gg[k]], (float)v[k]);
-          }
+      for (int k = k0; k < k1; ++k) {
+        const int idx = tf + 256 * k;
+        if (idx < n3) {
+          const double v = yb[idx];
+          const float xv = xb[idx];
+          const int r = sp_div(idx, px);
+          const int g = rb[r] + (idx - r * px);
+          yb[idx] = 0.0;
+          if (pap != nullptr) pap_acc += (double)xv * v;
+          if (!masked || freem[g] != 0) unsafeAtomicAdd(&y[g], (float)v);
         }
       }
     };
-    sp_h8 ahi[3][3], alo[3][3];
-    sp_h4 ahi3[3], alo3[3];
+    auto flush_steps = [&](int i) {
+      const Tile t = tiles[i];
+      return (3 * (t.nx + 2) * (t.ny + 2) * (t.nz + 2) + 255) >> 8;
+    };
+    sp_h8 ahi[6][3], alo[6][3];
+    sp_h4 ahi3[6], alo3[6];
     int cur_type = -1;
     __syncthreads();
     __syncthreads();
     for (int ti = first; ti < last; ++ti) {
       const int b = ti & 1;
       const Tile t = tiles[ti];
-      if (t.type != cur_type) {  // wave-uniform: (re)load this wave's A fragments
-        const uint4* af = afrag + ((size_t)(t.type * 2 + h) * 3 * 4) * 128 + lane * 2;
+      if (t.type != cur_type) {  // wave-uniform: (re)load the A fragments of all six M tiles
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) {
+        for (int mt = 0; mt < 6; ++mt) {
+          const uint4* af = afrag + ((size_t)(t.type * 2 + mt / 3) * 12 + (mt % 3) * 4) * 128 + lane * 2;
 #pragma unroll
           for (int ks = 0; ks < 3; ++ks) {
-            const uint4* p = af + (mm * 4 + ks) * 128;
-            ahi[mm][ks] = __builtin_bit_cast(sp_h8, p[0]);
-            alo[mm][ks] = __builtin_bit_cast(sp_h8, p[1]);
+            ahi[mt][ks] = __builtin_bit_cast(sp_h8, af[ks * 128]);
+            alo[mt][ks] = __builtin_bit_cast(sp_h8, af[ks * 128 + 1]);
           }
-          const uint4* p = af + (mm * 4 + 3) * 128;
-          const uint4 a0 = p[0], a1 = p[1];
-          ahi3[mm] = __builtin_bit_cast(sp_h4, make_uint2(a0.x, a0.y));
-          alo3[mm] = __builtin_bit_cast(sp_h4, make_uint2(a1.x, a1.y));
+          const uint4 a0 = af[3 * 128], a1 = af[3 * 128 + 1];
+          ahi3[mt] = __builtin_bit_cast(sp_h4, make_uint2(a0.x, a0.y));
+          alo3[mt] = __builtin_bit_cast(sp_h4, make_uint2(a1.x, a1.y));
         }
         // consumed here, so the loads are waited for on this (rare) path and
         // the group loop carries no vector-memory waits, which would also
         // wait for the flush's atomics
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) {
+        for (int mt = 0; mt < 6; ++mt) {
 #pragma unroll
-          for (int ks = 0; ks < 3; ++ks) asm volatile("" ::"v"(ahi[mm][ks]), "v"(alo[mm][ks]));
-          asm volatile("" ::"v"(ahi3[mm]), "v"(alo3[mm]));
+          for (int ks = 0; ks < 3; ++ks) asm volatile("" ::"v"(ahi[mt][ks]), "v"(alo[mt][ks]));
+          asm volatile("" ::"v"(ahi3[mt]), "v"(alo3[mt]));
         }
         cur_type = t.type;
       }
@@ -436,11 +429,12 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
       // across the tile loop
       int kq = kq0;
       asm volatile("" : "+v"(kq));
-      const int pxp = sp_pxp(nx);      // padded row length of the FP16 parts
+      const int pxp = sp_pxp(nx);        // padded row length of the FP16 parts
       const int wplane = pxp * py * pz;  // padded points per component
-      const int pplane = px * py * pz;  // points per component
-      // point offsets of this lane's B triplets (K steps 0..2: 8 ks + 2 kq
-      // and + 1; step 3: 24 + kq, kq = 3 reads triplet 26 against zero A)
+      const int pplane = px * py * pz;   // points per component
+      // point offsets (halved: dwords of a copy) of this lane's B triplets
+      // (K steps 0..2: 8 ks + 2 kq and + 1; step 3: 24 + kq, kq = 3 reads
+      // triplet 26 against zero A)
       int woff[7];
 #pragma unroll
       for (int ks = 0; ks < 3; ++ks)
@@ -448,28 +442,27 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
         for (int u = 0; u < 2; ++u) {
           int c, ay, az;
           sp_triplet(8 * ks + 2 * kq + u, c, ay, az);
-          woff[2 * ks + u] = c * wplane + pxp * (ay + py * az);
+          woff[2 * ks + u] = (c * wplane + pxp * (ay + py * az)) >> 1;
         }
       {
         int c, ay, az;
         sp_triplet(kq < 3 ? 24 + kq : 26, c, ay, az);
-        woff[6] = c * wplane + pxp * (ay + py * az);
+        woff[6] = (c * wplane + pxp * (ay + py * az)) >> 1;
       }
-      // output offsets of this lane's rows: triplet 4 (3h + mm) + kq, and
-      // for h = 0 the fourth register's triplet 24 + kq (kq < 3)
-      int roff[3], roff3 = 0;
+      // output offsets of this lane's rows: triplet 4 mt + kq of M tile mt,
+      // and the fourth registers' triplet 24 + kq (kq < 3)
+      int roff[6], roff3 = -1;
 #pragma unroll
-      for (int mm = 0; mm < 3; ++mm) {
+      for (int mt = 0; mt < 6; ++mt) {
         int c, ay, az;
-        sp_triplet(4 * (3 * h + mm) + kq, c, ay, az);
-        roff[mm] = c * pplane + px * (ay + py * az);
+        sp_triplet(4 * mt + kq, c, ay, az);
+        roff[mt] = c * pplane + px * (ay + py * az);
       }
-      if (h == 0 && kq < 3) {
+      if (kq < 3) {
         int c, ay, az;
         sp_triplet(24 + kq, c, ay, az);
         roff3 = c * pplane + px * (ay + py * az);
       }
-      // a 4-point run starting at point p: dword p / 2 of copy (p & 1)
       const uint32_t* whb0 = reinterpret_cast<const uint32_t*>(hh[b][0]);
       const uint32_t* whb1 = reinterpret_cast<const uint32_t*>(hh[b][1]);
       const uint32_t* wlb0 = reinterpret_cast<const uint32_t*>(hl[b][0]);
@@ -477,14 +470,31 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
       double* yb = ys[b];
       const float* sb = ss[b];
       const bool cmask = col <= nx + 1, emask = col < 2 && nx >= 15 + col;
-      // element of group g seen by this lane: window base, output base, scale index
-      auto group = [&](int g, int& wbase, int& pbase, int& eidx, bool& valid) {
+      // flush of tile ti - 1, spread over this wave's groups: FPG steps per
+      // group, the rest after the loop
+      const int fsteps = ti > first ? flush_steps(ti - 1) : 0;
+      constexpr int FPG = 2;
+      int fk = 0;
+      // the triplet offsets are even (rows are padded to a multiple of 8), so
+      // a group's runs all start with the parity of its element base: the
+      // row copy is picked once per group, and a run at point wbase + 2 o is
+      // dwords (wbase >> 1) + o .. + 1 of that copy
+      auto rd4 = [&](const uint32_t* w, int o) { return make_uint2(w[o], w[o + 1]); };
+      auto rd8 = [&](const uint32_t* w, int o1, int o2) {  // two runs as one 8-half operand
+        const uint2 a = rd4(w, o1), c = rd4(w, o2);
+        return __builtin_bit_cast(sp_h8, make_uint4(a.x, a.y, c.x, c.y));
+      };
+      for (int g = gs; g < ngroups; g += 4) {
+        // element of group g seen by this lane: point-row base, output base,
+        // scale index
+        int wbase, pb, eidx;
+        bool valid;
         if (lines) {
           const int lz = sp_div(g, t.ny), ly = g - lz * t.ny;
           valid = col < nx;
           eidx = col + nx * g;
           wbase = valid ? col + pxp * (ly + py * lz) : 0;
-          pbase = px * (ly + py * lz);
+          pb = px * (ly + py * lz);
         } else {
           const int e = 16 * g + col;
           valid = e < nel;
@@ -497,128 +507,127 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
           }
           eidx = valid ? e : 0;
           wbase = ex + pxp * (ey + py * ez);
-          pbase = ex + px * (ey + py * ez);
+          pb = ex + px * (ey + py * ez);
         }
-      };
-      // The triplet offsets are even (rows are padded to a multiple of 8), so
-      // a group's runs all start with the parity of its element base: the
-      // copy is picked once per group, and a run at point wbase + o is
-      // dwords (wbase >> 1) + o / 2 .. + 1 of that copy.
-      auto rd4 = [&](const uint32_t* w, int o) { return make_uint2(w[o], w[o + 1]); };
-      auto rd8 = [&](const uint32_t* w, int o1, int o2) {  // two runs as one 8-half operand
-        const uint2 a = rd4(w, o1), c = rd4(w, o2);
-        return __builtin_bit_cast(sp_h8, make_uint4(a.x, a.y, c.x, c.y));
-      };
-      auto copies = [&](int wb, const uint32_t*& wh, const uint32_t*& wl) {
-        wh = ((wb & 1) ? whb1 : whb0) + (wb >> 1);
-        wl = ((wb & 1) ? wlb1 : wlb0) + (wb >> 1);
-      };
-#pragma unroll
-      for (int i = 0; i < 7; ++i) woff[i] >>= 1;
-      // B operands in a two-deep rolling pipeline: the reads of K step
-      // ks + 1 (and of the next group's first step) are issued before the
-      // MFMAs of step ks; scheduling barriers keep the compiler from
-      // hoisting all of them (register pressure).
-      int g = gs, wbase = 0, pbase = 0, eidx = 0;
-      bool valid = false;
-      sp_h8 bh0, bl0;
-      const uint32_t *wh = whb0, *wl = wlb0;
-      if (g < ngroups) {
-        group(g, wbase, pbase, eidx, valid);
-        copies(wbase, wh, wl);
-        bh0 = rd8(wh, woff[0], woff[1]);
-        bl0 = rd8(wl, woff[0], woff[1]);
-      }
-      for (; g < ngroups; g += 4) {
         const float s = valid ? sb[eidx] * mult : 0.0f;
-        const int pb = pbase;
-        int wn = 0, pn = 0, en = 0;
-        bool vn = false;
-        const bool more = g + 4 < ngroups;
-        if (more) group(g + 4, wn, pn, en, vn);
-        sp_f4 acc[3];
+        sp_f4 acc[6];
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = sp_f4{0.0f, 0.0f, 0.0f, 0.0f};
-        sp_h8 bh1 = rd8(wh, woff[2], woff[3]);
-        sp_h8 bl1 = rd8(wl, woff[2], woff[3]);
-        __builtin_amdgcn_sched_barrier(0);
+        for (int mt = 0; mt < 6; ++mt) acc[mt] = sp_f4{0.0f, 0.0f, 0.0f, 0.0f};
+        // M tiles [m0, m0 + 3): three passes per K step, the three tiles'
+        // chains interleaved; B is streamed from LDS one K step ahead (and
+        // read again for the second half), which keeps 16 of its registers
+        // live instead of 28, and work(ks) is issued behind step ks's MFMAs
+        auto mfma_half = [&](int m0, auto&& work) {
+          int o = wbase >> 1;
+          asm volatile("" : "+v"(o));  // no reuse of the first half's B reads
+          const uint32_t* wh = ((wbase & 1) ? whb1 : whb0) + o;
+          const uint32_t* wl = ((wbase & 1) ? wlb1 : wlb0) + o;
+          sp_h8 bh[2], bl[2];
+          sp_h4 bh3, bl3;
+          bh[0] = rd8(wh, woff[0], woff[1]);
+          bl[0] = rd8(wl, woff[0], woff[1]);
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][0], bh0, acc[mm], 0, 0, 0);
+          for (int ks = 0; ks < 3; ++ks) {
+            const int c = ks & 1;
+            if (ks < 2) {
+              bh[c ^ 1] = rd8(wh, woff[2 * ks + 2], woff[2 * ks + 3]);
+              bl[c ^ 1] = rd8(wl, woff[2 * ks + 2], woff[2 * ks + 3]);
+            } else {
+              bh3 = __builtin_bit_cast(sp_h4, rd4(wh, woff[6]));
+              bl3 = __builtin_bit_cast(sp_h4, rd4(wl, woff[6]));
+            }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[mm][0], bh0, acc[mm], 0, 0, 0);
+            for (int m = m0; m < m0 + 3; ++m)
+              acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[m][ks], bh[c], acc[m], 0, 0, 0);
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][0], bl0, acc[mm], 0, 0, 0);
-        __builtin_amdgcn_sched_barrier(0);
-        bh0 = rd8(wh, woff[4], woff[5]);
-        bl0 = rd8(wl, woff[4], woff[5]);
-        __builtin_amdgcn_sched_barrier(0);
+            for (int m = m0; m < m0 + 3; ++m)
+              acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[m][ks], bh[c], acc[m], 0, 0, 0);
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][1], bh1, acc[mm], 0, 0, 0);
+            for (int m = m0; m < m0 + 3; ++m)
+              acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[m][ks], bl[c], acc[m], 0, 0, 0);
+            work(ks);
+            __builtin_amdgcn_sched_barrier(0);
+          }
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[mm][1], bh1, acc[mm], 0, 0, 0);
+          for (int m = m0; m < m0 + 3; ++m) acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi3[m], bh3, acc[m], 0, 0, 0);
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][1], bl1, acc[mm], 0, 0, 0);
-        __builtin_amdgcn_sched_barrier(0);
-        const sp_h4 bh3 = __builtin_bit_cast(sp_h4, rd4(wh, woff[6]));
-        const sp_h4 bl3 = __builtin_bit_cast(sp_h4, rd4(wl, woff[6]));
-        __builtin_amdgcn_sched_barrier(0);
+          for (int m = m0; m < m0 + 3; ++m) acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo3[m], bh3, acc[m], 0, 0, 0);
 #pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][2], bh0, acc[mm], 0, 0, 0);
-#pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[mm][2], bh0, acc[mm], 0, 0, 0);
-#pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[mm][2], bl0, acc[mm], 0, 0, 0);
-        __builtin_amdgcn_sched_barrier(0);
-        if (more) {
-          copies(wn, wh, wl);
-          bh0 = rd8(wh, woff[0], woff[1]);
-          bl0 = rd8(wl, woff[0], woff[1]);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi3[mm], bh3, acc[mm], 0, 0, 0);
-#pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(alo3[mm], bh3, acc[mm], 0, 0, 0);
-#pragma unroll
-        for (int mm = 0; mm < 3; ++mm) acc[mm] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi3[mm], bl3, acc[mm], 0, 0, 0);
+          for (int m = m0; m < m0 + 3; ++m) acc[m] = __builtin_amdgcn_mfma_f32_16x16x16f16(ahi3[m], bl3, acc[m], 0, 0, 0);
+          work(3);
+        };
+        auto none = [](int) {};
+        mfma_half(0, none);
         if (lines) {
+          // element scales shifted like the DPP-combined rows (full wave)
           float s1 = sp_shr<1>(s), s2 = sp_shr<2>(s), r2 = sp_ror<2>(s), q1 = sp_ror<1>(s);
-          asm("" : "+v"(s1), "+v"(s2), "+v"(r2), "+v"(q1));  // full-wave DPP, as above
+          asm("" : "+v"(s1), "+v"(s2), "+v"(r2), "+v"(q1));
           const float r1 = col == 0 ? q1 : 0.0f;
           double* yl = yb + pb + col;
+          // point x = col of the line receives s v0[col] + s1 v1[col - 1] +
+          // s2 v2[col - 2]: branch-free (lanes past the line add to a dummy
+          // slot), so it interleaves with the MFMAs of M tiles 3..5
+          auto comb_add = [&](int ro, bool on, float v0, float v1, float v2) {
+            float cb = v0 * s;
+            cb = fmaf(sp_shr<1>(v1), s1, cb);
+            cb = fmaf(sp_shr<2>(v2), s2, cb);
+            unsafeAtomicAdd((on && cmask) ? yl + ro : dummy, (double)cb);
+          };
+          // points 16 and 17 (lines of 15 and 16 elements) from lanes 15 and
+          // 14, rotated to lanes 0 and 1
+          auto ext_val = [&](float v1, float v2) {
+            float ex = sp_ror<2>(v2) * r2;
+            return fmaf(sp_ror<1>(v1), r1, ex);
+          };
+          // M tiles 0..2 (and their fourth registers) scattered behind the
+          // K steps of M tiles 3..5
+          mfma_half(3, [&](int ks) {
+            if (ks < 3)
+              comb_add(roff[ks], true, acc[ks][0], acc[ks][1], acc[ks][2]);
+            else
+              comb_add(roff3, kq < 3, acc[0][3], acc[1][3], acc[2][3]);
+          });
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm)
-            sp_line_scatter(yl + roff[mm], cmask, emask, s, s1, s2, r1, r2, acc[mm][0], acc[mm][1], acc[mm][2]);
-          // the fourth registers hold ax = mm of triplet 24 + kq (h = 0,
-          // kq < 3): the same combine across the three M tiles
-          if (h == 0)
-            sp_line_scatter(yl + roff3, cmask && kq < 3, emask && kq < 3, s, s1, s2, r1, r2, acc[0][3], acc[1][3],
-                            acc[2][3]);
-        } else if (valid) {
+          for (int m = 3; m < 6; ++m) comb_add(roff[m], true, acc[m][0], acc[m][1], acc[m][2]);
+          if (nx >= 15) {  // uniform
+            float e[7];
 #pragma unroll
-          for (int mm = 0; mm < 3; ++mm) {
-            double* ye = yb + pb + roff[mm];
-            unsafeAtomicAdd(ye + 0, (double)(acc[mm][0] * s));
-            unsafeAtomicAdd(ye + 1, (double)(acc[mm][1] * s));
-            unsafeAtomicAdd(ye + 2, (double)(acc[mm][2] * s));
+            for (int m = 0; m < 6; ++m) e[m] = ext_val(acc[m][1], acc[m][2]);
+            e[6] = ext_val(acc[1][3], acc[2][3]);
+            asm("" : "+v"(e[0]), "+v"(e[1]), "+v"(e[2]), "+v"(e[3]), "+v"(e[4]), "+v"(e[5]), "+v"(e[6]));
+            if (emask) {
+#pragma unroll
+              for (int m = 0; m < 6; ++m) unsafeAtomicAdd(yl + roff[m] + 16, (double)e[m]);
+              if (kq < 3) unsafeAtomicAdd(yl + roff3 + 16, (double)e[6]);
+            }
           }
-          if (h == 0 && kq < 3) {
-            double* ye = yb + pb + roff3;
+        } else {
+          mfma_half(3, none);
+          if (valid) {
 #pragma unroll
-            for (int mm = 0; mm < 3; ++mm) unsafeAtomicAdd(ye + mm, (double)(acc[mm][3] * s));
+            for (int m = 0; m < 6; ++m) {
+              double* ye = yb + pb + roff[m];
+              unsafeAtomicAdd(ye + 0, (double)(acc[m][0] * s));
+              unsafeAtomicAdd(ye + 1, (double)(acc[m][1] * s));
+              unsafeAtomicAdd(ye + 2, (double)(acc[m][2] * s));
+            }
+            if (kq < 3) {
+              double* ye = yb + pb + roff3;
+#pragma unroll
+              for (int m = 0; m < 3; ++m) unsafeAtomicAdd(ye + m, (double)(acc[m][3] * s));
+            }
           }
         }
-        wbase = wn;
-        pbase = pn;
-        eidx = en;
-        valid = vn;
+        if (fk < fsteps) {
+          flush(ti - 1, fk, min(fk + FPG, fsteps));
+          fk += FPG;
+        }
       }
-      // tile ti - 1's output box (complete since the last barrier) goes out
-      // while other waves still run their MFMAs
-      if (ti > first) flush(ti - 1);
+      if (fk < fsteps) flush(ti - 1, fk, fsteps);
       __syncthreads();
     }
-    flush(last - 1);
+    flush(last - 1, 0, flush_steps(last - 1));
     if (pap != nullptr) {
       for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
       if (lane == 0) red[wave] = pap_acc;
@@ -627,11 +636,7 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
   }
   if (pap != nullptr) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double sum = 0.0;
-      for (int w = 0; w < 8; ++w) sum += red[w];
-      pap[blockIdx.x] = sum;
-    }
+    if (threadIdx.x == 0) pap[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
   }
 }
 
