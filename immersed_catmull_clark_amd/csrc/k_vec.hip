@@ -130,6 +130,42 @@ __device__ __forceinline__ double sum_parts_all(const double* __restrict__ parts
   return s;  // valid on every thread
 }
 
+// 16-byte runs of V = 16 / sizeof(T) consecutive entries: the update kernels
+// move whole runs (one dwordx4 per array), the last run of a vector entry
+// by entry.  Each thread takes RU runs a grid stride apart per step, all
+// their loads issued before any arithmetic.
+constexpr int RU = 2;
+template <typename T>
+struct Run {
+  T v[16 / sizeof(T)];
+};
+template <typename T>
+__device__ __forceinline__ Run<T> ld_run(const T* __restrict__ a, int64_t e, int64_t n) {
+  constexpr int V = 16 / (int)sizeof(T);
+  Run<T> q;
+  if (e + V <= n) {
+    const uint4 w = *reinterpret_cast<const uint4*>(a + e);
+    __builtin_memcpy(&q, &w, 16);
+  } else {
+#pragma unroll
+    for (int j = 0; j < V; ++j) q.v[j] = e + j < n ? a[e + j] : T(0);
+  }
+  return q;
+}
+template <typename T>
+__device__ __forceinline__ void st_run(T* __restrict__ a, int64_t e, int64_t n, const Run<T>& q) {
+  constexpr int V = 16 / (int)sizeof(T);
+  if (e + V <= n) {
+    uint4 w;
+    __builtin_memcpy(&w, &q, 16);
+    *reinterpret_cast<uint4*>(a + e) = w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < V; ++j)
+      if (e + j < n) a[e + j] = q.v[j];
+  }
+}
+
 // alpha = rz / p.Ap (p.Ap summed from the apply's partials), x += alpha p,
 // r -= alpha Ap, per-block partials of r.r and r.D^-1 r.  Only block 0 writes
 // the state (it_cur, breakdown / limit flags).
@@ -139,9 +175,30 @@ __global__ __launch_bounds__(VB) void pcg_update1(int64_t n, PcgState* st, const
                                                   T* __restrict__ x, T* __restrict__ r, const T* __restrict__ dinv,
                                                   double* __restrict__ parts) {
   __shared__ double sh[16];
-  if (st->done) return;
+  constexpr int V = 16 / (int)sizeof(T);
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  Run<T> pv[RU], av[RU], xv[RU], rv[RU], dv[RU];
+  auto load_step = [&](int64_t v) {
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int64_t e = (v + u * gs) * V;
+      if (e < n) {
+        pv[u] = ld_run(p, e, n);
+        av[u] = ld_run(ap, e, n);
+        xv[u] = ld_run(x, e, n);
+        rv[u] = ld_run(r, e, n);
+        dv[u] = ld_run(dinv, e, n);
+      }
+    }
+  };
+  const int32_t done = st->done;
   const int it = st->iters;
-  if (it >= st->maxit) {  // iteration limit
+  const int maxit = st->maxit;
+  int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // the first step's loads overlap the state checks and the p.Ap sum
+  load_step(v);
+  if (done) return;
+  if (it >= maxit) {  // iteration limit
     if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
     return;
   }
@@ -157,13 +214,24 @@ __global__ __launch_bounds__(VB) void pcg_update1(int64_t n, PcgState* st, const
   }
   const T alpha = (T)(st->rz[it & 1] / pap);
   double rr = 0.0, rz = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    T xi = x[i] + alpha * p[i];
-    T ri = r[i] - alpha * ap[i];
-    x[i] = xi;
-    r[i] = ri;
-    rr += (double)ri * (double)ri;
-    rz += (double)ri * (double)(dinv[i] * ri);
+  for (; v * V < n; v += RU * gs) {
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int64_t e = (v + u * gs) * V;
+      if (e < n) {
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          const T ri = rv[u].v[j] - alpha * av[u].v[j];
+          xv[u].v[j] = xv[u].v[j] + alpha * pv[u].v[j];
+          rv[u].v[j] = ri;
+          rr += (double)ri * (double)ri;
+          rz += (double)ri * (double)(dv[u].v[j] * ri);
+        }
+        st_run(x, e, n, xv[u]);
+        st_run(r, e, n, rv[u]);
+      }
+    }
+    if ((v + RU * gs) * V < n) load_step(v + RU * gs);
   }
   double s1 = block_sum(rr, sh);
   double s2 = block_sum(rz, sh);
@@ -185,8 +253,26 @@ __global__ __launch_bounds__(VB) void pcg_update2(int64_t n, PcgState* st, const
                                                   T* __restrict__ p, const T* __restrict__ r,
                                                   const T* __restrict__ dinv, T* __restrict__ ap) {
   __shared__ double sh[16];
-  if (st->done) return;
+  constexpr int V = 16 / (int)sizeof(T);
+  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  Run<T> pv[RU], rv[RU], dv[RU];
+  auto load_step = [&](int64_t v) {
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int64_t e = (v + u * gs) * V;
+      if (e < n) {
+        pv[u] = ld_run(p, e, n);
+        rv[u] = ld_run(r, e, n);
+        dv[u] = ld_run(dinv, e, n);
+      }
+    }
+  };
+  const int32_t done = st->done;
   const int it = st->it_cur;
+  int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  // the first step's loads overlap the state check and the sums
+  load_step(v);
+  if (done) return;
   const double rr = sum_parts_all(parts, gridDim.x, sh);
   const double rz = sum_parts_all(parts + gridDim.x, gridDim.x, sh);
   const double res = sqrt(rr) / st->norm_b;
@@ -200,9 +286,21 @@ __global__ __launch_bounds__(VB) void pcg_update2(int64_t n, PcgState* st, const
   }
   if (conv) return;
   const T beta = (T)(rz / st->rz[it & 1]);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    p[i] = dinv[i] * r[i] + beta * p[i];
-    ap[i] = T(0);
+  Run<T> zero;
+#pragma unroll
+  for (int j = 0; j < V; ++j) zero.v[j] = T(0);
+  for (; v * V < n; v += RU * gs) {
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int64_t e = (v + u * gs) * V;
+      if (e < n) {
+#pragma unroll
+        for (int j = 0; j < V; ++j) pv[u].v[j] = dv[u].v[j] * rv[u].v[j] + beta * pv[u].v[j];
+        st_run(p, e, n, pv[u]);
+        st_run(ap, e, n, zero);
+      }
+    }
+    if ((v + RU * gs) * V < n) load_step(v + RU * gs);
   }
 }
 
