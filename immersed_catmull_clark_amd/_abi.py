@@ -46,6 +46,7 @@ SIGNATURES = [
     ("icc_diag_dev", cint, [vp, vp]),
     ("icc_pcg", cint, [vp, vp, vp, dbl, i64, vp, vp, vp]),
     ("icc_pcg_dev", cint, [vp, vp, vp, dbl, i64, vp, vp, vp]),
+    ("icc_pcg_dev_noalloc", cint, [vp, vp, vp, dbl, i64, vp, vp, vp]),
     ("icc_solve_ir_dev", cint, [vp, cint, vp, dbl, dbl, i64, i64, vp, vp]),
     ("icc_solve_ir", cint, [vp, cint, vp, dbl, dbl, i64, i64, vp, vp]),
     ("icc_solve_mixed_dev", cint, [vp, vp, dbl, dbl, i64, vp, vp]),

@@ -33,6 +33,7 @@ SOURCES = [
     ("k_topopt.hip", ["-ffp-contract=off"]),
     ("k_wq.hip", []),
     ("k_split.hip", []),
+    ("k_noalloc.hip", []),
     ("setup.cpp", None),
     ("wq.cpp", None),
     ("api.cpp", None),

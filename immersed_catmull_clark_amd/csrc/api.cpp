@@ -98,6 +98,9 @@ ICC_API int icc_ctx_create(int device, icc_ctx** out) {
     set_error(std::string("icc_ctx_create: built for gfx950, device is ") + prop.gcnArchName);
     return ICC_ENOTSUP;
   }
+  // the no-allocation PCG's fixed workspace on this device (its failure is
+  // reported by that entry point, not here)
+  (void)noalloc_init(device);
   auto* c = new icc_ctx();
   c->device = device;
   c->num_cu = prop.multiProcessorCount;
@@ -1262,6 +1265,64 @@ ICC_API int icc_pcg_dev(icc_op* op, const double* b, const double* diag, double 
   }
   set_error("icc_pcg: iteration limit reached");
   return ICC_ENOCONV;  // status 3
+}
+
+// icc_pcg_dev without allocation: the vectors, partials and state live in
+// the library's fixed per-device workspace (k_noalloc.hip), the iterations
+// are launched directly (no graph: instantiating one may allocate) and the
+// state is polled every PCG_BATCH iterations into a host variable.  Calls
+// that overlap on one GPU share that workspace and race.
+ICC_API int icc_pcg_dev_noalloc(icc_op* op, const double* b, const double* diag, double tol, int64_t maxit,
+                                double* x, int64_t* iters, double* res) {
+  if (!op || !b || !diag || !x || !iters || !res) return ICC_EINVAL;
+  const int64_t n = op->g->s.ndof;
+  if (n > NA_MAX_NDOF) {
+    set_error("icc_pcg_dev_noalloc: " + std::to_string(n) + " unknowns exceed the fixed workspace's " +
+              std::to_string(NA_MAX_NDOF));
+    return ICC_EINVAL;
+  }
+  if (npap_of(op) > NA_PAP) {
+    set_error("icc_pcg_dev_noalloc: the operator's p.Ap partials exceed the fixed workspace");
+    return ICC_ENOTSUP;
+  }
+  double* base = noalloc_scratch(op->g->ctx->device);
+  if (base == nullptr) {
+    set_error("icc_pcg_dev_noalloc: the device workspace's address could not be looked up");
+    return ICC_EHIP;
+  }
+  hipStream_t st = op->g->ctx->stream;
+  Work w;  // views into the workspace; nothing here owns or frees memory
+  w.n = n;
+  w.st = reinterpret_cast<PcgState*>(base);
+  w.parts = base + NA_STATE;
+  w.pap = w.parts + 2 * VEC_MAX_BLOCKS;
+  w.r = w.pap + NA_PAP;
+  w.p = w.r + NA_MAX_NDOF;
+  w.ap = w.p + NA_MAX_NDOF;
+  w.dinv = w.ap + NA_MAX_NDOF;
+  // the state keeps the previous call's contents: clear it (a fill kernel)
+  ICC_HIP(launch_fill<double>(base, 0.0, NA_STATE, st));
+  ICC_HIP(launch_inv_diag<double>(diag, w.dinv, n, 0.0, 0, st));
+  ICC_TRY(set_limits(op, &w, tol, maxit));
+  ICC_HIP((launch_pcg_init<double, double>(n, b, 1.0, w.dinv, x, w.r, w.p, w.ap, w.st, w.parts, st)));
+  PcgState hs;
+  for (;;) {
+    if (maxit > 0)
+      for (int i = 0; i < PCG_BATCH; ++i) ICC_HIP(enqueue_iteration<double>(op, &w, x, w.r, w.p, w.ap, w.dinv, st));
+    ICC_HIP(hipMemcpyAsync(&hs, w.st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+    ICC_HIP(hipStreamSynchronize(st));
+    if (maxit <= 0) hs.done = 3;
+    if (hs.done) break;
+  }
+  *iters = hs.iters;
+  *res = hs.res;
+  if (hs.done == 1) return ICC_OK;
+  if (hs.done == 2) {
+    set_error("icc_pcg_dev_noalloc: breakdown (p.Ap <= 0)");
+    return ICC_EBREAKDOWN;
+  }
+  set_error("icc_pcg_dev_noalloc: iteration limit reached");
+  return ICC_ENOCONV;
 }
 
 ICC_API int icc_pcg(icc_op* op, const double* b, const double* diag, double tol, int64_t maxit, double* x,

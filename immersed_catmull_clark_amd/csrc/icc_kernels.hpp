@@ -89,6 +89,19 @@ hipError_t launch_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
 hipError_t launch_wq_diag(const icc_grid* g, const double* tab, const double* x, double penal, double emin,
                           double* d, hipStream_t st);
 
+// k_noalloc.hip: fixed workspace of icc_pcg_dev_noalloc, in doubles:
+// [PcgState (NA_STATE) | 2 * VEC_MAX_BLOCKS vector partials | NA_PAP p.Ap
+// partials | r | p | Ap | D^-1 (NA_MAX_NDOF each)], every part 256-byte aligned
+constexpr int64_t NA_MAX_NDOF = 1 << 20;
+constexpr int64_t NA_STATE = 32;
+constexpr int64_t NA_PAP = 4096;
+constexpr int64_t NA_SCRATCH_DOUBLES = NA_STATE + 2 * VEC_MAX_BLOCKS + NA_PAP + 4 * NA_MAX_NDOF;
+static_assert(sizeof(PcgState) <= NA_STATE * 8 && (2 * VEC_MAX_BLOCKS) % 32 == 0 && NA_PAP % 32 == 0, "workspace alignment");
+// looks up this device's workspace (device current; once per device)
+hipError_t noalloc_init(int device);
+// the address found by noalloc_init, or null before it ran or if it failed
+double* noalloc_scratch(int device);
+
 // k_faces.hip: matrix-free face stabilisation, at most FACE_MAX_BLOCKS p.Ap partials
 constexpr int FACE_MAX_BLOCKS = 512;
 constexpr int FACE_LEGACY = 1, FACE_GHOST = 2;

@@ -163,6 +163,17 @@ int icc_pcg(icc_op* op, const double* b, const double* diag, double tol, int64_t
 int icc_pcg_dev(icc_op* op, const double* b_dev, const double* diag_dev, double tol, int64_t maxit, double* x_dev,
                 int64_t* iters, double* res);
 
+/* icc_pcg_dev without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (serving loops, harnesses that
+ * forbid hipMalloc).  The solver's vectors, partials and state live in a
+ * fixed per-GPU workspace of the library (about 34 MB, held from the
+ * library's load until the process ends) sized for at most 1,048,576
+ * unknowns; larger operators return ICC_EINVAL.  Same arguments, results and
+ * status codes as icc_pcg_dev.  Every call on one GPU shares that workspace:
+ * calls that overlap on one GPU (two host threads, two contexts) race. */
+int icc_pcg_dev_noalloc(icc_op* op, const double* b_dev, const double* diag_dev, double tol, int64_t maxit,
+                        double* x_dev, int64_t* iters, double* res);
+
 /* Mixed-precision iterative refinement: FP64 outer residual r = f - A u with
  * compensated dot products, FP64 update u += d, inner Jacobi PCG in `inner`
  * precision (ICC_F64, ICC_F32, or ICC_BF16: FP32 vectors with the BF16
