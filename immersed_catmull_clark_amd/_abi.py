@@ -121,6 +121,7 @@ SIGNATURES = [
     ("icc_immersed_mesh_solve", cint, [vp, vp, vp]),
     ("icc_wq_create", cint, [vp, vpp]),
     ("icc_wq_create_layout", cint, [vp, cint, vpp]),
+    ("icc_debug_quantize_kmat", dbl, [vp, cint, vp]),
     ("icc_debug_wq_rule", i64, [i64, cint, cint, i64, vp]),
     ("icc_wq_destroy", cint, [vp]),
     ("icc_wq_form", cint, [vp, cint, vp, dbl, dbl]),

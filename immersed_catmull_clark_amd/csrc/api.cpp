@@ -290,6 +290,7 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
   std::vector<double> kd((size_t)g->ntypes * NSH);
   for (int t = 0; t < g->ntypes; ++t) {
     const double* K = s.kmat[t].data();
+    const double* Kq = s.kmat_q32[t].data();  // FP32: zero block row sums, exact in FP32
     for (int mt = 0; mt < MT; ++mt)
       for (int ks = 0; ks < KS; ++ks)
         for (int l = 0; l < 64; ++l) {
@@ -297,7 +298,7 @@ ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bou
           const int row64 = perm_row(mt, i % 4, i / 4), row32 = perm_row(mt, i / 4, i % 4);
           size_t idx = (((size_t)t * MT + mt) * KS + ks) * 64 + l;
           f64[idx] = (row64 >= 0 && colk < NSH) ? K[row64 * NSH + colk] : 0.0;
-          f32[idx] = (float)((row32 >= 0 && colk < NSH) ? K[row32 * NSH + colk] : 0.0);
+          f32[idx] = (float)((row32 >= 0 && colk < NSH) ? Kq[row32 * NSH + colk] : 0.0);
         }
     for (int mt = 0; mt < MT; ++mt)
       for (int ks = 0; ks < 3; ++ks)

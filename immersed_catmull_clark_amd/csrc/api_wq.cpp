@@ -136,6 +136,11 @@ ICC_API int icc_wq_create_layout(icc_grid* g, int layout, icc_wq** out) {
 
 ICC_API int icc_wq_create(icc_grid* g, icc_wq** out) { return icc_wq_create_layout(g, ICC_WQ_INTERIOR, out); }
 
+ICC_API double icc_debug_quantize_kmat(const double* k, int bits, double* out) {
+  if (!k || !out || bits < 2 || bits > 50) return -1.0;
+  return icc::quantize_kmat(k, bits, out);
+}
+
 ICC_API int64_t icc_debug_wq_rule(int64_t nel, int layout, int which, int64_t i, double* out) {
   if (nel < 1 || (layout != ICC_WQ_INTERIOR && layout != ICC_WQ_CALABRO) || which < 0 || which > 4)
     return -(int64_t)ICC_EINVAL;

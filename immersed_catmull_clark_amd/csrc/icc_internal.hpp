@@ -88,12 +88,17 @@ struct GridSetup {
   // element types (first-appearance order) as (t0, t1, t2)
   std::vector<std::array<int32_t, 3>> type_list;
   std::vector<int32_t> type_of;  // index t0 + T0*(t1 + T1*t2) -> element type
-  std::vector<std::vector<double>> kmat;  // [ntypes][81*81]
+  std::vector<std::vector<double>> kmat;    // [ntypes][81*81]
+  std::vector<std::vector<double>> kmat_q32;  // kmat on 24-bit row grids with exact zero block row sums (FP32 apply)
   std::vector<Tile> tiles;
   std::vector<Tile> stiles;  // split-FP16 apply tiles
 };
 
 int build_setup(const int64_t nel[3], const double bounds[6], double young, double poisson, GridSetup& s);
+// K (81x81) with each row rounded onto its grid 2^(e_i - bits), |K_i.| <
+// 2^e_i, and every 27x27 component block's row sums exactly zero; returns the
+// largest entry deviation in its row's quanta.
+double quantize_kmat(const double* K, int bits, double* out);
 // (Re)builds s.tiles with tiling variant v (0 .. n_tile_variants() - 1; the
 // shape order is v % 6, and v < 6 asks for the balanced partition).
 void build_tiles(GridSetup& s, int variant);

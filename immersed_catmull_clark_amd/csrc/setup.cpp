@@ -268,10 +268,63 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
           }
     s.kmat.push_back(std::move(K));
   }
+  s.kmat_q32.assign(s.kmat.size(), std::vector<double>(NSH * NSH));
+  for (size_t t = 0; t < s.kmat.size(); ++t) quantize_kmat(s.kmat[t].data(), 24, s.kmat_q32[t].data());
 
   build_tiles(s, 0);
   build_split_tiles(s);
   return ICC_OK;
+}
+
+// ---- element matrix of the FP32 contraction -----------------------------------
+//
+// Rounding K_t entry by entry to FP32 breaks K_t t_c = 0 for the rigid
+// translations t_c (every row of every 27x27 component block sums to zero) by
+// the same residual in every element of the type.  That error slows the FP32
+// Jacobi-PCG recurrence of the mixed solver: at 96x48x48 with the cantilever
+// traction, 1.58x the FP64 iteration count with plain FP32 rounding against
+// 1.38x with the block sums kept (1.15x with the FP64 matrix, the floor the
+// FP32 vectors set).  quantize_kmat rounds each row i onto its own
+// fixed-point grid q_i = 2^(e_i - bits), max_j |K_ij| < 2^e_i (bits = 24: every
+// entry exact in FP32), then moves each block's rounding residual onto the
+// row's entry at its own control point (column (c, i mod 27)), which leaves
+// every block row sum exactly zero.  The result is symmetric only up to a few
+// quanta.  (Symmetric zero-sum roundings on one grid per type, and the same
+// on 22-bit grids for the split-FP16 pair, were measured slower: the first
+// loses precision on the small entries, the second is limited by the split x.)
+double quantize_kmat(const double* K, int bits, double* out) {
+  double dev = 0.0;
+  const double lim = std::ldexp(1.0, bits);
+  for (int i = 0; i < NSH; ++i) {
+    const double* row = K + (size_t)i * NSH;
+    double* o = out + (size_t)i * NSH;
+    double mx = 0.0;
+    for (int j = 0; j < NSH; ++j) mx = std::max(mx, std::fabs(row[j]));
+    if (mx == 0.0) {
+      std::fill(o, o + NSH, 0.0);
+      continue;
+    }
+    int e = 0;
+    std::frexp(mx, &e);
+    const double q = std::ldexp(1.0, e - bits);  // |row| / q < 2^bits
+    double X[NSH];
+    for (int j = 0; j < NSH; ++j) X[j] = std::nearbyint(row[j] / q);
+    for (int c = 0; c < 3; ++c) {
+      double sum = 0.0;
+      for (int b = 0; b < NSC; ++b) sum += X[c * NSC + b];
+      int j = c * NSC + i % NSC;
+      if (std::fabs(X[j] - sum) >= lim) {  // no headroom there: the block's smallest entry takes it
+        for (int b = 0; b < NSC; ++b)
+          if (std::fabs(X[c * NSC + b]) < std::fabs(X[j])) j = c * NSC + b;
+      }
+      X[j] -= sum;
+    }
+    for (int j = 0; j < NSH; ++j) {
+      dev = std::max(dev, std::fabs(X[j] - row[j] / q));
+      o[j] = X[j] * q;
+    }
+  }
+  return dev;
 }
 
 // Tiles of constant-type sub-boxes.  Sub-boxes at least 8 elements long in x

