@@ -12,5 +12,8 @@ from .batch import Batch  # noqa: F401
 from .imm_topopt import (ImmersedOptions, ImmersedProblem, ImmersedSolution, ImmersedTopOpt,  # noqa: F401
                          ImmersedTopOptConfig, ImmersedTopOptHistory, Stabilization, TriangleMesh3D,
                          build_immersed_problem, element_weights, padded_bounds, solve_immersed_problem)
+from .surface import (Dirichlet, NeumannPressure, NeumannTraction, RobinFoundation, SurfaceTerms,  # noqa: F401
+                      dirichlet_penalty_fn, eval_vector_at, scalar_basis_at, surface_boundary_terms,
+                      surface_quadrature)
 
 __all__ = ["Context", "Grid", "Operator", "Filter", "classify_cylinders", "ICC_F64", "ICC_F32", "ICC_BF16", "ICC_F16X", "IccError"]

@@ -119,6 +119,18 @@ SIGNATURES = [
     ("icc_immersed_mesh_set_solver", cint, [vp, dbl, i64, cint]),
     ("icc_immersed_mesh_apply", cint, [vp, vp, vp]),
     ("icc_immersed_mesh_solve", cint, [vp, vp, vp]),
+    ("icc_immersed_mesh_set_force", cint, [vp, vp]),
+    ("icc_immersed_mesh_grid", cint, [vp, vpp]),
+    ("icc_immersed_mesh_add_surface", cint, [vp, vp]),
+    ("icc_op_add_triplets", cint, [vp, i64, vp, vp, vp]),
+    ("icc_surface_quadrature", cint, [vp, i64, vp, i64, vp, i64, dbl, i64, vp, vp]),
+    ("icc_scalar_basis_at", cint, [vp, vp, vp, vp]),
+    ("icc_surface_create", cint, [vp, vpp]),
+    ("icc_surface_destroy", cint, [vp]),
+    ("icc_surface_add", cint, [vp, cint, vp, vp, i64, vp, i64, vp, i64, dbl]),
+    ("icc_surface_add_dirichlet_fn", cint, [vp, dbl, vp, vp, vp, i64, vp, i64, vp, i64, dbl]),
+    ("icc_surface_sizes", cint, [vp, vp]),
+    ("icc_surface_terms", cint, [vp, vp, vp, vp, vp]),
     ("icc_wq_create", cint, [vp, vpp]),
     ("icc_wq_create_layout", cint, [vp, cint, vpp]),
     ("icc_debug_quantize_kmat", dbl, [vp, cint, vp]),

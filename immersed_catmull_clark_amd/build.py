@@ -34,6 +34,7 @@ SOURCES = [
     ("k_wq.hip", []),
     ("k_split.hip", []),
     ("k_noalloc.hip", []),
+    ("k_surface.hip", ["-ffp-contract=off"]),
     ("setup.cpp", None),
     ("wq.cpp", None),
     ("api.cpp", None),
@@ -41,6 +42,7 @@ SOURCES = [
     ("api_imm.cpp", None),
     ("api_imm_topopt.cpp", None),
     ("api_wq.cpp", None),
+    ("api_surface.cpp", None),
 ]
 
 

@@ -666,6 +666,9 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
   op->d_s_data32 = nullptr;
   op->d_s_diag = nullptr;
   op->s_nnz = 0;
+  op->h_s_indptr.clear();
+  op->h_s_indices.clear();
+  op->h_s_data.clear();
   if (nrows == 0 || !indptr) return ICC_OK;
   if (nrows != n) {
     set_error("icc_op_set_sparse: nrows must equal ndof");
@@ -699,6 +702,9 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
   ICC_HIP(hipMemcpyAsync(op->d_s_diag, diag.data(), n * 8, hipMemcpyHostToDevice, st));
   ICC_HIP(hipStreamSynchronize(st));
   op->s_nnz = nnz;
+  op->h_s_indptr.assign(indptr, indptr + n + 1);
+  op->h_s_indices.assign(indices, indices + nnz);
+  op->h_s_data.assign(data, data + nnz);
   return ICC_OK;
 }
 

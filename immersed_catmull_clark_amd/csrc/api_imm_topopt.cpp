@@ -675,3 +675,46 @@ ICC_API int icc_immersed_mesh_solve(icc_immersed_mesh* h, double* u, double* sta
   if (stats) std::memcpy(stats, st8, sizeof(st8));
   return rc;
 }
+
+ICC_API int icc_immersed_mesh_grid(icc_immersed_mesh* h, icc_grid** out) {
+  if (!h || !out) return ICC_EINVAL;
+  *out = h->g;
+  return ICC_OK;
+}
+
+// ImmersedProblem::add_surface_terms (rust/src/immersed.rs:541-548): the
+// surface triplets join the operator's sparse term, the load joins the force
+// (and the right-hand side on the free DOFs).
+ICC_API int icc_immersed_mesh_add_surface(icc_immersed_mesh* h, icc_surface* s) {
+  if (!h || !s) return ICC_EINVAL;
+  if (surface_grid(s) != h->g) {
+    set_error("icc_immersed_mesh_add_surface: the surface terms belong to another grid");
+    return ICC_EINVAL;
+  }
+  int64_t sz[3];
+  TRY(icc_surface_sizes(s, sz));
+  const int64_t n = h->ndof;
+  std::vector<int64_t> r(sz[0]), c(sz[0]);
+  std::vector<double> v(sz[0]), f(n);
+  TRY(icc_surface_terms(s, r.data(), c.data(), v.data(), f.data()));
+  TRY(icc_op_add_triplets(h->op, sz[0], r.data(), c.data(), v.data()));
+  std::vector<double> b(n);
+  for (int64_t i = 0; i < n; ++i) {
+    h->force[i] += f[i];
+    b[i] = h->fr[i] ? h->force[i] : 0.0;
+  }
+  ICC_HIP(hipMemcpyAsync(h->d_f, b.data(), n * 8, hipMemcpyHostToDevice, h->ctx->stream));
+  ICC_HIP(hipStreamSynchronize(h->ctx->stream));
+  return ICC_OK;
+}
+
+ICC_API int icc_immersed_mesh_set_force(icc_immersed_mesh* h, const double* force) {
+  if (!h || !force) return ICC_EINVAL;
+  const int64_t n = h->ndof;
+  h->force.assign(force, force + n);
+  std::vector<double> b(n);
+  for (int64_t i = 0; i < n; ++i) b[i] = h->fr[i] ? h->force[i] : 0.0;
+  ICC_HIP(hipMemcpyAsync(h->d_f, b.data(), n * 8, hipMemcpyHostToDevice, h->ctx->stream));
+  ICC_HIP(hipStreamSynchronize(h->ctx->stream));
+  return ICC_OK;
+}

@@ -67,6 +67,8 @@ ICC_HD inline int perm_row(int w, int kq, int r) {
 
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+// the grid an icc_surface belongs to (api_surface.cpp)
+const icc_grid* surface_grid(const icc_surface* s);
 
 #define ICC_HIP(call)                                 \
   do {                                                \
@@ -200,6 +202,8 @@ struct icc_op {
   double* d_s_data = nullptr;
   float* d_s_data32 = nullptr;
   double* d_s_diag = nullptr;
+  std::vector<int64_t> h_s_indptr, h_s_indices;  // host copy of the add-on (plus_triplets)
+  std::vector<double> h_s_data;
   // optional matrix-free cut-cell face stabilisation (k_faces.hip)
   int f_kind = 0;               // FACE_LEGACY or FACE_GHOST
   int64_t f_n = 0;              // faces

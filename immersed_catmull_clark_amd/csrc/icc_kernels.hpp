@@ -52,6 +52,16 @@ hipError_t launch_diag_f32(const icc_grid* g, const float* scale, float* d, hipS
 hipError_t launch_layout_probe(double* out, hipStream_t st);
 hipError_t launch_layout_probe_bf16(double* out, hipStream_t st);
 
+// k_surface.hip: basis at surface quadrature points (key = first scalar
+// index, 27 values per point) and the per-element 81 x 81 surface blocks and
+// 81 loads of runs of points with one key
+hipError_t launch_surface_basis(int64_t npts, const double* px, const double* const knots[3], const int nk[3],
+                                const double lo[3], const double len[3], int64_t ncp0, int64_t ncp1, int64_t* key,
+                                double* nval, hipStream_t st);
+hipError_t launch_surface_blocks(int64_t nseg, const int64_t* seg, const int64_t* perm, const double* nval,
+                                 const double* pw, const double* ks, const int8_t* fmode, const double* fs,
+                                 const double* fv, double* blocks, double* fe, hipStream_t st);
+
 // k_vec.hip
 int vec_blocks(int64_t n);
 template <typename T>

@@ -155,6 +155,29 @@ class ImmersedProblem:
         check(lib().icc_immersed_mesh_problem(h, _p(self.status), _p(self.weights), _p(self.force), _p(fr)))
         self.free = fr.astype(bool)
 
+    def grid_handle(self):
+        g = C.c_void_p()
+        check(lib().icc_immersed_mesh_grid(self.h, C.byref(g)))
+        return g
+
+    @property
+    def h_min(self):
+        """Smallest background element size (SpaceBox::element_size)."""
+        gb = self.grid_bounds
+        return min((gb[d, 1] - gb[d, 0]) / self.opts.grid_res[d] for d in range(3))
+
+    def set_force(self, force):
+        """Replaces the problem's force (e.g. zeroed before surface loads)."""
+        f = np.ascontiguousarray(force, dtype=np.float64).reshape(self.ndof)
+        check(lib().icc_immersed_mesh_set_force(self.h, _p(f)))
+        self.force = f.copy()
+
+    def add_surface_terms(self, st):
+        """ImmersedProblem::add_surface_terms (immersed.rs:541-548)."""
+        check(lib().icc_immersed_mesh_add_surface(self.h, st.h))
+        fr = np.zeros(self.ndof, dtype=np.uint8)
+        check(lib().icc_immersed_mesh_problem(self.h, None, None, _p(self.force), _p(fr)))
+
     def apply(self, x):
         """M (K + S) M x."""
         y = np.zeros(self.ndof)

@@ -118,6 +118,9 @@ int icc_op_set_free_mask(icc_op* op, const uint8_t* free_mask);
 /* adds a sparse term S (CSR, ndof x ndof, FP64) to the operator: stabilisation
  * or penalty surface terms (topopt_immersed.rs:174-183, immersed.rs:542-547) */
 int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, const int64_t* indices, const double* data);
+/* S = S + T for triplets T (rows, cols, vals; duplicates summed): replaces
+ * CsrMatrix::plus_triplets (rust/src/sparse.rs:98-113) on the sparse term */
+int icc_op_add_triplets(icc_op* op, int64_t n, const int64_t* rows, const int64_t* cols, const double* vals);
 /* Adds the legacy cut-cell stabilisation of legacy_stabilization
  * (rust/src/immersed.rs:168-222) as a matrix-free face term: for each face
  * between two active cells (status INSIDE 1 or CUT -1) with at least one CUT,
@@ -433,6 +436,56 @@ int icc_immersed_mesh_apply(icc_immersed_mesh* h, const double* x, double* y);
  * updates (solver 1), true relative residual, compliance F.u, 0, 0, 0, 0.
  * ICC_ENOCONV / ICC_EBREAKDOWN like icc_pcg with u filled. */
 int icc_immersed_mesh_solve(icc_immersed_mesh* h, double* u, double* stats);
+
+/* ---- surface boundary conditions on triangle surfaces ------------------------ */
+/* Replaces rust/src/immersed_bc.rs.  surface_quadrature (:109-148): the
+ * triangles sel of the mesh, split at edge midpoints until every edge is at
+ * most max_edge, with the 6-point Dunavant rule; out (may be NULL): 7 doubles
+ * per point (x[3], weight, unit normal[3]), at most cap points; *npts = the
+ * point count. */
+int icc_surface_quadrature(const double* vertices, int64_t nv, const int64_t* tris, int64_t ntri,
+                           const int64_t* sel, int64_t nsel, double max_edge, int64_t cap, double* out,
+                           int64_t* npts);
+/* scalar_basis_at (:25-50): global indices and values of the 27 nonzero
+ * scalar basis functions at x (parameter clamped to the grid) */
+int icc_scalar_basis_at(const icc_grid* g, const double x[3], int64_t idx[27], double val[27]);
+/* One SurfaceTerms (:152-160) on grid g: stiffness triplets and a load.  Its
+ * conditions accumulate in call order; the first icc_surface_sizes /
+ * icc_surface_terms forms the per-element surface blocks on the device. */
+typedef struct icc_surface icc_surface;
+int icc_surface_create(icc_grid* g, icc_surface** out);
+int icc_surface_destroy(icc_surface* s);
+/* One labelled condition of surface_boundary_terms (:186-247) on the
+ * triangles sel (the face; labelling is the caller's):
+ *   ICC_BC_DIRICHLET  par = {cx, cy, cz (nonzero = constrained), vx, vy, vz, beta}:
+ *                     penalty beta int (u - v).w on the constrained components
+ *   ICC_BC_TRACTION   par = {tx, ty, tz}: load int t.w
+ *   ICC_BC_PRESSURE   par = {p}: load -int p n.w
+ *   ICC_BC_ROBIN      par = {kn, kt}: int (kn n n^T + kt (I - n n^T)) u.w
+ * max_edge <= 0 takes the reference's 0.25 h_min.  The reference's beta is
+ * penalty_factor * E / h_min. */
+enum { ICC_BC_DIRICHLET = 0, ICC_BC_TRACTION = 1, ICC_BC_PRESSURE = 2, ICC_BC_ROBIN = 3 };
+int icc_surface_add(icc_surface* s, int kind, const double* par, const double* vertices, int64_t nv,
+                    const int64_t* tris, int64_t ntri, const int64_t* sel, int64_t nsel, double max_edge);
+/* dirichlet_penalty_fn (:161-182): beta int (u - g(x)).w on the triangles
+ * sel, g evaluated on the host at every quadrature point */
+typedef void (*icc_point_fn)(const double x[3], double g[3], void* user);
+int icc_surface_add_dirichlet_fn(icc_surface* s, double beta, icc_point_fn g, void* user, const double* vertices,
+                                 int64_t nv, const int64_t* tris, int64_t ntri, const int64_t* sel, int64_t nsel,
+                                 double max_edge);
+/* out = triplets, element blocks, quadrature points */
+int icc_surface_sizes(icc_surface* s, int64_t out[3]);
+/* triplets in SurfaceAccumulator::into_triplets order (blocks by increasing
+ * first basis index; exact zeros dropped) and the load [ndof]; any may be NULL */
+int icc_surface_terms(icc_surface* s, int64_t* rows, int64_t* cols, double* vals, double* force);
+/* ImmersedProblem::add_surface_terms (rust/src/immersed.rs:541-548): K += the
+ * triplets (as icc_op_add_triplets), force += the load */
+int icc_immersed_mesh_add_surface(icc_immersed_mesh* h, icc_surface* s);
+/* replaces the problem's force (ImmersedProblem::force, e.g. zeroed before
+ * surface loads are added, immersed_cad_bc.rs:129) */
+int icc_immersed_mesh_set_force(icc_immersed_mesh* h, const double* force);
+/* the background grid of the problem (owned by h) */
+int icc_immersed_mesh_grid(icc_immersed_mesh* h, icc_grid** out);
 
 /* ---- immersed SIMP topology optimisation in a triangle B-rep ------------------ */
 /* Mirrors ImmersedTopOpt::{new, run} (rust/src/topopt_immersed.rs:21-247):

@@ -604,3 +604,118 @@ ORC_API void orc_set_threads(i64 n) {
   (void)n;
 #endif
 }
+
+// ---- surface boundary terms (immersed_bc.rs) --------------------------------
+namespace {
+struct SurfH {
+  const SpaceBox* sp = nullptr;
+  SurfaceAccumulator acc;
+  SurfaceTerms st;
+  bool done = false;
+};
+std::vector<usize> tri_list(const i64* sel, i64 nsel) {
+  std::vector<usize> t(nsel);
+  for (i64 i = 0; i < nsel; ++i) t[i] = (usize)sel[i];
+  return t;
+}
+}  // namespace
+
+// (x, w, n) per point as 7 doubles; returns the count (out may be NULL)
+ORC_API i64 orc_surface_quadrature(const double* nodes, i64 nv, const i64* tris, i64 nt, const i64* sel, i64 nsel,
+                                   double max_edge, double* out) {
+  TriangleMesh3D m = make_mesh(nodes, nv, tris, nt);
+  auto q = surface_quadrature(m, tri_list(sel, nsel), max_edge);
+  if (out)
+    for (usize i = 0; i < q.size(); ++i) {
+      for (int d = 0; d < 3; ++d) out[7 * i + d] = q[i].x[d];
+      out[7 * i + 3] = q[i].w;
+      for (int d = 0; d < 3; ++d) out[7 * i + 4 + d] = q[i].n[d];
+    }
+  return (i64)q.size();
+}
+// space: an orc_space handle (imm == 0) or an orc_immersed handle (imm == 1)
+static const SpaceBox* space_of(void* p, i64 imm) { return imm ? &((ImmH*)p)->pb.sp : &((SpaceH*)p)->sp; }
+ORC_API i64 orc_scalar_basis_at(void* p, i64 imm, const double* x, i64* idx, double* val) {
+  auto nz = scalar_basis_at(*space_of(p, imm), {x[0], x[1], x[2]});
+  for (usize a = 0; a < nz.size(); ++a) {
+    idx[a] = (i64)nz[a].first;
+    val[a] = nz[a].second;
+  }
+  return (i64)nz.size();
+}
+ORC_API void* orc_surface_new(void* p, i64 imm) {
+  auto* h = new SurfH();
+  h->sp = space_of(p, imm);
+  h->st.force.assign(h->sp->ndof, 0.0);
+  return h;
+}
+ORC_API void orc_surface_free(void* h) { delete (SurfH*)h; }
+// one condition of surface_boundary_terms: kind 0 Dirichlet (par = comps[3], values[3]),
+// 1 traction (par[3..6] = t), 2 pressure (par[6]), 3 Robin (par[7], par[8] = kn, kt);
+// beta = the Dirichlet penalty
+ORC_API void orc_surface_add(void* hp, i64 kind, const double* nodes, i64 nv, const i64* tris, i64 nt, const i64* sel,
+                             i64 nsel, const double* par, double beta) {
+  auto* h = (SurfH*)hp;
+  TriangleMesh3D m = make_mesh(nodes, nv, tris, nt);
+  SurfaceBc bc;
+  bc.kind = (SurfaceBc::Kind)kind;
+  bc.tris = tri_list(sel, nsel);
+  for (int c = 0; c < 3; ++c) {
+    bc.components[c] = par[c] != 0.0;
+    bc.values[c] = par[3 + c];
+  }
+  bc.pressure = par[6];
+  bc.kn = par[7];
+  bc.kt = par[8];
+  surface_add_condition(*h->sp, m, bc, beta, h->acc, h->st);
+}
+// dirichlet_penalty_fn with the affine g(x) = g0 + G x (G row-major 3x3)
+ORC_API void orc_surface_dirichlet_affine(void* hp, const double* nodes, i64 nv, const i64* tris, i64 nt,
+                                          const i64* sel, i64 nsel, double beta, const double* g0, const double* G) {
+  auto* h = (SurfH*)hp;
+  TriangleMesh3D m = make_mesh(nodes, nv, tris, nt);
+  const std::array<double, 3> a = {g0[0], g0[1], g0[2]};
+  std::array<double, 9> M;
+  for (int i = 0; i < 9; ++i) M[i] = G[i];
+  SurfaceTerms st = dirichlet_penalty_fn(*h->sp, m, tri_list(sel, nsel), beta, [&](const std::array<double, 3>& x) {
+    std::array<double, 3> r;
+    for (int i = 0; i < 3; ++i) r[i] = a[i] + M[3 * i] * x[0] + M[3 * i + 1] * x[1] + M[3 * i + 2] * x[2];
+    return r;
+  });
+  h->st = st;
+  h->done = true;
+}
+// finalises (into_triplets) once; returns the triplet count; rows/cols/vals/force may be NULL
+ORC_API i64 orc_surface_get(void* hp, i64* rows, i64* cols, double* vals, double* force) {
+  auto* h = (SurfH*)hp;
+  if (!h->done) {
+    h->acc.into_triplets(h->sp->ndof_sc, h->st);
+    h->done = true;
+  }
+  const usize n = h->st.vals.size();
+  for (usize k = 0; k < n; ++k) {
+    if (rows) rows[k] = (i64)h->st.rows[k];
+    if (cols) cols[k] = (i64)h->st.cols[k];
+    if (vals) vals[k] = h->st.vals[k];
+  }
+  if (force) std::memcpy(force, h->st.force.data(), h->st.force.size() * sizeof(double));
+  return (i64)n;
+}
+// ImmersedProblem::add_surface_terms on an orc_immersed handle
+ORC_API void orc_immersed_add_surface(void* ip, void* hp) {
+  auto* h = (SurfH*)hp;
+  orc_surface_get(hp, nullptr, nullptr, nullptr, nullptr);
+  add_surface_terms(((ImmH*)ip)->pb, h->st);
+}
+// ImmersedProblem::force is zeroed by the callers of surface terms (immersed_cad_bc.rs:129)
+ORC_API void orc_immersed_zero_force(void* ip) {
+  auto& f = ((ImmH*)ip)->pb.force;
+  std::fill(f.begin(), f.end(), 0.0);
+}
+// u at a physical point (fields.rs eval_vector_at)
+ORC_API void orc_immersed_eval_vector_at(void* ip, const double* u, const double* x, double* out) {
+  auto& pb = ((ImmH*)ip)->pb;
+  Vec uu(u, u + pb.sp.ndof);
+  auto r = eval_vector_at(pb.sp, uu, {x[0], x[1], x[2]});
+  for (int d = 0; d < 3; ++d) out[d] = r[d];
+}

@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <vector>
 
 namespace orc {
@@ -221,6 +222,43 @@ struct ImmersedSolution {
   double residual = 0;
 };
 ImmersedSolution solve_immersed_problem(const ImmersedProblem& pb, double tol, usize maxit);
+
+// ---- rust/src/immersed_bc.rs ----------------------------------------------
+struct SurfacePoint {
+  std::array<double, 3> x;
+  double w;
+  std::array<double, 3> n;
+};
+struct SurfaceTerms {
+  std::vector<usize> rows, cols;
+  Vec vals, force;
+};
+struct SurfaceAccumulator {
+  std::map<usize, std::pair<std::vector<usize>, Vec>> blocks;  // key: first basis function of the element
+  void add(const std::vector<std::pair<usize, double>>& nz, const double ks[3][3], double w);
+  void into_triplets(usize nsc, SurfaceTerms& st) const;
+};
+// One labelled condition on the triangles `tris` (BoundaryConditionType,
+// cad_model.rs): DIRICHLET {components, values}, TRACTION {values = t},
+// PRESSURE {pressure}, ROBIN {kn, kt}.
+struct SurfaceBc {
+  enum Kind { DIRICHLET = 0, TRACTION = 1, PRESSURE = 2, ROBIN = 3 } kind = DIRICHLET;
+  std::vector<usize> tris;
+  bool components[3] = {true, true, true};
+  double values[3] = {0, 0, 0};
+  double pressure = 0.0, kn = 0.0, kt = 0.0;
+};
+std::vector<std::pair<usize, double>> scalar_basis_at(const SpaceBox& sp, const std::array<double, 3>& x);
+std::vector<SurfacePoint> surface_quadrature(const TriangleMesh3D& mesh, const std::vector<usize>& tris,
+                                             double max_edge);
+void surface_add_condition(const SpaceBox& sp, const TriangleMesh3D& mesh, const SurfaceBc& bc, double beta,
+                           SurfaceAccumulator& acc, SurfaceTerms& st);
+SurfaceTerms surface_boundary_terms(const SpaceBox& sp, const TriangleMesh3D& mesh, const std::vector<SurfaceBc>& bcs,
+                                    double penalty_factor, double young);
+SurfaceTerms dirichlet_penalty_fn(const SpaceBox& sp, const TriangleMesh3D& mesh, const std::vector<usize>& tris,
+                                  double beta,
+                                  const std::function<std::array<double, 3>(const std::array<double, 3>&)>& g);
+void add_surface_terms(ImmersedProblem& pb, const SurfaceTerms& st);
 
 // ---- rust/src/topopt_immersed.rs ------------------------------------------
 struct ImmersedTopOptCfg {

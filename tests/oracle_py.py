@@ -97,6 +97,20 @@ def lib():
         L.orc_compare_with_pattern.argtypes = [vp, vp]
         L.orc_num_threads.restype = i64
         L.orc_num_threads.argtypes = []
+        L.orc_surface_quadrature.restype = i64
+        L.orc_surface_quadrature.argtypes = [vp, i64, vp, i64, vp, i64, dbl, vp]
+        L.orc_scalar_basis_at.restype = i64
+        L.orc_scalar_basis_at.argtypes = [vp, i64, vp, vp, vp]
+        L.orc_surface_new.restype = vp
+        L.orc_surface_new.argtypes = [vp, i64]
+        L.orc_surface_free.argtypes = [vp]
+        L.orc_surface_add.argtypes = [vp, i64, vp, i64, vp, i64, vp, i64, vp, dbl]
+        L.orc_surface_dirichlet_affine.argtypes = [vp, vp, i64, vp, i64, vp, i64, dbl, vp, vp]
+        L.orc_surface_get.restype = i64
+        L.orc_surface_get.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_immersed_add_surface.argtypes = [vp, vp]
+        L.orc_immersed_zero_force.argtypes = [vp]
+        L.orc_immersed_eval_vector_at.argtypes = [vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -314,6 +328,88 @@ class Immersed:
         res = np.zeros(1)
         lib().orc_immersed_solve(self.h, tol, int(maxit), _p(u), _p(c), _p(it), _p(res))
         return u, float(c[0]), int(it[0]), float(res[0])
+
+    def zero_force(self):
+        lib().orc_immersed_zero_force(self.h)
+        self.force[:] = 0.0
+
+    def add_surface(self, surf):
+        """ImmersedProblem::add_surface_terms (immersed.rs:541-548)."""
+        lib().orc_immersed_add_surface(self.h, surf.h)
+        self.force += surf.terms()[3]
+
+    def eval_vector_at(self, u, x):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(3)
+        lib().orc_immersed_eval_vector_at(self.h, _p(u), _p(x), _p(out))
+        return out
+
+
+def _mesh_args(nodes, tris, sel):
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    tris = np.ascontiguousarray(tris, dtype=np.int64)
+    sel = np.ascontiguousarray(sel, dtype=np.int64)
+    return nodes, tris, sel
+
+
+def surface_quadrature(nodes, tris, sel, max_edge):
+    """(x [n,3], w [n], normal [n,3]) of surface_quadrature (immersed_bc.rs:109-148)."""
+    nodes, tris, sel = _mesh_args(nodes, tris, sel)
+    n = lib().orc_surface_quadrature(_p(nodes), len(nodes), _p(tris), len(tris), _p(sel), len(sel), max_edge, None)
+    out = np.zeros((n, 7))
+    lib().orc_surface_quadrature(_p(nodes), len(nodes), _p(tris), len(tris), _p(sel), len(sel), max_edge, _p(out))
+    return out[:, :3].copy(), out[:, 3].copy(), out[:, 4:].copy()
+
+
+def scalar_basis_at(space, x, immersed=False):
+    """(indices, values) of scalar_basis_at (immersed_bc.rs:25-50) on an oracle Space or Immersed."""
+    idx = np.zeros(64, dtype=np.int64)
+    val = np.zeros(64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = lib().orc_scalar_basis_at(space.h, 1 if immersed else 0, _p(x), _p(idx), _p(val))
+    return idx[:n].copy(), val[:n].copy()
+
+
+class Surface:
+    """Surface terms (immersed_bc.rs SurfaceTerms) on an oracle Space or Immersed problem."""
+
+    DIRICHLET, TRACTION, PRESSURE, ROBIN = 0, 1, 2, 3
+
+    def __init__(self, space, immersed=False):
+        self._owner = space
+        self.h = lib().orc_surface_new(space.h, 1 if immersed else 0)
+
+    def __del__(self):
+        try:
+            lib().orc_surface_free(self.h)
+        except Exception:
+            pass
+
+    def add(self, kind, nodes, tris, sel, beta=0.0, comps=(1, 1, 1), values=(0, 0, 0), pressure=0.0, kn=0.0, kt=0.0):
+        """One labelled condition of surface_boundary_terms (immersed_bc.rs:186-247)."""
+        nodes, tris, sel = _mesh_args(nodes, tris, sel)
+        par = np.array(list(comps) + list(values) + [pressure, kn, kt], dtype=np.float64)
+        lib().orc_surface_add(self.h, kind, _p(nodes), len(nodes), _p(tris), len(tris), _p(sel), len(sel), _p(par),
+                              beta)
+
+    def dirichlet_affine(self, nodes, tris, sel, beta, g0, G):
+        """dirichlet_penalty_fn (immersed_bc.rs:161-182) with g(x) = g0 + G x."""
+        nodes, tris, sel = _mesh_args(nodes, tris, sel)
+        g0 = np.ascontiguousarray(g0, dtype=np.float64)
+        G = np.ascontiguousarray(G, dtype=np.float64).reshape(9)
+        lib().orc_surface_dirichlet_affine(self.h, _p(nodes), len(nodes), _p(tris), len(tris), _p(sel), len(sel),
+                                           beta, _p(g0), _p(G))
+
+    def terms(self):
+        """(rows, cols, vals, force) after into_triplets."""
+        n = lib().orc_surface_get(self.h, None, None, None, None)
+        r = np.zeros(n, dtype=np.int64)
+        c = np.zeros(n, dtype=np.int64)
+        v = np.zeros(n)
+        f = np.zeros(self._owner.ndof)
+        lib().orc_surface_get(self.h, _p(r), _p(c), _p(v), _p(f))
+        return r, c, v, f
 
 
 def immersed_topopt_run(nodes, tris, res, max_iter, volfrac, penal, rmin, gamma, young=1.0, poisson=0.3, emin=1e-4,

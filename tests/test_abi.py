@@ -87,3 +87,24 @@ def test_host_wq_rules_match_oracle(layout):
     else:
         want = np.array(sorted({0.0, 1 / 15, 2 / 15, 0.2, 0.4, 0.6, 0.8, 13 / 15, 14 / 15, 1.0, 0.3, 0.5, 0.7}))
     assert p5.shape == want.shape and np.max(np.abs(p5 - want)) < 1e-15
+
+
+def test_host_surface_quadrature_matches_oracle():
+    """icc_surface_quadrature (host code, no device work) against the
+    oracle's surface_quadrature (immersed_bc.rs:109-148): the same points,
+    weights and normals bit for bit, in the same order, on a tilted box with
+    splitting at several edge limits, and a degenerate triangle dropped."""
+    import numpy as np
+
+    import oracle_py as O
+    from immersed_catmull_clark_amd import TriangleMesh3D, surface_quadrature
+    m = TriangleMesh3D.new_box([0.1, -0.2, 0.05], [1.3, 0.4, 0.55])
+    rot = np.array([[0.9, -0.3, 0.1], [0.3, 0.95, 0.05], [-0.1, 0.0, 1.0]])
+    verts = np.vstack([m.vertices @ rot.T, [[0.5, 0.5, 0.5]]])
+    tris = np.vstack([m.triangles, [[0, 0, 8]]])  # last: degenerate
+    mesh = TriangleMesh3D(verts, tris)
+    for sel in (np.arange(13), [3, 12, 7]):
+        for me in (10.0, 0.3, 0.07):
+            x, w, n = surface_quadrature(mesh, sel, me)
+            xo, wo, no = O.surface_quadrature(verts, tris, sel, me)
+            assert np.array_equal(x, xo) and np.array_equal(w, wo) and np.array_equal(n, no)
