@@ -1938,6 +1938,7 @@ struct icc_batch {
   std::vector<icc_grid*> g;
   std::vector<icc_op*> op;
   std::vector<double*> d_f, d_u;
+  int wg_per_cu[2] = {1, 2};  // batched apply workgroups per CU: FP64, FP32
 };
 
 ICC_API int icc_batch_destroy(icc_batch* b) {
@@ -2052,22 +2053,252 @@ ICC_API int icc_batch_set_traction(icc_batch* b, int64_t k, const double t[3]) {
 }
 
 namespace icc {
-// Captures PCG_BATCH iterations of every instance in `live`, one instance
-// after another, on the context stream.
+// Device data of one batched solve: every instance's PCG vectors in
+// precision T concatenated at 64-aligned offsets, their stiffness scales,
+// free masks, A fragments (types renumbered by instance) and PCG states; and
+// the plan of the unfinished instances: their tiles as one list with a
+// TileInst per tile, the workgroup runs of the batched apply, the p.Ap slots
+// of (run, instance) and the vector segments of the batched updates.  The
+// plan is rebuilt (and the iteration graph captured again) when instances
+// finish.
 template <typename T>
-static int capture_batched(icc_batch* b, const std::vector<int>& live, const std::vector<Work*>& w, IterGraph& G) {
+struct BatchRun {
+  std::vector<int64_t> off, eoff;
+  std::vector<int32_t> toff;
+  T *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *dinv = nullptr, *scale = nullptr, *kfrag = nullptr;
+  uint8_t* fr = nullptr;
+  PcgState* st = nullptr;
+  PcgState* st_host = nullptr;
+  double *pap = nullptr, *parts = nullptr;
+  int pap_cap = 0;
+  VecSeg* segs = nullptr;
+  // plan
+  Tile* tiles = nullptr;
+  int32_t *first = nullptr, *tmask = nullptr, *seg_of = nullptr;
+  TileInst* tinst = nullptr;
+  int G = 0, nblk = 0;
+  void free_plan() {
+    (void)hipFree(tiles);
+    (void)hipFree(first);
+    (void)hipFree(tmask);
+    (void)hipFree(seg_of);
+    (void)hipFree(tinst);
+    tiles = nullptr;
+    first = tmask = seg_of = nullptr;
+    tinst = nullptr;
+  }
+  ~BatchRun() {
+    free_plan();
+    for (void* q : {(void*)x, (void*)r, (void*)p, (void*)ap, (void*)dinv, (void*)scale, (void*)kfrag, (void*)fr,
+                    (void*)st, (void*)pap, (void*)parts, (void*)segs})
+      (void)hipFree(q);
+    if (st_host) (void)hipHostFree(st_host);
+  }
+};
+
+template <typename T>
+static const T* kfrag_of(const icc_grid* g);
+template <>
+const double* kfrag_of<double>(const icc_grid* g) {
+  return g->d_kfrag64;
+}
+template <>
+const float* kfrag_of<float>(const icc_grid* g) {
+  return g->d_kfrag32;
+}
+template <typename T>
+static const T* scale_of(const icc_op* op);
+template <>
+const double* scale_of<double>(const icc_op* op) {
+  return op->d_scale;
+}
+template <>
+const float* scale_of<float>(const icc_op* op) {
+  return op->d_scale32;
+}
+
+// Allocates the concatenated data of all instances and copies the scales,
+// free masks and A fragments in.
+template <typename T>
+static int batch_setup(icc_batch* b, BatchRun<T>& R) {
+  const int nb = (int)b->op.size();
   hipStream_t m = b->ctx->stream;
-  ICC_HIP(hipStreamBeginCapture(m, hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipSuccess;
+  int64_t n_tot = 0, e_tot = 0, t_tot = 0, blk_tot = 0;
+  for (int k = 0; k < nb; ++k) {
+    const GridSetup& s = b->g[k]->s;
+    R.off.push_back(n_tot);
+    R.eoff.push_back(e_tot);
+    R.toff.push_back((int32_t)t_tot);
+    n_tot += (s.ndof + 63) / 64 * 64;
+    e_tot += s.nel_tot;
+    t_tot += (int64_t)s.kmat.size();
+    blk_tot += vec_blocks(s.ndof);
+  }
+  if (n_tot >= ((int64_t)1 << 31)) {
+    set_error("icc_batch_solve: more than 2^31 unknowns in one batch");
+    return ICC_EINVAL;
+  }
+  const size_t fsz = (size_t)MT * KS * 64;  // fragment entries per type
+  const int gmax = APPLY_MAX_WG_PER_CU * (b->ctx->num_cu > 0 ? b->ctx->num_cu : 256);
+  R.pap_cap = gmax + nb;
+  int rc;
+  if ((rc = dalloc(&R.x, n_tot)) || (rc = dalloc(&R.r, n_tot)) || (rc = dalloc(&R.p, n_tot)) ||
+      (rc = dalloc(&R.ap, n_tot)) || (rc = dalloc(&R.dinv, n_tot)) || (rc = dalloc(&R.scale, e_tot)) ||
+      (rc = dalloc(&R.kfrag, t_tot * fsz)) || (rc = dalloc(&R.fr, n_tot)) || (rc = dalloc(&R.st, nb)) ||
+      (rc = dalloc(&R.pap, R.pap_cap)) || (rc = dalloc(&R.parts, 2 * blk_tot)) || (rc = dalloc(&R.segs, nb)))
+    return rc;
+  if (hipHostMalloc((void**)&R.st_host, nb * sizeof(PcgState), hipHostMallocDefault) != hipSuccess) {
+    set_error("hipHostMalloc failed");
+    return ICC_ENOMEM;
+  }
+  std::vector<PcgState> z(nb);
+  std::memset(z.data(), 0, nb * sizeof(PcgState));
+  ICC_HIP(hipMemcpyAsync(R.st, z.data(), nb * sizeof(PcgState), hipMemcpyHostToDevice, m));
+  ICC_HIP(launch_fill<T>(R.x, T(0), n_tot, m));
+  ICC_HIP(launch_fill<T>(R.p, T(0), n_tot, m));
+  ICC_HIP(launch_fill<T>(R.ap, T(0), n_tot, m));
+  for (int k = 0; k < nb; ++k) {
+    const GridSetup& s = b->g[k]->s;
+    const icc_op* op = b->op[k];
+    ICC_HIP(hipMemcpyAsync(R.scale + R.eoff[k], scale_of<T>(op), s.nel_tot * sizeof(T), hipMemcpyDeviceToDevice, m));
+    ICC_HIP(hipMemcpyAsync(R.kfrag + (size_t)R.toff[k] * fsz, kfrag_of<T>(b->g[k]), s.kmat.size() * fsz * sizeof(T),
+                           hipMemcpyDeviceToDevice, m));
+    if (op->d_free)
+      ICC_HIP(hipMemcpyAsync(R.fr + R.off[k], op->d_free, s.ndof, hipMemcpyDeviceToDevice, m));
+    else
+      ICC_HIP(hipMemsetAsync(R.fr + R.off[k], 1, s.ndof, m));
+  }
+  ICC_HIP(hipStreamSynchronize(m));
+  return ICC_OK;
+}
+
+// Plan of the instances `live` (in order): their tiles concatenated, split
+// into G contiguous runs of balanced work (tile_cost), p.Ap slots numbered
+// instance by instance (an instance's runs are contiguous), and each
+// instance's vector blocks.
+template <typename T>
+static int batch_plan(icc_batch* b, BatchRun<T>& R, const std::vector<int>& live, int wg_per_cu) {
+  R.free_plan();
+  std::vector<Tile> tl;
+  std::vector<int32_t> inst, tm;
   for (int k : live) {
-    Work* wk = w[k];
-    for (int i = 0; i < PCG_BATCH && e == hipSuccess; ++i) {
-      if constexpr (std::is_same<T, double>::value)
-        e = enqueue_iteration<double>(b->op[k], wk, b->d_u[k], wk->r, wk->p, wk->ap, wk->dinv, m);
-      else
-        e = enqueue_iteration<float>(b->op[k], wk, wk->x32, wk->r32, wk->p32, wk->ap32, wk->dinv32, m);
+    const GridSetup& s = b->g[k]->s;
+    const icc_op* op = b->op[k];
+    const std::vector<int32_t> m =
+        op->h_free.empty() ? std::vector<int32_t>(s.tiles.size(), 0) : tile_mask_of(s, s.tiles, op->h_free.data());
+    for (size_t i = 0; i < s.tiles.size(); ++i) {
+      Tile t = s.tiles[i];
+      t.type += R.toff[k];
+      tl.push_back(t);
+      inst.push_back(k);
+      tm.push_back(m[i]);
     }
   }
+  const int64_t nt = (int64_t)tl.size();
+  int G = wg_per_cu * (b->ctx->num_cu > 0 ? b->ctx->num_cu : 256);
+  if (G > nt) G = (int)nt;
+  if (G >= 8) G -= G % 8;
+  if (G < 1) G = 1;
+  // contiguous runs ending as close as possible to the cumulative targets
+  std::vector<double> cum(nt + 1, 0.0);
+  for (int64_t i = 0; i < nt; ++i) cum[i + 1] = cum[i] + apply_tile_cost(tl[i]);
+  std::vector<int32_t> first(G + 1, 0);
+  first[G] = (int32_t)nt;
+  int64_t i = 0;
+  for (int c = 1; c < G; ++c) {
+    const double target = cum[nt] * c / G;
+    while (i < nt && cum[i + 1] <= target) ++i;
+    if (i < nt && (target - cum[i]) > (cum[i + 1] - target)) ++i;
+    if (i < first[c - 1] + 1 && first[c - 1] < nt) i = first[c - 1] + 1;  // no empty runs while tiles remain
+    first[c] = (int32_t)std::min<int64_t>(i, nt);
+  }
+  // runs of each instance and its slots
+  std::vector<int> c_first(b->op.size(), -1), c_last(b->op.size(), -1);
+  for (int c = 0; c < G; ++c)
+    for (int64_t t = first[c]; t < first[c + 1]; ++t) {
+      const int k = inst[t];
+      if (c_first[k] < 0) c_first[k] = c;
+      c_last[k] = c;
+    }
+  std::vector<VecSeg> segs(b->op.size());
+  std::vector<int32_t> seg_of;
+  int32_t pap0 = 0, blk0 = 0;
+  for (int k : live) {
+    VecSeg& sg = segs[k];
+    sg.off = R.off[k];
+    sg.n = b->g[k]->s.ndof;
+    sg.blk0 = blk0;
+    sg.nblk = vec_blocks(sg.n);
+    sg.pap0 = pap0;
+    sg.npap = c_first[k] < 0 ? 0 : c_last[k] - c_first[k] + 1;
+    pap0 += sg.npap;
+    blk0 += sg.nblk;
+    for (int q = 0; q < sg.nblk; ++q) seg_of.push_back(k);
+  }
+  if (pap0 > R.pap_cap) {
+    set_error("icc_batch_solve: too many p.Ap slots");
+    return ICC_EINVAL;
+  }
+  std::vector<TileInst> ti(nt);
+  for (int c = 0; c < G; ++c)
+    for (int64_t t = first[c]; t < first[c + 1]; ++t) {
+      const int k = inst[t];
+      const GridSetup& s = b->g[k]->s;
+      TileInst& q = ti[t];
+      q.dbase = (int32_t)R.off[k];
+      q.ebase = (int32_t)R.eoff[k];
+      q.ncp0 = (int32_t)s.ncp[0];
+      q.ncp1 = (int32_t)s.ncp[1];
+      q.ndof_sc = (int32_t)s.ndof_sc;
+      q.nel0 = (int32_t)s.nel[0];
+      q.nel1 = (int32_t)s.nel[1];
+      q.slot = segs[k].pap0 + (c - c_first[k]);
+    }
+  int rc;
+  if ((rc = dalloc(&R.tiles, nt)) || (rc = dalloc(&R.first, G + 1)) || (rc = dalloc(&R.tmask, nt)) ||
+      (rc = dalloc(&R.seg_of, seg_of.size())) || (rc = dalloc(&R.tinst, nt)))
+    return rc;
+  hipStream_t m = b->ctx->stream;
+  ICC_HIP(hipMemcpyAsync(R.tiles, tl.data(), nt * sizeof(Tile), hipMemcpyHostToDevice, m));
+  ICC_HIP(hipMemcpyAsync(R.first, first.data(), (G + 1) * sizeof(int32_t), hipMemcpyHostToDevice, m));
+  ICC_HIP(hipMemcpyAsync(R.tmask, tm.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice, m));
+  ICC_HIP(hipMemcpyAsync(R.seg_of, seg_of.data(), seg_of.size() * sizeof(int32_t), hipMemcpyHostToDevice, m));
+  ICC_HIP(hipMemcpyAsync(R.tinst, ti.data(), nt * sizeof(TileInst), hipMemcpyHostToDevice, m));
+  ICC_HIP(hipMemcpyAsync(R.segs, segs.data(), segs.size() * sizeof(VecSeg), hipMemcpyHostToDevice, m));
+  ICC_HIP(hipStreamSynchronize(m));
+  R.G = G;
+  R.nblk = (int)seg_of.size();
+  return ICC_OK;
+}
+
+// One batched PCG iteration: the apply of every planned instance in one
+// launch, then the two update launches over their segments.
+template <typename T>
+static hipError_t batch_iteration(BatchRun<T>& R, hipStream_t m) {
+  BatchApply ba;
+  ba.tiles = R.tiles;
+  ba.wg_first = R.first;
+  ba.tinst = R.tinst;
+  ba.tmask = R.tmask;
+  ba.G = R.G;
+  hipError_t e;
+  if constexpr (std::is_same<T, double>::value) {
+    ba.kfrag64 = R.kfrag;
+    e = launch_apply_batch_f64(ba, R.p, R.ap, R.scale, R.fr, R.pap, m);
+  } else {
+    ba.kfrag32 = R.kfrag;
+    e = launch_apply_batch_f32(ba, R.p, R.ap, R.scale, R.fr, R.pap, m);
+  }
+  if (e == hipSuccess)
+    e = launch_pcg_update_batch<T>(R.nblk, R.segs, R.seg_of, R.st, R.pap, R.p, R.ap, R.x, R.r, R.dinv, R.parts, m);
+  return e;
+}
+
+template <typename T>
+static int batch_capture(BatchRun<T>& R, hipStream_t m, IterGraph& G) {
+  ICC_HIP(hipStreamBeginCapture(m, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < PCG_BATCH && e == hipSuccess; ++i) e = batch_iteration<T>(R, m);
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(m, &graph);
   if (e != hipSuccess) return hip_fail(e, "batched capture");
@@ -2077,151 +2308,141 @@ static int capture_batched(icc_batch* b, const std::vector<int>& live, const std
   return ICC_OK;
 }
 
-// Copies the device state of every instance in `ks` to its pinned host slot.
-static int read_states(icc_batch* b, const std::vector<int>& ks, const std::vector<Work*>& w) {
-  hipStream_t m = b->ctx->stream;
-  for (int k : ks) ICC_HIP(hipMemcpyAsync(w[k]->st_host, w[k]->st, sizeof(PcgState), hipMemcpyDeviceToHost, m));
-  ICC_HIP(hipStreamSynchronize(m));
-  return ICC_OK;
-}
-
-// FP64 true residual r = f - A u (masked) of instance k into its state (rr, norm_b squared).
-static hipError_t true_residual(icc_batch* b, int k, Work* w) {
+// FP64 true residual r = f - A u (masked) of instance k into state st.
+static hipError_t batch_residual(icc_batch* b, int k, Work* w, PcgState* st) {
   hipError_t e = apply_t<double>(b->op[k], b->d_u[k], w->ap, nullptr);
   if (e == hipSuccess)
-    e = launch_residual(b->g[k]->s.ndof, b->d_f[k], w->ap, b->op[k]->d_free, w->r, w->st, w->parts, b->ctx->stream);
+    e = launch_residual(b->g[k]->s.ndof, b->d_f[k], w->ap, b->op[k]->d_free, w->r, st, w->parts, b->ctx->stream);
   return e;
 }
-}  // namespace icc
 
-ICC_API int icc_batch_solve(icc_batch* b, int prec, double tol, double delta, int64_t maxit, double* u,
-                            double* stats) {
-  if (!b || (prec != ICC_F64 && prec != ICC_F32) || !(tol > 0.0)) return ICC_EINVAL;
-  if (prec == ICC_F32 && !(delta > 0.0 && delta < 1.0)) return ICC_EINVAL;
+template <typename T>
+static int batch_solve_t(icc_batch* b, double tol, double delta, int64_t maxit, double* u, double* stats) {
+  constexpr bool MIXED = std::is_same<T, float>::value;
   const int nb = (int)b->op.size();
   hipStream_t m = b->ctx->stream;
+  BatchRun<T> R;
+  ICC_TRY(batch_setup<T>(b, R));
   std::vector<Work*> w(nb, nullptr);
-  std::vector<int> status(nb, ICC_ENOCONV), live, finished(nb, 0);
+  std::vector<int> status(nb, ICC_ENOCONV), finished(nb, 0);
   std::vector<double> nf(nb, 1.0), res_true(nb, 1.0), updates(nb, 0.0);
   std::vector<int64_t> iters(nb, 0);
+  const int32_t mi = (int32_t)std::min<int64_t>(maxit > 0 ? maxit : (int64_t)2000000000, 2000000000);
+  const double inner_tol = MIXED ? delta : tol;
   for (int k = 0; k < nb; ++k) {
     ICC_TRY(ensure_work(b->op[k], w[k]));
-    const int64_t n = b->g[k]->s.ndof;
+    const int64_t n = b->g[k]->s.ndof, o = R.off[k];
     ICC_TRY(icc_diag_dev(b->op[k], w[k]->tmp));
+    ICC_HIP(launch_inv_diag<T>(w[k]->tmp, R.dinv + o, n, 1e-12, 1, m));
     ICC_HIP(launch_fill<double>(b->d_u[k], 0.0, n, m));
-    if (prec == ICC_F32) {
-      ICC_HIP(launch_inv_diag<float>(w[k]->tmp, w[k]->dinv32, n, 1e-12, 1, m));
-      ICC_HIP(true_residual(b, k, w[k]));
-    } else {
-      ICC_HIP(launch_inv_diag<double>(w[k]->tmp, w[k]->dinv, n, 1e-12, 1, m));
-    }
+    ICC_HIP(hipMemcpyAsync(&R.st[k].tol, &inner_tol, sizeof(double), hipMemcpyHostToDevice, m));
+    ICC_HIP(hipMemcpyAsync(&R.st[k].maxit, &mi, sizeof(int32_t), hipMemcpyHostToDevice, m));
+    if (MIXED) ICC_HIP(batch_residual(b, k, w[k], R.st + k));
   }
-  const int32_t mi = (int32_t)std::min<int64_t>(maxit > 0 ? maxit : (int64_t)2000000000, 2000000000);
-  if (prec == ICC_F32) {
-    std::vector<int> all(nb);
-    for (int k = 0; k < nb; ++k) all[k] = k;
-    ICC_TRY(read_states(b, all, w));
+  std::vector<int> live;
+  if (MIXED) {
+    ICC_HIP(hipMemcpyAsync(R.st_host, R.st, nb * sizeof(PcgState), hipMemcpyDeviceToHost, m));
+    ICC_HIP(hipStreamSynchronize(m));
     for (int k = 0; k < nb; ++k) {
-      const PcgState& hs = *w[k]->st_host;
+      const PcgState& hs = R.st_host[k];
       nf[k] = std::sqrt(hs.norm_b);
       res_true[k] = nf[k] > 0 ? std::sqrt(hs.rr) / nf[k] : std::sqrt(hs.rr);
       if (res_true[k] <= tol) {
         finished[k] = 1;
         status[k] = ICC_OK;
-        continue;
       }
-      const int64_t n = b->g[k]->s.ndof;
-      ICC_HIP(hipMemcpyAsync(&w[k]->st->tol, &delta, sizeof(double), hipMemcpyHostToDevice, m));
-      ICC_HIP(hipMemcpyAsync(&w[k]->st->maxit, &mi, sizeof(int32_t), hipMemcpyHostToDevice, m));
-      ICC_HIP((launch_pcg_init<float, double>(n, w[k]->r, 1.0, w[k]->dinv32, w[k]->x32, w[k]->r32, w[k]->p32,
-                                              w[k]->ap32, w[k]->st, w[k]->parts, m)));
-      live.push_back(k);
-    }
-  } else {
-    for (int k = 0; k < nb; ++k) {
-      const int64_t n = b->g[k]->s.ndof;
-      ICC_HIP(hipMemcpyAsync(&w[k]->st->tol, &tol, sizeof(double), hipMemcpyHostToDevice, m));
-      ICC_HIP(hipMemcpyAsync(&w[k]->st->maxit, &mi, sizeof(int32_t), hipMemcpyHostToDevice, m));
-      ICC_HIP((launch_pcg_init<double, double>(n, b->d_f[k], 1.0, w[k]->dinv, b->d_u[k], w[k]->r, w[k]->p, w[k]->ap,
-                                               w[k]->st, w[k]->parts, m)));
-      live.push_back(k);
     }
   }
-  if (!live.empty()) {
-    // the graph holds the unfinished instances only: it is captured again
-    // whenever instances finish, so finished ones cost no idle launches
-    std::unique_ptr<IterGraph> G;
-    std::vector<int> in_graph;
-    const int32_t done_fin = 4;  // done value that keeps a finished instance's kernels idle
-    std::vector<int> pend;
-    while (true) {
-      std::vector<int> running;
-      for (int k : live)
-        if (!finished[k]) running.push_back(k);
-      if (running != in_graph) {
-        G.reset(new IterGraph());
-        if (prec == ICC_F32)
-          ICC_TRY(capture_batched<float>(b, running, w, *G));
-        else
-          ICC_TRY(capture_batched<double>(b, running, w, *G));
-        in_graph = running;
-      }
-      ICC_HIP(hipGraphLaunch(G->exec, m));
-      ICC_TRY(read_states(b, live, w));
-      pend.clear();
-      bool any_running = false;
-      for (int k : live) {
-        if (finished[k]) continue;
-        const PcgState& hs = *w[k]->st_host;
-        iters[k] = hs.iters;
-        if (hs.done == 0) {
-          any_running = true;
-          continue;
-        }
-        if (prec == ICC_F64) {
-          finished[k] = 1;
-          status[k] = hs.done == 1 ? ICC_OK : hs.done == 2 ? ICC_EBREAKDOWN : ICC_ENOCONV;
-        } else {
-          pend.push_back(k);
-        }
-      }
-      if (prec == ICC_F32 && !pend.empty()) {
-        // reliable updates: u += x32, FP64 true residual
-        std::vector<int> done_code(nb, 0);
-        for (int k : pend) {
-          done_code[k] = w[k]->st_host->done;
-          ICC_HIP(launch_ir_update<float>(b->g[k]->s.ndof, b->d_u[k], w[k]->x32, 1.0, m));
-          ICC_HIP(true_residual(b, k, w[k]));
-          updates[k] += 1.0;
-        }
-        ICC_TRY(read_states(b, pend, w));
-        for (int k : pend) {
-          const PcgState& hr = *w[k]->st_host;
-          res_true[k] = nf[k] > 0 ? std::sqrt(hr.rr) / nf[k] : std::sqrt(hr.rr);
-          if (res_true[k] <= tol || done_code[k] == 3 || done_code[k] == 2) {
-            finished[k] = 1;
-            status[k] = res_true[k] <= tol ? ICC_OK : done_code[k] == 2 ? ICC_EBREAKDOWN : ICC_ENOCONV;
-            ICC_HIP(hipMemcpyAsync(&w[k]->st->done, &done_fin, sizeof(int32_t), hipMemcpyHostToDevice, m));
-          } else {
-            const int64_t n = b->g[k]->s.ndof;
-            ICC_HIP(launch_reliable_replace(n, w[k]->r, w[k]->dinv32, w[k]->r32, w[k]->x32, w[k]->st, w[k]->parts, m));
-            ICC_HIP(launch_pdir_update(n, w[k]->st, w[k]->p32, w[k]->r32, w[k]->dinv32, w[k]->ap32, m));
-            any_running = true;
-          }
-        }
-      }
-      if (!any_running) break;
+  for (int k = 0; k < nb; ++k) {
+    if (finished[k]) continue;
+    const int64_t n = b->g[k]->s.ndof, o = R.off[k];
+    if constexpr (MIXED)
+      ICC_HIP((launch_pcg_init<float, double>(n, w[k]->r, 1.0, R.dinv + o, R.x + o, R.r + o, R.p + o, R.ap + o,
+                                              R.st + k, w[k]->parts, m)));
+    else
+      ICC_HIP((launch_pcg_init<double, double>(n, b->d_f[k], 1.0, R.dinv + o, R.x + o, R.r + o, R.p + o, R.ap + o,
+                                               R.st + k, w[k]->parts, m)));
+    live.push_back(k);
+  }
+  // replays are bounded on the host as well: every replay advances each
+  // running instance by PCG_BATCH iterations unless it stopped, and a stop
+  // ends the instance or resets it through a reliable update
+  int64_t replays = 0;
+  const int64_t max_replays = 4 * ((int64_t)mi / PCG_BATCH + 2) * (MIXED ? 2 : 1) + 64;
+  std::unique_ptr<IterGraph> G;
+  std::vector<int> in_graph;
+  const int wg = std::max(1, std::min(APPLY_MAX_WG_PER_CU, b->wg_per_cu[MIXED ? 1 : 0]));
+  while (true) {
+    std::vector<int> running;
+    for (int k : live)
+      if (!finished[k]) running.push_back(k);
+    if (running.empty()) break;
+    if (running != in_graph) {
+      G.reset();
+      ICC_TRY(batch_plan<T>(b, R, running, wg));
+      G.reset(new IterGraph());
+      ICC_TRY(batch_capture<T>(R, m, *G));
+      in_graph = running;
     }
+    if (++replays > max_replays) {
+      set_error("icc_batch_solve: replay limit reached");
+      break;
+    }
+    ICC_HIP(hipGraphLaunch(G->exec, m));
+    ICC_HIP(hipMemcpyAsync(R.st_host, R.st, nb * sizeof(PcgState), hipMemcpyDeviceToHost, m));
     ICC_HIP(hipStreamSynchronize(m));
+    std::vector<int> pend;
+    for (int k : running) {
+      const PcgState& hs = R.st_host[k];
+      iters[k] = hs.iters;
+      if (hs.done == 0) continue;
+      if (!MIXED) {
+        finished[k] = 1;
+        status[k] = hs.done == 1 ? ICC_OK : hs.done == 2 ? ICC_EBREAKDOWN : ICC_ENOCONV;
+      } else {
+        pend.push_back(k);
+      }
+    }
+    if constexpr (MIXED) {
+      // reliable updates: u += x32, FP64 true residual
+      std::vector<int> done_code(nb, 0);
+      for (int k : pend) {
+        done_code[k] = R.st_host[k].done;
+        const int64_t n = b->g[k]->s.ndof;
+        ICC_HIP(launch_ir_update<float>(n, b->d_u[k], R.x + R.off[k], 1.0, m));
+        ICC_HIP(batch_residual(b, k, w[k], R.st + k));
+        updates[k] += 1.0;
+      }
+      if (!pend.empty()) {
+        ICC_HIP(hipMemcpyAsync(R.st_host, R.st, nb * sizeof(PcgState), hipMemcpyDeviceToHost, m));
+        ICC_HIP(hipStreamSynchronize(m));
+      }
+      for (int k : pend) {
+        const PcgState& hr = R.st_host[k];
+        res_true[k] = nf[k] > 0 ? std::sqrt(hr.rr) / nf[k] : std::sqrt(hr.rr);
+        if (res_true[k] <= tol || done_code[k] == 3 || done_code[k] == 2) {
+          finished[k] = 1;
+          status[k] = res_true[k] <= tol ? ICC_OK : done_code[k] == 2 ? ICC_EBREAKDOWN : ICC_ENOCONV;
+        } else {
+          const int64_t n = b->g[k]->s.ndof, o = R.off[k];
+          ICC_HIP(launch_reliable_replace(n, w[k]->r, R.dinv + o, R.r + o, R.x + o, R.st + k, w[k]->parts, m));
+          ICC_HIP(launch_pdir_update(n, R.st + k, R.p + o, R.r + o, R.dinv + o, R.ap + o, m));
+        }
+      }
+    }
   }
-  // true residuals of the FP64 path; copy out
+  ICC_HIP(hipStreamSynchronize(m));
+  G.reset();
+  // FP64: the solution is the recurrence's x; true residuals; copy out
   int64_t off = 0;
   for (int k = 0; k < nb; ++k) {
     const int64_t n = b->g[k]->s.ndof;
-    if (prec == ICC_F64) {
-      ICC_HIP(true_residual(b, k, w[k]));
-      ICC_TRY(read_states(b, std::vector<int>{k}, w));
-      const PcgState& hr = *w[k]->st_host;
+    if (!MIXED) {
+      ICC_HIP(hipMemcpyAsync(b->d_u[k], R.x + R.off[k], n * 8, hipMemcpyDeviceToDevice, m));
+      ICC_HIP(batch_residual(b, k, w[k], R.st + k));
+      ICC_HIP(hipMemcpyAsync(R.st_host + k, R.st + k, sizeof(PcgState), hipMemcpyDeviceToHost, m));
+      ICC_HIP(hipStreamSynchronize(m));
+      const PcgState& hr = R.st_host[k];
       const double nfk = std::sqrt(hr.norm_b);
       res_true[k] = nfk > 0 ? std::sqrt(hr.rr) / nfk : std::sqrt(hr.rr);
     }
@@ -2241,4 +2462,13 @@ ICC_API int icc_batch_solve(icc_batch* b, int prec, double tol, double delta, in
       return status[k];
     }
   return ICC_OK;
+}
+}  // namespace icc
+
+ICC_API int icc_batch_solve(icc_batch* b, int prec, double tol, double delta, int64_t maxit, double* u,
+                            double* stats) {
+  if (!b || (prec != ICC_F64 && prec != ICC_F32) || !(tol > 0.0)) return ICC_EINVAL;
+  if (prec == ICC_F32 && !(delta > 0.0 && delta < 1.0)) return ICC_EINVAL;
+  return prec == ICC_F32 ? batch_solve_t<float>(b, tol, delta, maxit, u, stats)
+                         : batch_solve_t<double>(b, tol, delta, maxit, u, stats);
 }

@@ -34,6 +34,12 @@ constexpr int SP_SCALE_BITS = 15;   // K_t and x scaled to a maximum in [2^14, 2
 struct Tile {
   int32_t i0, j0, k0, nx, ny, nz, type, lines;
 };
+// Instance of a tile of a batched (multi-instance) apply: the instance's
+// offsets in the concatenated DOF and element vectors, its grid extents, and
+// the p.Ap slot of (workgroup run, instance).
+struct TileInst {
+  int32_t dbase, ebase, ncp0, ncp1, ndof_sc, nel0, nel1, slot;
+};
 
 #if defined(__HIP__) || defined(__HIPCC__)
 #define ICC_HD __host__ __device__
@@ -106,6 +112,8 @@ double quantize_kmat(const double* K, int bits, double* out);
 void build_tiles(GridSetup& s, int variant);
 // Tiles of the split-FP16 apply (s.stiles), within the SP_* limits.
 void build_split_tiles(GridSetup& s);
+// Work of an apply tile (element-group pairs plus staging and flush).
+double apply_tile_cost(const Tile& t);
 // Work of a split-apply tile (its element groups plus a per-tile overhead).
 double split_tile_cost(const Tile& t);
 // Re-partitions s.tiles for G persistent workgroups: walks the tiles in

@@ -45,6 +45,21 @@ hipError_t launch_apply_bf16(const icc_grid* g, const icc_op* op, const float* x
 hipError_t launch_apply_split(const icc_grid* g, const icc_op* op, const float* x, float* y, double* pap,
                               const PcgState* pst, hipStream_t st);
 hipError_t launch_layout_probe_f16(double* out, hipStream_t st);
+// Batched apply over the concatenated tiles of several instances (one launch):
+// y += M A x per instance, p.Ap into pap[TileInst::slot]
+struct BatchApply {
+  const Tile* tiles = nullptr;
+  const int32_t* wg_first = nullptr;
+  const TileInst* tinst = nullptr;
+  const int32_t* tmask = nullptr;
+  const double* kfrag64 = nullptr;  // concatenated A fragments (the tiles' types index them)
+  const float* kfrag32 = nullptr;
+  int G = 0;
+};
+hipError_t launch_apply_batch_f64(const BatchApply& b, const double* x, double* y, const double* scale,
+                                  const uint8_t* freem, double* pap, hipStream_t st);
+hipError_t launch_apply_batch_f32(const BatchApply& b, const float* x, float* y, const float* scale,
+                                  const uint8_t* freem, double* pap, hipStream_t st);
 hipError_t launch_energies(const icc_grid* g, const double* u, double* ee, const double* x, double penal,
                            double emin, const double* w, double* dc, hipStream_t st);
 hipError_t launch_diag_f64(const icc_grid* g, const double* scale, double* d, hipStream_t st);
@@ -74,6 +89,17 @@ hipError_t launch_pcg_init(int64_t n, const TB* b, double bscale, const T* dinv,
 template <typename T>
 hipError_t launch_pcg_update(int64_t n, PcgState* st, const double* pap_parts, int npap, T* p, T* ap, T* x, T* r,
                              const T* dinv, double* parts, hipStream_t s);
+// Batched PCG updates: one launch of nblk blocks over the segments of several
+// instances (concatenated vectors; each instance's blocks, state, p.Ap slots
+// and partials as in its single-instance launch, see VecSeg)
+struct VecSeg {
+  int64_t off, n;    // the instance's vector offset (a multiple of 64) and length
+  int32_t blk0, nblk;  // its blocks (vec_blocks(n) of them) and partials at 2 blk0
+  int32_t pap0, npap;  // its p.Ap slots
+};
+template <typename T>
+hipError_t launch_pcg_update_batch(int nblk, const VecSeg* segs, const int32_t* seg_of, PcgState* st, const double* pap,
+                                   T* p, T* ap, T* x, T* r, const T* dinv, double* parts, hipStream_t s);
 hipError_t launch_pdir_update(int64_t n, const PcgState* st, float* p, const float* r, const float* dinv, float* ap,
                               hipStream_t s);
 hipError_t launch_reliable_replace(int64_t n, const double* r64, const float* dinv, float* r, float* x, PcgState* st,

@@ -248,13 +248,21 @@ __device__ __forceinline__ void tile_range(const int* __restrict__ wg_first, int
 // BF = true (T = float): K and the staged x are rounded to BF16 for
 // v_mfma_f32_16x16x32_bf16 (3 K steps of 32), FP32 accumulation; the rest as
 // the FP32 mode.
-template <typename T, bool BF, int W>
+//
+// MI = true (a batch of instances): the tiles of all instances form one list;
+// tinst[i] gives tile i's instance geometry and vector offsets (dof(), elem()
+// add dbase / ebase), and p.Ap goes to one slot per (workgroup run, instance):
+// each flush sums its tile's share across the flushing waves, and thread 0
+// adds the tile sums in tile order, writing an instance's slot when the run
+// moves on.  The tile types index the concatenated A fragments of all
+// instances.
+template <typename T, bool BF, int W, bool MI = false>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 ? 3 : 4))) void apply_tiles(
     const Tile* __restrict__ tiles, const int* __restrict__ wg_first, const void* __restrict__ kfrag_v,
     const T* __restrict__ x,
     T* __restrict__ y, const T* __restrict__ scale, const uint8_t* __restrict__ freem,
     const int32_t* __restrict__ tmask, int ncp0, int ncp1, int ndof_sc, int nel0, int nel1,
-    double* __restrict__ pap, const PcgState* st) {
+    double* __restrict__ pap, const PcgState* st, const TileInst* __restrict__ tinst = nullptr) {
   __shared__ T xs[2][3 * TILE_MAX_PTS];
   // output points accumulate in FP64 for both precisions: ds_add_f64 runs at
   // full rate, while FP32 LDS atomics back up the LDS data FIFO on gfx950
@@ -277,9 +285,29 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   int first, last;
   tile_range(wg_first, first, last);
   if (first >= last) {
-    if (pap != nullptr && threadIdx.x == 0) pap[blockIdx.x] = 0.0;
+    if (!MI && pap != nullptr && threadIdx.x == 0) pap[blockIdx.x] = 0.0;
     return;
   }
+  // geometry and vector offsets of tile i
+  const auto ctx_of = [&](int i) {
+    TileInst c;
+    if constexpr (MI) {
+      c = tinst[i];
+    } else {
+      c.dbase = 0;
+      c.ebase = 0;
+      c.ncp0 = ncp0;
+      c.ncp1 = ncp1;
+      c.ndof_sc = ndof_sc;
+      c.nel0 = nel0;
+      c.nel1 = nel1;
+      c.slot = 0;
+    }
+    return c;
+  };
+  __shared__ double red2[2][W];  // MI: per-wave p.Ap shares of the last two flushed tiles
+  double mi_acc = 0.0;           // MI, thread 0: the current instance's sum in this run
+  int mi_slot = -1;
 
   const int wave = threadIdx.x >> 6;
   const int mt = apply_mtile<W>(wave);
@@ -294,36 +322,65 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   // The p.Ap partial is the sum over the staged points of (M x)_i times the
   // tile's contribution to (A x)_i, which summed over tiles is x^T M K M x.
   double pap_acc = 0.0;  // this thread's share, tiles in order
-  auto flush = [&](const Tile& tf, bool maskf, int b) {
+  auto flush = [&](const Tile& tf, int tfi, bool maskf, int b) {
     if (!bgw) return;
     TileGeom Gf;
     Gf.init(tf);
+    const TileInst cf = ctx_of(tfi);
+    double pt = 0.0;
 #pragma unroll
     for (int j = 0; j < NPF; ++j) {
       const int idx = tb + j * NBG;
       if (idx < Gf.n3) {
         const double v = ys[b][idx];
         ys[b][idx] = 0.0;
-        if (pap != nullptr) pap_acc += (double)xs[b][idx] * v;
-        const int g = Gf.dof(tf, idx, ncp0, ncp1, ndof_sc);
+        if (pap != nullptr) {
+          if constexpr (MI)
+            pt += (double)xs[b][idx] * v;
+          else
+            pap_acc += (double)xs[b][idx] * v;
+        }
+        const int g = Gf.dof(tf, idx, cf.ncp0, cf.ncp1, cf.ndof_sc) + cf.dbase;
         if (!maskf || freem[g] != 0) unsafeAtomicAdd(&y[g], (T)v);
       }
+    }
+    if constexpr (MI) {
+      for (int off = 32; off > 0; off >>= 1) pt += __shfl_down(pt, off, 64);
+      if (lane == 0) red2[b][wave] = pt;
+    }
+  };
+  // MI, thread 0: adds flushed tile j's share (written before a barrier the
+  // caller has passed) to its instance's slot sum
+  auto collect = [&](int j) {
+    if constexpr (MI) {
+      const int b = (j - first) & 1;
+      double sj = 0.0;
+      for (int w = SPEC ? 6 : 0; w < W; ++w) sj += red2[b][w];
+      const int slot = tinst[j].slot;
+      if (slot != mi_slot) {
+        if (mi_slot >= 0) pap[mi_slot] = mi_acc;
+        mi_acc = 0.0;
+        mi_slot = slot;
+      }
+      mi_acc += sj;
     }
   };
   // loads of tile tl's points and scales into registers (unused slots are not
   // loaded)
   T pv[NPF];
   T psc = T(0);
-  auto load_regs = [&](const Tile& tl) {
+  auto load_regs = [&](int tli) {
     if (!bgw) return;
+    const Tile tl = tiles[tli];
+    const TileInst cl = ctx_of(tli);
     TileGeom Gl;
     Gl.init(tl);
 #pragma unroll
     for (int j = 0; j < NPF; ++j) {
       const int idx = tb + j * NBG;
-      if (idx < Gl.n3) pv[j] = x[Gl.dof(tl, idx, ncp0, ncp1, ndof_sc)];
+      if (idx < Gl.n3) pv[j] = x[Gl.dof(tl, idx, cl.ncp0, cl.ncp1, cl.ndof_sc) + cl.dbase];
     }
-    if (tb < Gl.nel) psc = scale[Gl.elem(tl, tb, nel0, nel1)];
+    if (tb < Gl.nel) psc = scale[Gl.elem(tl, tb, cl.nel0, cl.nel1) + cl.ebase];
   };
 
   // prologue: stage the first tile into buffer 0, clear both output buffers,
@@ -333,17 +390,18 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   G.init(t);
   {
     const bool masked = tmask != nullptr && tmask[first] != 0;
+    const TileInst c0 = ctx_of(first);
     for (int idx = threadIdx.x; idx < 3 * TILE_MAX_PTS; idx += NT) {
       ys[0][idx] = 0.0;
       ys[1][idx] = 0.0;
       if (idx < 16) claim[idx >> 3][idx & 7] = 0;
       if (idx < G.n3) {
-        const int g = G.dof(t, idx, ncp0, ncp1, ndof_sc);
+        const int g = G.dof(t, idx, c0.ncp0, c0.ncp1, c0.ndof_sc) + c0.dbase;
         xs[0][idx] = (!masked || freem[g] != 0) ? x[g] : T(0);
       }
     }
+    if (threadIdx.x < G.nel) ss[0][threadIdx.x] = scale[G.elem(t, threadIdx.x, c0.nel0, c0.nel1) + c0.ebase];
   }
-  if (threadIdx.x < G.nel) ss[0][threadIdx.x] = scale[G.elem(t, threadIdx.x, nel0, nel1)];
   offset_table<BF>(G, otab[0], threadIdx.x);
 
   // The tile pipeline, three deep.  At the top of tile i the background
@@ -357,7 +415,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   // their first use they sit in the scalar cache, and no descriptor has to
   // be carried (and spilled) across tiles.
   const auto mask_at = [&](int i) { return tmask != nullptr && tmask[i] != 0; };
-  if (first + 1 < last) load_regs(tiles[first + 1]);
+  if (first + 1 < last) load_regs(first + 1);
   __syncthreads();  // buffer 0 staged, output buffers clear
 
   T a[BF ? 1 : KS];
@@ -392,23 +450,26 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
       }
       cur_type = t.type;
     }
-    if (ti > first) flush(tiles[ti - 1], mask_at(ti - 1), buf ^ 1);
+    if (MI && threadIdx.x == 0 && ti - 2 >= first) collect(ti - 2);
+    if (ti > first) flush(tiles[ti - 1], ti - 1, mask_at(ti - 1), buf ^ 1);
     if (bgw && tb < 8) claim[buf ^ 1][tb] = 0;
     if (bgw && ti + 1 < last) {  // stage tile i + 1 into buffer buf ^ 1
       const Tile tn = tiles[ti + 1];
       const bool mask_n = mask_at(ti + 1);
+      const TileInst cn = ctx_of(ti + 1);
       TileGeom Gn;
       Gn.init(tn);
 #pragma unroll
       for (int j = 0; j < NPF; ++j) {
         const int idx = tb + j * NBG;
         if (idx < Gn.n3)
-          xs[buf ^ 1][idx] = (!mask_n || freem[Gn.dof(tn, idx, ncp0, ncp1, ndof_sc)] != 0) ? pv[j] : T(0);
+          xs[buf ^ 1][idx] =
+              (!mask_n || freem[Gn.dof(tn, idx, cn.ncp0, cn.ncp1, cn.ndof_sc) + cn.dbase] != 0) ? pv[j] : T(0);
       }
       if (tb < Gn.nel) ss[buf ^ 1][tb] = psc;
       offset_table<BF>(Gn, otab[buf ^ 1], tb);
     }
-    if (ti + 2 < last) load_regs(tiles[ti + 2]);
+    if (ti + 2 < last) load_regs(ti + 2);
 
     const int px = G.px, py = G.py, nel = G.nel;
     const bool r3 = perm_row(mt, kq, 3) >= 0;
@@ -559,7 +620,16 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     __syncthreads();  // tile i's output complete; buffer buf ^ 1 staged
     buf ^= 1;
   }
-  flush(tiles[last - 1], mask_at(last - 1), buf ^ 1);  // the last tile
+  flush(tiles[last - 1], last - 1, mask_at(last - 1), buf ^ 1);  // the last tile
+  if constexpr (MI) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (last - 2 >= first) collect(last - 2);
+      collect(last - 1);
+      pap[mi_slot] = mi_acc;
+    }
+    return;
+  }
   if (pap != nullptr) {
     // wave sums, then the waves in fixed order
     for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
@@ -754,6 +824,20 @@ hipError_t launch_apply_bf16(const icc_grid* g, const icc_op* op, const float* x
   hipLaunchKernelGGL((apply_tiles<float, true, 8>), dim3(apply_grid(g)), dim3(64 * 8), 0, st, g->d_tiles,
                      g->d_wgfirst, (const void*)g->d_kfragb, x, y, op->d_scale32, op->d_free, op->d_tmask,
                      (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0], (int)s.nel[1], pap, pst);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_batch_f64(const BatchApply& b, const double* x, double* y, const double* scale,
+                                  const uint8_t* freem, double* pap, hipStream_t st) {
+  hipLaunchKernelGGL((apply_tiles<double, false, 12, true>), dim3(b.G), dim3(64 * 12), 0, st, b.tiles, b.wg_first,
+                     (const void*)b.kfrag64, x, y, scale, freem, b.tmask, 0, 0, 0, 0, 0, pap, nullptr, b.tinst);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_batch_f32(const BatchApply& b, const float* x, float* y, const float* scale,
+                                  const uint8_t* freem, double* pap, hipStream_t st) {
+  hipLaunchKernelGGL((apply_tiles<float, false, 8, true>), dim3(b.G), dim3(64 * 8), 0, st, b.tiles, b.wg_first,
+                     (const void*)b.kfrag32, x, y, scale, freem, b.tmask, 0, 0, 0, 0, 0, pap, nullptr, b.tinst);
   return hipGetLastError();
 }
 

@@ -169,14 +169,17 @@ __device__ __forceinline__ void st_run(T* __restrict__ a, int64_t e, int64_t n, 
 // alpha = rz / p.Ap (p.Ap summed from the apply's partials), x += alpha p,
 // r -= alpha Ap, per-block partials of r.r and r.D^-1 r.  Only block 0 writes
 // the state (it_cur, breakdown / limit flags).
+// The bodies take the block's index lb among the nb blocks of its vector
+// (blockIdx.x / gridDim.x for one instance; the block's place in its
+// instance's segment for a batch).
 template <typename T>
-__global__ __launch_bounds__(VB) void pcg_update1(int64_t n, PcgState* st, const double* __restrict__ pap_parts,
-                                                  int npap, const T* __restrict__ p, const T* __restrict__ ap,
-                                                  T* __restrict__ x, T* __restrict__ r, const T* __restrict__ dinv,
-                                                  double* __restrict__ parts) {
-  __shared__ double sh[16];
+__device__ __forceinline__ void pcg_update1_body(int lb, int nb, int64_t n, PcgState* st,
+                                                 const double* __restrict__ pap_parts, int npap,
+                                                 const T* __restrict__ p, const T* __restrict__ ap,
+                                                 T* __restrict__ x, T* __restrict__ r, const T* __restrict__ dinv,
+                                                 double* __restrict__ parts, double* sh) {
   constexpr int V = 16 / (int)sizeof(T);
-  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gs = (int64_t)nb * blockDim.x;
   Run<T> pv[RU], av[RU], xv[RU], rv[RU], dv[RU];
   auto load_step = [&](int64_t v) {
 #pragma unroll
@@ -194,17 +197,17 @@ __global__ __launch_bounds__(VB) void pcg_update1(int64_t n, PcgState* st, const
   const int32_t done = st->done;
   const int it = st->iters;
   const int maxit = st->maxit;
-  int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t v = lb * (int64_t)blockDim.x + threadIdx.x;
   // the first step's loads overlap the state checks and the p.Ap sum
   load_step(v);
   if (done) return;
   if (it >= maxit) {  // iteration limit
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 3;
+    if (lb == 0 && threadIdx.x == 0) st->done = 3;
     return;
   }
   const double pap = sum_parts_all(pap_parts, npap, sh);
   if (!(pap > 0.0)) {  // breakdown (solver.rs:47-49): stop before updating x
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (lb == 0 && threadIdx.x == 0) {
       st->pap = pap;
       st->it_cur = it;
       st->iters = it + 1;  // the reference counts the breaking iteration (final_iter = iter + 1, solver.rs:41)
@@ -236,25 +239,34 @@ __global__ __launch_bounds__(VB) void pcg_update1(int64_t n, PcgState* st, const
   double s1 = block_sum(rr, sh);
   double s2 = block_sum(rz, sh);
   if (threadIdx.x == 0) {
-    parts[blockIdx.x] = s1;
-    parts[gridDim.x + blockIdx.x] = s2;
-    if (blockIdx.x == 0) {
+    parts[lb] = s1;
+    parts[nb + lb] = s2;
+    if (lb == 0) {
       st->pap = pap;
       st->it_cur = it;
     }
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(VB) void pcg_update1(int64_t n, PcgState* st, const double* __restrict__ pap_parts,
+                                                  int npap, const T* __restrict__ p, const T* __restrict__ ap,
+                                                  T* __restrict__ x, T* __restrict__ r, const T* __restrict__ dinv,
+                                                  double* __restrict__ parts) {
+  __shared__ double sh[16];
+  pcg_update1_body<T>(blockIdx.x, gridDim.x, n, st, pap_parts, npap, p, ap, x, r, dinv, parts, sh);
+}
+
 // Convergence test on |r| / norm_b (solver.rs:57-61), then
 // beta = rz_new / rz_old, p = D^-1 r + beta p, ap = 0.  Block 0 writes the
 // state (iters, rz_new, res, done).
 template <typename T>
-__global__ __launch_bounds__(VB) void pcg_update2(int64_t n, PcgState* st, const double* __restrict__ parts,
-                                                  T* __restrict__ p, const T* __restrict__ r,
-                                                  const T* __restrict__ dinv, T* __restrict__ ap) {
-  __shared__ double sh[16];
+__device__ __forceinline__ void pcg_update2_body(int lb, int nb, int64_t n, PcgState* st,
+                                                 const double* __restrict__ parts, T* __restrict__ p,
+                                                 const T* __restrict__ r, const T* __restrict__ dinv,
+                                                 T* __restrict__ ap, double* sh) {
   constexpr int V = 16 / (int)sizeof(T);
-  const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gs = (int64_t)nb * blockDim.x;
   Run<T> pv[RU], rv[RU], dv[RU];
   auto load_step = [&](int64_t v) {
 #pragma unroll
@@ -269,15 +281,15 @@ __global__ __launch_bounds__(VB) void pcg_update2(int64_t n, PcgState* st, const
   };
   const int32_t done = st->done;
   const int it = st->it_cur;
-  int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t v = lb * (int64_t)blockDim.x + threadIdx.x;
   // the first step's loads overlap the state check and the sums
   load_step(v);
   if (done) return;
-  const double rr = sum_parts_all(parts, gridDim.x, sh);
-  const double rz = sum_parts_all(parts + gridDim.x, gridDim.x, sh);
+  const double rr = sum_parts_all(parts, nb, sh);
+  const double rz = sum_parts_all(parts + nb, nb, sh);
   const double res = sqrt(rr) / st->norm_b;
   const bool conv = res < st->tol;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (lb == 0 && threadIdx.x == 0) {
     st->rr = rr;
     st->res = res;
     st->rz[(it + 1) & 1] = rz;
@@ -302,6 +314,42 @@ __global__ __launch_bounds__(VB) void pcg_update2(int64_t n, PcgState* st, const
     }
     if ((v + RU * gs) * V < n) load_step(v + RU * gs);
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(VB) void pcg_update2(int64_t n, PcgState* st, const double* __restrict__ parts,
+                                                  T* __restrict__ p, const T* __restrict__ r,
+                                                  const T* __restrict__ dinv, T* __restrict__ ap) {
+  __shared__ double sh[16];
+  pcg_update2_body<T>(blockIdx.x, gridDim.x, n, st, parts, p, r, dinv, ap, sh);
+}
+
+// Batched updates: block b works on instance seg_of[b]'s segment (its
+// vectors at seg.off, its blocks seg.blk0 .. seg.blk0 + seg.nblk, its state,
+// its p.Ap slots seg.pap0 .. seg.pap0 + seg.npap and partials at 2 seg.blk0).
+template <typename T>
+__global__ __launch_bounds__(VB) void pcg_update1_batch(const VecSeg* __restrict__ segs,
+                                                        const int32_t* __restrict__ seg_of, PcgState* st,
+                                                        const double* __restrict__ pap, const T* __restrict__ p,
+                                                        const T* __restrict__ ap, T* __restrict__ x, T* __restrict__ r,
+                                                        const T* __restrict__ dinv, double* __restrict__ parts) {
+  __shared__ double sh[16];
+  const int k = seg_of[blockIdx.x];
+  const VecSeg sg = segs[k];
+  pcg_update1_body<T>(blockIdx.x - sg.blk0, sg.nblk, sg.n, st + k, pap + sg.pap0, sg.npap, p + sg.off, ap + sg.off,
+                      x + sg.off, r + sg.off, dinv + sg.off, parts + 2 * sg.blk0, sh);
+}
+template <typename T>
+__global__ __launch_bounds__(VB) void pcg_update2_batch(const VecSeg* __restrict__ segs,
+                                                        const int32_t* __restrict__ seg_of, PcgState* st,
+                                                        const double* __restrict__ parts, T* __restrict__ p,
+                                                        const T* __restrict__ r, const T* __restrict__ dinv,
+                                                        T* __restrict__ ap) {
+  __shared__ double sh[16];
+  const int k = seg_of[blockIdx.x];
+  const VecSeg sg = segs[k];
+  pcg_update2_body<T>(blockIdx.x - sg.blk0, sg.nblk, sg.n, st + k, parts + 2 * sg.blk0, p + sg.off, r + sg.off,
+                      dinv + sg.off, ap + sg.off, sh);
 }
 
 // p = D^-1 r + beta p, beta = rz[iters & 1] / rz[(iters - 1) & 1], ap = 0:
@@ -567,6 +615,19 @@ hipError_t launch_pcg_update(int64_t n, PcgState* st, const double* pap_parts, i
 }
 template hipError_t launch_pcg_update<double>(int64_t, PcgState*, const double*, int, double*, double*, double*,
                                               double*, const double*, double*, hipStream_t);
+
+template <typename T>
+hipError_t launch_pcg_update_batch(int nblk, const VecSeg* segs, const int32_t* seg_of, PcgState* st, const double* pap,
+                                   T* p, T* ap, T* x, T* r, const T* dinv, double* parts, hipStream_t s) {
+  hipLaunchKernelGGL((pcg_update1_batch<T>), dim3(nblk), dim3(VB), 0, s, segs, seg_of, st, pap, p, ap, x, r, dinv, parts);
+  hipLaunchKernelGGL((pcg_update2_batch<T>), dim3(nblk), dim3(VB), 0, s, segs, seg_of, st, parts, p, r, dinv, ap);
+  return hipGetLastError();
+}
+template hipError_t launch_pcg_update_batch<double>(int, const VecSeg*, const int32_t*, PcgState*, const double*,
+                                                    double*, double*, double*, double*, const double*, double*,
+                                                    hipStream_t);
+template hipError_t launch_pcg_update_batch<float>(int, const VecSeg*, const int32_t*, PcgState*, const double*,
+                                                   float*, float*, float*, float*, const float*, double*, hipStream_t);
 template hipError_t launch_pcg_update<float>(int64_t, PcgState*, const double*, int, float*, float*, float*, float*,
                                              const float*, double*, hipStream_t);
 

@@ -410,6 +410,8 @@ static double tile_cost(const Tile& t) {
   return (double)((groups + 1) / 2) + pts / 300.0;
 }
 
+double apply_tile_cost(const Tile& t) { return tile_cost(t); }
+
 // One pass of balance_tiles for an assumed total work `total`: workgroup b
 // fills up to the cumulative target (b + 1) total / G, and the tile that
 // crosses it is taken whole, left for the next workgroup, or split at the
