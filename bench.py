@@ -91,6 +91,14 @@ def time_applies(torch, icc, ctx, op, x, y, prec, steps, warmup, accumulate=Fals
     return e0.elapsed_time(e1) / steps  # ms
 
 
+def op_apply_atomic(op, x):
+    """The operator's apply with the atomic tile scatter (deterministic mode off)."""
+    op.set_deterministic(False)
+    y = op.apply(x)
+    op.set_deterministic(True)
+    return y
+
+
 def pcg_c0(np, torch, icc, ctx):
     """FP64 Jacobi PCG to 1e-8 on the 16x8x8 grid (rho = 0.5, penal 3,
     Emin 1e-9, x = 0 clamped, unit downward traction on x = 2)."""
@@ -546,6 +554,17 @@ def main():
     if args.apply_only:
         print(json.dumps(out), flush=True)
         return
+
+    log("bench: deterministic FP64 applies")
+    op.set_deterministic(True)
+    ms_det = time_applies(torch, icc, ctx, op, x, y, icc.ICC_F64, args.steps, args.warmup)
+    yd = op.apply(xh)
+    out["apply_f64_deterministic"] = {
+        "applies_per_s": 1e3 / ms_det, "ms": ms_det,
+        "rel_diff_vs_atomic": float(np.linalg.norm(yd - op_apply_atomic(op, xh)) / np.linalg.norm(yd)),
+        "bitwise_repeatable": bool(np.array_equal(yd, op.apply(xh))),
+        "element_output_bytes": int(8 * 81 * nel)}
+    op.set_deterministic(False)
 
     log("bench: FP32 applies")
     tuned["f32"] = op.autotune(icc.ICC_F32, reps=5)

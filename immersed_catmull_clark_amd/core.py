@@ -173,6 +173,18 @@ class Operator:
         dd = _f64(data)
         check(lib().icc_op_set_sparse(self.h, len(ip) - 1, _p(ip), _p(ix), _p(dd)))
 
+    def set_deterministic(self, on=True):
+        """Bitwise-repeatable FP64 / FP32 applies and solves (per-element
+        outputs and a fixed-order gather instead of the atomic scatter)."""
+        check(lib().icc_op_set_deterministic(self.h, 1 if on else 0))
+
+    def add_triplets(self, rows, cols, vals):
+        """S = S + T (CsrMatrix::plus_triplets, sparse.rs:98-113)."""
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        v = _f64(vals)
+        check(lib().icc_op_add_triplets(self.h, len(v), _p(r), _p(c), _p(v)))
+
     def set_legacy_stabilization(self, status, gamma):
         """Legacy cut-cell stabilisation (immersed.rs:168-222) as a matrix-free
         face term; status None removes it.  Returns the number of faces."""

@@ -489,8 +489,11 @@ static void drop_work(icc_op* op) {
 }
 
 static int csr_blocks(const icc_op* op) { return op->s_nnz > 0 ? vec_blocks(op->g->s.ndof) : 0; }
-// p.Ap partial layout of one apply: [apply_grid | csr_blocks | face_blocks]
-static int npap_of(const icc_op* op) { return apply_grid(op->g) + csr_blocks(op) + face_blocks(op); }
+// partials of the element term: one per apply workgroup, or per gather block
+// in the deterministic mode
+static int elem_grid(const icc_op* op) { return op->det ? det_grid(op->g) : apply_grid(op->g); }
+// p.Ap partial layout of one apply: [elem_grid | csr_blocks | face_blocks]
+static int npap_of(const icc_op* op) { return elem_grid(op) + csr_blocks(op) + face_blocks(op); }
 
 // Lazily allocates the solver workspace of op.
 static int ensure_work(icc_op* op, Work*& out) {
@@ -542,6 +545,7 @@ ICC_API int icc_op_destroy(icc_op* op) {
   hipFree(op->d_s_data);
   hipFree(op->d_s_data32);
   hipFree(op->d_s_diag);
+  hipFree(op->d_E);
   hipFree(op->d_faces);
   hipFree(op->d_f_diag);
   hipFree(op->d_ftab);
@@ -628,6 +632,30 @@ ICC_API int icc_op_set_free_mask(icc_op* op, const uint8_t* fm) {
   if (!op->d_free) ICC_TRY(dalloc(&op->d_free, n));
   ICC_HIP(hipMemcpyAsync(op->d_free, fm, n, hipMemcpyHostToDevice, op->g->ctx->stream));
   return upload_tile_mask(op);
+}
+
+// Deterministic mode: the element term of every FP64 / FP32 apply of op (and
+// of its PCG iterations) runs as per-element outputs plus a fixed-order
+// gather instead of the atomic tile scatter, so applies and solves repeat
+// bit for bit.  Needs nel * 81 doubles of device memory; the face
+// stabilisation (atomic) is not covered.
+ICC_API int icc_op_set_deterministic(icc_op* op, int on) {
+  if (!op || (on != 0 && on != 1)) return ICC_EINVAL;
+  if (on && op->f_n > 0) {
+    set_error("icc_op_set_deterministic: the face stabilisation has no deterministic mode");
+    return ICC_ENOTSUP;
+  }
+  if (on && op->d_E == nullptr) {
+    double* e = nullptr;
+    ICC_TRY(dalloc(&e, (size_t)op->g->s.nel_tot * NSH));
+    op->d_E = e;
+  }
+  if (!on) {
+    (void)hipFree(op->d_E);
+    op->d_E = nullptr;
+  }
+  op->det = on;
+  return ICC_OK;
 }
 
 ICC_API int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu) {
@@ -764,6 +792,10 @@ static int stab_faces(const GridSetup& s, const int8_t* status, std::vector<int3
 // the face range of each block.
 static int upload_faces(icc_op* op, int kind, double coef, const std::vector<int32_t>& faces_in,
                         const std::vector<double>& dg) {
+  if (op->det && !faces_in.empty()) {
+    set_error("the face stabilisation has no deterministic mode (icc_op_set_deterministic)");
+    return ICC_ENOTSUP;
+  }
   const GridSetup& s = op->g->s;
   const int64_t nbx = (s.nel[0] + FACE_BLK - 1) / FACE_BLK, nby = (s.nel[1] + FACE_BLK - 1) / FACE_BLK;
   auto block_of = [&](int32_t code) {
@@ -992,15 +1024,16 @@ static hipError_t apply_t(icc_op* op, const T* x, T* y, double* pap) {
   if (op->s_nnz > 0) {
     if constexpr (std::is_same<T, double>::value)
       e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, x, y, op->d_free,
-                                          pap ? pap + apply_grid(g) : nullptr, st);
+                                          pap ? pap + elem_grid(op) : nullptr, st);
     else
       e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, x, y, op->d_free,
-                                        pap ? pap + apply_grid(g) : nullptr, st);
+                                        pap ? pap + elem_grid(op) : nullptr, st);
   } else {
     e = launch_fill<T>(y, T(0), n, st);
   }
-  if (e == hipSuccess) e = launch_faces<T>(op, x, y, pap ? pap + apply_grid(g) + csr_blocks(op) : nullptr, nullptr, st);
+  if (e == hipSuccess) e = launch_faces<T>(op, x, y, pap ? pap + elem_grid(op) + csr_blocks(op) : nullptr, nullptr, st);
   if (e != hipSuccess) return e;
+  if (op->det) return launch_apply_det<T>(g, op, x, y, pap, nullptr, (T*)op->d_E, st);
   if constexpr (std::is_same<T, double>::value)
     return launch_apply_f64(g, op, x, y, pap, nullptr, st);
   else
@@ -1149,16 +1182,19 @@ static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap
   if constexpr (std::is_same<T, double>::value) {
     if (op->s_nnz > 0)
       e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, p, ap, op->d_free,
-                                          w->pap + apply_grid(g), st);
-    if (e == hipSuccess) e = launch_faces<double>(op, p, ap, w->pap + apply_grid(g) + csr_blocks(op), w->st, st);
-    if (e == hipSuccess) e = launch_apply_f64(g, op, p, ap, w->pap, w->st, st);
+                                          w->pap + elem_grid(op), st);
+    if (e == hipSuccess) e = launch_faces<double>(op, p, ap, w->pap + elem_grid(op) + csr_blocks(op), w->st, st);
+    if (e == hipSuccess)
+      e = op->det ? launch_apply_det<double>(g, op, p, ap, w->pap, w->st, (double*)op->d_E, st)
+                  : launch_apply_f64(g, op, p, ap, w->pap, w->st, st);
   } else {
     if (op->s_nnz > 0)
       e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, p, ap, op->d_free,
-                                        w->pap + apply_grid(g), st);
-    if (e == hipSuccess) e = launch_faces<float>(op, p, ap, w->pap + apply_grid(g) + csr_blocks(op), w->st, st);
+                                        w->pap + elem_grid(op), st);
+    if (e == hipSuccess) e = launch_faces<float>(op, p, ap, w->pap + elem_grid(op) + csr_blocks(op), w->st, st);
     if (e == hipSuccess)
-      e = prec == ICC_F32 ? launch_apply_f32(g, op, p, ap, w->pap, w->st, st)
+      e = prec == ICC_F32 ? (op->det ? launch_apply_det<float>(g, op, p, ap, w->pap, w->st, (float*)op->d_E, st)
+                                     : launch_apply_f32(g, op, p, ap, w->pap, w->st, st))
                           : launch_apply_lowp(prec, g, op, p, ap, w->pap, w->st, st);
   }
   if (e == hipSuccess) e = launch_pcg_update<T>(n, w->st, w->pap, npap, p, ap, x, r, dinv, w->parts, st);
@@ -1356,6 +1392,10 @@ ICC_API int icc_solve_ir_dev(icc_op* op, int inner, const double* f, double tol,
     set_error("icc_solve_ir: unknown inner precision");
     return ICC_EINVAL;
   }
+  if (op->det && inner != ICC_F64 && inner != ICC_F32) {
+    set_error("icc_solve_ir: the deterministic mode covers the FP64 and FP32 operators only");
+    return ICC_EINVAL;
+  }
   Work* w;
   ICC_TRY(ensure_work(op, w));
   icc_grid* g = op->g;
@@ -1448,6 +1488,10 @@ ICC_API int icc_solve_mixed_prec_dev(icc_op* op, int inner, const double* f, dou
   if (!op || !f || !u || !(delta > 0.0 && delta < 1.0)) return ICC_EINVAL;
   if (inner != ICC_F32 && inner != ICC_BF16 && inner != ICC_F16X) {
     set_error("icc_solve_mixed: the inner precision must be ICC_F32, ICC_BF16 or ICC_F16X");
+    return ICC_EINVAL;
+  }
+  if (op->det && inner != ICC_F32) {
+    set_error("icc_solve_mixed: the deterministic mode covers the FP64 and FP32 operators only");
     return ICC_EINVAL;
   }
   Work* w;

@@ -225,6 +225,9 @@ struct icc_op {
   int ftab_joff[3] = {0, 0, 0}, ftab_moff[3] = {0, 0, 0}, ftab_mass = 0;
   std::vector<int32_t> h_faces;  // host copies (Frobenius norm of the face term)
   std::vector<double> h_ftab;
+  // deterministic mode: element outputs [nel * 81] of the two-pass apply
+  int det = 0;
+  void* d_E = nullptr;
   // scratch
   double* d_partials = nullptr;  // reduction partials
   int partials_cap = 0;

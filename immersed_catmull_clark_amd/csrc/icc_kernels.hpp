@@ -60,6 +60,12 @@ hipError_t launch_apply_batch_f64(const BatchApply& b, const double* x, double* 
                                   const uint8_t* freem, double* pap, hipStream_t st);
 hipError_t launch_apply_batch_f32(const BatchApply& b, const float* x, float* y, const float* scale,
                                   const uint8_t* freem, double* pap, hipStream_t st);
+// Deterministic apply (opt-in): y += M A M x by per-element outputs E
+// [nel * 81] and a fixed-order gather; pap gets det_grid(g) partials
+int det_grid(const icc_grid* g);
+template <typename T>
+hipError_t launch_apply_det(const icc_grid* g, const icc_op* op, const T* x, T* y, double* pap, const PcgState* pst,
+                            T* E, hipStream_t st);
 hipError_t launch_energies(const icc_grid* g, const double* u, double* ee, const double* x, double penal,
                            double emin, const double* w, double* dc, hipStream_t st);
 hipError_t launch_diag_f64(const icc_grid* g, const double* scale, double* d, hipStream_t st);

@@ -24,7 +24,9 @@
 // icc_debug_mfma_layout pins both on the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "icc_internal.hpp"
 #include "icc_kernels.hpp"
@@ -720,6 +722,117 @@ __global__ __launch_bounds__(384) void energies_tiles(const Tile* __restrict__ t
   }
 }
 
+// Deterministic apply, pass 1: every element's 81 outputs s_e K_t x_e
+// (x masked) into E[row][e] (row-major by output slot, so that the gather's
+// neighbouring control points read neighbouring elements), one workgroup of
+// six waves (one per M tile) per tile.  Each entry is written once, so the
+// result does not depend on the order in which the workgroups run.
+template <typename T>
+__global__ __launch_bounds__(384) void det_elem_kernel(const Tile* __restrict__ tiles, const void* __restrict__ kfrag_v,
+                                                       const T* __restrict__ x, const T* __restrict__ scale,
+                                                       const uint8_t* __restrict__ freem,
+                                                       const int32_t* __restrict__ tmask, int ncp0, int ncp1,
+                                                       int ndof_sc, int nel0, int nel1, int64_t nel_tot,
+                                                       T* __restrict__ E, const PcgState* st) {
+  if (st != nullptr && st->done != 0) return;
+  __shared__ T xs[3 * TILE_MAX_PTS];
+  const Tile t = tiles[blockIdx.x];
+  const int px = t.nx + 2, py = t.ny + 2, pz = t.nz + 2;
+  const int npts = px * py * pz;
+  const int nel = t.nx * t.ny * t.nz;
+  const bool masked = tmask != nullptr && tmask[blockIdx.x] != 0;
+  stage_tile<T>(t, x, masked ? freem : nullptr, ncp0, ncp1, ndof_sc, xs, nullptr, px, py, npts);
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15;
+  const int kq = lane >> 4;
+  T a[KS];
+  int boff[KS];
+  const T* kf = (const T*)kfrag_v + ((size_t)(t.type * MT + wave) * KS) * 64 + lane;
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    a[ks] = kf[ks * 64];
+    const int b = 4 * ks + kq;
+    boff[ks] = b < NSH ? local_offset(b, px, py, npts) : 0;
+  }
+  int rowi[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) rowi[r] = perm_row(wave, kq, r);
+  __syncthreads();
+  for (int g0 = 0; g0 < nel; g0 += 16) {
+    const int el = g0 + col;
+    const bool valid = el < nel;
+    int ex = 0, ey = 0, ez = 0;
+    if (valid) {
+      ex = el % t.nx;
+      const int rr = el / t.nx;
+      ey = rr % t.ny;
+      ez = rr / t.ny;
+    }
+    const int eb = ex + px * (ey + py * ez);
+    typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) acc = Mfma<T>::run(a[ks], xs[eb + boff[ks]], acc);
+    if (valid) {
+      const int64_t ge = (t.i0 + ex) + (int64_t)nel0 * ((t.j0 + ey) + (int64_t)nel1 * (t.k0 + ez));
+      const T sc = scale[ge];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (rowi[r] >= 0) E[rowi[r] * nel_tot + ge] = acc[r] * sc;
+    }
+  }
+}
+
+// Deterministic apply, pass 2: y(c, P) += sum over the <= 27 elements around
+// control point P, in a fixed order, of E[(c, a)][e] (masked), accumulated in
+// FP64 and rounded once; pap[block] = the block's share of (M x).y in a fixed
+// tree.
+template <typename T>
+__global__ __launch_bounds__(256) void det_gather_kernel(const T* __restrict__ E, const T* __restrict__ x,
+                                                         T* __restrict__ y, const uint8_t* __restrict__ freem,
+                                                         int ncp0, int ncp1, int ncp2, int nel0, int nel1, int nel2,
+                                                         double* __restrict__ pap, const PcgState* st) {
+  if (st != nullptr && st->done != 0) return;
+  __shared__ double sh[4];
+  const int64_t ndof_sc = (int64_t)ncp0 * ncp1 * ncp2;
+  const int64_t nel_tot = (int64_t)nel0 * nel1 * nel2;
+  double part = 0.0;
+  for (int64_t P = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; P < ndof_sc; P += (int64_t)gridDim.x * blockDim.x) {
+    const int pi = (int)(P % ncp0), pj = (int)((P / ncp0) % ncp1), pk = (int)(P / ((int64_t)ncp0 * ncp1));
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int az = 2; az >= 0; --az) {
+      const int ek = pk - az;
+      if (ek < 0 || ek >= nel2) continue;
+      for (int ay = 2; ay >= 0; --ay) {
+        const int ej = pj - ay;
+        if (ej < 0 || ej >= nel1) continue;
+        for (int ax = 2; ax >= 0; --ax) {
+          const int ei = pi - ax;
+          if (ei < 0 || ei >= nel0) continue;
+          const T* Ee = E + (int64_t)(ax + 3 * ay + 9 * az) * nel_tot + (ei + (int64_t)nel0 * (ej + (int64_t)nel1 * ek));
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[c] += (double)Ee[c * NSC * nel_tot];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t gi = c * ndof_sc + P;
+      const bool fr = freem == nullptr || freem[gi] != 0;
+      const T v = fr ? (T)acc[c] : T(0);
+      y[gi] += v;
+      if (fr) part += (double)x[gi] * (double)v;
+    }
+  }
+  if (pap != nullptr) {
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) pap[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  }
+}
+
 // Jacobi diagonal by gathering over the <= 27 elements around each control
 // point: d(c, P) = sum_{e ni P} s_e K_t(e)[(c,a), (c,a)], a = P - corner(e).
 // Deterministic (fixed element order), no atomics.
@@ -840,6 +953,37 @@ hipError_t launch_apply_batch_f32(const BatchApply& b, const float* x, float* y,
                      (const void*)b.kfrag32, x, y, scale, freem, b.tmask, 0, 0, 0, 0, 0, pap, nullptr, b.tinst);
   return hipGetLastError();
 }
+
+int det_grid(const icc_grid* g) {
+  const int64_t nb = (g->s.ndof_sc + 255) / 256;
+  const int cap = APPLY_MAX_WG_PER_CU * (g->ctx->num_cu > 0 ? g->ctx->num_cu : 256);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(nb, cap));
+}
+
+template <typename T>
+hipError_t launch_apply_det(const icc_grid* g, const icc_op* op, const T* x, T* y, double* pap, const PcgState* pst,
+                            T* E, hipStream_t st) {
+  const auto& s = g->s;
+  const void* kf;
+  const T* sc;
+  if constexpr (std::is_same<T, double>::value) {
+    kf = g->d_kfrag64;
+    sc = op->d_scale;
+  } else {
+    kf = g->d_kfrag32;
+    sc = op->d_scale32;
+  }
+  hipLaunchKernelGGL((det_elem_kernel<T>), dim3(g->ntiles), dim3(384), 0, st, g->d_tiles, kf, x, sc, op->d_free,
+                     op->d_tmask, (int)s.ncp[0], (int)s.ncp[1], (int)s.ndof_sc, (int)s.nel[0], (int)s.nel[1],
+                     (int64_t)s.nel_tot, E, pst);
+  hipLaunchKernelGGL((det_gather_kernel<T>), dim3(det_grid(g)), dim3(256), 0, st, E, x, y, op->d_free, (int)s.ncp[0],
+                     (int)s.ncp[1], (int)s.ncp[2], (int)s.nel[0], (int)s.nel[1], (int)s.nel[2], pap, pst);
+  return hipGetLastError();
+}
+template hipError_t launch_apply_det<double>(const icc_grid*, const icc_op*, const double*, double*, double*,
+                                             const PcgState*, double*, hipStream_t);
+template hipError_t launch_apply_det<float>(const icc_grid*, const icc_op*, const float*, float*, double*,
+                                            const PcgState*, float*, hipStream_t);
 
 hipError_t launch_energies(const icc_grid* g, const double* u, double* ee, const double* x, double penal,
                            double emin, const double* w, double* dc, hipStream_t st) {

@@ -118,6 +118,14 @@ int icc_op_set_free_mask(icc_op* op, const uint8_t* free_mask);
 /* adds a sparse term S (CSR, ndof x ndof, FP64) to the operator: stabilisation
  * or penalty surface terms (topopt_immersed.rs:174-183, immersed.rs:542-547) */
 int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, const int64_t* indices, const double* data);
+/* on = 1: the element term of every FP64 / FP32 apply of op, and of its PCG
+ * iterations, runs as per-element outputs (nel x 81, device memory held by
+ * op) and a fixed-order gather instead of the atomic tile scatter, so
+ * applies and solves repeat bit for bit, like the reference's row-parallel
+ * CSR product (rust/src/sparse.rs:60-68); about 1.6x the time of the atomic
+ * apply (149 against 94 us at 96x48x48).  ICC_ENOTSUP with the face stabilisation; the BF16 and split-FP16
+ * operators are not covered (ICC_EINVAL in the solvers).  on = 0 frees it. */
+int icc_op_set_deterministic(icc_op* op, int on);
 /* S = S + T for triplets T (rows, cols, vals; duplicates summed): replaces
  * CsrMatrix::plus_triplets (rust/src/sparse.rs:98-113) on the sparse term */
 int icc_op_add_triplets(icc_op* op, int64_t n, const int64_t* rows, const int64_t* cols, const double* vals);
