@@ -462,6 +462,8 @@ def main():
     ap.add_argument("--tiling", default=None, metavar="V,W",
                     help="fixed tiling variant and workgroups per CU for the apply legs (skips the autotuner; "
                          "for profiling the configuration a default run picked)")
+    ap.add_argument("--split-groups", type=int, default=0,
+                    help="element groups per split-FP16 tile (1..16; 0 keeps the default)")
     ap.add_argument("--wq-only", action="store_true",
                     help="run only the WQ formation leg on the 96x48x48 grid and exit")
     ap.add_argument("--imm-only", action="store_true",
@@ -524,6 +526,8 @@ def main():
         tuned = {"f64": (fixed[0], fixed[1], None)}
     else:
         tuned = {"f64": op.autotune(icc.ICC_F64, reps=5)}
+    if args.split_groups:
+        grid.set_split_tiling(args.split_groups)
     if args.apply_only and args.apply_prec in ("f32", "bf16", "f16x"):
         pr = {"f32": icc.ICC_F32, "bf16": icc.ICC_BF16, "f16x": icc.ICC_F16X}[args.apply_prec]
         if not fixed and pr != icc.ICC_F16X:

@@ -124,6 +124,7 @@ SIGNATURES = [
     ("icc_immersed_mesh_add_surface", cint, [vp, vp]),
     ("icc_op_add_triplets", cint, [vp, i64, vp, vp, vp]),
     ("icc_op_set_deterministic", cint, [vp, cint]),
+    ("icc_grid_set_split_tiling", cint, [vp, cint]),
     ("icc_surface_quadrature", cint, [vp, i64, vp, i64, vp, i64, dbl, i64, vp, vp]),
     ("icc_scalar_basis_at", cint, [vp, vp, vp, vp]),
     ("icc_surface_create", cint, [vp, vpp]),

@@ -93,6 +93,10 @@ class Grid:
         self.young, self.poisson = float(young), float(poisson)
         self.bounds = b.reshape(3, 2)
 
+    def set_split_tiling(self, max_groups):
+        """Element groups per tile of the split-FP16 apply (1..16)."""
+        check(lib().icc_grid_set_split_tiling(self.h, int(max_groups)))
+
     def connectivity(self):
         out = np.zeros(self.nel_tot * 81, dtype=np.int64)
         check(lib().icc_grid_connectivity(self.h, _p(out)))

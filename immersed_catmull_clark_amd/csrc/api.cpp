@@ -671,6 +671,20 @@ ICC_API int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu) {
   return ICC_OK;
 }
 
+// Element groups (x-lines of <= 16 elements, or 16-element groups) per tile
+// of the split-FP16 apply: fewer groups give more, smaller tiles per CU (a
+// deeper pipeline) at more staged halo points per element.
+ICC_API int icc_grid_set_split_tiling(icc_grid* g, int max_groups) {
+  if (!g || max_groups < 1 || max_groups > SP_MAX_GROUPS) return ICC_EINVAL;
+  ICC_HIP(hipStreamSynchronize(g->ctx->stream));
+  g->s.split_max_groups = max_groups;
+  build_split_tiles(g->s);
+  ICC_TRY(upload_split_tiles(g));
+  for (icc_op* op : g->ops)
+    if (op->d_free) ICC_TRY(upload_tile_mask(op));
+  return ICC_OK;
+}
+
 ICC_API int icc_grid_tiling(const icc_grid* g, int64_t out[3]) {
   if (!g || !out) return ICC_EINVAL;
   out[0] = g->tile_variant;

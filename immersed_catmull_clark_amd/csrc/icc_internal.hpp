@@ -100,6 +100,7 @@ struct GridSetup {
   std::vector<std::vector<double>> kmat_q32;  // kmat on 24-bit row grids with exact zero block row sums (FP32 apply)
   std::vector<Tile> tiles;
   std::vector<Tile> stiles;  // split-FP16 apply tiles
+  int split_max_groups = 16;  // element groups per split tile (<= SP_MAX_GROUPS)
 };
 
 int build_setup(const int64_t nel[3], const double bounds[6], double young, double poisson, GridSetup& s);

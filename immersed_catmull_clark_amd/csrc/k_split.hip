@@ -27,24 +27,24 @@
 // one 16-deep step (four slots per lane group).  Executed MFMA work per
 // element: 96 rows x 112 columns x 3 passes.
 //
-// Work decomposition.  One workgroup of twelve waves per CU walks a
+// Work decomposition.  One workgroup of eight waves per CU walks a
 // contiguous run of tiles (sub-boxes of constant element type, up to 16
 // x-lines of <= 16 elements, or 16-element groups of an x-boundary box):
-//   * waves 0..7 (MFMA waves): wave w owns M tiles 3h .. 3h + 2 (h = w / 4)
-//     of every element group g = w % 4 (mod 4) of the tile, with its 84 VGPRs
-//     of A fragments (both parts) resident; scale, DPP combine along the
-//     x-line (rows are permuted so that accumulator registers 0..2 hold the
-//     ax = 0, 1, 2 rows of one triplet, as in the FP64 apply) and ds_add_f64
-//     into the tile's LDS output box; after their groups of tile i they
-//     flush tile i - 1's output box to y with global atomic adds (masked),
-//     accumulating the p.Ap partial;
-//   * waves 8..11 (memory waves): one tile ahead they load the points of
-//     tile i + 2 and reduce their maximum, and convert and write the windows
-//     of tile i + 1.  They issue no global stores or atomics, so their waits
-//     for the loads wait for nothing else.
+//   * waves 0..3 (MFMA waves, one per SIMD): all six M tiles of element
+//     groups w, w + 4, ... of each tile, with the A fragments of both parts
+//     resident; scale, DPP combine along the x-line (rows are permuted so
+//     that accumulator registers 0..2 hold the ax = 0, 1, 2 rows of one
+//     triplet, as in the FP64 apply) and ds_add_f64 into the tile's LDS
+//     output box;
+//   * waves 4..7 (memory waves): per tile i they issue the loads of tile
+//     i + 2, convert and write the windows of tile i + 1, publish tile
+//     i + 2's maximum, then flush tile i - 1's output box to y with global
+//     atomic adds (masked), accumulating the p.Ap partial.  The atomics are
+//     issued after every load the memory waves wait for in that tile, so
+//     those waits never wait for the atomics.
 // One barrier per tile.  Windows and output boxes are double-buffered, the
-// FP32 copy of the staged points (the p.Ap partial of the flush) is
-// triple-buffered.
+// FP32 copy of the staged points (the p.Ap partial of the flush) and the
+// row bases are triple-buffered.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -298,6 +298,29 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
   const int tm = threadIdx.x - 256;  // memory thread index (memory waves)
   const auto masked_at = [&](int i) { return tmask != nullptr && tmask[i] != 0; };
 
+  // Flush of tile i's output box to y (masked) with global atomic adds,
+  // clearing it, by the memory waves (thread tf = tm); the p.Ap partial is
+  // the sum over the staged points of (M x)_i times the tile's contribution
+  // to (A x)_i, which summed over tiles is x^T M K M x.
+  double pap_acc = 0.0;
+  const auto flush = [&](int i) {
+    const Tile t = tiles[i];
+    const bool masked = masked_at(i);
+    const int px = t.nx + 2;
+    const int n3 = 3 * px * (t.ny + 2) * (t.nz + 2);
+    double* yb = ys[i & 1];
+    const float* xb = xs[i % 3];
+    const int* rb = rowg[i % 3];
+    for (int idx = tm; idx < n3; idx += 256) {
+      const double v = yb[idx];
+      const float xv = xb[idx];
+      const int r = sp_div(idx, px);
+      const int g = rb[r] + (idx - r * px);
+      yb[idx] = 0.0;
+      if (pap != nullptr) pap_acc += (double)xv * v;
+      if (!masked || freem[g] != 0) unsafeAtomicAdd(&y[g], (float)v);
+    }
+  };
   // The two roles run separate loops with the same barrier sequence: two in
   // the prologue, one per tile, and one before the p.Ap reduction.
   if (memw) {
@@ -326,11 +349,17 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
                rowg[i % 3], ncp0, ncp1, ndof_sc);
       if (tm == 0) sexp[i & 1] = ex + kexp[t.type];
     };
-    // tile i: stage i + 1 from Lh, load i + 2 into Lf
+    // tile i: load i + 2 into Lf, stage i + 1 from Lh, publish i + 2's
+    // maximum, then flush tile i - 1 (its output box is complete since the
+    // barrier that ended tile i - 1, and the MFMA waves write it again only
+    // after the barrier that ends tile i).  The flush's atomics are issued
+    // after every load this body waits for; the next wait (the staging of
+    // i + 2, a barrier later) finds them long retired.
     auto body = [&](int i, SpLoad& Lh, SpLoad& Lf) {
       if (i + 2 < last) load(i + 2, Lf);
       if (i + 1 < last) stage(i + 1, Lh);
       if (i + 2 < last) publish_max(sp_absmax(tiles[i + 2], tm, Lf), (i + 2) & 1);
+      if (i > first) flush(i - 1);
       __syncthreads();
     };
     for (int idx = tm; idx < 3 * SP_MAX_PTS; idx += 256) {
@@ -348,45 +377,19 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
       body(ti, Lb, La);
       if (ti + 1 < last) body(ti + 1, La, Lb);
     }
-    __syncthreads();  // the MFMA waves' flush of the last tile
+    flush(last - 1);  // the last tile
+    if (pap != nullptr) {
+      for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
+      if (lane == 0) red[wave - 4] = pap_acc;
+    }
+    __syncthreads();
   } else {
     // MFMA waves 0..3, one per SIMD: all six M tiles of element groups
     // w, w + 4, ... of each tile.
     const int gs = wave;
     const int col = lane & 15;
     const int kq0 = lane >> 4;
-    const int tf = threadIdx.x;  // flush thread index (0..255)
     double* dummy = &sdummy[wave][lane];  // target of the lanes whose output point does not exist
-    double pap_acc = 0.0;
-    // Part [k0, k1) of the flush of tile i's output box to y (masked) with
-    // global atomic adds, clearing it; the p.Ap partial is the sum over the
-    // staged points of (M x)_i times the tile's contribution to (A x)_i,
-    // which summed over tiles is x^T M K M x.
-    auto flush = [&](int i, int k0, int k1) {
-      const Tile t = tiles[i];
-      const bool masked = masked_at(i);
-      const int px = t.nx + 2;
-      const int n3 = 3 * px * (t.ny + 2) * (t.nz + 2);
-      double* yb = ys[i & 1];
-      const float* xb = xs[i % 3];
-      const int* rb = rowg[i % 3];
-      for (int k = k0; k < k1; ++k) {
-        const int idx = tf + 256 * k;
-        if (idx < n3) {
-          const double v = yb[idx];
-          const float xv = xb[idx];
-          const int r = sp_div(idx, px);
-          const int g = rb[r] + (idx - r * px);
-          yb[idx] = 0.0;
-          if (pap != nullptr) pap_acc += (double)xv * v;
-          if (!masked || freem[g] != 0) unsafeAtomicAdd(&y[g], (float)v);
-        }
-      }
-    };
-    auto flush_steps = [&](int i) {
-      const Tile t = tiles[i];
-      return (3 * (t.nx + 2) * (t.ny + 2) * (t.nz + 2) + 255) >> 8;
-    };
     sp_h8 ahi[6][3], alo[6][3];
     sp_h4 ahi3[6], alo3[6];
     int cur_type = -1;
@@ -470,11 +473,6 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
       double* yb = ys[b];
       const float* sb = ss[b];
       const bool cmask = col <= nx + 1, emask = col < 2 && nx >= 15 + col;
-      // flush of tile ti - 1, spread over this wave's groups: FPG steps per
-      // group, the rest after the loop
-      const int fsteps = ti > first ? flush_steps(ti - 1) : 0;
-      constexpr int FPG = 2;
-      int fk = 0;
       // the triplet offsets are even (rows are padded to a multiple of 8), so
       // a group's runs all start with the parity of its element base: the
       // row copy is picked once per group, and a run at point wbase + 2 o is
@@ -619,20 +617,10 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
             }
           }
         }
-        if (fk < fsteps) {
-          flush(ti - 1, fk, min(fk + FPG, fsteps));
-          fk += FPG;
-        }
       }
-      if (fk < fsteps) flush(ti - 1, fk, fsteps);
       __syncthreads();
     }
-    flush(last - 1, 0, flush_steps(last - 1));
-    if (pap != nullptr) {
-      for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
-      if (lane == 0) red[wave] = pap_acc;
-    }
-    __syncthreads();
+    __syncthreads();  // the memory waves' flush of the last tile
   }
   if (pap != nullptr) {
     __syncthreads();

@@ -677,9 +677,10 @@ void build_split_tiles(GridSetup& s) {
           tx = (int)((ex + ntx - 1) / ntx);
           lines = 1;
           int best_lines = 0, best_pts = 0;
-          for (int by = 1; by <= std::min<int64_t>(ey, SP_MAX_GROUPS); ++by)
-            for (int bz = 1; bz <= std::min<int64_t>(ez, SP_MAX_GROUPS); ++bz) {
-              if (by * bz > SP_MAX_GROUPS || !split_fits(tx, by, bz)) continue;
+          const int mg = std::max(1, std::min(s.split_max_groups, SP_MAX_GROUPS));
+          for (int by = 1; by <= std::min<int64_t>(ey, mg); ++by)
+            for (int bz = 1; bz <= std::min<int64_t>(ez, mg); ++bz) {
+              if (by * bz > mg || !split_fits(tx, by, bz)) continue;
               const int pts = (tx + 2) * (by + 2) * (bz + 2);
               if (by * bz > best_lines || (by * bz == best_lines && pts < best_pts)) {
                 best_lines = by * bz;
@@ -696,7 +697,8 @@ void build_split_tiles(GridSetup& s) {
           for (int by = 1; by <= std::min<int64_t>(ey, 64); ++by)
             for (int bz = 1; bz <= std::min<int64_t>(ez, 64); ++bz) {
               const int nel = tx * by * bz;
-              if ((nel + 15) / 16 > SP_MAX_GROUPS || !split_fits(tx, by, bz)) continue;
+              if ((nel + 15) / 16 > std::max(1, std::min(s.split_max_groups, SP_MAX_GROUPS)) || !split_fits(tx, by, bz))
+                continue;
               const double ppe = (double)((tx + 2) * (by + 2) * (bz + 2)) / nel;
               if (nel > best_el || (nel == best_el && ppe < best_ppe)) {
                 best_el = nel;

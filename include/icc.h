@@ -97,6 +97,9 @@ int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out);
  * 0 is the default.  wg_per_cu: apply workgroups per CU (1, 2 or 4).
  * Operators on the grid follow the new tiling. */
 int icc_grid_set_tiling(icc_grid* g, int variant, int wg_per_cu);
+/* element groups per tile of the split-FP16 apply (1 .. 16, default 16):
+ * fewer give more, smaller tiles per CU at more staged halo points per element */
+int icc_grid_set_split_tiling(icc_grid* g, int max_groups);
 /* out = variant, workgroups per CU, tile count */
 int icc_grid_tiling(const icc_grid* g, int64_t out[3]);
 /* Times `reps` applies of op (precision prec) for every variant and
