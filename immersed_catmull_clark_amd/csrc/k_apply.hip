@@ -351,13 +351,13 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
       if (lane == 0) red2[b][wave] = pt;
     }
   };
-  // MI, thread 0: adds flushed tile j's share (written before a barrier the
-  // caller has passed) to its instance's slot sum
-  auto collect = [&](int j) {
+  // MI, thread 0: adds flushed tile j's share (written by waves w0..W-1
+  // before a barrier the caller has passed) to its instance's slot sum
+  auto collect = [&](int j, int w0) {
     if constexpr (MI) {
       const int b = (j - first) & 1;
       double sj = 0.0;
-      for (int w = SPEC ? 6 : 0; w < W; ++w) sj += red2[b][w];
+      for (int w = w0; w < W; ++w) sj += red2[b][w];
       const int slot = tinst[j].slot;
       if (slot != mi_slot) {
         if (mi_slot >= 0) pap[mi_slot] = mi_acc;
@@ -385,11 +385,41 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     if (tb < Gl.nel) psc = scale[Gl.elem(tl, tb, cl.nel0, cl.nel1) + cl.ebase];
   };
 
+  // A fragments of this wave's M tile mt for element type `type`; consumed
+  // by an empty asm each at the point given by the caller, so that the loads
+  // are waited for there (never in the MFMA loop, whose waits would also
+  // wait for the flush's atomics and the prefetch of tile i + 2)
+  T a[BF ? 1 : KS];
+  bf8_t ab[BF ? KSB : 1];
+  const auto load_a = [&](int type) {
+    if constexpr (BF) {
+      const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(type * MT + mt) * KSB) * 64 + lane;
+#pragma unroll
+      for (int ks = 0; ks < KSB; ++ks) ab[ks] = kf[ks * 64];
+    } else {
+      const T* kf = (const T*)kfrag_v + ((size_t)(type * MT + mt) * KS) * 64 + lane;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) a[ks] = kf[ks * 64];
+    }
+  };
+  const auto use_a = [&]() {
+    if constexpr (BF) {
+#pragma unroll
+      for (int ks = 0; ks < KSB; ++ks) asm volatile("" ::"v"(ab[ks]));
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) asm volatile("" ::"v"(a[ks]));
+    }
+  };
+
   // prologue: stage the first tile into buffer 0, clear both output buffers,
-  // load the second tile into registers
+  // load the second tile into registers.  The first tile's A fragments are
+  // loaded first: their latency overlaps the staging's.
   Tile t = tiles[first];
   TileGeom G;
   G.init(t);
+  int cur_type = t.type;
+  load_a(cur_type);
   {
     const bool masked = tmask != nullptr && tmask[first] != 0;
     const TileInst c0 = ctx_of(first);
@@ -417,12 +447,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
   // their first use they sit in the scalar cache, and no descriptor has to
   // be carried (and spilled) across tiles.
   const auto mask_at = [&](int i) { return tmask != nullptr && tmask[i] != 0; };
+  use_a();  // arrived with the staged points
   if (first + 1 < last) load_regs(first + 1);
   __syncthreads();  // buffer 0 staged, output buffers clear
 
-  T a[BF ? 1 : KS];
-  bf8_t ab[BF ? KSB : 1];
-  int cur_type = -1;
   int buf = 0;
   for (int ti = first; ti < last; ++ti) {
     if (ti > first) {
@@ -430,29 +458,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
       G.init(t);
     }
     // the A fragments are (re)loaded before any other memory operation of
-    // the tile, so that waiting for them never waits for the flush's atomics
-    // or the prefetch below (the vector memory counter retires in order)
-    if (t.type != cur_type) {  // uniform: the A fragments of this wave's M tile mt
-      // each fragment is consumed by an empty asm right here, so the loads are
-      // waited for on this (rare) path and are never pending in the MFMA
-      // loop, whose waits would otherwise also wait for the prefetch of tile
-      // i + 2 issued below
-      if constexpr (BF) {
-        const bf8_t* kf = (const bf8_t*)kfrag_v + ((size_t)(t.type * MT + mt) * KSB) * 64 + lane;
-#pragma unroll
-        for (int ks = 0; ks < KSB; ++ks) ab[ks] = kf[ks * 64];
-#pragma unroll
-        for (int ks = 0; ks < KSB; ++ks) asm volatile("" ::"v"(ab[ks]));
-      } else {
-        const T* kf = (const T*)kfrag_v + ((size_t)(t.type * MT + mt) * KS) * 64 + lane;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) a[ks] = kf[ks * 64];
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) asm volatile("" ::"v"(a[ks]));
-      }
+    // the tile and waited for on this (rare, uniform) path, so that waiting
+    // for them never waits for the flush's atomics or the prefetch below (the
+    // vector memory counter retires in order)
+    if (t.type != cur_type) {
+      load_a(t.type);
+      use_a();
       cur_type = t.type;
     }
-    if (MI && threadIdx.x == 0 && ti - 2 >= first) collect(ti - 2);
+    if (MI && threadIdx.x == 0 && ti - 2 >= first) collect(ti - 2, SPEC ? 6 : 0);
     if (ti > first) flush(tiles[ti - 1], ti - 1, mask_at(ti - 1), buf ^ 1);
     if (bgw && tb < 8) claim[buf ^ 1][tb] = 0;
     if (bgw && ti + 1 < last) {  // stage tile i + 1 into buffer buf ^ 1
@@ -622,12 +636,37 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     __syncthreads();  // tile i's output complete; buffer buf ^ 1 staged
     buf ^= 1;
   }
-  flush(tiles[last - 1], last - 1, mask_at(last - 1), buf ^ 1);  // the last tile
+  {
+    // the last tile: nothing is left to overlap with, so every wave flushes
+    // (the same point order as flush() when all waves are background waves)
+    const int b = buf ^ 1;
+    const Tile tf = tiles[last - 1];
+    TileGeom Gf;
+    Gf.init(tf);
+    const TileInst cf = ctx_of(last - 1);
+    const bool maskf = mask_at(last - 1);
+    double pt = 0.0;
+    for (int idx = threadIdx.x; idx < Gf.n3; idx += NT) {
+      const double v = ys[b][idx];
+      if (pap != nullptr) {
+        if constexpr (MI)
+          pt += (double)xs[b][idx] * v;
+        else
+          pap_acc += (double)xs[b][idx] * v;
+      }
+      const int g = Gf.dof(tf, idx, cf.ncp0, cf.ncp1, cf.ndof_sc) + cf.dbase;
+      if (!maskf || freem[g] != 0) unsafeAtomicAdd(&y[g], (T)v);
+    }
+    if constexpr (MI) {
+      for (int off = 32; off > 0; off >>= 1) pt += __shfl_down(pt, off, 64);
+      if (lane == 0) red2[b][wave] = pt;
+    }
+  }
   if constexpr (MI) {
     __syncthreads();
     if (threadIdx.x == 0) {
-      if (last - 2 >= first) collect(last - 2);
-      collect(last - 1);
+      if (last - 2 >= first) collect(last - 2, SPEC ? 6 : 0);
+      collect(last - 1, 0);
       pap[mi_slot] = mi_acc;
     }
     return;

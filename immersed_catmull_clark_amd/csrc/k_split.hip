@@ -42,7 +42,8 @@
 //     atomic adds (masked), accumulating the p.Ap partial.  The atomics are
 //     issued after every load the memory waves wait for in that tile, so
 //     those waits never wait for the atomics.
-// One barrier per tile.  Windows and output boxes are double-buffered, the
+// The MFMA waves load the first tile's A fragments while the memory waves
+// stage it, and all eight waves flush the last tile.  One barrier per tile.  Windows and output boxes are double-buffered, the
 // FP32 copy of the staged points (the p.Ap partial of the flush) and the
 // row bases are triple-buffered.
 #include <hip/hip_runtime.h>
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
   __shared__ float ss[2][SP_MAX_EL];
   __shared__ uint32_t pmax[2][4];
   __shared__ int sexp[2];
-  __shared__ double red[4];
+  __shared__ double red[SP_THREADS / 64];
   __shared__ double sdummy[4][64];
 
   if (st != nullptr && st->done != 0) return;  // solver already finished
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
   // the sum over the staged points of (M x)_i times the tile's contribution
   // to (A x)_i, which summed over tiles is x^T M K M x.
   double pap_acc = 0.0;
-  const auto flush = [&](int i) {
+  const auto flush = [&](int i, int t0, int nt) {  // points t0, t0 + nt, ... of tile i
     const Tile t = tiles[i];
     const bool masked = masked_at(i);
     const int px = t.nx + 2;
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
     double* yb = ys[i & 1];
     const float* xb = xs[i % 3];
     const int* rb = rowg[i % 3];
-    for (int idx = tm; idx < n3; idx += 256) {
+    for (int idx = t0; idx < n3; idx += nt) {
       const double v = yb[idx];
       const float xv = xb[idx];
       const int r = sp_div(idx, px);
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
       if (i + 2 < last) load(i + 2, Lf);
       if (i + 1 < last) stage(i + 1, Lh);
       if (i + 2 < last) publish_max(sp_absmax(tiles[i + 2], tm, Lf), (i + 2) & 1);
-      if (i > first) flush(i - 1);
+      if (i > first) flush(i - 1, tm, 256);
       __syncthreads();
     };
     for (int idx = tm; idx < 3 * SP_MAX_PTS; idx += 256) {
@@ -377,12 +378,6 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
       body(ti, Lb, La);
       if (ti + 1 < last) body(ti + 1, La, Lb);
     }
-    flush(last - 1);  // the last tile
-    if (pap != nullptr) {
-      for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
-      if (lane == 0) red[wave - 4] = pap_acc;
-    }
-    __syncthreads();
   } else {
     // MFMA waves 0..3, one per SIMD: all six M tiles of element groups
     // w, w + 4, ... of each tile.
@@ -392,34 +387,40 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
     double* dummy = &sdummy[wave][lane];  // target of the lanes whose output point does not exist
     sp_h8 ahi[6][3], alo[6][3];
     sp_h4 ahi3[6], alo3[6];
-    int cur_type = -1;
+    // (re)loads the A fragments of all six M tiles of type `type`
+    // (wave-uniform); they are consumed right here, so the loads are waited
+    // for on this (rare) path and the group loop carries no vector-memory
+    // waits
+    const auto load_a = [&](int type) {
+#pragma unroll
+      for (int mt = 0; mt < 6; ++mt) {
+        const uint4* af = afrag + ((size_t)(type * 2 + mt / 3) * 12 + (mt % 3) * 4) * 128 + lane * 2;
+#pragma unroll
+        for (int ks = 0; ks < 3; ++ks) {
+          ahi[mt][ks] = __builtin_bit_cast(sp_h8, af[ks * 128]);
+          alo[mt][ks] = __builtin_bit_cast(sp_h8, af[ks * 128 + 1]);
+        }
+        const uint4 a0 = af[3 * 128], a1 = af[3 * 128 + 1];
+        ahi3[mt] = __builtin_bit_cast(sp_h4, make_uint2(a0.x, a0.y));
+        alo3[mt] = __builtin_bit_cast(sp_h4, make_uint2(a1.x, a1.y));
+      }
+#pragma unroll
+      for (int mt = 0; mt < 6; ++mt) {
+#pragma unroll
+        for (int ks = 0; ks < 3; ++ks) asm volatile("" ::"v"(ahi[mt][ks]), "v"(alo[mt][ks]));
+        asm volatile("" ::"v"(ahi3[mt]), "v"(alo3[mt]));
+      }
+    };
+    // the first tile's fragments load while the memory waves stage it
+    int cur_type = tiles[first].type;
+    load_a(cur_type);
     __syncthreads();
     __syncthreads();
     for (int ti = first; ti < last; ++ti) {
       const int b = ti & 1;
       const Tile t = tiles[ti];
-      if (t.type != cur_type) {  // wave-uniform: (re)load the A fragments of all six M tiles
-#pragma unroll
-        for (int mt = 0; mt < 6; ++mt) {
-          const uint4* af = afrag + ((size_t)(t.type * 2 + mt / 3) * 12 + (mt % 3) * 4) * 128 + lane * 2;
-#pragma unroll
-          for (int ks = 0; ks < 3; ++ks) {
-            ahi[mt][ks] = __builtin_bit_cast(sp_h8, af[ks * 128]);
-            alo[mt][ks] = __builtin_bit_cast(sp_h8, af[ks * 128 + 1]);
-          }
-          const uint4 a0 = af[3 * 128], a1 = af[3 * 128 + 1];
-          ahi3[mt] = __builtin_bit_cast(sp_h4, make_uint2(a0.x, a0.y));
-          alo3[mt] = __builtin_bit_cast(sp_h4, make_uint2(a1.x, a1.y));
-        }
-        // consumed here, so the loads are waited for on this (rare) path and
-        // the group loop carries no vector-memory waits, which would also
-        // wait for the flush's atomics
-#pragma unroll
-        for (int mt = 0; mt < 6; ++mt) {
-#pragma unroll
-          for (int ks = 0; ks < 3; ++ks) asm volatile("" ::"v"(ahi[mt][ks]), "v"(alo[mt][ks]));
-          asm volatile("" ::"v"(ahi3[mt]), "v"(alo3[mt]));
-        }
+      if (t.type != cur_type) {
+        load_a(t.type);
         cur_type = t.type;
       }
       const int nx = t.nx, px = nx + 2, py = t.ny + 2, pz = t.nz + 2;
@@ -620,11 +621,19 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
       }
       __syncthreads();
     }
-    __syncthreads();  // the memory waves' flush of the last tile
   }
+  // The last tile's output box (complete since the barrier that ended the
+  // tile) is flushed by all eight waves: nothing is left to overlap with.
+  flush(last - 1, threadIdx.x, SP_THREADS);
   if (pap != nullptr) {
+    for (int off = 32; off > 0; off >>= 1) pap_acc += __shfl_down(pap_acc, off, 64);
+    if (lane == 0) red[wave] = pap_acc;
     __syncthreads();
-    if (threadIdx.x == 0) pap[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < SP_THREADS / 64; ++w) sum += red[w];
+      pap[blockIdx.x] = sum;
+    }
   }
 }
 
