@@ -37,7 +37,7 @@
 //     triplet, as in the FP64 apply) and ds_add_f64 into the tile's LDS
 //     output box;
 //   * waves 4..7 (memory waves): per tile i they issue the loads of tile
-//     i + 2, convert and write the windows of tile i + 1, publish tile
+//     i + 3, convert and write the windows of tile i + 1, publish tile
 //     i + 2's maximum, then flush tile i - 1's output box to y with global
 //     atomic adds (masked), accumulating the p.Ap partial.  The atomics are
 //     issued after every load the memory waves wait for in that tile, so
@@ -325,13 +325,12 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
   // The two roles run separate loops with the same barrier sequence: two in
   // the prologue, one per tile, and one before the p.Ap reduction.
   if (memw) {
-    // Two register sets alternate: at the top of tile i one holds tile i + 1
-    // (loaded during tile i - 1, its maximum published) and the other is
-    // free; the loads of tile i + 2 go into the free set first, so their
-    // latency is covered by the staging.  The memory waves issue no stores
-    // or atomics to global memory, so waiting for these loads never waits
-    // for anything else.
-    SpLoad La, Lb;
+    // Three register sets rotate: at the top of tile i one holds tile i + 1
+    // (its maximum published), one tile i + 2 (loaded during tile i - 1) and
+    // one is free; the loads of tile i + 3 go into the free set first.  So
+    // every load has a whole tile's work before its first use (the maximum,
+    // one tile later).
+    SpLoad La, Lb, Lc;
     auto publish_max = [&](float mx, int slot) {
       for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
       if (lane == 0) pmax[slot][wave - 4] = __builtin_bit_cast(uint32_t, mx);
@@ -350,16 +349,16 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
                rowg[i % 3], ncp0, ncp1, ndof_sc);
       if (tm == 0) sexp[i & 1] = ex + kexp[t.type];
     };
-    // tile i: load i + 2 into Lf, stage i + 1 from Lh, publish i + 2's
-    // maximum, then flush tile i - 1 (its output box is complete since the
-    // barrier that ended tile i - 1, and the MFMA waves write it again only
-    // after the barrier that ends tile i).  The flush's atomics are issued
-    // after every load this body waits for; the next wait (the staging of
-    // i + 2, a barrier later) finds them long retired.
-    auto body = [&](int i, SpLoad& Lh, SpLoad& Lf) {
-      if (i + 2 < last) load(i + 2, Lf);
-      if (i + 1 < last) stage(i + 1, Lh);
-      if (i + 2 < last) publish_max(sp_absmax(tiles[i + 2], tm, Lf), (i + 2) & 1);
+    // tile i: load i + 3 into Ll, stage i + 1 from Ls, publish i + 2's
+    // maximum from Lp, then flush tile i - 1 (its output box is complete
+    // since the barrier that ended tile i - 1, and the MFMA waves write it
+    // again only after the barrier that ends tile i).  The flush's atomics
+    // are issued after every load this body waits for; the next waits (a
+    // barrier later) find them long retired.
+    auto body = [&](int i, const SpLoad& Ls, SpLoad& Lp, SpLoad& Ll) {
+      if (i + 3 < last) load(i + 3, Ll);
+      if (i + 1 < last) stage(i + 1, Ls);
+      if (i + 2 < last) publish_max(sp_absmax(tiles[i + 2], tm, Lp), (i + 2) & 1);
       if (i > first) flush(i - 1, tm, 256);
       __syncthreads();
     };
@@ -369,14 +368,17 @@ __global__ __launch_bounds__(SP_THREADS, 1) void apply_split_kernel(
     }
     load(first, La);
     if (first + 1 < last) load(first + 1, Lb);
+    if (first + 2 < last) load(first + 2, Lc);
     publish_max(sp_absmax(tiles[first], tm, La), first & 1);
     __syncthreads();
     stage(first, La);
     if (first + 1 < last) publish_max(sp_absmax(tiles[first + 1], tm, Lb), (first + 1) & 1);
     __syncthreads();
-    for (int ti = first; ti < last; ti += 2) {
-      body(ti, Lb, La);
-      if (ti + 1 < last) body(ti + 1, La, Lb);
+    // set of tile k: La, Lb, Lc for (k - first) mod 3 = 0, 1, 2
+    for (int ti = first; ti < last; ti += 3) {
+      body(ti, Lb, Lc, La);
+      if (ti + 1 < last) body(ti + 1, Lc, La, Lb);
+      if (ti + 2 < last) body(ti + 2, La, Lb, Lc);
     }
   } else {
     // MFMA waves 0..3, one per SIMD: all six M tiles of element groups
