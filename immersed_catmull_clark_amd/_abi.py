@@ -32,6 +32,7 @@ SIGNATURES = [
     ("icc_grid_sizes", cint, [vp, vp]),
     ("icc_grid_connectivity", cint, [vp, vp]),
     ("icc_grid_type_id", cint, [vp, vp]),
+    ("icc_element_types", cint, [vp, vp, vp, vp]),
     ("icc_grid_element_matrix", cint, [vp, i64, vp]),
     ("icc_op_create", cint, [vp, vpp]),
     ("icc_op_destroy", cint, [vp]),

@@ -74,6 +74,18 @@ class Context:
         return out[:256].reshape(64, 4), out[256:].reshape(64, 4)
 
 
+def element_types(nel, bounds):
+    """(ntypes, type_id) of elasticity_element_matrices (rust/src/iga/
+    elasticity.rs:30-98) for a p = 2 grid, on the host without a device:
+    the numbering Grid.type_id() reports."""
+    n = np.ascontiguousarray(np.asarray(nel, dtype=np.int64).reshape(3))
+    b = _f64(np.asarray(bounds, dtype=np.float64).ravel())
+    nt = np.zeros(1, dtype=np.int64)
+    tid = np.zeros(int(np.prod(n)), dtype=np.int64)
+    check(lib().icc_element_types(_p(n), _p(b), _p(nt), _p(tid)))
+    return int(nt[0]), tid
+
+
 class Grid:
     """p = 2 B-spline space on a box with its exact element matrices
     (SpaceBox::new + elasticity_element_matrices, rust/src/iga)."""

@@ -253,6 +253,26 @@ static void split_fragments(const GridSetup& s, std::vector<uint16_t>& frag, std
 }
 }  // namespace icc
 
+ICC_API int icc_element_types(const int64_t nel[3], const double bounds[6], int64_t* ntypes, int64_t* type_id) {
+  if (!nel || !bounds || !ntypes) return ICC_EINVAL;
+  try {
+    GridSetup s;
+    const int rc = build_setup(nel, bounds, 1.0, 0.3, s);
+    if (rc != ICC_OK) return rc;
+    *ntypes = (int64_t)s.rtype_internal.size();
+    if (type_id) {
+      int64_t e = 0;
+      for (int64_t k = 0; k < s.nel[2]; ++k)
+        for (int64_t j = 0; j < s.nel[1]; ++j)
+          for (int64_t i = 0; i < s.nel[0]; ++i, ++e) type_id[e] = reference_type(s, i, j, k);
+    }
+  } catch (const std::exception& ex) {
+    set_error(std::string("icc_element_types: ") + ex.what());
+    return ICC_EINVAL;
+  }
+  return ICC_OK;
+}
+
 ICC_API int icc_grid_create(icc_ctx* ctx, const int64_t nel[3], const double bounds[6], int64_t degree, double young,
                             double poisson, icc_grid** out) {
   if (!ctx || !nel || !bounds || !out) {
@@ -384,7 +404,7 @@ ICC_API int icc_grid_sizes(const icc_grid* g, int64_t out[11]) {
   out[0] = s.nel_tot;
   out[1] = s.ndof_sc;
   out[2] = s.ndof;
-  out[3] = g->ntypes;
+  out[3] = (int64_t)g->s.rtype_internal.size();  // the reference's type count
   out[4] = NSH;
   for (int d = 0; d < 3; ++d) {
     out[5 + d] = s.ncp[d];
@@ -416,13 +436,13 @@ ICC_API int icc_grid_type_id(const icc_grid* g, int64_t* out) {
   int64_t e = 0;
   for (int64_t k = 0; k < s.nel[2]; ++k)
     for (int64_t j = 0; j < s.nel[1]; ++j)
-      for (int64_t i = 0; i < s.nel[0]; ++i, ++e) out[e] = element_type(s, i, j, k);
+      for (int64_t i = 0; i < s.nel[0]; ++i, ++e) out[e] = reference_type(s, i, j, k);
   return ICC_OK;
 }
 
 ICC_API int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out) {
-  if (!g || !out || type < 0 || type >= g->ntypes) return ICC_EINVAL;
-  std::memcpy(out, g->s.kmat[type].data(), sizeof(double) * NSH * NSH);
+  if (!g || !out || type < 0 || type >= (int64_t)g->s.rtype_internal.size()) return ICC_EINVAL;
+  std::memcpy(out, g->s.kmat[g->s.rtype_internal[type]].data(), sizeof(double) * NSH * NSH);
   return ICC_OK;
 }
 

@@ -96,6 +96,11 @@ struct GridSetup {
   // element types (first-appearance order) as (t0, t1, t2)
   std::vector<std::array<int32_t, 3>> type_list;
   std::vector<int32_t> type_of;  // index t0 + T0*(t1 + T1*t2) -> element type
+  // the reference's numbering (rounded keys, elasticity.rs:73-98), reported by
+  // icc_grid_type_id: 1D types, combination -> type, type -> element type above
+  std::vector<int32_t> rtid1[3];
+  std::vector<int32_t> rtype_of;  // index r0 + R0*(r1 + R1*r2)
+  std::vector<int32_t> rtype_internal;
   std::vector<std::vector<double>> kmat;    // [ntypes][81*81]
   std::vector<std::vector<double>> kmat_q32;  // kmat on 24-bit row grids with exact zero block row sums (FP32 apply)
   std::vector<Tile> tiles;
@@ -131,6 +136,8 @@ void partition_split_tiles(std::vector<Tile>& tiles, int G, std::vector<int32_t>
 void balance_tile_list(std::vector<Tile>& tiles, int G, double (*cost)(const Tile&), std::vector<int32_t>& first);
 int n_tile_variants();
 int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
+// the element's type in the reference's numbering
+int32_t reference_type(const GridSetup& s, int64_t i, int64_t j, int64_t k);
 
 void ghost_tables_1d(int64_t nel, double length, std::vector<double>& jump, std::vector<double>& mass);
 // Design-domain filter offsets (topopt_immersed.rs:117-131): radius

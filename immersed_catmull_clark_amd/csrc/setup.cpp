@@ -4,9 +4,10 @@
 //
 // Semantics follow the reference (rust/src/iga/{quadrature,space,elasticity}.rs):
 // types are the distinct combinations of per-direction 1D types, numbered in
-// order of first appearance (elasticity.rs:85-98); 1D types are grouped by a
-// tolerance rather than the reference's rounded key (see build_setup);
-// K = lambda G_ij + mu G_ji + mu delta_ij sum_k G_kk (elasticity.rs:125-145).
+// order of first appearance (elasticity.rs:85-98); the kernels group 1D types
+// by a tolerance, while the reported numbering uses the reference's rounded
+// key (see build_setup); K = lambda G_ij + mu G_ji + mu delta_ij sum_k G_kk
+// (elasticity.rs:125-145).
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -172,6 +173,12 @@ int32_t element_type(const GridSetup& s, int64_t i, int64_t j, int64_t k) {
   return s.type_of[s.tid1[0][i] + T0 * (s.tid1[1][j] + T1 * s.tid1[2][k])];
 }
 
+int32_t reference_type(const GridSetup& s, int64_t i, int64_t j, int64_t k) {
+  const int64_t R0 = 1 + *std::max_element(s.rtid1[0].begin(), s.rtid1[0].end());
+  const int64_t R1 = 1 + *std::max_element(s.rtid1[1].begin(), s.rtid1[1].end());
+  return s.rtype_of[s.rtid1[0][i] + R0 * (s.rtid1[1][j] + R1 * s.rtid1[2][k])];
+}
+
 int build_setup(const int64_t nel[3], const double bounds[6], double young, double poisson, GridSetup& s) {
   for (int d = 0; d < 3; ++d) {
     if (nel[d] < 1) {
@@ -209,6 +216,20 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
     double scale = 0.0;
     for (auto& m : per)
       for (double v : m) scale = std::max(scale, std::fabs(v));
+    // the reference's 1D types: equal keys round(v / scale * 1e12) over the
+    // four matrices in order (elasticity.rs:72-82).  element_mats_1d repeats
+    // the reference's arithmetic operation for operation, so the keys are
+    // the reference's.
+    {
+      std::map<std::array<int64_t, 36>, int32_t> keys;
+      s.rtid1[d].clear();
+      for (auto& m : per) {
+        std::array<int64_t, 36> key;
+        for (int q = 0; q < 36; ++q) key[q] = (int64_t)std::round(m[q] / scale * 1e12);
+        auto it = keys.emplace(key, (int32_t)keys.size()).first;
+        s.rtid1[d].push_back(it->second);
+      }
+    }
     const double tol = 1e-9 * scale;
     s.tid1[d].clear();
     s.mats1[d].clear();
@@ -239,6 +260,24 @@ int build_setup(const int64_t nel[3], const double bounds[6], double young, doub
           s.type_list.push_back({t0, t1, t2});
         }
       }
+
+  // the reference's element types: distinct (r0, r1, r2) in element order
+  {
+    const int64_t R0 = 1 + *std::max_element(s.rtid1[0].begin(), s.rtid1[0].end());
+    const int64_t R1 = 1 + *std::max_element(s.rtid1[1].begin(), s.rtid1[1].end());
+    const int64_t R2 = 1 + *std::max_element(s.rtid1[2].begin(), s.rtid1[2].end());
+    s.rtype_of.assign((size_t)(R0 * R1 * R2), -1);
+    s.rtype_internal.clear();
+    for (int64_t k = 0; k < s.nel[2]; ++k)
+      for (int64_t j = 0; j < s.nel[1]; ++j)
+        for (int64_t i = 0; i < s.nel[0]; ++i) {
+          int32_t& slot = s.rtype_of[s.rtid1[0][i] + R0 * (s.rtid1[1][j] + R1 * s.rtid1[2][k])];
+          if (slot < 0) {
+            slot = (int32_t)s.rtype_internal.size();
+            s.rtype_internal.push_back(element_type(s, i, j, k));
+          }
+        }
+  }
 
   // 81x81 matrices: G_ij = (x) over d of m_d^{k(d,i,j)}, direction 0 fastest
   s.kmat.clear();

@@ -75,19 +75,25 @@ int icc_grid_destroy(icc_grid* g);
 int icc_grid_sizes(const icc_grid* g, int64_t out[11]);
 /* connectivity[e*81 + c*27 + a] (space.rs:133-155), bit-exact */
 int icc_grid_connectivity(const icc_grid* g, int64_t* out);
-/* type_id[e] in first-appearance order (elasticity.rs:85-98).
- * Deviation: the reference groups 1D element matrices by the rounded key
- * round(v / scale * 1e12) (elasticity.rs:73); here a 1D type is every
- * element whose matrices agree with the type's first element to
- * 1e-9 * scale.  Where the knot spacing's rounding noise straddles the
- * reference's key (a length-2 side with 32 or 160 elements) the reference
- * splits one mathematical type in two and this map merges them, so its
- * numbering (and ntypes) then differs from the reference's; every element's
- * matrix (icc_grid_element_matrix of its type) equals the reference's matrix
- * for that element to round-off. */
+/* type_id[e] in first-appearance order, the reference's numbering
+ * (elasticity_element_matrices, rust/src/iga/elasticity.rs:30-98): 1D types
+ * are the distinct keys round(v / scale * 1e12) of the four 1D element
+ * matrices (computed with the reference's arithmetic, so the keys are bit for
+ * bit the reference's), element types the distinct combinations in element
+ * order.  Where rounding noise of the knot spacing straddles a key (a
+ * length-2 side with 32 or 160 elements) the reference splits one
+ * mathematical type in two and so does this map; the kernels group by a
+ * tolerance and use one matrix for both (icc_grid_element_matrix returns the
+ * same matrix for the two). */
 int icc_grid_type_id(const icc_grid* g, int64_t* out);
-/* row-major 81x81 element matrix of a type (test x trial) */
+/* row-major 81x81 element matrix of a type (test x trial), type in the
+ * numbering of icc_grid_type_id (0 <= type < ntypes of icc_grid_sizes) */
 int icc_grid_element_matrix(const icc_grid* g, int64_t type, double* out);
+/* The same type map without a grid or a device (host code): *ntypes and, if
+ * type_id is not NULL, type_id[nel0 nel1 nel2] (elasticity.rs:30-98, the
+ * types of elasticity_element_matrices; they do not depend on the Lame
+ * parameters). */
+int icc_element_types(const int64_t nel[3], const double bounds[6], int64_t* ntypes, int64_t* type_id);
 
 /* ---- tiling (no counterpart in the reference) ---------------------------------- */
 /* The apply walks the grid in tiles of constant element type.  variant selects

@@ -108,3 +108,28 @@ def test_host_surface_quadrature_matches_oracle():
             x, w, n = surface_quadrature(mesh, sel, me)
             xo, wo, no = O.surface_quadrature(verts, tris, sel, me)
             assert np.array_equal(x, xo) and np.array_equal(w, wo) and np.array_equal(n, no)
+
+
+def test_host_element_types_match_oracle():
+    """icc_element_types (host code) against the oracle's restatement of
+    elasticity_element_matrices (elasticity.rs:30-98), whose type ids the
+    reference's fixtures pin: the same count and the same map element by
+    element, including the sizes where the reference's rounded key splits a
+    mathematical type (length-2 sides with 32 or 160 elements: 36 and 90
+    types where a tolerance finds 27)."""
+    import numpy as np
+
+    import oracle_py as O
+    from immersed_catmull_clark_amd import element_types
+    cases = [([[0.5, 2.5], [-1, 0.2], [0.1, 0.4]], [4, 3, 3]), ([[0, 1.2], [0, 0.6], [0, 0.3]], [8, 4, 2]),
+             ([[0, 1.0], [0, 1.0], [0, 1.0]], [1, 1, 1]), ([[0, 1.0], [0, 0.5], [0, 2.0]], [2, 1, 5]),
+             ([[0, 1.0], [0, 1.0], [0, 1.0]], [7, 11, 13]), ([[0, 2.0], [0, 1.0], [0, 1.0]], [32, 3, 3]),
+             ([[0, 2.0], [0, 1.0], [0, 1.0]], [160, 3, 3]), ([[0, 2.0], [0, 2.0], [0, 2.0]], [32, 32, 32])]
+    counts = {}
+    for bounds, nel in cases:
+        nt, tid = element_types(nel, bounds)
+        sp = O.Space(bounds, nel, 2)
+        _, tref = sp.elem_mats(*O.lame(1.0, 0.3))
+        assert nt == sp.ntypes and np.array_equal(tid, tref), (nel, nt, sp.ntypes)
+        counts[tuple(nel)] = nt
+    assert counts[(32, 3, 3)] == 36 and counts[(160, 3, 3)] == 90 and counts[(32, 32, 32)] == 64
