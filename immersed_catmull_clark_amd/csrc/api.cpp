@@ -509,6 +509,17 @@ static void drop_work(icc_op* op) {
 }
 
 static int csr_blocks(const icc_op* op) { return op->s_nnz > 0 ? vec_blocks(op->g->s.ndof) : 0; }
+// y = S x (overwrites y) in precision T, csr_blocks(op) partials into parts
+template <typename T>
+static hipError_t op_csr(const icc_op* op, const T* x, T* y, double* parts, hipStream_t st) {
+  const int64_t n = op->g->s.ndof;
+  if constexpr (std::is_same<T, double>::value)
+    return launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, op->d_s_rows,
+                                           op->s_nrows, csr_blocks(op), x, y, op->d_free, parts, st);
+  else
+    return launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, op->d_s_rows,
+                                         op->s_nrows, csr_blocks(op), x, y, op->d_free, parts, st);
+}
 // partials of the element term: one per apply workgroup, or per gather block
 // in the deterministic mode
 static int elem_grid(const icc_op* op) { return op->det ? det_grid(op->g) : apply_grid(op->g); }
@@ -565,6 +576,7 @@ ICC_API int icc_op_destroy(icc_op* op) {
   hipFree(op->d_s_data);
   hipFree(op->d_s_data32);
   hipFree(op->d_s_diag);
+  hipFree(op->d_s_rows);
   hipFree(op->d_E);
   hipFree(op->d_faces);
   hipFree(op->d_f_diag);
@@ -722,12 +734,15 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
   hipFree(op->d_s_data);
   hipFree(op->d_s_data32);
   hipFree(op->d_s_diag);
+  hipFree(op->d_s_rows);
   op->d_s_indptr = nullptr;
   op->d_s_indices = nullptr;
   op->d_s_data = nullptr;
   op->d_s_data32 = nullptr;
   op->d_s_diag = nullptr;
+  op->d_s_rows = nullptr;
   op->s_nnz = 0;
+  op->s_nrows = 0;
   op->h_s_indptr.clear();
   op->h_s_indices.clear();
   op->h_s_data.clear();
@@ -751,6 +766,10 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
       d32[k] = (float)data[k];
       if (indices[k] == i) diag[i] += data[k];
     }
+  std::vector<int32_t> rows;
+  for (int64_t i = 0; i < n; ++i)
+    if (indptr[i + 1] > indptr[i]) rows.push_back((int32_t)i);
+  ICC_TRY(dalloc(&op->d_s_rows, std::max<int64_t>((int64_t)rows.size(), 1)));
   ICC_TRY(dalloc(&op->d_s_indptr, n + 1));
   ICC_TRY(dalloc(&op->d_s_indices, nnz));
   ICC_TRY(dalloc(&op->d_s_data, nnz));
@@ -762,8 +781,11 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
   ICC_HIP(hipMemcpyAsync(op->d_s_data, data, nnz * 8, hipMemcpyHostToDevice, st));
   ICC_HIP(hipMemcpyAsync(op->d_s_data32, d32.data(), nnz * 4, hipMemcpyHostToDevice, st));
   ICC_HIP(hipMemcpyAsync(op->d_s_diag, diag.data(), n * 8, hipMemcpyHostToDevice, st));
+  if (!rows.empty())
+    ICC_HIP(hipMemcpyAsync(op->d_s_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, st));
   ICC_HIP(hipStreamSynchronize(st));
   op->s_nnz = nnz;
+  op->s_nrows = (int64_t)rows.size();
   op->h_s_indptr.assign(indptr, indptr + n + 1);
   op->h_s_indices.assign(indices, indices + nnz);
   op->h_s_data.assign(data, data + nnz);
@@ -1057,11 +1079,9 @@ static hipError_t apply_t(icc_op* op, const T* x, T* y, double* pap) {
   const int64_t n = g->s.ndof;
   if (op->s_nnz > 0) {
     if constexpr (std::is_same<T, double>::value)
-      e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, x, y, op->d_free,
-                                          pap ? pap + elem_grid(op) : nullptr, st);
+      e = op_csr<double>(op, x, y, pap ? pap + elem_grid(op) : nullptr, st);
     else
-      e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, x, y, op->d_free,
-                                        pap ? pap + elem_grid(op) : nullptr, st);
+      e = op_csr<float>(op, x, y, pap ? pap + elem_grid(op) : nullptr, st);
   } else {
     e = launch_fill<T>(y, T(0), n, st);
   }
@@ -1087,8 +1107,7 @@ static hipError_t apply_lowp(icc_op* op, int prec, const float* x, float* y) {
   hipError_t e;
   const int64_t n = g->s.ndof;
   if (op->s_nnz > 0)
-    e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, x, y, op->d_free, nullptr,
-                                      st);
+    e = op_csr<float>(op, x, y, nullptr, st);
   else
     e = launch_fill<float>(y, 0.0f, n, st);
   if (e == hipSuccess) e = launch_faces<float>(op, x, y, nullptr, nullptr, st);
@@ -1215,16 +1234,14 @@ static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap
   hipError_t e = hipSuccess;
   if constexpr (std::is_same<T, double>::value) {
     if (op->s_nnz > 0)
-      e = launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, p, ap, op->d_free,
-                                          w->pap + elem_grid(op), st);
+      e = op_csr<double>(op, p, ap, w->pap + elem_grid(op), st);
     if (e == hipSuccess) e = launch_faces<double>(op, p, ap, w->pap + elem_grid(op) + csr_blocks(op), w->st, st);
     if (e == hipSuccess)
       e = op->det ? launch_apply_det<double>(g, op, p, ap, w->pap, w->st, (double*)op->d_E, st)
                   : launch_apply_f64(g, op, p, ap, w->pap, w->st, st);
   } else {
     if (op->s_nnz > 0)
-      e = launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, p, ap, op->d_free,
-                                        w->pap + elem_grid(op), st);
+      e = op_csr<float>(op, p, ap, w->pap + elem_grid(op), st);
     if (e == hipSuccess) e = launch_faces<float>(op, p, ap, w->pap + elem_grid(op) + csr_blocks(op), w->st, st);
     if (e == hipSuccess)
       e = prec == ICC_F32 ? (op->det ? launch_apply_det<float>(g, op, p, ap, w->pap, w->st, (float*)op->d_E, st)

@@ -17,6 +17,7 @@
 // Ogita, Rump and Oishi 2005).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "icc_internal.hpp"
@@ -496,31 +497,60 @@ __global__ __launch_bounds__(VB) void ir_update_kernel(int64_t n, double* __rest
     u[i] += s * (double)d[i];
 }
 
-// ---- CSR add-on: y = S x (overwrites y), per-row partials of x.(S x) -----------
+// ---- CSR add-on: y = S x (overwrites y), per-block partials of x.(S x) ----------
+// The add-on's rows with entries (`rows`, nrows of them) are few (penalty
+// and surface terms) and the rest of y must be zeroed, so the launch has two
+// parts:
+//   * blocks [0, rb): one wave per row with entries (rows w, w + waves, ...),
+//     the lanes over the row's entries, all of them loaded at once; the row
+//     sum is reduced across the wave in a fixed order, and each block writes
+//     one partial of x.(S x) (its waves' rows in order);
+//   * blocks [rb, gridDim.x): y = 0 on the rows without entries, four rows per
+//     thread.
+// Masked: rows and columns with freem = 0 contribute nothing (y = 0 there).
 template <typename T, typename TD>
 __global__ __launch_bounds__(VB) void csr_spmv_kernel(int64_t n, const int32_t* __restrict__ indptr,
                                                       const int32_t* __restrict__ indices, const TD* __restrict__ data,
+                                                      const int32_t* __restrict__ rows, int64_t nrows, int rb,
                                                       const T* __restrict__ x, T* __restrict__ y,
                                                       const uint8_t* __restrict__ freem, double* __restrict__ parts) {
   __shared__ double sh[16];
-  double acc = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    bool fr = freem == nullptr || freem[i] != 0;
-    T s = T(0);
-    if (fr) {
-      for (int k = indptr[i]; k < indptr[i + 1]; ++k) {
-        int c = indices[k];
-        T xv = x[c];
-        if (freem != nullptr && freem[c] == 0) xv = T(0);
+  if ((int)blockIdx.x < rb) {
+    constexpr int WPB = VB / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)rb * WPB;
+    double acc = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6); q < nrows; q += nw) {
+      const int i = rows[q];
+      const int k0 = indptr[i], k1 = indptr[i + 1];
+      T s = T(0);
+      for (int k = k0 + lane; k < k1; k += 64) {
+        const int c = indices[k];
+        const T xv = (freem == nullptr || freem[c] != 0) ? x[c] : T(0);
         s += (T)data[k] * xv;
       }
-      acc += (double)x[i] * (double)s;
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      const bool fr = freem == nullptr || freem[i] != 0;
+      if (!fr) s = T(0);
+      if (lane == 0) {
+        y[i] = s;
+        acc += (double)x[i] * (double)s;
+      }
     }
-    y[i] = s;
-  }
-  if (parts) {
-    double b = block_sum(acc, sh);
-    if (threadIdx.x == 0) parts[blockIdx.x] = b;
+    if (parts) {
+      const double b = block_sum(acc, sh);
+      if (threadIdx.x == 0) parts[blockIdx.x] = b;
+    }
+  } else {
+    const int64_t stride = (int64_t)(gridDim.x - rb) * VB * 4;
+    for (int64_t i0 = ((int64_t)(blockIdx.x - rb) * VB + threadIdx.x) * 4; i0 < n; i0 += stride) {
+      int32_t ip[5];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) ip[j] = i0 + j <= n ? indptr[i0 + j] : 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (i0 + j < n && ip[j] == ip[j + 1]) y[i0 + j] = T(0);
+    }
   }
 }
 
@@ -658,15 +688,19 @@ template hipError_t launch_ir_update<double>(int64_t, double*, const double*, do
 template hipError_t launch_ir_update<float>(int64_t, double*, const float*, double, hipStream_t);
 
 template <typename T, typename TD>
-hipError_t launch_csr_spmv(int64_t n, const int32_t* indptr, const int32_t* indices, const TD* data, const T* x, T* y,
-                           const uint8_t* freem, double* parts, hipStream_t s) {
-  hipLaunchKernelGGL((csr_spmv_kernel<T, TD>), dim3(vec_blocks(n)), dim3(VB), 0, s, n, indptr, indices, data, x, y,
-                     freem, parts);
+hipError_t launch_csr_spmv(int64_t n, const int32_t* indptr, const int32_t* indices, const TD* data,
+                           const int32_t* rows, int64_t nrows, int rb, const T* x, T* y, const uint8_t* freem,
+                           double* parts, hipStream_t s) {
+  const int64_t fb = std::min<int64_t>((n + 4 * VB - 1) / (4 * VB), 2048);
+  hipLaunchKernelGGL((csr_spmv_kernel<T, TD>), dim3((unsigned)(rb + fb)), dim3(VB), 0, s, n, indptr, indices, data,
+                     rows, nrows, rb, x, y, freem, parts);
   return hipGetLastError();
 }
 template hipError_t launch_csr_spmv<double, double>(int64_t, const int32_t*, const int32_t*, const double*,
-                                                    const double*, double*, const uint8_t*, double*, hipStream_t);
-template hipError_t launch_csr_spmv<float, float>(int64_t, const int32_t*, const int32_t*, const float*, const float*,
-                                                  float*, const uint8_t*, double*, hipStream_t);
+                                                    const int32_t*, int64_t, int, const double*, double*,
+                                                    const uint8_t*, double*, hipStream_t);
+template hipError_t launch_csr_spmv<float, float>(int64_t, const int32_t*, const int32_t*, const float*,
+                                                  const int32_t*, int64_t, int, const float*, float*, const uint8_t*,
+                                                  double*, hipStream_t);
 
 }  // namespace icc
