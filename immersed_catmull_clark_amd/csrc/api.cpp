@@ -522,9 +522,16 @@ static hipError_t op_csr(const icc_op* op, const T* x, T* y, double* parts, hipS
 }
 // partials of the element term: one per apply workgroup, or per gather block
 // in the deterministic mode
-static int elem_grid(const icc_op* op) { return op->det ? det_grid(op->g) : apply_grid(op->g); }
+// (the split-FP16 apply, prec = ICC_F16X, has its own tiles and one partial per
+// split-tile workgroup, so the layout below depends on the precision)
+static int elem_grid(const icc_op* op, int prec = ICC_F64) {
+  if (op->det) return det_grid(op->g);
+  return prec == ICC_F16X ? op->g->split_G : apply_grid(op->g);
+}
 // p.Ap partial layout of one apply: [elem_grid | csr_blocks | face_blocks]
-static int npap_of(const icc_op* op) { return elem_grid(op) + csr_blocks(op) + face_blocks(op); }
+static int npap_of(const icc_op* op, int prec = ICC_F64) {
+  return elem_grid(op, prec) + csr_blocks(op) + face_blocks(op);
+}
 
 // Lazily allocates the solver workspace of op.
 static int ensure_work(icc_op* op, Work*& out) {
@@ -1230,7 +1237,7 @@ static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap
                                     int prec = ICC_F32) {
   icc_grid* g = op->g;
   const int64_t n = g->s.ndof;
-  const int npap = npap_of(op);
+  const int npap = npap_of(op, std::is_same<T, double>::value ? ICC_F64 : prec);
   hipError_t e = hipSuccess;
   if constexpr (std::is_same<T, double>::value) {
     if (op->s_nnz > 0)
@@ -1241,8 +1248,9 @@ static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap
                   : launch_apply_f64(g, op, p, ap, w->pap, w->st, st);
   } else {
     if (op->s_nnz > 0)
-      e = op_csr<float>(op, p, ap, w->pap + elem_grid(op), st);
-    if (e == hipSuccess) e = launch_faces<float>(op, p, ap, w->pap + elem_grid(op) + csr_blocks(op), w->st, st);
+      e = op_csr<float>(op, p, ap, w->pap + elem_grid(op, prec), st);
+    if (e == hipSuccess)
+      e = launch_faces<float>(op, p, ap, w->pap + elem_grid(op, prec) + csr_blocks(op), w->st, st);
     if (e == hipSuccess)
       e = prec == ICC_F32 ? (op->det ? launch_apply_det<float>(g, op, p, ap, w->pap, w->st, (float*)op->d_E, st)
                                      : launch_apply_f32(g, op, p, ap, w->pap, w->st, st))

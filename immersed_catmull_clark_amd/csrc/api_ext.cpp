@@ -96,7 +96,8 @@ int solve_state_dev(icc_topopt* t, const double* x_dev, const double* u0_dev, do
   } else {
     ICC_HIP(launch_fill<double>(t->du, 0.0, n, st));
     double stats[8];
-    int rc = icc_solve_mixed_dev(t->op, t->b, tol, 0.1, 20 * n, t->du, stats);
+    int rc = icc_solve_mixed_prec_dev(t->op, c.solver == 2 ? ICC_F16X : ICC_F32, t->b, tol, 0.1, 20 * n, t->du,
+                                      stats);
     if (rc != ICC_OK && rc != ICC_ENOCONV && rc != ICC_EBREAKDOWN) return rc;
     its = (int64_t)stats[1];
     res = stats[2];
@@ -200,6 +201,10 @@ ICC_API int icc_topopt_destroy(icc_topopt* t) {
 
 ICC_API int icc_topopt_create(icc_ctx* ctx, const icc_topopt_config* cfg, icc_topopt** out) {
   if (!ctx || !cfg || !out) return ICC_EINVAL;
+  if (cfg->solver < 0 || cfg->solver > 2) {
+    set_error("icc_topopt_create: solver must be 0, 1 or 2");
+    return ICC_EINVAL;
+  }
   auto* t = new icc_topopt();
   t->ctx = ctx;
   t->cfg = *cfg;

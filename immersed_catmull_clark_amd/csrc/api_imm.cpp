@@ -118,6 +118,10 @@ ICC_API int icc_immersed_create(icc_ctx* ctx, const icc_immersed_config* cfg, co
     set_error("icc_immersed_create: mode must be 0 or 1");
     return ICC_EINVAL;
   }
+  if (cfg->solver < 0 || cfg->solver > 2) {
+    set_error("icc_immersed_create: solver must be 0, 1 or 2");
+    return ICC_EINVAL;
+  }
   for (int d = 0; d < 3; ++d)
     if (cfg->sub[d] < 1 || cfg->res[d] < 1) {
       set_error("icc_immersed_create: res and sub must be >= 1");
@@ -252,7 +256,8 @@ ICC_API int icc_immersed_solve_dev(icc_immersed* h, double* u, double* stats) {
     st8[4] = (double)iters;
   } else {
     ICC_HIP(launch_fill<double>(u, 0.0, n, st));
-    rc = icc_solve_mixed_dev(h->op, h->d_f, h->cfg.pcg_tol, 0.1, maxit, u, st8);
+    rc = icc_solve_mixed_prec_dev(h->op, h->cfg.solver == 2 ? ICC_F16X : ICC_F32, h->d_f, h->cfg.pcg_tol, 0.1, maxit,
+                                  u, st8);
     if (rc != ICC_OK && rc != ICC_ENOCONV && rc != ICC_EBREAKDOWN) return rc;
     const double upd = st8[0], its = st8[1], f64 = st8[3];
     st8[0] = its;

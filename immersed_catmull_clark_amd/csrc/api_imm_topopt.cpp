@@ -206,7 +206,7 @@ ICC_API int icc_imm_topopt_create(icc_ctx* ctx, const icc_imm_topopt_config* cfg
     set_error("icc_imm_topopt_create: unknown stabilization");
     return ICC_EINVAL;
   }
-  if (cfg->solver != 0 && cfg->solver != 1) return ICC_EINVAL;
+  if (cfg->solver < 0 || cfg->solver > 2) return ICC_EINVAL;
   auto* t = new icc_imm_topopt();
   t->ctx = ctx;
   t->cfg = *cfg;
@@ -368,7 +368,8 @@ int imm_iteration(icc_imm_topopt* t, double* c_out, double* vol_out, double* ch_
   } else {
     ICC_HIP(launch_fill<double>(t->du, 0.0, n, st));
     double stats[8];
-    int rc = icc_solve_mixed_dev(t->op, t->b, tol, 0.1, cf.pcg_maxit, t->du, stats);
+    int rc = icc_solve_mixed_prec_dev(t->op, cf.solver == 2 ? ICC_F16X : ICC_F32, t->b, tol, 0.1, cf.pcg_maxit, t->du,
+                                      stats);
     if (rc != ICC_OK && rc != ICC_ENOCONV && rc != ICC_EBREAKDOWN) return rc;
     its = (int64_t)stats[1];
   }
@@ -517,7 +518,7 @@ ICC_API int icc_immersed_mesh_create(icc_ctx* ctx, const icc_immersed_mesh_confi
     }
   if ((cfg->stabilization != ICC_STAB_NONE && cfg->stabilization != ICC_STAB_LEGACY &&
        cfg->stabilization != ICC_STAB_GHOST) ||
-      cfg->clamped_face < ICC_CLAMP_NONE || cfg->clamped_face > ICC_CLAMP_LEFT || (cfg->solver != 0 && cfg->solver != 1)) {
+      cfg->clamped_face < ICC_CLAMP_NONE || cfg->clamped_face > ICC_CLAMP_LEFT || cfg->solver < 0 || cfg->solver > 2) {
     set_error("icc_immersed_mesh_create: unknown stabilization, clamped face or solver");
     return ICC_EINVAL;
   }
@@ -617,7 +618,7 @@ ICC_API int icc_immersed_mesh_problem(const icc_immersed_mesh* h, int8_t* status
 }
 
 ICC_API int icc_immersed_mesh_set_solver(icc_immersed_mesh* h, double tol, int64_t maxit, int solver) {
-  if (!h || !(tol > 0.0) || maxit < 1 || (solver != 0 && solver != 1)) return ICC_EINVAL;
+  if (!h || !(tol > 0.0) || maxit < 1 || solver < 0 || solver > 2) return ICC_EINVAL;
   h->cfg.pcg_tol = tol;
   h->cfg.pcg_maxit = maxit;
   h->cfg.solver = solver;
@@ -646,7 +647,8 @@ ICC_API int icc_immersed_mesh_solve(icc_immersed_mesh* h, double* u, double* sta
     st8[1] = res;
   } else {
     ICC_HIP(launch_fill<double>(h->d_u, 0.0, n, st));
-    rc = icc_solve_mixed_dev(h->op, h->d_f, h->cfg.pcg_tol, 0.1, h->cfg.pcg_maxit, h->d_u, st8);
+    rc = icc_solve_mixed_prec_dev(h->op, h->cfg.solver == 2 ? ICC_F16X : ICC_F32, h->d_f, h->cfg.pcg_tol, 0.1,
+                                  h->cfg.pcg_maxit, h->d_u, st8);
     if (rc != ICC_OK && rc != ICC_ENOCONV && rc != ICC_EBREAKDOWN) return rc;
     const double upd = st8[0], its = st8[1];
     st8[0] = its;

@@ -87,7 +87,8 @@ class ImmersedOptions:
     """Fields and defaults of ImmersedOptions (immersed.rs:392-429); degree
     must be 2.  clamped_face: "bottom", "top", "left" or None; force: None
     for the default top-centre load.  Extension: solver 0 = FP64 Jacobi
-    PCG, 1 = FP32 PCG with FP64 reliable updates."""
+    PCG, 1 = FP32 PCG with FP64 reliable updates, 2 = the same with the
+    split-FP16 inner contraction."""
     grid_res: tuple = (24, 16, 12)
     padding: float = 0.05
     degree: int = 2
@@ -220,7 +221,8 @@ def solve_immersed_problem(pb, tol, maxit):
 class ImmersedTopOptConfig:
     """Fields and defaults of ImmersedTopOptConfig (topopt_immersed.rs:21-52),
     ghost penalty gamma 1e-2 by default.  Extension: solver 0 = FP64 Jacobi
-    PCG; 1 = FP32 PCG with FP64 reliable updates."""
+    PCG; 1 = FP32 PCG with FP64 reliable updates; 2 = the same with the
+    split-FP16 inner contraction."""
     grid_res: tuple = (28, 20, 16)
     volfrac: float = 0.35
     penal: float = 3.0

@@ -21,7 +21,8 @@ class ImmersedConfig:
     mode 0: scale emin + (1 - emin) w on every cell (immersed.rs:484); mode 1:
     scale w on cells with w >= active_min, other cells dropped.  solver 0: FP64
     Jacobi PCG (|r|/|b| < pcg_tol); 1: FP32 PCG with FP64 reliable updates to a
-    true relative residual <= pcg_tol."""
+    true relative residual <= pcg_tol; 2: the same with the split-FP16 inner
+    contraction."""
     box_lo: tuple = (0.0, 0.0, 0.0)
     box_hi: tuple = (2.0, 1.0, 1.0)
     res: tuple = (32, 16, 16)

@@ -19,7 +19,8 @@ class CantileverTopOptConfig:
     """Defaults of CantileverTopOptConfig::default (topopt_iga.rs:35-51).
     Extensions: load 0 = reference nodal load, 1 = unit downward traction on
     x = L; filter 0 = sensitivity filter, 1 = density filter (adjoint);
-    solver 0 = FP64 PCG, 1 = FP32 PCG with FP64 reliable updates."""
+    solver 0 = FP64 PCG, 1 = FP32 PCG with FP64 reliable updates, 2 = the same with the
+    split-FP16 inner contraction."""
     nel: tuple = (16, 8, 4)
     size: tuple = (1.2, 0.6, 0.3)
     degree: int = 2

@@ -327,7 +327,8 @@ typedef struct icc_topopt_config {
   double young, poisson, emin, move_limit, pcg_tol;
   int32_t load;   /* 0: reference nodal load (topopt_iga.rs:79-98); 1: unit downward traction on x = L */
   int32_t filter; /* 0: sensitivity filter (topopt_iga.rs:200-205); 1: density filter with adjoint backward */
-  int32_t solver; /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates to the same true residual */
+  int32_t solver; /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates to the same true residual;
+                     2: the same with the split-FP16 inner contraction (ICC_F16X) */
   int32_t pad;
 } icc_topopt_config;
 /* CantileverTopOptConfig::default (topopt_iga.rs:35-51) */
@@ -368,7 +369,7 @@ int icc_topopt_run(icc_topopt* t, icc_report_fn report, void* user, double* comp
  *  - consistent traction `traction` on the face x = box_hi[0];
  *  - Jacobi PCG on the free DOFs, diagonal clamped at 1e-6 (immersed.rs:555),
  *    stop at |r| / |b| < pcg_tol (solver 0), or the mixed solver of
- *    icc_solve_mixed_dev to a true relative residual <= pcg_tol (solver 1). */
+ *    icc_solve_mixed_prec_dev to a true relative residual <= pcg_tol (solvers 1, 2). */
 typedef struct icc_immersed icc_immersed;
 typedef struct icc_immersed_config {
   double box_lo[3], box_hi[3];
@@ -379,7 +380,8 @@ typedef struct icc_immersed_config {
   double pcg_tol;
   int64_t pcg_maxit; /* 0: 20 * ndof */
   int32_t mode;      /* 0: reference scale on all cells; 1: drop cells with w < active_min */
-  int32_t solver;    /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates */
+  int32_t solver;    /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates; 2: the same with the
+                        split-FP16 inner contraction (ICC_F16X) */
 } icc_immersed_config;
 /* box [0,2]x[0,1]x[0,1], 32x16x16 elements, 4^3 sub-cells, E = 1, nu = 0.3,
  * emin = 1e-4, active_min = 0.05, penalty factor 1e3, traction (0,-1,0),
@@ -398,7 +400,7 @@ int icc_immersed_problem(const icc_immersed* h, int8_t* status, double* weights,
 /* y = M (K_s + P) M x on host vectors (the solver's operator) */
 int icc_immersed_apply(icc_immersed* h, const double* x, double* y);
 /* u [ndof] from zero; stats (8 doubles, may be NULL): iterations, recursive
- * relative residual (solver 0) or reliable updates (solver 1), true relative
+ * relative residual (solver 0) or reliable updates (solvers 1, 2), true relative
  * residual, compliance F.u, FP64 applies, 0, 0, 0.  Returns ICC_ENOCONV /
  * ICC_EBREAKDOWN like icc_pcg with u filled. */
 int icc_immersed_solve(icc_immersed* h, double* u, double* stats);
@@ -413,7 +415,7 @@ int icc_immersed_solve_dev(icc_immersed* h, double* u_dev, double* stats);
  * with force = NULL, -1/count on z at the top-face control points with
  * |i - round(n/2)| <= 2), and Jacobi PCG on the free DOFs with the diagonal
  * clamped at 1e-6 (solver 0) or the mixed solver to the same true residual
- * (solver 1). */
+ * (solvers 1, 2). */
 enum { ICC_CLAMP_NONE = -1, ICC_CLAMP_BOTTOM = 0, ICC_CLAMP_TOP = 1, ICC_CLAMP_LEFT = 2 };
 typedef struct icc_immersed_mesh icc_immersed_mesh;
 typedef struct icc_immersed_mesh_config {
@@ -427,7 +429,8 @@ typedef struct icc_immersed_mesh_config {
   int64_t subcell_res[3];
   double pcg_tol;
   int64_t pcg_maxit;
-  int32_t solver; /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates */
+  int32_t solver; /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates; 2: the same with the
+                     split-FP16 inner contraction (ICC_F16X) */
   int32_t pad;
 } icc_immersed_mesh_config;
 /* ImmersedOptions::default: 24x16x12, padding 0.05, E = 1, nu = 0.3, ghost
@@ -450,7 +453,7 @@ int icc_immersed_mesh_set_solver(icc_immersed_mesh* h, double tol, int64_t maxit
 int icc_immersed_mesh_apply(icc_immersed_mesh* h, const double* x, double* y);
 /* u [ndof] (zero on fixed DOFs, like ImmersedSolution::u); stats (8, may be
  * NULL): iterations, recursive relative residual (solver 0) or reliable
- * updates (solver 1), true relative residual, compliance F.u, 0, 0, 0, 0.
+ * updates (solvers 1, 2), true relative residual, compliance F.u, 0, 0, 0, 0.
  * ICC_ENOCONV / ICC_EBREAKDOWN like icc_pcg with u filled. */
 int icc_immersed_mesh_solve(icc_immersed_mesh* h, double* u, double* stats);
 
@@ -525,7 +528,8 @@ typedef struct icc_imm_topopt_config {
   int64_t max_iter;
   double young, poisson, emin;
   int32_t stabilization; /* ICC_STAB_NONE, ICC_STAB_LEGACY or ICC_STAB_GHOST */
-  int32_t solver;        /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates */
+  int32_t solver;        /* 0: FP64 Jacobi PCG; 1: FP32 PCG with FP64 reliable updates; 2: the same with
+                            the split-FP16 inner contraction (ICC_F16X) */
   double gamma;          /* stabilisation parameter */
   double pcg_tol;
   int64_t pcg_maxit;

@@ -6,8 +6,9 @@ properties that do not depend on the oracle assembling the whole system:
   central finite differences of the compliance (h = 1e-6) on seeded
   elements, every solve to 1e-12; two OC iterations keep the volume;
 * the immersed 128 x 64 x 64 problem (box minus three seeded cylinders): the
-  mixed solve's true residual recomputed with the problem's FP64 operator,
-  and its solution against the FP64 solve;
+  mixed solves' (FP32 and split-FP16 inner operators) true residuals
+  recomputed with the problem's FP64 operator, and their solutions against
+  the FP64 solve;
 * the batch of 24 uneven instances, 24 x 12 x 12 up to 160 x 80 x 80 (24.2M
   DOF), autotuned, in one mixed solve: every instance's reported true
   residual <= 1e-8, recomputed independently for three of them (the oracle's
@@ -79,7 +80,7 @@ def test_immersed_full_size_mixed_vs_fp64(ctx):
     rng = np.random.default_rng(20261204)
     cyl = np.column_stack([rng.uniform(0.4, 1.6, 3), rng.uniform(0.3, 0.7, 3), rng.uniform(0.08, 0.15, 3)])
     sols = {}
-    for solver in (0, 1):
+    for solver in (0, 1, 2):
         h = icc.ImmersedSdf(ctx, icc.ImmersedConfig(res=(128, 64, 64), mode=1, solver=solver, pcg_tol=1e-8), cyl)
         u, st, rc = h.solve()
         assert rc == icc.ICC_OK and st["true_res"] <= 1e-8
@@ -89,8 +90,9 @@ def test_immersed_full_size_mixed_vs_fp64(ctx):
         assert abs(st["compliance"] - float(h.force @ u)) <= 1e-12 * abs(st["compliance"])
         sols[solver] = (u, st["compliance"])
         h.close()
-    (u64, c64), (um, cm) = sols[0], sols[1]
-    assert O.rel_err(um, u64) < 1e-6 and abs(cm - c64) / abs(c64) < 1e-8
+    for s in (1, 2):
+        (u64, c64), (um, cm) = sols[0], sols[s]
+        assert O.rel_err(um, u64) < 1e-6 and abs(cm - c64) / abs(c64) < 1e-8, s
 
 
 def test_batch_24_instances_full_size(ctx):

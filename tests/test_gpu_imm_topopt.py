@@ -186,13 +186,15 @@ def test_imm_topopt_matches_oracle(ctx, res):
     assert np.all(np.abs(np.asarray(h.pcg_iterations) - its) <= 2 + 0.06 * its), (h.pcg_iterations, its)
 
 
-def test_imm_topopt_mixed_solver(ctx):
-    """solver 1 (FP32 PCG with FP64 reliable updates) reaches the same true
-    residual, so the loop tracks the FP64 one closely."""
+@pytest.mark.parametrize("solver", [1, 2])
+def test_imm_topopt_mixed_solver(ctx, solver):
+    """solver 1 (FP32 PCG with FP64 reliable updates; 2: split-FP16 inner
+    contraction) reaches the same true residual, so the loop tracks the FP64
+    one closely."""
     d, m = _tilted("imm_topopt_tilted_box.json")
     res = (10, 8, 6)
     h64 = icc.ImmersedTopOpt(ctx, m, _topopt_cfg(d, max_iter=4, res=res)).run()
-    h32 = icc.ImmersedTopOpt(ctx, m, _topopt_cfg(d, max_iter=4, res=res, solver=1)).run()
+    h32 = icc.ImmersedTopOpt(ctx, m, _topopt_cfg(d, max_iter=4, res=res, solver=solver)).run()
     c64, c32 = np.asarray(h64.compliance), np.asarray(h32.compliance)
     assert np.all(np.abs(c32 - c64) / c64 < 1e-6), (c32, c64)
     assert np.max(np.abs(h32.density - h64.density)) < 1e-4
@@ -352,8 +354,9 @@ def test_build_immersed_problem_matches_oracle(ctx, stab, clamp):
     u, c, it, res_ = ref.solve(1e-12, 200000)
     assert O.rel_err(sol.u, u) < 1e-8, (O.rel_err(sol.u, u), sol.iterations, it)
     assert abs(sol.compliance - c) / abs(c) < 1e-9
-    mixed = pb.solve(tol=1e-10, solver=1)
-    assert mixed.true_residual <= 1e-10 and O.rel_err(mixed.u, u) < 1e-6
+    for solver in (1, 2):
+        mixed = pb.solve(tol=1e-10, solver=solver)
+        assert mixed.true_residual <= 1e-10 and O.rel_err(mixed.u, u) < 1e-6, solver
 
 
 def test_build_immersed_problem_custom_force_top_clamp(ctx):

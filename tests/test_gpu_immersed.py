@@ -70,10 +70,12 @@ def test_fp64_solve_matches_oracle(ctx):
     assert np.linalg.norm(r) / np.linalg.norm(o.force) < 2e-10
 
 
-def test_mixed_solve_true_residual(ctx):
-    """FP32 PCG with FP64 reliable updates: true residual (oracle operator)
-    <= 1e-8 and the solution within the FP64 one to the solve's conditioning."""
-    g, o = _pair(ctx, [24, 12, 12], 1, solver=1, tol=1e-8)
+@pytest.mark.parametrize("solver", [1, 2])
+def test_mixed_solve_true_residual(ctx, solver):
+    """FP32 PCG with FP64 reliable updates (solver 1; 2: split-FP16 inner
+    contraction): true residual (oracle operator) <= 1e-8 and the solution
+    within the FP64 one to the solve's conditioning."""
+    g, o = _pair(ctx, [24, 12, 12], 1, solver=solver, tol=1e-8)
     u, st, rc = g.solve()
     assert rc == icc.ICC_OK, st
     r = o.free * (o.force - o.matvec(u))

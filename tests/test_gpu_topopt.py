@@ -153,12 +153,14 @@ def test_traction_load_total(ctx):
     assert abs(t.force[nsc:2 * nsc].sum() + 0.5) < 1e-13
 
 
-def test_mixed_solver_state_matches_fp64(ctx):
-    """solver 1 (FP32 PCG with FP64 reliable updates) reaches the state of the
-    FP64 solve: compliance and dc/dx to the tolerance's level."""
+@pytest.mark.parametrize("solver", [1, 2])
+def test_mixed_solver_state_matches_fp64(ctx, solver):
+    """solver 1 (FP32 PCG with FP64 reliable updates; 2: split-FP16 inner
+    contraction) reaches the state of the FP64 solve: compliance and dc/dx to
+    the tolerance's level."""
     base = dict(nel=(16, 8, 8), size=(2.0, 1.0, 1.0), emin=1e-9, pcg_tol=1e-10, load=1, filter=1)
     t64 = icc.CantileverTopOpt(ctx, icc.CantileverTopOptConfig(solver=0, **base))
-    t32 = icc.CantileverTopOpt(ctx, icc.CantileverTopOptConfig(solver=1, **base))
+    t32 = icc.CantileverTopOpt(ctx, icc.CantileverTopOptConfig(solver=solver, **base))
     x = np.random.default_rng(9).uniform(0.05, 1.0, t64.nel)
     c64, g64, u64 = t64.compliance(x)
     c32, g32, u32 = t32.compliance(x)
