@@ -471,6 +471,8 @@ def main():
                     help="run only the WQ formation leg on the 96x48x48 grid and exit")
     ap.add_argument("--imm-only", action="store_true",
                     help="run one design iteration of the 96x72x42 immersed topology optimisation and exit")
+    ap.add_argument("--immersed-only", action="store_true",
+                    help="run only the immersed 128x64x64 leg (FP64 and both mixed solvers) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy of 735,000 doubles 20 times and exit")
     args = ap.parse_args()
@@ -502,6 +504,11 @@ def main():
         print(json.dumps({"workload": "immersed topopt, tilted box, 96x72x42, 1 iteration, ghost penalty 1e-2",
                           "ndof": opt.ndof, "faces": opt.n_faces, "run_s": time.time() - t0,
                           "pcg_iterations": h.pcg_iterations, "compliance": h.compliance}), flush=True)
+        return
+
+    if args.immersed_only:
+        ctx = icc.Context(0)
+        print(json.dumps({"immersed_128x64x64": immersed_leg(np, icc, ctx)}), flush=True)
         return
 
     log("bench: setting up the 96x48x48 grid")

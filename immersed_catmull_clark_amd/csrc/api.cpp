@@ -773,10 +773,11 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
       d32[k] = (float)data[k];
       if (indices[k] == i) diag[i] += data[k];
     }
+  // {row, first entry, end, 0} of every row with entries
   std::vector<int32_t> rows;
   for (int64_t i = 0; i < n; ++i)
-    if (indptr[i + 1] > indptr[i]) rows.push_back((int32_t)i);
-  ICC_TRY(dalloc(&op->d_s_rows, std::max<int64_t>((int64_t)rows.size(), 1)));
+    if (indptr[i + 1] > indptr[i]) rows.insert(rows.end(), {(int32_t)i, ip[i], ip[i + 1], 0});
+  ICC_TRY(dalloc(&op->d_s_rows, std::max<int64_t>((int64_t)rows.size(), 4)));
   ICC_TRY(dalloc(&op->d_s_indptr, n + 1));
   ICC_TRY(dalloc(&op->d_s_indices, nnz));
   ICC_TRY(dalloc(&op->d_s_data, nnz));
@@ -792,7 +793,7 @@ ICC_API int icc_op_set_sparse(icc_op* op, int64_t nrows, const int64_t* indptr, 
     ICC_HIP(hipMemcpyAsync(op->d_s_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, st));
   ICC_HIP(hipStreamSynchronize(st));
   op->s_nnz = nnz;
-  op->s_nrows = (int64_t)rows.size();
+  op->s_nrows = (int64_t)rows.size() / 4;
   op->h_s_indptr.assign(indptr, indptr + n + 1);
   op->h_s_indices.assign(indices, indices + nnz);
   op->h_s_data.assign(data, data + nnz);

@@ -218,7 +218,7 @@ struct icc_op {
   double* d_s_data = nullptr;
   float* d_s_data32 = nullptr;
   double* d_s_diag = nullptr;
-  int32_t* d_s_rows = nullptr;  // the rows with entries
+  int32_t* d_s_rows = nullptr;  // {row, first entry, end, 0} per row with entries
   int64_t s_nrows = 0;
   std::vector<int64_t> h_s_indptr, h_s_indices;  // host copy of the add-on (plus_triplets)
   std::vector<double> h_s_data;

@@ -501,10 +501,11 @@ __global__ __launch_bounds__(VB) void ir_update_kernel(int64_t n, double* __rest
 // The add-on's rows with entries (`rows`, nrows of them) are few (penalty
 // and surface terms) and the rest of y must be zeroed, so the launch has two
 // parts:
-//   * blocks [0, rb): one wave per row with entries (rows w, w + waves, ...),
-//     the lanes over the row's entries, all of them loaded at once; the row
-//     sum is reduced across the wave in a fixed order, and each block writes
-//     one partial of x.(S x) (its waves' rows in order);
+//   * blocks [0, rb): each wave takes four rows with entries per step (a
+//     {row, first entry, end, 0} record each), the lanes over a row's
+//     entries, the first 64 entries of all four loaded at once; each row sum
+//     is reduced across the wave in a fixed order, and each block writes one
+//     partial of x.(S x) (its waves' rows in order);
 //   * blocks [rb, gridDim.x): y = 0 on the rows without entries, four rows per
 //     thread.
 // Masked: rows and columns with freem = 0 contribute nothing (y = 0 there).
@@ -516,25 +517,61 @@ __global__ __launch_bounds__(VB) void csr_spmv_kernel(int64_t n, const int32_t* 
                                                       const uint8_t* __restrict__ freem, double* __restrict__ parts) {
   __shared__ double sh[16];
   if ((int)blockIdx.x < rb) {
-    constexpr int WPB = VB / 64;
+    // each wave takes CG listed rows per step ({row, first entry, end, 0}
+    // records) and issues the first 64 entries' loads of all of them before
+    // any arithmetic, so their dependent loads (entry -> column -> x) overlap
+    constexpr int WPB = VB / 64, CG = 4;
     const int lane = threadIdx.x & 63;
     const int64_t nw = (int64_t)rb * WPB;
+    const int4* __restrict__ info = reinterpret_cast<const int4*>(rows);
     double acc = 0.0;
-    for (int64_t q = (int64_t)blockIdx.x * WPB + (threadIdx.x >> 6); q < nrows; q += nw) {
-      const int i = rows[q];
-      const int k0 = indptr[i], k1 = indptr[i + 1];
-      T s = T(0);
-      for (int k = k0 + lane; k < k1; k += 64) {
-        const int c = indices[k];
-        const T xv = (freem == nullptr || freem[c] != 0) ? x[c] : T(0);
-        s += (T)data[k] * xv;
+    for (int64_t q0 = ((int64_t)blockIdx.x * WPB + (threadIdx.x >> 6)) * CG; q0 < nrows; q0 += nw * CG) {
+      int ri[CG], k0[CG], k1[CG], c[CG];
+      TD dv[CG];
+      T s[CG];
+#pragma unroll
+      for (int j = 0; j < CG; ++j) {
+        if (q0 + j < nrows) {
+          const int4 v = info[q0 + j];
+          ri[j] = v.x;
+          k0[j] = v.y;
+          k1[j] = v.z;
+        } else {
+          ri[j] = -1;
+          k0[j] = k1[j] = 0;
+        }
       }
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      const bool fr = freem == nullptr || freem[i] != 0;
-      if (!fr) s = T(0);
-      if (lane == 0) {
-        y[i] = s;
-        acc += (double)x[i] * (double)s;
+#pragma unroll
+      for (int j = 0; j < CG; ++j) {
+        const int k = k0[j] + lane;
+        c[j] = k < k1[j] ? indices[k] : -1;
+        dv[j] = k < k1[j] ? data[k] : TD(0);
+      }
+#pragma unroll
+      for (int j = 0; j < CG; ++j) {
+        const bool on = c[j] >= 0 && (freem == nullptr || freem[c[j] >= 0 ? c[j] : 0] != 0);
+        const T xv = c[j] >= 0 ? x[c[j]] : T(0);
+        s[j] = on ? (T)dv[j] * xv : T(0);
+      }
+      // rows of more than 64 entries: the rest in order, 64 at a time
+#pragma unroll
+      for (int j = 0; j < CG; ++j)
+        for (int k = k0[j] + 64 + lane; k < k1[j]; k += 64) {
+          const int cc = indices[k];
+          const T xv = (freem == nullptr || freem[cc] != 0) ? x[cc] : T(0);
+          s[j] += (T)data[k] * xv;
+        }
+#pragma unroll
+      for (int j = 0; j < CG; ++j) {
+        for (int off = 32; off > 0; off >>= 1) s[j] += __shfl_xor(s[j], off, 64);
+        if (ri[j] >= 0) {
+          const bool fr = freem == nullptr || freem[ri[j]] != 0;
+          if (!fr) s[j] = T(0);
+          if (lane == 0) {
+            y[ri[j]] = s[j];
+            acc += (double)x[ri[j]] * (double)s[j];
+          }
+        }
       }
     }
     if (parts) {
