@@ -16,9 +16,10 @@
 // mu (d_ij d_{k1 k2} + d_{i k2} d_{j k1})).
 //
 // One wave per row: s_loc and the three 4 x 5 x 9 tables are staged in
-// LDS; the first mode product is shared by all (k1, k2) (4 x 405 values),
-// the second and third are done per (k1, k2), and each lane keeps the 9
-// component blocks of its 2 stencil entries in registers.
+// LDS; the first mode product is shared by the (k1, k2) pairs that use the
+// same direction-0 table (four groups, one 405-value product held at a
+// time), the second and third are done per (k1, k2), and each lane keeps the
+// 9 component blocks of its 2 stencil entries in registers.
 //
 // Storage: S[(3 i + j) * 125 + o][P] with o = (dx+2) + 5 (dy+2) + 25 (dz+2),
 // coalesced over P; entries whose column falls outside the grid are zero.
@@ -73,6 +74,14 @@ __global__ __launch_bounds__(256) void wq_simp_kernel(int kind, int np0, int np1
   }
 }
 
+// The (k1, k2) pairs grouped by the table of direction 0 they use (t0), so
+// that only one t0's mode-0 product is held at a time: t0 = 3 for (0, 0);
+// 0 for (1, 1), (2, 2), (1, 2), (2, 1); 1 for (0, 1), (0, 2); 2 for (1, 0),
+// (2, 0).  Entries: k1 * 3 + k2, groups at WQ_GRP.
+__constant__ int WQ_PAIRS[9] = {0, 4, 8, 5, 7, 1, 2, 3, 6};
+__constant__ int WQ_GRP[5] = {0, 1, 5, 7, 9};
+__constant__ int WQ_GRP_T0[4] = {3, 0, 1, 2};
+
 __global__ __launch_bounds__(64 * WQ_WAVES) void wq_form_kernel(int ncp0, int ncp1, int ncp2, int np0, int np1,
                                                                 int np2, const int* __restrict__ pstart,
                                                                 const double* __restrict__ tab0,
@@ -80,8 +89,10 @@ __global__ __launch_bounds__(64 * WQ_WAVES) void wq_form_kernel(int ncp0, int nc
                                                                 const double* __restrict__ tab2,
                                                                 const double* __restrict__ simp, WqCoef cf,
                                                                 double* __restrict__ S) {
+  // 15 KB of LDS per wave (one t0's mode-0 product at a time), so ten
+  // waves fit on a CU
   __shared__ double sm[WQ_WAVES][729];
-  __shared__ double m0[WQ_WAVES][4 * 405];
+  __shared__ double m0[WQ_WAVES][405];
   __shared__ double tm[WQ_WAVES][225];
   __shared__ double tb[WQ_WAVES][3][WQ_TAB];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -105,31 +116,32 @@ __global__ __launch_bounds__(64 * WQ_WAVES) void wq_form_kernel(int ncp0, int nc
     tb[w][2][k] = tab2[(int64_t)i2 * WQ_TAB + k];
   }
   __syncthreads();
-  // mode 0 for t0 = 0..3: m0[t0][jr0][q1][q2] = sum_q0 B0[t0][jr0][q0] s[q0][q1][q2]
-  for (int k = l; k < 4 * 405; k += 64) {
-    const int t0 = k / 405, r = k - 405 * t0;
-    const int jr0 = r / 81, q12 = r - 81 * jr0;
-    const double* B = &tb[w][0][(t0 * 5 + jr0) * 9];
-    const double* s = &sm[w][q12 * 9];
-    double a = 0.0;
-#pragma unroll
-    for (int q0 = 0; q0 < 9; ++q0) a += B[q0] * s[q0];
-    m0[w][k] = a;
-  }
-  __syncthreads();
   double acc[9][2];
 #pragma unroll
   for (int a = 0; a < 9; ++a) acc[a][0] = acc[a][1] = 0.0;
-  for (int k1 = 0; k1 < 3; ++k1)
-    for (int k2 = 0; k2 < 3; ++k2) {
+  for (int gi = 0; gi < 4; ++gi) {
+    const int t0 = WQ_GRP_T0[gi];
+    // mode 0: m0[jr0][q1][q2] = sum_q0 B0[t0][jr0][q0] s[q0][q1][q2]
+    for (int k = l; k < 405; k += 64) {
+      const int jr0 = k / 81, q12 = k - 81 * jr0;
+      const double* B = &tb[w][0][(t0 * 5 + jr0) * 9];
+      const double* sv = &sm[w][q12 * 9];
+      double a = 0.0;
+#pragma unroll
+      for (int q0 = 0; q0 < 9; ++q0) a += B[q0] * sv[q0];
+      m0[w][k] = a;
+    }
+    __syncthreads();
+    for (int pi = WQ_GRP[gi]; pi < WQ_GRP[gi + 1]; ++pi) {
+      const int k1 = WQ_PAIRS[pi] / 3, k2 = WQ_PAIRS[pi] % 3;
       int t[3];
       for (int d = 0; d < 3; ++d)
         t[d] = (k1 == k2) ? (d == k1 ? 3 : 0) : (d == k1 ? 1 : (d == k2 ? 2 : 0));
-      // mode 1: tm[jr0][jr1][q2] = sum_q1 B1[t1][jr1][q1] m0[t0][jr0][q1][q2]
+      // mode 1: tm[jr0][jr1][q2] = sum_q1 B1[t1][jr1][q1] m0[jr0][q1][q2]
       for (int k = l; k < 225; k += 64) {
         const int jr0 = k % 5, jr1 = (k / 5) % 5, q2 = k / 25;
         const double* B = &tb[w][1][(t[1] * 5 + jr1) * 9];
-        const double* m = &m0[w][t[0] * 405 + jr0 * 81 + q2 * 9];
+        const double* m = &m0[w][jr0 * 81 + q2 * 9];
         double a = 0.0;
 #pragma unroll
         for (int q1 = 0; q1 < 9; ++q1) a += B[q1] * m[q1];
@@ -152,6 +164,7 @@ __global__ __launch_bounds__(64 * WQ_WAVES) void wq_form_kernel(int ncp0, int nc
       }
       __syncthreads();
     }
+  }
   if (!live) return;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
