@@ -377,3 +377,59 @@ def test_balanced_partition_splits_tiles_and_keeps_the_operator(ctx):
         assert O.rel_err(op.apply(x, prec=icc.ICC_BF16), refb) < 1e-5, (v, wg)
         xv, _, _, _ = op.pcg(b, diag, 0.0, 30)
         assert O.rel_err(xv, xp) < 1e-10, (v, wg)
+
+
+def _random_spd_addon(n, rng, rows_by_len):
+    """Symmetric, diagonally dominant sparse add-on whose rows have the given
+    numbers of off-diagonal entries (before symmetrisation), as CSR arrays."""
+    import scipy.sparse as sps
+    r, c, v = [], [], []
+    for ln, rows in rows_by_len:
+        for i in rows:
+            cols = rng.choice(n, ln, replace=False)
+            r += [i] * ln
+            c += list(cols)
+            v += list(rng.uniform(-1.0, 1.0, ln))
+    t = sps.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+    s = (t + t.T).tocsr()
+    s = s + sps.diags(np.asarray(abs(s).sum(axis=1)).ravel() + 1e-3 * (np.asarray(abs(s).sum(axis=1)).ravel() > 0))
+    s = s.tocsr()
+    s.sort_indices()
+    return s
+
+
+def test_sparse_addon_long_rows_apply_and_pcg(ctx):
+    # the CSR add-on (y = S x beside the element term) with rows of 0 to a few
+    # hundred entries: the wave-per-row product takes 64 entries per pass, so
+    # rows of 64, 65, 130 and more exercise its remainder passes; applies in
+    # FP64 and FP32 against scipy, and a PCG solve of M (K + S) M u = f whose
+    # true residual is recomputed with the oracle's K and scipy's S
+    nel = [8, 4, 4]
+    bounds, sp, s, free, f = _cantilever_problem(nel, 17, rho=(0.3, 1.0), emin=1e-3)
+    n = sp.ndof
+    rng = np.random.default_rng(29)
+    pick = rng.permutation(n)
+    S = _random_spd_addon(n, rng, [(10, pick[:40]), (63, pick[40:50]), (64, pick[50:55]), (65, pick[55:60]),
+                                   (130, pick[60:64]), (300, pick[64:66])])
+    assert S.getnnz(axis=1).max() > 300
+    g = icc.Grid(ctx, nel, bounds)
+    op = icc.Operator(g, scale=s, free=free)
+    ops = icc.Operator(g, scale=s, free=free)
+    ops.set_sparse(S.indptr, S.indices, S.data)
+    m = free.astype(np.float64)
+    for trial in range(2):
+        x = rng.standard_normal(n)
+        ref = m * (S @ (m * x))
+        y = ops.apply(x) - op.apply(x)
+        assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+        y32 = ops.apply(x, prec=icc.ICC_F32) - op.apply(x, prec=icc.ICC_F32)
+        assert np.linalg.norm(y32 - ref) <= 2e-6 * np.linalg.norm(ops.apply(x))
+    d = np.maximum(ops.diag(), 1e-12)
+    u, it, res, rc = ops.pcg(f, d, 1e-10, 20 * n)
+    assert rc == icc.ICC_OK and res < 1e-10
+    sp.assemble(s)
+    r = m * (f - sp.matvec(m * u) - S @ (m * u))
+    assert np.linalg.norm(r) / np.linalg.norm(f) < 1e-9
+    op.close()
+    ops.close()
+    g.close()
