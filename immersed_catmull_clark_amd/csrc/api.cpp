@@ -509,16 +509,17 @@ static void drop_work(icc_op* op) {
 }
 
 static int csr_blocks(const icc_op* op) { return op->s_nnz > 0 ? vec_blocks(op->g->s.ndof) : 0; }
-// y = S x (overwrites y) in precision T, csr_blocks(op) partials into parts
+// y = S x (overwrites y) in precision T, csr_blocks(op) partials into parts;
+// y_zeroed: y is known to be zero already (rows without entries left alone)
 template <typename T>
-static hipError_t op_csr(const icc_op* op, const T* x, T* y, double* parts, hipStream_t st) {
+static hipError_t op_csr(const icc_op* op, const T* x, T* y, double* parts, hipStream_t st, bool y_zeroed = false) {
   const int64_t n = op->g->s.ndof;
   if constexpr (std::is_same<T, double>::value)
     return launch_csr_spmv<double, double>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data, op->d_s_rows,
-                                           op->s_nrows, csr_blocks(op), x, y, op->d_free, parts, st);
+                                           op->s_nrows, csr_blocks(op), x, y, op->d_free, parts, y_zeroed, st);
   else
     return launch_csr_spmv<float, float>(n, op->d_s_indptr, op->d_s_indices, op->d_s_data32, op->d_s_rows,
-                                         op->s_nrows, csr_blocks(op), x, y, op->d_free, parts, st);
+                                         op->s_nrows, csr_blocks(op), x, y, op->d_free, parts, y_zeroed, st);
 }
 // partials of the element term: one per apply workgroup, or per gather block
 // in the deterministic mode
@@ -1232,7 +1233,9 @@ static int write_tol(icc_op* op, Work* w, double tol) {
 }
 
 // One PCG iteration in precision T enqueued on the op stream (sparse term
-// first, then the tiles, then the two vector kernels).
+// first, then the tiles, then the two vector kernels).  Ap arrives zeroed
+// (pcg_init1, the previous pcg_update2, or pdir_update after a reliable
+// update), so the terms only add to it and the sparse term skips zeroing.
 template <typename T>
 static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap, const T* dinv, hipStream_t st,
                                     int prec = ICC_F32) {
@@ -1242,14 +1245,14 @@ static hipError_t enqueue_iteration(icc_op* op, Work* w, T* x, T* r, T* p, T* ap
   hipError_t e = hipSuccess;
   if constexpr (std::is_same<T, double>::value) {
     if (op->s_nnz > 0)
-      e = op_csr<double>(op, p, ap, w->pap + elem_grid(op), st);
+      e = op_csr<double>(op, p, ap, w->pap + elem_grid(op), st, true);
     if (e == hipSuccess) e = launch_faces<double>(op, p, ap, w->pap + elem_grid(op) + csr_blocks(op), w->st, st);
     if (e == hipSuccess)
       e = op->det ? launch_apply_det<double>(g, op, p, ap, w->pap, w->st, (double*)op->d_E, st)
                   : launch_apply_f64(g, op, p, ap, w->pap, w->st, st);
   } else {
     if (op->s_nnz > 0)
-      e = op_csr<float>(op, p, ap, w->pap + elem_grid(op, prec), st);
+      e = op_csr<float>(op, p, ap, w->pap + elem_grid(op, prec), st, true);
     if (e == hipSuccess)
       e = launch_faces<float>(op, p, ap, w->pap + elem_grid(op, prec) + csr_blocks(op), w->st, st);
     if (e == hipSuccess)

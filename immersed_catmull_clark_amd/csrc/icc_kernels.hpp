@@ -116,11 +116,12 @@ template <typename T>
 hipError_t launch_ir_update(int64_t n, double* u, const T* d, double scale, hipStream_t s);
 // y = S x (overwrites y), masked; rows[nrows]: the rows with entries; rb
 // blocks for those rows (one partial each in parts, if not null), the rest of
-// the launch zeroes the other rows
+// the launch zeroes the other rows; y_zeroed: y is already zero (the PCG
+// iteration's Ap, cleared by the previous update), so the zeroing is skipped
 template <typename T, typename TD>
 hipError_t launch_csr_spmv(int64_t n, const int32_t* indptr, const int32_t* indices, const TD* data,
                            const int32_t* rows, int64_t nrows, int rb, const T* x, T* y, const uint8_t* freem,
-                           double* parts, hipStream_t s);
+                           double* parts, bool y_zeroed, hipStream_t s);
 
 hipError_t launch_clamp_min(int64_t n, const double* a, double lo, double* b, hipStream_t st);
 hipError_t launch_simp_scale(int64_t n, const double* x, double penal, double emin, const double* w,

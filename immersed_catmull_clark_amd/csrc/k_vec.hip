@@ -727,17 +727,17 @@ template hipError_t launch_ir_update<float>(int64_t, double*, const float*, doub
 template <typename T, typename TD>
 hipError_t launch_csr_spmv(int64_t n, const int32_t* indptr, const int32_t* indices, const TD* data,
                            const int32_t* rows, int64_t nrows, int rb, const T* x, T* y, const uint8_t* freem,
-                           double* parts, hipStream_t s) {
-  const int64_t fb = std::min<int64_t>((n + 4 * VB - 1) / (4 * VB), 2048);
+                           double* parts, bool y_zeroed, hipStream_t s) {
+  const int64_t fb = y_zeroed ? 0 : std::min<int64_t>((n + 4 * VB - 1) / (4 * VB), 2048);
   hipLaunchKernelGGL((csr_spmv_kernel<T, TD>), dim3((unsigned)(rb + fb)), dim3(VB), 0, s, n, indptr, indices, data,
                      rows, nrows, rb, x, y, freem, parts);
   return hipGetLastError();
 }
 template hipError_t launch_csr_spmv<double, double>(int64_t, const int32_t*, const int32_t*, const double*,
                                                     const int32_t*, int64_t, int, const double*, double*,
-                                                    const uint8_t*, double*, hipStream_t);
+                                                    const uint8_t*, double*, bool, hipStream_t);
 template hipError_t launch_csr_spmv<float, float>(int64_t, const int32_t*, const int32_t*, const float*,
                                                   const int32_t*, int64_t, int, const float*, float*, const uint8_t*,
-                                                  double*, hipStream_t);
+                                                  double*, bool, hipStream_t);
 
 }  // namespace icc
